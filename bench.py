@@ -1,0 +1,189 @@
+#!/usr/bin/env python3
+"""Benchmark of the nlchains hot path on one MI355X.
+
+Workload (default): DNKG m=1, beta=1, N=1024 sites, 16384 copies, dt=0.05,
+FP64 plain method, initial mode energies equal in the band k < N/8 at energy
+density 0.5 (seed 2).  One bench step = one batch of --batch Yoshida-6 steps
+(default 1000, the dump interval of this workload) over the whole ensemble
+followed by the linear-mode energy pass and the entropies (the reference's
+per-batch loop, DNKG_FPUT_Toda.cpp:108-143).
+
+value  = site-steps/s over the timed region (whole job, inputs resident in HBM).
+roofline: FP64 algorithmic FLOP of the step kernel (70 FLOP per DNKG
+          site-step) / the kernel's mean duration from HIP events recorded on
+          the context's stream, against the MI355X FP64 vector peak.
+cpu_baseline: the oracle's release-flags build (OpenMP over copies) on the
+          host cores, on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "site-steps/s plain FP64 vs mixed/batched at stated energy drift; % of roofline"
+FP64_PEAK_TFLOPS = 78.6
+HBM_PEAK_GBS = 8000.0
+
+# algorithmic FLOP per site-step (FMA = 2, exp = 0): 7 kicks + 7 drifts
+FLOP_PER_SITE_STEP = {"DNKG": 70, "dDNKG": 70, "FPUT": 119, "Toda": 56}
+
+WORKLOADS = {
+    "dnkg": dict(model="DNKG", n=1024, copies=16384, dt=0.05, m=1.0, alpha=0.0, beta=1.0, k_lo=0, k_hi=128,
+                 dens=0.5, seed=2, batch=1000),
+    "fput": dict(model="FPUT", n=1024, copies=16384, dt=0.05, m=0.0, alpha=0.0, beta=1.0, k_lo=1, k_hi=128,
+                 dens=0.5, seed=2, batch=1000),
+    "toda": dict(model="Toda", n=1024, copies=16384, dt=0.05, m=0.0, alpha=1.0, beta=0.0, k_lo=1, k_hi=52,
+                 dens=0.25, seed=4, batch=1000),
+}
+
+
+class HipEvents:
+    """hipEvent timing on an arbitrary stream (torch's events only see torch's stream)."""
+
+    def __init__(self):
+        self.hip = C.CDLL("libamdhip64.so")
+        self.hip.hipEventCreate.argtypes = [C.POINTER(C.c_void_p)]
+        self.hip.hipEventRecord.argtypes = [C.c_void_p, C.c_void_p]
+        self.hip.hipEventSynchronize.argtypes = [C.c_void_p]
+        self.hip.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), C.c_void_p, C.c_void_p]
+        self.hip.hipEventDestroy.argtypes = [C.c_void_p]
+
+    def create(self):
+        ev = C.c_void_p()
+        assert self.hip.hipEventCreate(C.byref(ev)) == 0
+        return ev
+
+    def record(self, ev, stream):
+        assert self.hip.hipEventRecord(ev, C.c_void_p(stream)) == 0
+
+    def elapsed_ms(self, a, b):
+        assert self.hip.hipEventSynchronize(b) == 0
+        ms = C.c_float()
+        assert self.hip.hipEventElapsedTime(C.byref(ms), a, b) == 0
+        return float(ms.value)
+
+    def destroy(self, ev):
+        self.hip.hipEventDestroy(ev)
+
+
+def cpu_baseline(w, s0, seconds_budget: float):
+    """Oracle release build, OpenMP over copies, on a bounded sample of the workload."""
+    from oracle import oracle as O
+    lib = O.fast()
+    cores = max(1, min(len(os.sched_getaffinity(0)), 16))
+    model = {"DNKG": O.DNKG, "FPUT": O.FPUT, "Toda": O.TODA}[w["model"]]
+    # calibrate on a small slice, then size the sample to the time budget
+    probe_copies = min(w["copies"], 64 * cores)
+    t = time.perf_counter()
+    O.move_real(model, s0[:probe_copies], 10, w["dt"], m=w["m"], alpha=w["alpha"], beta=w["beta"], threads=cores,
+                lib=lib)
+    rate = probe_copies * w["n"] * 10 / (time.perf_counter() - t)
+    copies = w["copies"]
+    steps = int(max(10, min(w["batch"], seconds_budget * rate / (copies * w["n"]))))
+    t = time.perf_counter()
+    O.move_real(model, s0, steps, w["dt"], m=w["m"], alpha=w["alpha"], beta=w["beta"], threads=cores, lib=lib)
+    el = time.perf_counter() - t
+    return {"value": copies * w["n"] * steps / el, "unit": "site-steps/s", "cores": cores, "kind": "port",
+            "sample": f"oracle/liboracle_fast.so (-O3 -ffast-math, OpenMP over copies), {w['model']} "
+                      f"N={w['n']} x {copies} copies x {steps} steps, {el:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50, help="timed bench steps (batches)")
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--batch", type=int, default=None, help="Yoshida steps per bench step")
+    ap.add_argument("--workload", default="dnkg", choices=sorted(WORKLOADS))
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU-baseline time budget (0 = skip)")
+    ap.add_argument("--lanes", type=int, default=0)
+    ap.add_argument("--sites", type=int, default=0)
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print(json.dumps({"error": "nlchains_amd runs on one GPU per process with no sharding; "
+                                   "multi-GPU runs are not part of this build (bench.py --gpus 1)"}))
+        sys.exit(2)
+
+    import nlchains_amd as N
+    w = dict(WORKLOADS[args.workload])
+    if args.batch:
+        w["batch"] = args.batch
+    model = N.MODELS[w["model"]]
+    n, copies, b = w["n"], w["copies"], w["batch"]
+    s0 = N.ic_plane_waves(n, copies, w["m"], w["k_lo"], w["k_hi"], w["dens"], w["seed"])
+
+    ev = HipEvents()
+    with N.Ensemble(model, n, copies, w["dt"], m=w["m"], alpha=w["alpha"], beta=w["beta"],
+                    lanes_per_chain=args.lanes, sites_per_lane=args.sites) as ens:
+        ens.upload(s0)
+        desc = ens.describe()
+        stream = ens.stream()
+        E0, _ = ens.chain_energies()
+        for _ in range(args.warmup):
+            ens.move(b)
+            ens.linenergies()
+        ens.sync()
+        starts = [ev.create() for _ in range(args.steps)]
+        stops = [ev.create() for _ in range(args.steps)]
+        t0 = time.perf_counter()
+        for i in range(args.steps):
+            ev.record(starts[i], stream)
+            ens.move(b)
+            ev.record(stops[i], stream)
+            ens.linenergies()  # energy pass + entropies, synchronous (results reach the host every batch)
+        ens.sync()
+        elapsed = time.perf_counter() - t0
+        kernel_ms = [ev.elapsed_ms(a, z) for a, z in zip(starts, stops)]
+        for e_ in starts + stops:
+            ev.destroy(e_)
+        E1, _ = ens.chain_energies()
+        e_lin, ent = ens.linenergies()
+
+    total_steps = (args.warmup + args.steps) * b
+    drift = float(np.mean(np.abs(E1 - E0) / np.abs(E0)))
+    site_steps = float(n) * copies * b * args.steps
+    value = site_steps / elapsed
+    k_ms = float(np.mean(kernel_ms))
+    flop = FLOP_PER_SITE_STEP[w["model"]] * float(n) * copies * b
+    achieved = flop / (k_ms * 1e-3) / 1e12
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "site-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded plane-wave initial conditions)",
+        "config": {"workload": f"{w['model']} m={w['m']} alpha={w['alpha']} beta={w['beta']} N={n} copies={copies} "
+                               f"dt={w['dt']}, plain FP64, {b} steps + energy pass + entropies per bench step",
+                   "kernel": desc, "yoshida_steps_per_bench_step": b},
+        "energy_drift": {"mean_rel": drift, "over_steps": total_steps},
+        "entropies_final": [float(ent[0]), float(ent[1])],
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "kernel": "k_step_wave (FP64 vector ALU bound)", "kernel_ms": k_ms,
+                     "flop_per_launch": flop},
+    }
+    if args.cpu_seconds > 0:
+        out["cpu_baseline"] = cpu_baseline(w, s0, args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,194 @@
+/*
+ * nlc.h -- C ABI of libnlchains_amd.so, the MI355X (gfx950) hot path of nlchains.
+ *
+ * The reference exposes its hot path as C++ device entry points per model
+ * (reference paths, relative to the reference tree):
+ *   DNKG_FPUT_Toda/DNKG_FPUT_Toda.hpp:19-23  move<Model>(plane2split*&, cudaStream_t),
+ *                                            make_linenergies(fft_phis, fft_pis, omegas, stream)
+ *   dDNKG/dDNKG.hpp:12-14                    move(plane2split&, bool& use_split, mp2, stream),
+ *                                            make_linenergies(proj_phi, proj_pi, stream)
+ *   DNLS/DNLS.hpp:16-20                      evolve_nonlinear / evolve_linear / make_linenergies
+ *   common/results.hpp:26-31                 results::calc_linenergies(norm).calc_entropies()
+ * with state in the globals gres.shard_gpu / gres.linenergies_gpu
+ * (common/configuration.hpp:67-78) and parameters in __constant__ symbols.
+ * This ABI replaces that seam with an opaque context: plain pointers and sizes,
+ * no exceptions (a negative return code plus nlc_last_error()).
+ *
+ * State layout is the reference shard/dump layout (README.md:74-78):
+ *   real models: double[copies][chain_length][2] = {q, p}
+ *   DNLS:        complex<double>[copies][chain_length]  (= double[..][..][2] {re, im})
+ *
+ * Threading: a context is used by one host thread at a time.  Device work is
+ * enqueued on the context's own HIP stream; calls marked "asynchronous" return
+ * before the work completes, all others synchronise the stream before returning.
+ */
+#ifndef NLC_H
+#define NLC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NLC_ABI_VERSION 1
+
+/* Return codes.  The reference throws instead (common/main.cpp:162-176 maps
+ * exceptions to exit code 1); here every failure is a negative code. */
+#define NLC_OK 0
+#define NLC_EARG (-1)        /* invalid argument: std::invalid_argument, common/main.cpp:83-103 */
+#define NLC_EHIP (-2)        /* HIP runtime failure: cuda_error from assertcu, common/utilities_cuda.cuh:19-37 */
+#define NLC_ENOMEM (-3)      /* device or pinned host allocation failed */
+#define NLC_EUNSUPPORTED (-4)/* valid request this build has no kernel for */
+#define NLC_ESTATE (-5)      /* call not valid in the context's current state */
+
+/* Subprograms of the reference registry (common/main.cpp:33-36, DNKG_FPUT_Toda.cpp:156-162,
+ * dDNKG.cpp:168-170, DNLS.cpp:193-195). */
+typedef enum { NLC_DNKG = 0, NLC_FPUT = 1, NLC_TODA = 2, NLC_DDNKG = 3, NLC_DNLS = 4 } nlc_model;
+
+/* Arithmetic of the step.  NLC_FP64 is the reference's all-double method. */
+typedef enum { NLC_FP64 = 0, NLC_FP32C = 1 } nlc_precision;
+
+/* Step-kernel family.  NLC_KERNEL_SPLIT is the reference's --split_kernel
+ * (DNKG_FPUT_Toda.cpp:43,91-94; dDNKG.cpp:34): HBM-streaming over the
+ * ensemble-major (split) layout.  NLC_KERNEL_WAVE keeps each chain in the
+ * registers of one or more wavefront lanes (the reference's move_chain_in_warp /
+ * move_chain_in_thread).  NLC_KERNEL_AUTO picks WAVE when a shape exists for
+ * the chain length, else SPLIT (the reference's move<> dispatch,
+ * DNKG_FPUT_Toda.cu:262-297). */
+typedef enum { NLC_KERNEL_AUTO = 0, NLC_KERNEL_WAVE = 1, NLC_KERNEL_SPLIT = 2 } nlc_kernel;
+
+/* Model and ensemble parameters: the reference's common options (gconf,
+ * common/configuration.hpp:20-39) and model options (DNKG_FPUT_Toda.cpp:26-43,
+ * dDNKG.cpp:31-34, DNLS.cpp:34-37). */
+typedef struct nlc_params {
+	uint32_t struct_size;     /* sizeof(nlc_params), set by nlc_params_init */
+	int32_t model;            /* nlc_model */
+	int32_t precision;        /* nlc_precision */
+	int32_t kernel;           /* nlc_kernel */
+	uint32_t chain_length;    /* -n, >= 2 */
+	uint32_t copies;          /* -c, >= 1 */
+	double dt;                /* --dt, > 0 */
+	double m;                 /* -m (DNKG linear parameter) */
+	double alpha;             /* --alpha (FPUT cubic term; Toda steepness, != 0) */
+	double beta;              /* --beta (DNKG, FPUT, dDNKG, DNLS) */
+	const double *masses;     /* dDNKG: m_x, chain_length doubles (content of the -m file); copied */
+	uint32_t lanes_per_chain; /* wave shape override (0 = default); used by the autotuner and tests */
+	uint32_t sites_per_lane;  /* wave shape override (0 = default) */
+} nlc_params;
+
+typedef struct nlc_ctx nlc_ctx;
+
+/* ---- library ---------------------------------------------------------- */
+
+int nlc_abi_version(void);
+const char *nlc_strerror(int code);
+/* Message of the last failure on ctx; with ctx == NULL, of the last failed
+ * nlc_create in the calling thread. */
+const char *nlc_last_error(const nlc_ctx *ctx);
+
+/* Defaults: FP64, AUTO kernel, everything else zero. */
+int nlc_params_init(nlc_params *p);
+
+/* ---- context life cycle ------------------------------------------------ */
+
+/* Validates p (the rules of common/main.cpp:83-103 that concern the hot path),
+ * selects the step kernel, allocates the device shard and work buffers on the
+ * current HIP device, creates the stream.  Replaces cuda_ctx.activate
+ * (common/utilities_cuda.cpp:13-28) plus the per-driver setup
+ * (DNKG_FPUT_Toda.cpp:47-95, dDNKG.cpp:36-70 incl. the eigensystem of
+ * dDNKG/eigensystem.cpp:13-36, DNLS.cpp:55-126). */
+int nlc_create(nlc_ctx **out, const nlc_params *p);
+void nlc_destroy(nlc_ctx *ctx);
+
+/* ---- state ------------------------------------------------------------- */
+
+/* Host -> device copy of the whole shard (the initial-state read of
+ * common/main.cpp:112-118 plus the H2D of utilities_cuda.cpp:22). */
+int nlc_upload(nlc_ctx *ctx, const void *state);
+/* Device -> host copy of the shard (the dumper of DNKG_FPUT_Toda.cpp:98-104). */
+int nlc_download(nlc_ctx *ctx, void *state);
+/* Device address and size of the shard (reference layout), for callers that
+ * move it with their own device copies. */
+int nlc_device_state(nlc_ctx *ctx, void **dptr, size_t *bytes);
+
+/* ---- hot path ---------------------------------------------------------- */
+
+/* Asynchronous.  Advances every chain by `steps` Yoshida-6 steps: the
+ * reference's move<Model> (DNKG_FPUT_Toda.cu:262-297), dDNKG::move
+ * (dDNKG.cu:199-234), and the DNLS split-step batch (DNLS.cpp:159-175), each
+ * with kernel_batching = steps.  Drift/kick merging as the reference: the
+ * c[7] drift of a step is fused with the c[0] drift of the next, and one final
+ * c[7] drift closes the batch. */
+int nlc_move(nlc_ctx *ctx, uint32_t steps);
+
+/* Entropy mode set: --entropymask semantics (common/main.cpp:119-128,
+ * common/results.cpp:8-15).  mask: chain_length bytes, non-zero = mode included,
+ * or NULL for "all modes" (minus mode 0 when a0is0).  a0is0: the reference's
+ * results(a0is0) flag; nlc_create sets the reference default per model
+ * (m == 0 for DNKG/FPUT/Toda, DNKG_FPUT_Toda.cpp:52; true for DNLS, DNLS.cpp:49;
+ * false for dDNKG, dDNKG.cpp:83).  Returns NLC_EARG for fewer than 2 modes. */
+int nlc_set_entropy_mask(nlc_ctx *ctx, const uint8_t *mask, int a0is0);
+
+/* Synchronous.  Linear-mode energy pass plus results::calc_linenergies(norm)
+ * and calc_entropies (common/results_entropy.cpp:11-46), all FP64 on the GPU:
+ *   linenergies[k] = norm * sum_copies e_k   (e_0 := 0 when a0is0)
+ *   entropies[0] = S_WT, entropies[1] = S_INF
+ * norm <= 0 selects the reference's normalisation for the model:
+ *   0.5/(copies*n) (DNKG_FPUT_Toda.cpp:129), 0.5/copies (dDNKG.cpp:141),
+ *   1/(n*copies) (DNLS.cpp:150).
+ * Either output pointer may be NULL.  Replaces the per-batch energy chain
+ * DNKG_FPUT_Toda.cpp:113-131, dDNKG.cpp:118-143, DNLS.cpp:142-152. */
+int nlc_linenergies(nlc_ctx *ctx, double norm, double *linenergies, double *entropies);
+
+/* Synchronous.  Raw per-mode sums over copies, before normalisation (the
+ * content of gres.linenergies after make_linenergies, e.g.
+ * DNKG_FPUT_Toda.cu:304-321). */
+int nlc_linenergies_raw(nlc_ctx *ctx, double *e_raw);
+
+/* Synchronous.  Total Hamiltonian of every copy, FP64 on the GPU, with the
+ * energy functions of supporting-material/post-processing/nlchains.m:40-48.
+ * For DNLS also norms[c] = sum |psi|^2 (norms may be NULL; ignored otherwise). */
+int nlc_chain_energies(nlc_ctx *ctx, double *energies, double *norms);
+
+/* Waits for all work enqueued on the context. */
+int nlc_sync(nlc_ctx *ctx);
+/* The context's hipStream_t (for events and graph capture by the caller). */
+void *nlc_stream(nlc_ctx *ctx);
+/* Human-readable description of the selected kernels and launch shapes. */
+int nlc_describe(const nlc_ctx *ctx, char *buf, size_t len);
+
+/* ---- host helpers (no device needed) ------------------------------------ */
+
+/* Yoshida-6 weights c[8], d[8]: common/symplectic.hpp:5-8. */
+int nlc_yoshida(double *c, double *d);
+/* Linear dispersion omega[n]: DNKG_FPUT_Toda/dispersion.cpp:9-16 for
+ * DNKG/FPUT/Toda (FPUT/Toda use m = 0), DNLS/dispersion.cpp:10-17 for DNLS. */
+int nlc_dispersion(int model, uint32_t n, double m, double *omega);
+/* Eigensystem of diag(mp2) - periodic adjacency (dDNKG/eigensystem.cpp:13-36):
+ * omegas ascending = sqrt(eigenvalues); eigenvectors column-major n x n.
+ * Householder tridiagonalisation + implicit QL. */
+int nlc_eigensystem(uint32_t n, const double *mp2, double *omegas, double *eigenvectors);
+
+/* Seeded initial conditions (SplitMix64 streams, one per copy).  The
+ * reference has no generator (it only reads -i files); these define the
+ * synthetic workloads.  All write the reference layout. */
+/* Running waves of the harmonic chain with omega_k^2 = m + 4 sin^2(pi k/n):
+ * modes k in [k_lo, k_hi) get energy e_density*n/(k_hi-k_lo) each, with
+ * uniform random phases; modes with omega_k == 0 are skipped. */
+int nlc_ic_plane_waves(uint32_t n, uint32_t copies, double m, uint32_t k_lo, uint32_t k_hi, double e_density,
+                       uint64_t seed, double *state);
+/* DNLS: psi_x = sum_{|k|<kmax} a e^{i(2 pi k x/n + phi_k)}, |a|^2 = norm_density/(2kmax-1). */
+int nlc_ic_dnls(uint32_t n, uint32_t copies, uint32_t kmax, double norm_density, uint64_t seed, double *state);
+/* dDNKG: standing eigenmodes 0..modes-1 of (omegas, eigenvectors), energy
+ * e_density*n/modes each, uniform random phases. */
+int nlc_ic_eigenmodes(uint32_t n, uint32_t copies, const double *omegas, const double *eigenvectors, uint32_t modes,
+                      double e_density, uint64_t seed, double *state);
+/* Per-site masses m_x ~ U[lo, hi) (dDNKG masses file content). */
+int nlc_ic_masses(uint32_t n, double lo, double hi, uint64_t seed, double *masses);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NLC_H */

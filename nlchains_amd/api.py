@@ -1,0 +1,248 @@
+"""ctypes binding of libnlchains_amd.so (include/nlc.h).
+
+The hot path runs only on the GPU through the C ABI; there is no CPU fallback.
+Importing this module fails loudly when the shared library is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+_PKG = Path(__file__).resolve().parent
+LIB_PATH = Path(os.environ.get("NLC_LIBRARY", _PKG / "libnlchains_amd.so"))
+
+if not LIB_PATH.exists():
+    raise ImportError(f"{LIB_PATH} is not built: run `python -m nlchains_amd.build` (or __graft_entry__.build())")
+
+lib = C.CDLL(str(LIB_PATH))
+
+DNKG, FPUT, TODA, DDNKG, DNLS = range(5)
+MODELS = {"DNKG": DNKG, "FPUT": FPUT, "Toda": TODA, "dDNKG": DDNKG, "DNLS": DNLS}
+FP64, FP32C = 0, 1
+KERNEL_AUTO, KERNEL_WAVE, KERNEL_SPLIT = 0, 1, 2
+
+NLC_OK, NLC_EARG, NLC_EHIP, NLC_ENOMEM, NLC_EUNSUPPORTED, NLC_ESTATE = 0, -1, -2, -3, -4, -5
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("struct_size", C.c_uint32),
+        ("model", C.c_int32),
+        ("precision", C.c_int32),
+        ("kernel", C.c_int32),
+        ("chain_length", C.c_uint32),
+        ("copies", C.c_uint32),
+        ("dt", C.c_double),
+        ("m", C.c_double),
+        ("alpha", C.c_double),
+        ("beta", C.c_double),
+        ("masses", C.POINTER(C.c_double)),
+        ("lanes_per_chain", C.c_uint32),
+        ("sites_per_lane", C.c_uint32),
+    ]
+
+
+_dp = C.POINTER(C.c_double)
+_ctx_p = C.c_void_p
+
+
+def _sig(name, res, *args):
+    f = getattr(lib, name)
+    f.restype = res
+    f.argtypes = list(args)
+    return f
+
+
+_sig("nlc_abi_version", C.c_int)
+_sig("nlc_strerror", C.c_char_p, C.c_int)
+_sig("nlc_last_error", C.c_char_p, _ctx_p)
+_sig("nlc_params_init", C.c_int, C.POINTER(Params))
+_sig("nlc_create", C.c_int, C.POINTER(_ctx_p), C.POINTER(Params))
+_sig("nlc_destroy", None, _ctx_p)
+_sig("nlc_upload", C.c_int, _ctx_p, C.c_void_p)
+_sig("nlc_download", C.c_int, _ctx_p, C.c_void_p)
+_sig("nlc_device_state", C.c_int, _ctx_p, C.POINTER(C.c_void_p), C.POINTER(C.c_size_t))
+_sig("nlc_move", C.c_int, _ctx_p, C.c_uint32)
+_sig("nlc_set_entropy_mask", C.c_int, _ctx_p, C.c_void_p, C.c_int)
+_sig("nlc_linenergies", C.c_int, _ctx_p, C.c_double, _dp, _dp)
+_sig("nlc_linenergies_raw", C.c_int, _ctx_p, _dp)
+_sig("nlc_chain_energies", C.c_int, _ctx_p, _dp, _dp)
+_sig("nlc_sync", C.c_int, _ctx_p)
+_sig("nlc_stream", C.c_void_p, _ctx_p)
+_sig("nlc_describe", C.c_int, _ctx_p, C.c_char_p, C.c_size_t)
+_sig("nlc_yoshida", C.c_int, _dp, _dp)
+_sig("nlc_dispersion", C.c_int, C.c_int, C.c_uint32, C.c_double, _dp)
+_sig("nlc_eigensystem", C.c_int, C.c_uint32, _dp, _dp, _dp)
+_sig("nlc_ic_plane_waves", C.c_int, C.c_uint32, C.c_uint32, C.c_double, C.c_uint32, C.c_uint32, C.c_double,
+     C.c_uint64, _dp)
+_sig("nlc_ic_dnls", C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, C.c_double, C.c_uint64, _dp)
+_sig("nlc_ic_eigenmodes", C.c_int, C.c_uint32, C.c_uint32, _dp, _dp, C.c_uint32, C.c_double, C.c_uint64, _dp)
+_sig("nlc_ic_masses", C.c_int, C.c_uint32, C.c_double, C.c_double, C.c_uint64, _dp)
+
+
+class NlcError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{lib.nlc_strerror(code).decode()} ({code}): {msg}")
+        self.code = code
+
+
+def _ptr(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_dp)
+
+
+def _check(rc: int, ctx=None):
+    if rc != NLC_OK:
+        raise NlcError(rc, lib.nlc_last_error(ctx).decode(errors="replace"))
+
+
+# ---- host helpers -----------------------------------------------------------
+
+def yoshida() -> tuple[np.ndarray, np.ndarray]:
+    c, d = np.zeros(8), np.zeros(8)
+    _check(lib.nlc_yoshida(_ptr(c), _ptr(d)))
+    return c, d
+
+
+def dispersion(model: int, n: int, m: float = 0.0) -> np.ndarray:
+    w = np.zeros(n)
+    _check(lib.nlc_dispersion(model, n, m, _ptr(w)))
+    return w
+
+
+def eigensystem(mp2: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
+    """(omegas, V) with V[:, j] the j-th eigenvector (stored column-major)."""
+    mp2 = np.ascontiguousarray(mp2, dtype=np.float64)
+    n = mp2.size
+    om = np.zeros(n)
+    vt = np.zeros((n, n))  # row j = eigenvector j (column-major storage of V)
+    _check(lib.nlc_eigensystem(n, _ptr(mp2), _ptr(om), _ptr(vt)))
+    return om, vt.T.copy()
+
+
+def ic_plane_waves(n: int, copies: int, m: float, k_lo: int, k_hi: int, e_density: float, seed: int) -> np.ndarray:
+    s = np.zeros((copies, n, 2))
+    _check(lib.nlc_ic_plane_waves(n, copies, m, k_lo, k_hi, e_density, seed, _ptr(s)))
+    return s
+
+
+def ic_dnls(n: int, copies: int, kmax: int, norm_density: float, seed: int) -> np.ndarray:
+    s = np.zeros((copies, n, 2))
+    _check(lib.nlc_ic_dnls(n, copies, kmax, norm_density, seed, _ptr(s)))
+    return s
+
+
+def ic_eigenmodes(n: int, copies: int, omegas: np.ndarray, V: np.ndarray, modes: int, e_density: float,
+                  seed: int) -> np.ndarray:
+    s = np.zeros((copies, n, 2))
+    vt = np.ascontiguousarray(V.T)  # column-major V
+    om = np.ascontiguousarray(omegas, dtype=np.float64)
+    _check(lib.nlc_ic_eigenmodes(n, copies, _ptr(om), _ptr(vt), modes, e_density, seed, _ptr(s)))
+    return s
+
+
+def ic_masses(n: int, lo: float, hi: float, seed: int) -> np.ndarray:
+    m = np.zeros(n)
+    _check(lib.nlc_ic_masses(n, lo, hi, seed, _ptr(m)))
+    return m
+
+
+# ---- ensembles on the GPU -----------------------------------------------------
+
+class Ensemble:
+    """One nlc_ctx: an ensemble of `copies` chains of length `n` resident on the GPU."""
+
+    def __init__(self, model: int, n: int, copies: int, dt: float, *, m: float = 0.0, alpha: float = 0.0,
+                 beta: float = 0.0, masses: np.ndarray | None = None, precision: int = FP64,
+                 kernel: int = KERNEL_AUTO, lanes_per_chain: int = 0, sites_per_lane: int = 0):
+        p = Params()
+        _check(lib.nlc_params_init(C.byref(p)))
+        p.model, p.precision, p.kernel = model, precision, kernel
+        p.chain_length, p.copies, p.dt, p.m, p.alpha, p.beta = n, copies, dt, m, alpha, beta
+        p.lanes_per_chain, p.sites_per_lane = lanes_per_chain, sites_per_lane
+        self._masses = None
+        if masses is not None:
+            self._masses = np.ascontiguousarray(masses, dtype=np.float64)
+            p.masses = _ptr(self._masses)
+        self.model, self.n, self.copies = model, n, copies
+        self._ctx = _ctx_p()
+        _check(lib.nlc_create(C.byref(self._ctx), C.byref(p)))
+
+    def close(self):
+        if self._ctx:
+            lib.nlc_destroy(self._ctx)
+            self._ctx = _ctx_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def handle(self):
+        return self._ctx
+
+    def upload(self, state: np.ndarray) -> None:
+        state = np.ascontiguousarray(state, dtype=np.float64)
+        assert state.shape == (self.copies, self.n, 2), state.shape
+        _check(lib.nlc_upload(self._ctx, state.ctypes.data_as(C.c_void_p)), self._ctx)
+
+    def download(self) -> np.ndarray:
+        s = np.empty((self.copies, self.n, 2))
+        _check(lib.nlc_download(self._ctx, s.ctypes.data_as(C.c_void_p)), self._ctx)
+        return s
+
+    def move(self, steps: int) -> None:
+        _check(lib.nlc_move(self._ctx, steps), self._ctx)
+
+    def sync(self) -> None:
+        _check(lib.nlc_sync(self._ctx), self._ctx)
+
+    def set_entropy_mask(self, mask: np.ndarray | None, a0is0: bool) -> None:
+        if mask is None:
+            _check(lib.nlc_set_entropy_mask(self._ctx, None, int(a0is0)), self._ctx)
+        else:
+            mk = np.ascontiguousarray(mask, dtype=np.uint8)
+            assert mk.size == self.n
+            _check(lib.nlc_set_entropy_mask(self._ctx, mk.ctypes.data_as(C.c_void_p), int(a0is0)), self._ctx)
+
+    def linenergies(self, norm: float = 0.0) -> tuple[np.ndarray, np.ndarray]:
+        e, s = np.zeros(self.n), np.zeros(2)
+        _check(lib.nlc_linenergies(self._ctx, norm, _ptr(e), _ptr(s)), self._ctx)
+        return e, s
+
+    def linenergies_raw(self) -> np.ndarray:
+        e = np.zeros(self.n)
+        _check(lib.nlc_linenergies_raw(self._ctx, _ptr(e)), self._ctx)
+        return e
+
+    def chain_energies(self) -> tuple[np.ndarray, np.ndarray]:
+        e, nr = np.zeros(self.copies), np.zeros(self.copies)
+        _check(lib.nlc_chain_energies(self._ctx, _ptr(e), _ptr(nr)), self._ctx)
+        return e, nr
+
+    def stream(self) -> int:
+        return lib.nlc_stream(self._ctx) or 0
+
+    def describe(self) -> str:
+        buf = C.create_string_buffer(256)
+        _check(lib.nlc_describe(self._ctx, buf, 256), self._ctx)
+        return buf.value.decode()
+
+
+def declared_symbols(header: Path | None = None) -> list[str]:
+    """Function names declared in include/nlc.h."""
+    import re
+    header = header or (_PKG.parent / "include" / "nlc.h")
+    text = header.read_text()
+    return sorted(set(re.findall(r"^[A-Za-z_][\w\s\*]*?\b(nlc_\w+)\s*\(", text, flags=re.M)))

@@ -1,0 +1,334 @@
+// Linear-mode energy pass, per-mode reduction over copies, normalisation and
+// entropies, and per-chain total energies (all FP64).
+//
+// Reference behaviour followed:
+//   plane-wave energies: DNKG_FPUT_Toda.cpp:113-123 (memset + 2x memcpy2D +
+//     cufftExecZ2Z of q and p separately) and make_linenergies_kernel
+//     DNKG_FPUT_Toda.cu:304-321 (thread per mode, serial loop over copies);
+//   DNLS energies: DNLS.cpp:142-144, DNLS.cu:70-90;
+//   normalisation and entropies: common/results_entropy.cpp:11-46 (host code
+//     in the reference, FP64 on the GPU here);
+//   chain energies: supporting-material/post-processing/nlchains.m:40-48.
+//
+// Design: one workgroup owns a block of chains; each chain is loaded once
+// (coalesced, 16 B/site), transformed in LDS with an in-place radix-4 Stockham
+// FFT, and its per-mode energies are accumulated in registers.  Because the
+// planar pair {q, p} already is the complex number z = q + i p, a single
+// complex FFT per chain yields both spectra:
+//   Q_k = (Z_k + conj Z_{n-k})/2,  P_k = (Z_k - conj Z_{n-k})/(2i)
+// and the reference's e_k = w(w|Q|^2 + 2(Q_i P_r - Q_r P_i)) + |P|^2 = |P - i w Q|^2
+// collapses to e_k = |(1+w) Z_k - (1-w) conj Z_{n-k}|^2 / 4 (half the FFT work
+// of transforming q and p separately).  Per-block partial sums are reduced in
+// a fixed order, so results are bit-reproducible run to run.
+#include "nlc_internal.hpp"
+
+namespace nlc {
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kMaxPerThread = kEnergyFftMax / 4 / kThreads; // radix-4 butterflies per thread per pass (n <= 4096)
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+	return make_double2(fma(a.x, b.x, -a.y * b.y), fma(a.x, b.y, a.y * b.x));
+}
+
+// In-place radix-4 (plus one final radix-2 pass when log2 n is odd) Stockham
+// FFT of buf[0..n), forward sign, natural-order output.  tw[j] = exp(-2 pi i j/n).
+// Each pass reads all inputs into registers, synchronises, then writes.
+__device__ void lds_fft(double2 *buf, uint32_t n, const double2 *__restrict__ tw) {
+	const uint32_t tid = threadIdx.x;
+	uint32_t ns = 1;
+	const uint32_t quarter = n >> 2;
+	for (; ns * 4 <= n; ns *= 4) {
+		double2 v[kMaxPerThread][4];
+		const uint32_t span = n / (ns * 4); // twiddle stride
+#pragma unroll
+		for (int t = 0; t < kMaxPerThread; ++t) {
+			const uint32_t j = tid + t * kThreads;
+			if (j < quarter) {
+#pragma unroll
+				for (int r = 0; r < 4; ++r) v[t][r] = buf[j + r * quarter];
+			}
+		}
+		__syncthreads();
+#pragma unroll
+		for (int t = 0; t < kMaxPerThread; ++t) {
+			const uint32_t j = tid + t * kThreads;
+			if (j < quarter) {
+				const uint32_t k = j % ns;
+				const double2 a0 = v[t][0];
+				const double2 a1 = cmul(v[t][1], tw[k * span]);
+				const double2 a2 = cmul(v[t][2], tw[2 * k * span]);
+				const double2 a3 = cmul(v[t][3], tw[3 * k * span]);
+				const double2 t0 = make_double2(a0.x + a2.x, a0.y + a2.y);
+				const double2 t1 = make_double2(a0.x - a2.x, a0.y - a2.y);
+				const double2 t2 = make_double2(a1.x + a3.x, a1.y + a3.y);
+				const double2 t3 = make_double2(a1.x - a3.x, a1.y - a3.y);
+				const uint32_t o = (j / ns) * ns * 4 + k;
+				buf[o] = make_double2(t0.x + t2.x, t0.y + t2.y);
+				buf[o + ns] = make_double2(t1.x + t3.y, t1.y - t3.x);     // t1 - i t3
+				buf[o + 2 * ns] = make_double2(t0.x - t2.x, t0.y - t2.y);
+				buf[o + 3 * ns] = make_double2(t1.x - t3.y, t1.y + t3.x); // t1 + i t3
+			}
+		}
+		__syncthreads();
+	}
+	if (ns < n) { // final radix-2 pass
+		const uint32_t half = n >> 1;
+		double2 v[2 * kMaxPerThread][2];
+#pragma unroll
+		for (int t = 0; t < 2 * kMaxPerThread; ++t) {
+			const uint32_t j = tid + t * kThreads;
+			if (j < half) v[t][0] = buf[j], v[t][1] = buf[j + half];
+		}
+		__syncthreads();
+#pragma unroll
+		for (int t = 0; t < 2 * kMaxPerThread; ++t) {
+			const uint32_t j = tid + t * kThreads;
+			if (j < half) {
+				const uint32_t k = j % ns;
+				const double2 b = cmul(v[t][1], tw[k * (n / (2 * ns))]);
+				const uint32_t o = (j / ns) * ns * 2 + k;
+				buf[o] = make_double2(v[t][0].x + b.x, v[t][0].y + b.y);
+				buf[o + ns] = make_double2(v[t][0].x - b.x, v[t][0].y - b.y);
+			}
+		}
+		__syncthreads();
+	}
+}
+
+// Direct DFT into buf2 (any n; used when n is not a power of two or exceeds
+// the LDS FFT): Z_k = sum_x z_x W^{kx mod n}.
+__device__ void lds_dft(const double2 *buf, double2 *out_k, uint32_t k, uint32_t n, const double2 *__restrict__ tw) {
+	double sr = 0, si = 0;
+	uint32_t idx = 0;
+	for (uint32_t x = 0; x < n; ++x) {
+		const double2 w = tw[idx], z = buf[x];
+		sr = fma(z.x, w.x, fma(-z.y, w.y, sr));
+		si = fma(z.x, w.y, fma(z.y, w.x, si));
+		idx += k;
+		if (idx >= n) idx -= n;
+	}
+	*out_k = make_double2(sr, si);
+}
+
+template <bool DNLS_MODE>
+__global__ void __launch_bounds__(kThreads)
+k_energy(const double2 *__restrict__ state, uint32_t n, uint32_t copies, const double2 *__restrict__ tw,
+         const double *__restrict__ omega, double *__restrict__ partial, int use_fft) {
+	extern __shared__ double2 lds[]; // n (FFT) or 2n (DFT) entries
+	constexpr int kMaxModes = kEnergyFftMax / kThreads;
+	double acc[kMaxModes];
+#pragma unroll
+	for (int t = 0; t < kMaxModes; ++t) acc[t] = 0.0;
+	const uint32_t tid = threadIdx.x;
+	// modes owned by this thread: k = tid + t*kThreads; the DFT path (n > 4096
+	// allowed) accumulates directly into partial instead.
+	for (uint32_t c = blockIdx.x; c < copies; c += gridDim.x) {
+		const double2 *chain = state + size_t(c) * n;
+		for (uint32_t x = tid; x < n; x += kThreads) lds[x] = chain[x];
+		__syncthreads();
+		if (use_fft) {
+			lds_fft(lds, n, tw);
+#pragma unroll
+			for (int t = 0; t < kMaxModes; ++t) {
+				const uint32_t k = tid + t * kThreads;
+				if (k < n) {
+					const double2 z = lds[k];
+					if constexpr (DNLS_MODE) {
+						acc[t] += z.x * z.x + z.y * z.y;
+					} else {
+						const double2 zm = lds[(n - k) & (n - 1)];
+						const double w = omega[k];
+						const double a = 1.0 + w, b = 1.0 - w;
+						const double er = fma(a, z.x, -b * zm.x); // Re[(1+w)Z - (1-w)conj(Zm)]
+						const double ei = fma(a, z.y, b * zm.y);  // Im[...]
+						acc[t] += 0.25 * fma(er, er, ei * ei);
+					}
+				}
+			}
+			__syncthreads();
+		} else {
+			double2 *spec = lds + n;
+			for (uint32_t k = tid; k < n; k += kThreads) lds_dft(lds, &spec[k], k, n, tw);
+			__syncthreads();
+			for (uint32_t k = tid; k < n; k += kThreads) {
+				const double2 z = spec[k];
+				double e;
+				if constexpr (DNLS_MODE) {
+					e = z.x * z.x + z.y * z.y;
+				} else {
+					const double2 zm = spec[k ? n - k : 0];
+					const double w = omega[k];
+					const double a = 1.0 + w, b = 1.0 - w;
+					const double er = fma(a, z.x, -b * zm.x);
+					const double ei = fma(a, z.y, b * zm.y);
+					e = 0.25 * fma(er, er, ei * ei);
+				}
+				partial[size_t(blockIdx.x) * n + k] += e;
+			}
+			__syncthreads();
+		}
+	}
+	if (use_fft) {
+#pragma unroll
+		for (int t = 0; t < kMaxModes; ++t) {
+			const uint32_t k = tid + t * kThreads;
+			if (k < n) partial[size_t(blockIdx.x) * n + k] = acc[t];
+		}
+	}
+}
+
+__global__ void k_zero(double *p, size_t count) {
+	for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < count; i += size_t(gridDim.x) * blockDim.x)
+		p[i] = 0.0;
+}
+
+__global__ void k_reduce_partial(const double *__restrict__ partial, uint32_t grid, uint32_t n,
+                                 const double *__restrict__ scale, double *__restrict__ e_raw) {
+	const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+	if (k >= n) return;
+	double s = 0.0;
+	for (uint32_t g = 0; g < grid; ++g) s += partial[size_t(g) * n + k];
+	e_raw[k] = scale ? s * scale[k] : s;
+}
+
+// Single workgroup: normalisation (results_entropy.cpp:11-19) and entropies
+// (results_entropy.cpp:21-46) with a fixed-order tree reduction.
+__global__ void __launch_bounds__(kThreads)
+k_results(const double *__restrict__ e_raw, uint32_t n, double norm, int a0is0, const uint32_t *__restrict__ idx,
+          uint32_t nidx, double *__restrict__ out) {
+	__shared__ double red[3][kThreads];
+	const uint32_t tid = threadIdx.x;
+	for (uint32_t k = tid; k < n; k += kThreads) out[k] = (a0is0 && k == 0) ? 0.0 : e_raw[k] * norm;
+	double s_e = 0, s_log = 0, s_elog = 0;
+	const uint32_t first = a0is0 ? 1 : 0;
+	const uint32_t cnt = nidx ? nidx : n - first;
+	for (uint32_t j = tid; j < cnt; j += kThreads) {
+		const uint32_t k = nidx ? idx[j] : first + j;
+		const double e = (a0is0 && k == 0) ? 0.0 : e_raw[k] * norm;
+		const double lg = log(e);
+		s_e += e;
+		s_log += lg;
+		s_elog += e * lg;
+	}
+	red[0][tid] = s_e;
+	red[1][tid] = s_log;
+	red[2][tid] = s_elog;
+	__syncthreads();
+	for (uint32_t w = kThreads / 2; w > 0; w >>= 1) {
+		if (tid < w) {
+			red[0][tid] += red[0][tid + w];
+			red[1][tid] += red[1][tid + w];
+			red[2][tid] += red[2][tid + w];
+		}
+		__syncthreads();
+	}
+	if (tid == 0) {
+		const double tot = red[0][0], totlog = red[1][0], totelog = red[2][0];
+		const double normalization = double(cnt) / tot, lognorm = log(normalization);
+		out[n] = -(totlog + double(cnt) * lognorm);
+		out[n + 1] = normalization * (totelog + lognorm * tot);
+	}
+}
+
+// One workgroup per chain: total energy (and DNLS norm), fixed-order reduction.
+__global__ void __launch_bounds__(kThreads)
+k_chain_energy(int model, const double2 *__restrict__ state, uint32_t n, StepArgs a,
+               const double *__restrict__ masses, double *__restrict__ energy, double *__restrict__ norm) {
+	__shared__ double red[2][kThreads];
+	const uint32_t c = blockIdx.x, tid = threadIdx.x;
+	const double2 *s = state + size_t(c) * n;
+	double e = 0, nr = 0;
+	for (uint32_t x = tid; x < n; x += kThreads) {
+		const double2 v = s[x], w = s[x + 1 == n ? 0 : x + 1];
+		if (model == DNLS) {
+			const double dr = w.x - v.x, di = w.y - v.y, a2 = v.x * v.x + v.y * v.y;
+			e += dr * dr + di * di + a.beta * 0.5 * a2 * a2;
+			nr += a2;
+		} else {
+			const double q = v.x, p = v.y, d = w.x - q;
+			double h = 0.5 * p * p;
+			switch (model) {
+			case DNKG: h += 0.5 * d * d + 0.5 * a.m * q * q + 0.25 * a.beta * q * q * q * q; break;
+			case DDNKG: h += 0.5 * d * d + 0.5 * masses[x] * q * q + 0.25 * a.beta * q * q * q * q; break;
+			case FPUT: h += 0.5 * d * d + a.alpha * d * d * d / 3.0 + 0.25 * a.beta * d * d * d * d; break;
+			case TODA: {
+				const double al = 0.5 * a.a2, inv4a2 = 1.0 / (4.0 * al * al);
+				h += exp(a.a2 * d) * inv4a2 - d * a.a2i - inv4a2;
+				break;
+			}
+			}
+			e += h;
+		}
+	}
+	red[0][tid] = e;
+	red[1][tid] = nr;
+	__syncthreads();
+	for (uint32_t w = kThreads / 2; w > 0; w >>= 1) {
+		if (tid < w) red[0][tid] += red[0][tid + w], red[1][tid] += red[1][tid + w];
+		__syncthreads();
+	}
+	if (tid == 0) {
+		energy[c] = red[0][0];
+		if (norm) norm[c] = red[1][0];
+	}
+}
+
+bool fft_ok(uint32_t n) { return n >= 2 && (n & (n - 1)) == 0 && n <= kEnergyFftMax; }
+
+} // namespace
+
+static hipError_t launch_energy_common(hipStream_t st, bool dnls, const double2 *state, uint32_t n, uint32_t copies,
+                                       const double2 *tw, const double *omega, double *partial, uint32_t grid) {
+	const int use_fft = fft_ok(n) ? 1 : 0;
+	const size_t lds = sizeof(double2) * n * (use_fft ? 1 : 2);
+	if (lds > 160 * 1024) return hipErrorInvalidValue;
+	if (!use_fft) {
+		const size_t cnt = size_t(grid) * n;
+		hipLaunchKernelGGL(k_zero, dim3(1024), dim3(256), 0, st, partial, cnt);
+	}
+	if (lds > 64 * 1024) {
+		hipError_t e = hipFuncSetAttribute(dnls ? (const void *) k_energy<true> : (const void *) k_energy<false>,
+		                                   hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+		if (e != hipSuccess) return e;
+	}
+	if (dnls)
+		hipLaunchKernelGGL(k_energy<true>, dim3(grid), dim3(kThreads), lds, st, state, n, copies, tw, omega, partial,
+		                   use_fft);
+	else
+		hipLaunchKernelGGL(k_energy<false>, dim3(grid), dim3(kThreads), lds, st, state, n, copies, tw, omega, partial,
+		                   use_fft);
+	return hipGetLastError();
+}
+
+hipError_t launch_energy_real(hipStream_t st, const double2 *state, uint32_t n, uint32_t copies, const double2 *tw,
+                              const double *omega, double *partial, uint32_t grid) {
+	return launch_energy_common(st, false, state, n, copies, tw, omega, partial, grid);
+}
+
+hipError_t launch_energy_dnls(hipStream_t st, const double2 *state, uint32_t n, uint32_t copies, const double2 *tw,
+                              double *partial, uint32_t grid) {
+	return launch_energy_common(st, true, state, n, copies, tw, nullptr, partial, grid);
+}
+
+hipError_t launch_reduce_partial(hipStream_t st, const double *partial, uint32_t grid, uint32_t n, const double *scale,
+                                 double *e_raw) {
+	hipLaunchKernelGGL(k_reduce_partial, dim3((n + 255) / 256), dim3(256), 0, st, partial, grid, n, scale, e_raw);
+	return hipGetLastError();
+}
+
+hipError_t launch_results(hipStream_t st, const double *e_raw, uint32_t n, double norm, int a0is0, const uint32_t *idx,
+                          uint32_t nidx, double *out) {
+	hipLaunchKernelGGL(k_results, dim3(1), dim3(kThreads), 0, st, e_raw, n, norm, a0is0, idx, nidx, out);
+	return hipGetLastError();
+}
+
+hipError_t launch_chain_energy(hipStream_t st, int model, const double2 *state, uint32_t n, uint32_t copies,
+                               const StepArgs &args, const double *masses, double *energy, double *norm) {
+	hipLaunchKernelGGL(k_chain_energy, dim3(copies), dim3(kThreads), 0, st, model, state, n, args, masses, energy,
+	                   norm);
+	return hipGetLastError();
+}
+
+} // namespace nlc

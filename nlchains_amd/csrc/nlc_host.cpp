@@ -1,0 +1,324 @@
+// Host-side helpers of libnlchains_amd: Yoshida weights, dispersion relations,
+// the dDNKG eigensystem and the seeded initial-condition generators.  Plain
+// C++ (no device code); exported through include/nlc.h.
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <thread>
+#include <vector>
+
+#include "../../include/nlc.h"
+
+namespace nlc {
+
+// Yoshida (1990) 6th-order weights, common/symplectic.hpp:5-8: c[8] drift, d[8] kick.
+extern const double kYoshidaC[8];
+extern const double kYoshidaD[8];
+const double kYoshidaC[8] = {0.392256805238780,  0.510043411918458,  -0.471053385409757, 0.068753168252518,
+                             0.068753168252518,  -0.471053385409757, 0.510043411918458,  0.392256805238780};
+const double kYoshidaD[8] = {0.784513610477560,  0.235573213359357, -1.177679984178870, 1.315186320683906,
+                             -1.177679984178870, 0.235573213359357, 0.784513610477560,  0};
+
+namespace {
+
+// Parallel for over [0, count) on the host's hardware threads.
+template <class F>
+void parallel_for(uint32_t count, F &&f) {
+	unsigned nt = std::max(1u, std::min(std::thread::hardware_concurrency(), 32u));
+	if (count < 2 * nt) nt = 1;
+	if (nt == 1) {
+		for (uint32_t i = 0; i < count; ++i) f(i);
+		return;
+	}
+	std::vector<std::thread> pool;
+	for (unsigned t = 0; t < nt; ++t)
+		pool.emplace_back([&, t] {
+			for (uint32_t i = t; i < count; i += nt) f(i);
+		});
+	for (auto &th : pool) th.join();
+}
+
+// SplitMix64 (Steele, Lea, Flood 2014): a 64-bit counter-based generator.
+struct SplitMix64 {
+	uint64_t s;
+	explicit SplitMix64(uint64_t seed) : s(seed) {}
+	uint64_t next() {
+		uint64_t z = (s += 0x9E3779B97F4A7C15ULL);
+		z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+		z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+		return z ^ (z >> 31);
+	}
+	double uniform() { return double(next() >> 11) * 0x1.0p-53; } // [0, 1), 53 bits
+};
+
+// Independent stream per copy: seed mixed with the copy index.
+SplitMix64 copy_stream(uint64_t seed, uint64_t copy) {
+	SplitMix64 h(seed ^ (0xD1B54A32D192ED03ULL * (copy + 1)));
+	return SplitMix64(h.next());
+}
+
+// cos/sin(2 pi j / n), j < n
+void unit_roots(uint32_t n, std::vector<double> &cs, std::vector<double> &sn) {
+	cs.resize(n);
+	sn.resize(n);
+	for (uint32_t j = 0; j < n; ++j) {
+		const double ang = 2.0 * M_PI * double(j) / double(n);
+		cs[j] = std::cos(ang);
+		sn[j] = std::sin(ang);
+	}
+}
+
+// Householder reduction of the symmetric matrix a (column-major n x n) to
+// tridiagonal form T = Q^T A Q (Golub & Van Loan, Matrix Computations, 8.3.1).
+// On return d = diag(T), e[i] = T(i+1, i) (e[n-1] = 0) and q = Q (column-major).
+void tridiagonalize(uint32_t n, std::vector<double> &a, std::vector<double> &q, std::vector<double> &d,
+                    std::vector<double> &e) {
+	auto A = [&](uint32_t i, uint32_t j) -> double & { return a[size_t(j) * n + i]; };
+	std::vector<std::vector<double>> reflectors(n > 2 ? n - 2 : 0);
+	std::vector<double> v(n), w(n);
+	for (uint32_t k = 0; k + 2 < n; ++k) {
+		// reflector mapping x = A(k+1.., k) onto alpha e_1
+		double norm2 = 0;
+		for (uint32_t i = k + 1; i < n; ++i) norm2 += A(i, k) * A(i, k);
+		const double x0 = A(k + 1, k);
+		const double alpha = -std::copysign(std::sqrt(norm2), x0);
+		double vv = 0;
+		for (uint32_t i = k + 1; i < n; ++i) {
+			v[i] = A(i, k) - (i == k + 1 ? alpha : 0.0);
+			vv += v[i] * v[i];
+		}
+		auto &ref = reflectors[k];
+		if (vv == 0) continue; // column already reduced
+		const double tau = 2.0 / vv;
+		ref.assign(v.begin() + k + 1, v.end());
+		// symmetric rank-2 update of the trailing block: A <- H A H
+		for (uint32_t i = k + 1; i < n; ++i) {
+			double s = 0;
+			for (uint32_t j = k + 1; j < n; ++j) s += A(i, j) * v[j];
+			w[i] = tau * s;
+		}
+		double kk = 0;
+		for (uint32_t i = k + 1; i < n; ++i) kk += v[i] * w[i];
+		kk *= 0.5 * tau;
+		for (uint32_t i = k + 1; i < n; ++i) w[i] -= kk * v[i];
+		for (uint32_t j = k + 1; j < n; ++j)
+			for (uint32_t i = k + 1; i < n; ++i) A(i, j) -= v[i] * w[j] + w[i] * v[j];
+		A(k + 1, k) = A(k, k + 1) = alpha;
+		for (uint32_t i = k + 2; i < n; ++i) A(i, k) = A(k, i) = 0.0;
+	}
+	d.assign(n, 0.0);
+	e.assign(n, 0.0);
+	for (uint32_t i = 0; i < n; ++i) d[i] = A(i, i);
+	for (uint32_t i = 0; i + 1 < n; ++i) e[i] = A(i + 1, i);
+	// Q = H_0 H_1 ... H_{n-3}, accumulated backwards onto the identity
+	q.assign(size_t(n) * n, 0.0);
+	for (uint32_t i = 0; i < n; ++i) q[size_t(i) * n + i] = 1.0;
+	for (int64_t k = int64_t(n) - 3; k >= 0; --k) {
+		const auto &ref = reflectors[k];
+		if (ref.empty()) continue;
+		double vv = 0;
+		for (double x : ref) vv += x * x;
+		const double tau = 2.0 / vv;
+		for (uint32_t j = uint32_t(k) + 1; j < n; ++j) {
+			double s = 0;
+			for (uint32_t i = uint32_t(k) + 1; i < n; ++i) s += ref[i - k - 1] * q[size_t(j) * n + i];
+			s *= tau;
+			for (uint32_t i = uint32_t(k) + 1; i < n; ++i) q[size_t(j) * n + i] -= s * ref[i - k - 1];
+		}
+	}
+}
+
+// Implicit symmetric QL iteration with Wilkinson-type shifts on the
+// tridiagonal (d, e) (Bowdler, Martin, Reinsch & Wilkinson, Numer. Math. 11,
+// 1968), applying every plane rotation to the columns of z.
+bool tridiagonal_ql(uint32_t n, std::vector<double> &d, std::vector<double> &e, std::vector<double> &z) {
+	const double eps = std::numeric_limits<double>::epsilon();
+	for (uint32_t l = 0; l < n; ++l) {
+		for (int iter = 0;; ++iter) {
+			uint32_t m = l;
+			for (; m + 1 < n; ++m) {
+				const double dd = std::fabs(d[m]) + std::fabs(d[m + 1]);
+				if (std::fabs(e[m]) <= eps * dd) break;
+			}
+			if (m == l) break;
+			if (iter == 64) return false;
+			// shift from the leading 2x2 block
+			double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+			double r = std::hypot(g, 1.0);
+			g = d[m] - d[l] + e[l] / (g + std::copysign(r, g));
+			double s = 1.0, c = 1.0, p = 0.0;
+			bool deflated = false;
+			for (int64_t i = int64_t(m) - 1; i >= int64_t(l); --i) {
+				const double f = s * e[i], b = c * e[i];
+				r = std::hypot(f, g);
+				e[i + 1] = r;
+				if (r == 0.0) { // underflow: split the matrix here
+					d[i + 1] -= p;
+					e[m] = 0.0;
+					deflated = true;
+					break;
+				}
+				s = f / r;
+				c = g / r;
+				g = d[i + 1] - p;
+				r = (d[i] - g) * s + 2.0 * c * b;
+				p = s * r;
+				d[i + 1] = g + p;
+				g = c * r - b;
+				double *zi = &z[size_t(i) * n], *zi1 = &z[size_t(i + 1) * n];
+				for (uint32_t k = 0; k < n; ++k) {
+					const double t = zi1[k];
+					zi1[k] = s * zi[k] + c * t;
+					zi[k] = c * zi[k] - s * t;
+				}
+			}
+			if (deflated) continue;
+			d[l] -= p;
+			e[l] = g;
+			e[m] = 0.0;
+		}
+	}
+	return true;
+}
+
+} // namespace
+
+int eigensystem(uint32_t n, const double *mp2, double *omegas, double *V) {
+	if (n < 2 || !mp2 || !omegas || !V) return NLC_EARG;
+	// A = diag(m_x + 2) - adjacency, periodic corners (dDNKG/eigensystem.cpp:15-18)
+	std::vector<double> a(size_t(n) * n, 0.0), q, d, e;
+	for (uint32_t i = 0; i < n; ++i) {
+		a[size_t(i) * n + i] = mp2[i];
+		if (i + 1 < n) a[size_t(i + 1) * n + i] = a[size_t(i) * n + i + 1] = -1.0;
+	}
+	a[size_t(n - 1) * n + 0] = a[0 * size_t(n) + n - 1] = -1.0;
+	tridiagonalize(n, a, q, d, e);
+	if (!tridiagonal_ql(n, d, e, q)) return NLC_EUNSUPPORTED;
+	std::vector<uint32_t> ord(n);
+	for (uint32_t i = 0; i < n; ++i) ord[i] = i;
+	std::stable_sort(ord.begin(), ord.end(), [&](uint32_t x, uint32_t y) { return d[x] < d[y]; });
+	for (uint32_t j = 0; j < n; ++j) {
+		omegas[j] = std::sqrt(d[ord[j]]); // omega = sqrt(lambda), dDNKG/eigensystem.cpp:20
+		std::memcpy(V + size_t(j) * n, &q[size_t(ord[j]) * n], sizeof(double) * n);
+	}
+	return NLC_OK;
+}
+
+} // namespace nlc
+
+using namespace nlc;
+
+extern "C" {
+
+int nlc_yoshida(double *c, double *d) {
+	if (!c || !d) return NLC_EARG;
+	std::memcpy(c, kYoshidaC, sizeof kYoshidaC);
+	std::memcpy(d, kYoshidaD, sizeof kYoshidaD);
+	return NLC_OK;
+}
+
+int nlc_dispersion(int model, uint32_t n, double m, double *omega) {
+	if (!omega || n < 2) return NLC_EARG;
+	for (uint32_t k = 0; k < n; ++k) {
+		const double s2 = 2 * std::sin(k * M_PI / n);
+		if (model == NLC_DNLS) omega[k] = s2 * s2;              // DNLS/dispersion.cpp:10-17
+		else if (model == NLC_DNKG || model == NLC_FPUT || model == NLC_TODA)
+			omega[k] = std::sqrt((model == NLC_DNKG ? m : 0.0) + s2 * s2); // DNKG_FPUT_Toda/dispersion.cpp:9-16
+		else return NLC_EARG;
+	}
+	return NLC_OK;
+}
+
+int nlc_eigensystem(uint32_t n, const double *mp2, double *omegas, double *eigenvectors) {
+	return eigensystem(n, mp2, omegas, eigenvectors);
+}
+
+int nlc_ic_plane_waves(uint32_t n, uint32_t copies, double m, uint32_t k_lo, uint32_t k_hi, double e_density,
+                       uint64_t seed, double *state) {
+	if (n < 2 || !copies || !state || k_lo >= k_hi || k_hi > n || e_density < 0) return NLC_EARG;
+	std::vector<double> cs, sn, omega(n);
+	unit_roots(n, cs, sn);
+	for (uint32_t k = 0; k < n; ++k) {
+		const double s2 = 2 * std::sin(k * M_PI / n);
+		omega[k] = std::sqrt(m + s2 * s2);
+	}
+	const double e_mode = e_density * n / double(k_hi - k_lo);
+	parallel_for(copies, [&](uint32_t c) {
+		SplitMix64 rng = copy_stream(seed, c);
+		double *chain = state + size_t(c) * n * 2;
+		for (uint32_t x = 0; x < 2 * n; ++x) chain[x] = 0.0;
+		for (uint32_t k = k_lo; k < k_hi; ++k) {
+			const double phi = 2.0 * M_PI * rng.uniform();
+			if (!(omega[k] > 0)) continue;
+			const double amp = std::sqrt(2.0 * e_mode / (n * omega[k] * omega[k]));
+			const double cp = std::cos(phi), sp = std::sin(phi);
+			uint64_t j = 0;
+			for (uint32_t x = 0; x < n; ++x, j += k) {
+				const uint32_t jj = uint32_t(j % n);
+				// q = A cos(2 pi k x/n + phi), p = A w sin(2 pi k x/n + phi): a running wave
+				chain[2 * x] += amp * (cs[jj] * cp - sn[jj] * sp);
+				chain[2 * x + 1] += amp * omega[k] * (sn[jj] * cp + cs[jj] * sp);
+			}
+		}
+	});
+	return NLC_OK;
+}
+
+int nlc_ic_dnls(uint32_t n, uint32_t copies, uint32_t kmax, double norm_density, uint64_t seed, double *state) {
+	if (n < 2 || !copies || !state || !kmax || 2 * kmax - 1 > n || norm_density < 0) return NLC_EARG;
+	std::vector<double> cs, sn;
+	unit_roots(n, cs, sn);
+	const double amp = std::sqrt(norm_density / double(2 * kmax - 1));
+	parallel_for(copies, [&](uint32_t c) {
+		SplitMix64 rng = copy_stream(seed, c);
+		double *chain = state + size_t(c) * n * 2;
+		for (uint32_t x = 0; x < 2 * n; ++x) chain[x] = 0.0;
+		for (int64_t k = -int64_t(kmax) + 1; k < int64_t(kmax); ++k) {
+			const double phi = 2.0 * M_PI * rng.uniform();
+			const double cp = std::cos(phi), sp = std::sin(phi);
+			const uint64_t kk = uint64_t((k % int64_t(n) + int64_t(n)) % int64_t(n));
+			uint64_t j = 0;
+			for (uint32_t x = 0; x < n; ++x, j += kk) {
+				const uint32_t jj = uint32_t(j % n);
+				chain[2 * x] += amp * (cs[jj] * cp - sn[jj] * sp);
+				chain[2 * x + 1] += amp * (sn[jj] * cp + cs[jj] * sp);
+			}
+		}
+	});
+	return NLC_OK;
+}
+
+int nlc_ic_eigenmodes(uint32_t n, uint32_t copies, const double *omegas, const double *V, uint32_t modes,
+                      double e_density, uint64_t seed, double *state) {
+	if (n < 2 || !copies || !state || !omegas || !V || !modes || modes > n || e_density < 0) return NLC_EARG;
+	const double e_mode = e_density * n / double(modes);
+	parallel_for(copies, [&](uint32_t c) {
+		SplitMix64 rng = copy_stream(seed, c);
+		double *chain = state + size_t(c) * n * 2;
+		for (uint32_t x = 0; x < 2 * n; ++x) chain[x] = 0.0;
+		for (uint32_t j = 0; j < modes; ++j) {
+			const double phi = 2.0 * M_PI * rng.uniform();
+			if (!(omegas[j] > 0)) continue;
+			// standing mode q = A cos(phi) v_j, p = -A w sin(phi) v_j with A^2 w^2 / 2 = e_mode
+			const double amp = std::sqrt(2.0 * e_mode) / omegas[j];
+			const double cq = amp * std::cos(phi), cp = -amp * omegas[j] * std::sin(phi);
+			const double *vj = V + size_t(j) * n;
+			for (uint32_t x = 0; x < n; ++x) {
+				chain[2 * x] += cq * vj[x];
+				chain[2 * x + 1] += cp * vj[x];
+			}
+		}
+	});
+	return NLC_OK;
+}
+
+int nlc_ic_masses(uint32_t n, double lo, double hi, uint64_t seed, double *masses) {
+	if (n < 1 || !masses || !(hi >= lo)) return NLC_EARG;
+	SplitMix64 rng = copy_stream(seed, 0xFFFFFFFFULL);
+	for (uint32_t x = 0; x < n; ++x) masses[x] = lo + (hi - lo) * rng.uniform();
+	return NLC_OK;
+}
+
+} // extern "C"

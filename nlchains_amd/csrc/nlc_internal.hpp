@@ -1,0 +1,77 @@
+// Internal declarations shared by the gfx950 kernels and the C-ABI layer.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+
+namespace nlc {
+
+enum Model { DNKG = 0, FPUT = 1, TODA = 2, DDNKG = 3, DNLS = 4 };
+
+// Scaled Yoshida weights and model constants, passed by value as a kernel
+// argument (they land in SGPRs).  dc[k] = c_k*dt, dd[k] = d_k*dt, computed on
+// the host exactly as the reference does (DNKG_FPUT_Toda.cpp:76-77).
+struct StepArgs {
+	double dc[8];
+	double dd[8];
+	double m;     // DNKG linear parameter
+	double m2;    // DNKG: m + 2
+	double beta;
+	double alpha; // FPUT cubic coefficient
+	double a2;    // Toda: 2*alpha
+	double a2i;   // Toda: 1/(2*alpha)
+};
+
+// Enqueues `steps` Yoshida steps of a wave-resident step kernel.
+using StepLauncher = hipError_t (*)(hipStream_t, double2 *state, uint32_t copies, uint32_t n, uint32_t steps,
+                                    const StepArgs &args, const double *mp2);
+
+// Wave shapes: a chain occupies L lanes of a wavefront, S consecutive sites
+// per lane (64/L chains per wavefront).  `exact` means n == L*S; otherwise the
+// last L*S-n lanes hold S-1 sites (requires (S-1)*L < n).
+StepLauncher lookup_step_dnkg(int L, int S, bool exact);
+StepLauncher lookup_step_fput(int L, int S, bool exact);
+StepLauncher lookup_step_toda(int L, int S, bool exact);
+StepLauncher lookup_step_ddnkg(int L, int S, bool exact);
+
+// HBM-streaming step over the split layout q[n][copies], p[n][copies].
+hipError_t launch_step_split(hipStream_t, int model, double *q, double *p, uint32_t copies, uint32_t n,
+                             uint32_t steps, const StepArgs &args, const double *mp2);
+// planar double2[copies][n] <-> split q[n][copies], p[n][copies]
+hipError_t launch_planar_to_split(hipStream_t, const double2 *planar, double *q, double *p, uint32_t copies,
+                                  uint32_t n);
+hipError_t launch_split_to_planar(hipStream_t, const double *q, const double *p, double2 *planar, uint32_t copies,
+                                  uint32_t n);
+
+// Energy pass of the plane-wave models (DNKG/FPUT/Toda): per-block partial sums
+// of e_k over copies into partial[grid][n].  tw[j] = exp(-2 pi i j/n).
+hipError_t launch_energy_real(hipStream_t, const double2 *state, uint32_t n, uint32_t copies, const double2 *tw,
+                              const double *omega, double *partial, uint32_t grid);
+// DNLS energies: partial[g][k] = sum |psi_hat_k|^2 (omega applied in the reduction).
+hipError_t launch_energy_dnls(hipStream_t, const double2 *state, uint32_t n, uint32_t copies, const double2 *tw,
+                              double *partial, uint32_t grid);
+// e_raw[k] = scale_k * sum_g partial[g][k] (scale = omega for DNLS, else NULL = 1).
+hipError_t launch_reduce_partial(hipStream_t, const double *partial, uint32_t grid, uint32_t n,
+                                 const double *scale, double *e_raw);
+// out[0..n) = normalised linenergies, out[n] = S_WT, out[n+1] = S_INF.
+hipError_t launch_results(hipStream_t, const double *e_raw, uint32_t n, double norm, int a0is0,
+                          const uint32_t *idx, uint32_t nidx, double *out);
+// Total energy per copy (and DNLS norm).
+hipError_t launch_chain_energy(hipStream_t, int model, const double2 *state, uint32_t n, uint32_t copies,
+                               const StepArgs &args, const double *masses, double *energy, double *norm);
+
+// Largest chain length the LDS FFT of the energy pass handles (power of two).
+constexpr uint32_t kEnergyFftMax = 4096;
+
+} // namespace nlc
+
+namespace nlc {
+// dDNKG projection energies: partial[rows][n] with rows = ddnkg_partial_rows(copies).
+uint32_t ddnkg_partial_rows(uint32_t copies);
+hipError_t launch_energy_ddnkg(hipStream_t, const double2 *state, uint32_t n, uint32_t copies, const double *V,
+                               const double *omega, double *partial);
+// Host eigensystem (nlc_host.cpp).
+int eigensystem(uint32_t n, const double *mp2, double *omegas, double *V);
+extern const double kYoshidaC[8], kYoshidaD[8];
+} // namespace nlc

@@ -1,0 +1,231 @@
+// Wave-resident Yoshida-6 step kernels for the real models (gfx950, wave64).
+//
+// Included once per model by nlc_step_<model>.hip with NLC_STEP_MODEL and
+// NLC_STEP_LOOKUP defined, so the models compile in parallel.
+//
+// Reference behaviour followed: DNKG_FPUT_Toda.cu:82-147 (move_chain_in_warp),
+// :156-190 (move_chain_in_thread), dDNKG.cu:25-128; forces DNKG_FPUT_Toda.cu:21-74
+// and dDNKG.cu:17-19.
+//
+// Design (MI355X-first, not a translation of the warp kernel):
+//  * a chain occupies L lanes of a 64-wide wavefront, S consecutive sites per
+//    lane held in VGPRs for the whole batch of steps; 64/L chains share a
+//    wavefront, so short chains do not waste lanes (L = 1 is thread-per-chain);
+//  * nearest-neighbour halos move between lanes with DPP (wave_ror/wave_rol for
+//    L = 64, row_ror for L = 16, quad_perm for L = 4 and 2) or ds_bpermute
+//    (L = 32, 8), never through LDS memory;
+//  * the kick of DNKG/dDNKG is 5 VALU ops per site, the drift 1 FMA;
+//    FPUT and Toda exchange the bond force of the lane's first bond instead of
+//    recomputing it, so each bond term (a cubic or an FP64 exp) is evaluated
+//    once per substep;
+//  * the 7 substeps are unrolled with their weights in SGPRs; the c7 drift of a
+//    step is fused with the c0 drift of the next (the reference's 2*c0 merge).
+//  * HBM is touched once per batch: 16 B/site in, 16 B/site out.
+#include "nlc_internal.hpp"
+
+namespace nlc {
+namespace {
+
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {
+	const int lo = __double2loint(v), hi = __double2hiint(v);
+	// every control used here reads a valid lane, so bound_ctrl never applies
+	const int rlo = __builtin_amdgcn_mov_dpp(lo, CTRL, 0xF, 0xF, true);
+	const int rhi = __builtin_amdgcn_mov_dpp(hi, CTRL, 0xF, 0xF, true);
+	return __hiloint2double(rhi, rlo);
+}
+
+__device__ __forceinline__ double bperm_f64(double v, int src_lane) {
+	const int addr = src_lane << 2;
+	const int lo = __builtin_amdgcn_ds_bpermute(addr, __double2loint(v));
+	const int hi = __builtin_amdgcn_ds_bpermute(addr, __double2hiint(v));
+	return __hiloint2double(hi, lo);
+}
+
+// DPP controls (GFX9 encoding): wave_rol:1 = 0x134, wave_ror:1 = 0x13C,
+// row_ror:n = 0x120+n, quad_perm = 8-bit lane selector.
+constexpr int kWaveRol1 = 0x134;
+constexpr int kWaveRor1 = 0x13C;
+constexpr int kRowRor1 = 0x121;
+constexpr int kRowRor15 = 0x12F;
+constexpr int kQuadLeft = (3 << 0) | (0 << 2) | (1 << 4) | (2 << 6);  // lane i <- lane (i-1)&3
+constexpr int kQuadRight = (1 << 0) | (2 << 2) | (3 << 4) | (0 << 6); // lane i <- lane (i+1)&3
+constexpr int kQuadSwap = (1 << 0) | (0 << 2) | (3 << 4) | (2 << 6);  // lane i <- lane i^1
+
+// Value held by the lane on the left (lane-1 within the chain's L-lane ring).
+template <int L>
+__device__ __forceinline__ double from_left(double v, int lane) {
+	if constexpr (L == 1) return v;
+	else if constexpr (L == 2) return dpp_f64<kQuadSwap>(v);
+	else if constexpr (L == 4) return dpp_f64<kQuadLeft>(v);
+	else if constexpr (L == 16) return dpp_f64<kRowRor1>(v);
+	else if constexpr (L == 64) return dpp_f64<kWaveRor1>(v);
+	else return bperm_f64(v, (lane & ~(L - 1)) | ((lane - 1) & (L - 1)));
+}
+
+// Value held by the lane on the right (lane+1 within the ring).
+template <int L>
+__device__ __forceinline__ double from_right(double v, int lane) {
+	if constexpr (L == 1) return v;
+	else if constexpr (L == 2) return dpp_f64<kQuadSwap>(v);
+	else if constexpr (L == 4) return dpp_f64<kQuadRight>(v);
+	else if constexpr (L == 16) return dpp_f64<kRowRor15>(v);
+	else if constexpr (L == 64) return dpp_f64<kWaveRol1>(v);
+	else return bperm_f64(v, (lane & ~(L - 1)) | ((lane + 1) & (L - 1)));
+}
+
+// Bond force term of FPUT / Toda for the bond x = q_{i} - q_{i-1}; the force on
+// site i is bond(x_{i+1}) - bond(x_i).
+//   FPUT: V'(x) = x + alpha x^2 + beta x^3  (Horner), the same force as
+//         (l+r-2c)(1+alpha(r-l)) + beta(d_r^3-d_l^3) of DNKG_FPUT_Toda.cu:41-58;
+//   Toda: e^{2 alpha x}/(2 alpha)  (DNKG_FPUT_Toda.cu:62-74; the constant
+//         -1/(2 alpha) of V' cancels in the difference).  FP64 exp of ROCm's
+//         device math library.
+template <int MODEL>
+__device__ __forceinline__ double bond_force(double x, const StepArgs &a) {
+	if constexpr (MODEL == FPUT) return x * fma(x, fma(a.beta, x, a.alpha), 1.0);
+	else return a.a2i * exp(a.a2 * x);
+}
+
+template <int MODEL, int L, int S, bool EXACT>
+struct Chain {
+	double q[S], p[S];
+	double mp[MODEL == DDNKG ? S : 1];
+	bool full; // lane holds S real sites (else S-1 and a dummy last slot)
+	int lane;
+
+	__device__ __forceinline__ void drift(double c) {
+#pragma unroll
+		for (int j = 0; j < S; ++j) q[j] = fma(c, p[j], q[j]);
+	}
+
+	__device__ __forceinline__ void kick(double d, const StepArgs &a) {
+		// value of this lane's last real site, which the right neighbour needs
+		const double send = (EXACT || full) ? q[S - 1] : q[S > 1 ? S - 2 : 0];
+		const double hl = from_left<L>(send, lane);
+		if constexpr (MODEL == DNKG || MODEL == DDNKG) {
+			const double hr = from_right<L>(q[0], lane);
+			if (!EXACT && !full) q[S - 1] = hr;
+#pragma unroll
+			for (int j = 0; j < S; ++j) {
+				const double lft = (j == 0) ? hl : q[j - 1];
+				const double rgt = (j == S - 1) ? hr : q[j + 1];
+				const double mass = (MODEL == DDNKG) ? mp[j] : a.m2;
+				// F = l + r - q (m + 2 + beta q^2)
+				const double t = fma(a.beta, q[j] * q[j], mass);
+				const double f = fma(-q[j], t, lft + rgt);
+				p[j] = fma(d, f, p[j]);
+			}
+			if (!EXACT && !full) p[S - 1] = 0.0;
+		} else {
+			double b[S];
+			b[0] = bond_force<MODEL>(q[0] - hl, a);
+#pragma unroll
+			for (int j = 1; j < S; ++j) b[j] = bond_force<MODEL>(q[j] - q[j - 1], a);
+			const double bs = from_right<L>(b[0], lane);
+			if (!EXACT && !full) b[S - 1] = bs; // the dummy slot gets zero force
+#pragma unroll
+			for (int j = 0; j < S; ++j) {
+				const double br = (j == S - 1) ? bs : b[j + 1];
+				p[j] = fma(d, br - b[j], p[j]);
+			}
+		}
+	}
+};
+
+template <int MODEL, int L, int S, bool EXACT>
+__global__ void __launch_bounds__(256)
+k_step_wave(double2 *__restrict__ state, uint32_t copies, uint32_t n, uint32_t steps, StepArgs a,
+            const double *__restrict__ mp2g) {
+	constexpr int CPW = 64 / L; // chains per wavefront
+	const int lane = threadIdx.x & 63;
+	const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+	const uint32_t first_chain = wave * CPW;
+	if (first_chain >= copies) return; // whole wavefront idle: uniform exit
+	const int l = lane & (L - 1);
+	const uint32_t chain = first_chain + lane / L;
+	const bool live = chain < copies; // lanes of a missing chain still take part in the halo exchange
+
+	Chain<MODEL, L, S, EXACT> ch;
+	ch.lane = lane;
+	const int full_lanes = EXACT ? L : int(n) - (S - 1) * L;
+	ch.full = EXACT || l < full_lanes;
+	const uint32_t off = ch.full ? uint32_t(l * S) : uint32_t(full_lanes * S + (l - full_lanes) * (S - 1));
+	double2 *base = state + size_t(live ? chain : 0) * n + off;
+
+#pragma unroll
+	for (int j = 0; j < S; ++j) {
+		const bool real_site = EXACT || j < S - 1 || ch.full;
+		double2 v = make_double2(0.0, 0.0);
+		if (live && real_site) v = base[j];
+		ch.q[j] = v.x;
+		ch.p[j] = v.y;
+		if constexpr (MODEL == DDNKG) ch.mp[j] = real_site ? mp2g[off + j] : 0.0;
+	}
+
+	const double dc0 = a.dc[0];
+	ch.drift(dc0);
+	for (uint32_t s = 0; s < steps; ++s) {
+		const double c_last = (s + 1 < steps) ? 2.0 * dc0 : a.dc[7];
+		if constexpr (MODEL == TODA) {
+			// exp is long: keep the substep loop rolled to bound code size
+#pragma unroll 1
+			for (int k = 0; k < 7; ++k) {
+				ch.kick(a.dd[k], a);
+				ch.drift(k < 6 ? a.dc[k + 1] : c_last);
+			}
+		} else {
+#pragma unroll
+			for (int k = 0; k < 7; ++k) {
+				ch.kick(a.dd[k], a);
+				ch.drift(k < 6 ? a.dc[k + 1] : c_last);
+			}
+		}
+	}
+
+	if (live) {
+#pragma unroll
+		for (int j = 0; j < S; ++j)
+			if (EXACT || j < S - 1 || ch.full) base[j] = make_double2(ch.q[j], ch.p[j]);
+	}
+}
+
+template <int MODEL, int L, int S, bool EXACT>
+hipError_t launch_wave(hipStream_t st, double2 *state, uint32_t copies, uint32_t n, uint32_t steps,
+                       const StepArgs &a, const double *mp2) {
+	constexpr uint32_t CPW = 64 / L;
+	const uint64_t waves = (uint64_t(copies) + CPW - 1) / CPW;
+	const uint64_t blocks = (waves + 3) / 4;
+	hipLaunchKernelGGL((k_step_wave<MODEL, L, S, EXACT>), dim3(uint32_t(blocks)), dim3(256), 0, st, state, copies, n,
+	                   steps, a, mp2);
+	return hipGetLastError();
+}
+
+} // namespace
+
+// Shapes instantiated: every power-of-two (L, S) with 2 <= L*S <= 2048, S <= 32
+// (exact fit), plus L = 64 with S = 2..32 for any n in (64(S-1), 64S].
+StepLauncher NLC_STEP_LOOKUP(int L, int S, bool exact) {
+#define NLC_EXACT(LL, SS)                                                                                          \
+	if (exact && L == LL && S == SS) return &launch_wave<NLC_STEP_MODEL, LL, SS, true>;
+#define NLC_RAGGED64(SS)                                                                                           \
+	if (!exact && L == 64 && S == SS) return &launch_wave<NLC_STEP_MODEL, 64, SS, false>;
+	NLC_EXACT(1, 2) NLC_EXACT(1, 4) NLC_EXACT(1, 8) NLC_EXACT(1, 16) NLC_EXACT(1, 32)
+	NLC_EXACT(2, 1) NLC_EXACT(2, 2) NLC_EXACT(2, 4) NLC_EXACT(2, 8) NLC_EXACT(2, 16) NLC_EXACT(2, 32)
+	NLC_EXACT(4, 1) NLC_EXACT(4, 2) NLC_EXACT(4, 4) NLC_EXACT(4, 8) NLC_EXACT(4, 16) NLC_EXACT(4, 32)
+	NLC_EXACT(8, 1) NLC_EXACT(8, 2) NLC_EXACT(8, 4) NLC_EXACT(8, 8) NLC_EXACT(8, 16) NLC_EXACT(8, 32)
+	NLC_EXACT(16, 1) NLC_EXACT(16, 2) NLC_EXACT(16, 4) NLC_EXACT(16, 8) NLC_EXACT(16, 16) NLC_EXACT(16, 32)
+	NLC_EXACT(32, 1) NLC_EXACT(32, 2) NLC_EXACT(32, 4) NLC_EXACT(32, 8) NLC_EXACT(32, 16) NLC_EXACT(32, 32)
+	NLC_EXACT(64, 1) NLC_EXACT(64, 2) NLC_EXACT(64, 4) NLC_EXACT(64, 8) NLC_EXACT(64, 16) NLC_EXACT(64, 32)
+	NLC_RAGGED64(2) NLC_RAGGED64(3) NLC_RAGGED64(4) NLC_RAGGED64(5) NLC_RAGGED64(6) NLC_RAGGED64(7)
+	NLC_RAGGED64(8) NLC_RAGGED64(9) NLC_RAGGED64(10) NLC_RAGGED64(11) NLC_RAGGED64(12) NLC_RAGGED64(13)
+	NLC_RAGGED64(14) NLC_RAGGED64(15) NLC_RAGGED64(16) NLC_RAGGED64(17) NLC_RAGGED64(18) NLC_RAGGED64(19)
+	NLC_RAGGED64(20) NLC_RAGGED64(21) NLC_RAGGED64(22) NLC_RAGGED64(23) NLC_RAGGED64(24) NLC_RAGGED64(25)
+	NLC_RAGGED64(26) NLC_RAGGED64(27) NLC_RAGGED64(28) NLC_RAGGED64(29) NLC_RAGGED64(30) NLC_RAGGED64(31)
+	NLC_RAGGED64(32)
+#undef NLC_EXACT
+#undef NLC_RAGGED64
+	return nullptr;
+}
+
+} // namespace nlc

@@ -1,0 +1,221 @@
+"""GPU parity of the FP64 hot path against the CPU oracle, through the C ABI.
+
+Bars (the plain-form acceptance of the build):
+  * q, p after 100 steps within 1e-10 relative (max-norm over the shard);
+  * normalised linear energies and both entropies within 1e-9 relative;
+  * chain energies within 1e-12 relative.
+Inputs are seeded synthetic ensembles (the reference ships no initial states).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+STATE_TOL = 1e-10
+ENERGY_TOL = 1e-9
+
+
+def banner(msg):
+    print(f"[gpu-parity] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def rel_err(got, want):
+    return np.abs(got - want).max() / np.abs(want).max()
+
+
+MODEL_KW = {
+    "DNKG": dict(m=1.0, beta=1.0),
+    "FPUT": dict(alpha=0.25, beta=1.0),
+    "Toda": dict(alpha=1.0),
+    "dDNKG": dict(beta=1.0),
+}
+
+
+def make_case(N, model_name, n, copies, seed, dens=0.5):
+    model = N.MODELS[model_name]
+    kw = dict(MODEL_KW[model_name])
+    masses = None
+    if model == N.DDNKG:
+        masses = N.ic_masses(n, 0.5, 1.5, seed)
+        om, V = N.eigensystem(masses + 2)
+        s = N.ic_eigenmodes(n, copies, om, V, max(1, n // 8), dens, seed)
+    else:
+        m = kw.get("m", 0.0)
+        k_lo = 0 if m > 0 else 1
+        s = N.ic_plane_waves(n, copies, m, k_lo, max(k_lo + 1, n // 8), dens, seed)
+    return model, kw, masses, s
+
+
+def oracle_move(model, kw, masses, s, steps, dt):
+    mp2 = None if masses is None else masses + 2
+    okw = {k: v for k, v in kw.items()}
+    return O.move_real(model, s, steps, dt, mp2=mp2, threads=1, **okw)
+
+
+def check_move(N, model_name, n, copies, steps=100, dt=0.05, seed=1, **shape):
+    banner(f"move {model_name} n={n} copies={copies} steps={steps} shape={shape}")
+    model, kw, masses, s0 = make_case(N, model_name, n, copies, seed)
+    with N.Ensemble(model, n, copies, dt, masses=masses, **kw, **shape) as ens:
+        ens.upload(s0)
+        ens.move(steps)
+        got = ens.download()
+        desc = ens.describe()
+    want = oracle_move(model, kw, masses, s0, steps, dt)
+    err = rel_err(got, want)
+    banner(f"  {desc}: rel err {err:.3e}")
+    assert np.isfinite(got).all()
+    assert err <= STATE_TOL, (desc, err)
+    return got
+
+
+@pytest.mark.parametrize("model_name", ["DNKG", "FPUT", "Toda", "dDNKG"])
+@pytest.mark.parametrize("n,copies", [(64, 96), (1024, 40)])
+def test_move_default_shape(gpu, model_name, n, copies):
+    check_move(gpu, model_name, n, copies)
+
+
+@pytest.mark.parametrize("n,L,S", [(64, 8, 8), (64, 16, 4), (64, 32, 2), (64, 64, 1), (64, 2, 32), (64, 4, 16),
+                                   (32, 1, 32), (32, 4, 8), (16, 2, 8), (8, 1, 8), (2, 1, 2), (128, 64, 2),
+                                   (2048, 64, 32), (512, 16, 32), (256, 8, 32)])
+def test_move_wave_shapes(gpu, n, L, S):
+    check_move(gpu, "DNKG", n, 37, steps=60, lanes_per_chain=L, sites_per_lane=S)
+    check_move(gpu, "FPUT", n, 37, steps=60, lanes_per_chain=L, sites_per_lane=S)
+
+
+@pytest.mark.parametrize("n", [65, 100, 1000, 1500, 2047])
+@pytest.mark.parametrize("model_name", ["DNKG", "Toda", "dDNKG"])
+def test_move_ragged_lanes(gpu, model_name, n):
+    """n not a multiple of 64: the last lanes hold one site fewer."""
+    check_move(gpu, model_name, n, 21, steps=50)
+
+
+@pytest.mark.parametrize("model_name", ["DNKG", "FPUT", "Toda", "dDNKG"])
+@pytest.mark.parametrize("n", [3, 64, 2500])
+def test_move_split_kernel(gpu, model_name, n):
+    """The HBM-streaming kernel (reference --split_kernel, and the fallback
+    for chain lengths without a wave shape)."""
+    shape = dict(kernel=gpu.KERNEL_SPLIT) if n == 64 else {}
+    check_move(gpu, model_name, n, 33, steps=40, **shape)
+
+
+def test_move_single_copy_and_single_step(gpu):
+    check_move(gpu, "DNKG", 1024, 1, steps=1)
+    check_move(gpu, "FPUT", 64, 1, steps=1)
+
+
+def test_move_batches_compose(gpu):
+    """b + b steps in one call equals two calls of b (the merged c7/c0 drift
+    is undone at the end of every batch), as the reference's resume semantics need."""
+    N = gpu
+    model, kw, masses, s0 = make_case(N, "DNKG", 1024, 8, 3)
+    with N.Ensemble(model, 1024, 8, 0.05, **kw) as a, N.Ensemble(model, 1024, 8, 0.05, **kw) as b:
+        a.upload(s0)
+        b.upload(s0)
+        a.move(40)
+        b.move(20)
+        b.move(20)
+        assert rel_err(a.download(), b.download()) < 1e-13
+
+
+def test_move_is_deterministic(gpu):
+    N = gpu
+    model, kw, masses, s0 = make_case(N, "Toda", 512, 64, 4)
+    outs = []
+    for _ in range(2):
+        with N.Ensemble(model, 512, 64, 0.05, **kw) as e:
+            e.upload(s0)
+            e.move(30)
+            outs.append(e.download())
+    assert np.array_equal(outs[0], outs[1])
+
+
+# ------------------------------------------------------------ energy pass
+
+@pytest.mark.parametrize("model_name,n,copies", [("DNKG", 64, 50), ("DNKG", 1024, 64), ("FPUT", 256, 33),
+                                                 ("Toda", 128, 20), ("dDNKG", 512, 70), ("dDNKG", 100, 9),
+                                                 ("DNKG", 3000, 3), ("FPUT", 96, 5)])
+def test_linenergies_and_entropy(gpu, model_name, n, copies):
+    N = gpu
+    banner(f"linenergies {model_name} n={n} copies={copies}")
+    model, kw, masses, s0 = make_case(N, model_name, n, copies, 9)
+    s0 = s0 + 0.01 * np.random.default_rng(1).normal(size=s0.shape)  # populate every mode
+    with N.Ensemble(model, n, copies, 0.05, masses=masses, **kw) as ens:
+        ens.upload(s0)
+        e_gpu, s_gpu = ens.linenergies()
+        raw_gpu = ens.linenergies_raw()
+    if model == N.DDNKG:
+        om, V = N.eigensystem(masses + 2)
+        raw = O.linenergies_ddnkg(s0, V, om)
+        norm, a0is0 = 0.5 / copies, False
+    else:
+        raw = O.linenergies_real(s0, O.dispersion_real(n, kw.get("m", 0.0)))
+        norm, a0is0 = 0.5 / copies / n, kw.get("m", 0.0) == 0.0
+    e = O.calc_linenergies(raw, norm, a0is0)
+    wt, inf = O.entropies(e, a0is0)
+    banner(f"  e rel {rel_err(e_gpu, e):.3e}  S_WT {s_gpu[0]:.12g}/{wt:.12g}  S_INF {s_gpu[1]:.12g}/{inf:.12g}")
+    assert rel_err(raw_gpu, raw) <= ENERGY_TOL
+    assert rel_err(e_gpu, e) <= ENERGY_TOL
+    assert abs(s_gpu[0] - wt) <= ENERGY_TOL * max(1, abs(wt))
+    assert abs(s_gpu[1] - inf) <= ENERGY_TOL * max(1, abs(inf))
+
+
+def test_entropy_mask_on_gpu(gpu):
+    N = gpu
+    n, copies = 128, 16
+    model, kw, masses, s0 = make_case(N, "FPUT", n, copies, 5)
+    s0 = s0 + 0.01 * np.random.default_rng(2).normal(size=s0.shape)
+    mask = np.zeros(n, np.uint8)
+    mask[[0, 1, 2, 5, 9, 40, 77]] = 1
+    with N.Ensemble(model, n, copies, 0.05, **kw) as ens:
+        ens.upload(s0)
+        ens.set_entropy_mask(mask, True)
+        e_gpu, s_gpu = ens.linenergies()
+        with pytest.raises(N.NlcError):
+            ens.set_entropy_mask(np.eye(1, n, 0, dtype=np.uint8)[0] + np.eye(1, n, 1, dtype=np.uint8)[0], True)
+    e = O.calc_linenergies(O.linenergies_real(s0, O.dispersion_real(n, 0.0)), 0.5 / copies / n, True)
+    idx = O.entropy_modes(n, mask, True)
+    wt, inf = O.entropies(e, True, idx)
+    assert abs(s_gpu[0] - wt) <= ENERGY_TOL * abs(wt) and abs(s_gpu[1] - inf) <= ENERGY_TOL * abs(inf)
+
+
+@pytest.mark.parametrize("model_name", ["DNKG", "FPUT", "Toda", "dDNKG"])
+def test_chain_energies(gpu, model_name):
+    N = gpu
+    model, kw, masses, s0 = make_case(N, model_name, 256, 12, 6)
+    with N.Ensemble(model, 256, 12, 0.05, masses=masses, **kw) as ens:
+        ens.upload(s0)
+        E, _ = ens.chain_energies()
+    want = O.chain_energy(model, s0, masses=masses, **kw)
+    np.testing.assert_allclose(E, want, rtol=1e-12)
+
+
+# ------------------------------------------------ full-size property checks
+
+def test_full_size_dnkg_sampled_parity_and_drift(gpu):
+    """At the benchmark shape (1024 sites x 16384 copies): chains are
+    independent, so a sample of copies re-run on the oracle checks parity at
+    full size; energy drift over 1000 steps and run-to-run determinism are
+    size-independent properties."""
+    N = gpu
+    n, copies, dt, steps = 1024, 16384, 0.05, 200
+    banner("full-size DNKG 1024 x 16384")
+    s0 = N.ic_plane_waves(n, copies, 1.0, 0, n // 8, 0.5, 2)
+    with N.Ensemble(N.DNKG, n, copies, dt, m=1.0, beta=1.0) as ens:
+        ens.upload(s0)
+        E0, _ = ens.chain_energies()
+        ens.move(steps)
+        got = ens.download()
+        E1, _ = ens.chain_energies()
+        e_lin, ent = ens.linenergies()
+    pick = np.random.default_rng(0).choice(copies, 6, replace=False)
+    want = O.move_real(O.DNKG, s0[pick], steps, dt, m=1.0, beta=1.0)
+    assert rel_err(got[pick], want) <= STATE_TOL
+    drift = np.mean(np.abs(E1 - E0) / E0)
+    banner(f"  sampled parity ok, mean relative energy drift {drift:.3e}, entropies {ent}")
+    assert drift < 1e-9
+    assert np.isfinite(e_lin).all() and np.isfinite(ent).all()
