@@ -216,6 +216,11 @@ def test_full_size_dnkg_sampled_parity_and_drift(gpu):
     want = O.move_real(O.DNKG, s0[pick], steps, dt, m=1.0, beta=1.0)
     assert rel_err(got[pick], want) <= STATE_TOL
     drift = np.mean(np.abs(E1 - E0) / E0)
-    banner(f"  sampled parity ok, mean relative energy drift {drift:.3e}, entropies {ent}")
-    assert drift < 1e-9
+    banner(f"  sampled parity ok, mean relative energy error {drift:.3e}, entropies {ent}")
+    # The energy error at this dt is the integrator's truncation error (same
+    # method on both sides): the GPU's must equal the oracle's chain by chain.
+    E0_o = O.chain_energy(O.DNKG, s0[pick], m=1.0, beta=1.0)
+    E1_o = O.chain_energy(O.DNKG, want, m=1.0, beta=1.0)
+    np.testing.assert_allclose((E1[pick] - E0[pick]) / E0[pick], (E1_o - E0_o) / E0_o, rtol=0, atol=1e-12)
+    assert drift < 1e-6
     assert np.isfinite(e_lin).all() and np.isfinite(ent).all()
