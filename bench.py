@@ -98,6 +98,27 @@ def cpu_baseline(w, s0, seconds_budget: float):
                       f"N={w['n']} x {copies} copies x {steps} steps, {el:.1f} s"}
 
 
+def copy_calibration(mib: int) -> None:
+    """Device-to-device copies of a known size (ROCm runs them as a blit
+    kernel on the compute queue): the reference point that scales the
+    FETCH_SIZE / WRITE_SIZE counters of a profiled run to bytes."""
+    hip = C.CDLL("libamdhip64.so")
+    hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    hip.hipMemset.argtypes = [C.c_void_p, C.c_int, C.c_size_t]
+    hip.hipFree.argtypes = [C.c_void_p]
+    nbytes = mib << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), nbytes) == 0 and hip.hipMalloc(C.byref(b), nbytes) == 0
+    assert hip.hipMemset(a, 1, nbytes) == 0
+    for _ in range(3):
+        assert hip.hipMemcpy(b, a, nbytes, 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_calibration_bytes": nbytes, "copies": 3}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -108,7 +129,12 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU-baseline time budget (0 = skip)")
     ap.add_argument("--lanes", type=int, default=0)
     ap.add_argument("--sites", type=int, default=0)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MiB",
+                    help="only run device-to-device copies of this size (counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print(json.dumps({"error": "nlchains_amd runs on one GPU per process with no sharding; "

@@ -31,6 +31,7 @@ struct nlc_ctx {
 	double *omega = nullptr;   // n
 	double2 *tw = nullptr;     // n roots of unity exp(-2 pi i j/n)
 	double *partial = nullptr; // energy partial sums
+	double *reduce_scratch = nullptr;
 	uint32_t partial_rows = 0;
 	double *e_raw = nullptr;   // n
 	double *results = nullptr; // n + 2
@@ -42,6 +43,8 @@ struct nlc_ctx {
 	double *eigvec = nullptr;  // dDNKG V, column-major
 	double *split_q = nullptr, *split_p = nullptr;
 	double *energy_d = nullptr, *norm_d = nullptr;
+	double2 *dnls_table = nullptr; // DNLS Fourier multipliers
+	double theta[9] = {};          // DNLS nonlinear angles
 	double *h_results = nullptr; // pinned, n + 2
 
 	nlc::StepLauncher launcher = nullptr;
@@ -103,7 +106,7 @@ bool default_shape(uint32_t n, int &L, int &S, bool &exact) {
 
 void free_all(nlc_ctx *c) {
 	if (c->stream) hipStreamSynchronize(c->stream);
-	for (void *ptr : {(void *) c->state, (void *) c->omega, (void *) c->tw, (void *) c->partial, (void *) c->e_raw,
+	for (void *ptr : {(void *) c->state, (void *) c->omega, (void *) c->tw, (void *) c->partial, (void *) c->reduce_scratch, (void *) c->e_raw,
 	                  (void *) c->results, (void *) c->modes, (void *) c->mp2, (void *) c->masses_d,
 	                  (void *) c->eigvec, (void *) c->split_q, (void *) c->split_p, (void *) c->energy_d,
 	                  (void *) c->norm_d})
@@ -181,8 +184,34 @@ int setup(nlc_ctx *c) {
 	}
 	NLC_HIP(c, hipMemcpy(c->omega, omega.data(), sizeof(double) * n, hipMemcpyHostToDevice));
 	if ((rc = dev_alloc(c, &c->partial, size_t(c->partial_rows) * n, "energy partials"))) return rc;
+	if ((rc = dev_alloc(c, &c->reduce_scratch, size_t(nlc::reduce_scratch_rows(c->partial_rows)) * n,
+	                    "reduction scratch")))
+		return rc;
 
-	if (p.model == NLC_DNLS) return c->fail(NLC_EUNSUPPORTED, "DNLS is not available in this build");
+	if (p.model == NLC_DNLS) {
+		// DNLS/dispersion.cpp:19-26 (multipliers) and DNLS.cpp:96-100,161-175 (angles)
+		if (!is_pow2(n) || n > nlc::kEnergyFftMax)
+			return c->fail(NLC_EUNSUPPORTED, "DNLS needs a power-of-two chain_length <= %u in this build",
+			               nlc::kEnergyFftMax);
+		const double bdt = p.beta * p.dt;
+		for (int k = 0; k < 8; ++k) c->theta[k] = (bdt * 1.) * nlc::kYoshidaC[k];
+		c->theta[8] = (bdt * 2.) * nlc::kYoshidaC[0];
+		std::vector<double2> tab(size_t(7) * n), perm(size_t(7) * n);
+		const double nrm = 1.0 / n;
+		for (int k = 0; k < 7; ++k)
+			for (uint32_t j = 0; j < n; ++j) {
+				const double ang = ((-p.dt) * nlc::kYoshidaD[k]) * omega[j];
+				tab[size_t(k) * n + j] = make_double2(std::cos(ang) * nrm, std::sin(ang) * nrm);
+			}
+		const double2 *src = tab.data();
+		if (nlc::dnls_fast_path(n)) {
+			nlc::dnls_permute_tables(n, tab.data(), perm.data());
+			src = perm.data();
+		}
+		if ((rc = dev_alloc(c, &c->dnls_table, size_t(7) * n, "DNLS multipliers"))) return rc;
+		NLC_HIP(c, hipMemcpy(c->dnls_table, src, sizeof(double2) * 7 * n, hipMemcpyHostToDevice));
+		return nlc_set_entropy_mask(c, nullptr, c->a0is0);
+	}
 
 	// step kernel selection (the reference's move<> dispatch, DNKG_FPUT_Toda.cu:262-297)
 	bool want_split = p.kernel == NLC_KERNEL_SPLIT;
@@ -321,6 +350,10 @@ int nlc_device_state(nlc_ctx *c, void **dptr, size_t *bytes) {
 int nlc_move(nlc_ctx *c, uint32_t steps) {
 	if (!c) return NLC_EARG;
 	if (!steps) return c->fail(NLC_EARG, "--kernel_batching must be a positive number");
+	if (c->p.model == NLC_DNLS) {
+		NLC_HIP(c, nlc::launch_dnls(c->stream, c->state, c->n, c->copies, steps, c->theta, c->tw, c->dnls_table));
+		return NLC_OK;
+	}
 	if (c->use_split) {
 		NLC_HIP(c, nlc::launch_planar_to_split(c->stream, c->state, c->split_q, c->split_p, c->copies, c->n));
 		NLC_HIP(c, nlc::launch_step_split(c->stream, c->p.model, c->split_q, c->split_p, c->copies, c->n, steps,
@@ -361,15 +394,18 @@ static int energy_pass(nlc_ctx *c) {
 	case NLC_TODA:
 		NLC_HIP(c, nlc::launch_energy_real(c->stream, c->state, n, c->copies, c->tw, c->omega, c->partial,
 		                                   c->partial_rows));
-		NLC_HIP(c, nlc::launch_reduce_partial(c->stream, c->partial, c->partial_rows, n, nullptr, c->e_raw));
+		NLC_HIP(c, nlc::launch_reduce_partial(c->stream, c->partial, c->partial_rows, n, nullptr, c->e_raw,
+		                                      c->reduce_scratch));
 		break;
 	case NLC_DDNKG:
 		NLC_HIP(c, nlc::launch_energy_ddnkg(c->stream, c->state, n, c->copies, c->eigvec, c->omega, c->partial));
-		NLC_HIP(c, nlc::launch_reduce_partial(c->stream, c->partial, c->partial_rows, n, nullptr, c->e_raw));
+		NLC_HIP(c, nlc::launch_reduce_partial(c->stream, c->partial, c->partial_rows, n, nullptr, c->e_raw,
+		                                      c->reduce_scratch));
 		break;
 	case NLC_DNLS:
 		NLC_HIP(c, nlc::launch_energy_dnls(c->stream, c->state, n, c->copies, c->tw, c->partial, c->partial_rows));
-		NLC_HIP(c, nlc::launch_reduce_partial(c->stream, c->partial, c->partial_rows, n, c->omega, c->e_raw));
+		NLC_HIP(c, nlc::launch_reduce_partial(c->stream, c->partial, c->partial_rows, n, c->omega, c->e_raw,
+		                                      c->reduce_scratch));
 		break;
 	default: return c->fail(NLC_EUNSUPPORTED, "no energy pass for this model");
 	}
@@ -429,7 +465,10 @@ void *nlc_stream(nlc_ctx *c) { return c ? (void *) c->stream : nullptr; }
 int nlc_describe(const nlc_ctx *c, char *buf, size_t len) {
 	if (!c || !buf || !len) return NLC_EARG;
 	static const char *names[] = {"DNKG", "FPUT", "Toda", "dDNKG", "DNLS"};
-	if (c->use_split)
+	if (c->p.model == NLC_DNLS)
+		snprintf(buf, len, "model=DNLS n=%u copies=%u kernel=%s", c->n, c->copies,
+		         nlc::dnls_fast_path(c->n) ? "dnls_radix16_registers" : "dnls_lds_stockham");
+	else if (c->use_split)
 		snprintf(buf, len, "model=%s n=%u copies=%u kernel=split", names[c->p.model], c->n, c->copies);
 	else
 		snprintf(buf, len, "model=%s n=%u copies=%u kernel=wave lanes_per_chain=%d sites_per_lane=%d exact=%d",

@@ -1,0 +1,377 @@
+// DNLS split-step Yoshida-6 integrator, chain resident on chip for a whole batch.
+//
+// Reference behaviour followed: DNLS.cpp:159-175 (per substep k: nonlinear
+// rotation by beta dt c_k with the merged 2 c_0, forward FFT, Fourier multiplier
+// T_k = exp(-i dt d_k w)/N (DNLS/dispersion.cpp:19-26), inverse FFT; a final
+// beta dt c_7 rotation after the batch), kernels DNLS.cu:11-25 and the cuFFT
+// load callbacks DNLS.cu:45-67; CPU form DNLS/cpu-implementation/DNLS.cpp:110-128.
+//
+// Design (MI355X-first): the reference issues 14 cuFFT launches per step, each
+// a full HBM pass over the ensemble.  Here one workgroup owns one chain for all
+// steps of the batch, so HBM is touched once per batch (16 B/site in and out).
+//  * k_dnls_4096: 256 threads x 16 complex values in VGPRs.  The 4096-point FFT
+//    is three radix-16 passes computed in registers (decimation in frequency);
+//    between passes the data moves through LDS: one cross-wavefront exchange
+//    (workgroup barrier) and one exchange within each wavefront.  The forward
+//    transform leaves the spectrum in digit-reversed order, the Fourier
+//    multiplier is read from a table stored in that order, and the inverse
+//    transform (decimation in time, conjugate twiddles) returns natural order,
+//    so no reordering pass is ever made.  Padded LDS layouts keep both
+//    exchanges free of bank conflicts.
+//  * k_dnls_lds: any power-of-two n <= 4096, chain in LDS, radix-4 Stockham
+//    passes (the inverse transform as conj(FFT(conj(.)))).
+//  * the nonlinear phase e^{-i theta} uses Taylor polynomials of sin and cos
+//    (degree 15 and 16, error below 1e-16 for |theta| <= pi/4) and ROCm's
+//    device sincos for larger angles.
+#include "nlc_internal.hpp"
+
+namespace nlc {
+namespace {
+
+constexpr int kT = 256; // threads per chain
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+	return make_double2(fma(a.x, b.x, -a.y * b.y), fma(a.x, b.y, a.y * b.x));
+}
+__device__ __forceinline__ double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+
+// (cos a, sin a).  Taylor series about 0 in x = a^2 for |a| <= pi/4: the first
+// omitted terms are below 5e-17 relative; otherwise the library sincos.
+__device__ __forceinline__ double2 expi(double a) {
+	if (fabs(a) <= 0.78539816339744830962) {
+		const double x = a * a;
+		double s = -7.6471637318198164759e-13;          // -1/15!
+		s = fma(s, x, 1.6059043836821614599e-10);        // 1/13!
+		s = fma(s, x, -2.5052108385441718775e-08);       // -1/11!
+		s = fma(s, x, 2.7557319223985890653e-06);        // 1/9!
+		s = fma(s, x, -1.9841269841269841270e-04);       // -1/7!
+		s = fma(s, x, 8.3333333333333333333e-03);        // 1/5!
+		s = fma(s, x, -1.6666666666666666667e-01);       // -1/3!
+		double c = 4.7794773323873852974e-14;           // 1/16!
+		c = fma(c, x, -1.1470745597729724714e-11);       // -1/14!
+		c = fma(c, x, 2.0876756987868098979e-09);        // 1/12!
+		c = fma(c, x, -2.7557319223985890653e-07);       // -1/10!
+		c = fma(c, x, 2.4801587301587301587e-05);        // 1/8!
+		c = fma(c, x, -1.3888888888888888889e-03);       // -1/6!
+		c = fma(c, x, 4.1666666666666666667e-02);        // 1/4!
+		c = fma(c, x, -0.5);                             // -1/2!
+		return make_double2(fma(c, x, 1.0), fma(a * x, s, a));
+	}
+	double sn, cs;
+	sincos(a, &sn, &cs);
+	return make_double2(cs, sn);
+}
+
+// psi * e^{-i theta |psi|^2} (DNLS.cu:11-17)
+__device__ __forceinline__ double2 nonlinear(double2 p, double theta) {
+	const double a = -theta * fma(p.x, p.x, p.y * p.y);
+	const double2 e = expi(a);
+	return make_double2(fma(p.x, e.x, -p.y * e.y), fma(p.x, e.y, p.y * e.x));
+}
+
+// ---------------------------------------------------------------- radix 16
+
+// 4-point DFT in place, SIGN = -1 forward, +1 inverse.
+template <int SIGN>
+__device__ __forceinline__ void dft4(double2 &a0, double2 &a1, double2 &a2, double2 &a3) {
+	const double2 t0 = cadd(a0, a2), t1 = csub(a0, a2), t2 = cadd(a1, a3), t3 = csub(a1, a3);
+	a0 = cadd(t0, t2);
+	a2 = csub(t0, t2);
+	// (SIGN i) t3
+	const double2 it3 = SIGN < 0 ? make_double2(t3.y, -t3.x) : make_double2(-t3.y, t3.x);
+	a1 = cadd(t1, it3);
+	a3 = csub(t1, it3);
+}
+
+// exp(SIGN 2 pi i e / 16) for e in {1, 2, 3, 6, 9}
+template <int SIGN, int E>
+__device__ __forceinline__ double2 w16() {
+	constexpr double c1 = 0.92387953251128675613, s1 = 0.38268343236508977173, h = 0.70710678118654752440;
+	if constexpr (E == 1) return make_double2(c1, SIGN * s1);
+	else if constexpr (E == 2) return make_double2(h, SIGN * h);
+	else if constexpr (E == 3) return make_double2(s1, SIGN * c1);
+	else if constexpr (E == 6) return make_double2(-h, SIGN * h);
+	else return make_double2(-c1, -SIGN * s1); // E == 9
+}
+
+// 16-point DFT in registers: v[k] <- sum_j v[j] exp(SIGN 2 pi i jk/16), natural
+// order in and out (j = 4a + b, k = c + 4d: DFT4 over a, twiddle W16^{bc},
+// DFT4 over b).
+template <int SIGN>
+__device__ __forceinline__ void dft16(double2 v[16]) {
+#pragma unroll
+	for (int b = 0; b < 4; ++b) dft4<SIGN>(v[b], v[4 + b], v[8 + b], v[12 + b]); // u[b][c] at v[4c + b]
+	v[4 * 1 + 1] = cmul(v[4 * 1 + 1], w16<SIGN, 1>());
+	v[4 * 2 + 1] = cmul(v[4 * 2 + 1], w16<SIGN, 2>());
+	v[4 * 3 + 1] = cmul(v[4 * 3 + 1], w16<SIGN, 3>());
+	v[4 * 1 + 2] = cmul(v[4 * 1 + 2], w16<SIGN, 2>());
+	{ // W16^4 = SIGN i
+		const double2 t = v[4 * 2 + 2];
+		v[4 * 2 + 2] = SIGN < 0 ? make_double2(t.y, -t.x) : make_double2(-t.y, t.x);
+	}
+	v[4 * 3 + 2] = cmul(v[4 * 3 + 2], w16<SIGN, 6>());
+	v[4 * 1 + 3] = cmul(v[4 * 1 + 3], w16<SIGN, 3>());
+	v[4 * 2 + 3] = cmul(v[4 * 2 + 3], w16<SIGN, 6>());
+	v[4 * 3 + 3] = cmul(v[4 * 3 + 3], w16<SIGN, 9>());
+	double2 o[16];
+#pragma unroll
+	for (int c = 0; c < 4; ++c) {
+		double2 a0 = v[4 * c], a1 = v[4 * c + 1], a2 = v[4 * c + 2], a3 = v[4 * c + 3];
+		dft4<SIGN>(a0, a1, a2, a3);
+		o[c] = a0, o[c + 4] = a1, o[c + 8] = a2, o[c + 12] = a3;
+	}
+#pragma unroll
+	for (int k = 0; k < 16; ++k) v[k] = o[k];
+}
+
+// v[j] *= W^{e0 + j de} for j = 0..15 with W = exp(-2 pi i/4096) = tw[1]
+// (conjugated when CONJ); ONE0 asserts e0 == 0, so v[0] is left alone.  The
+// running power is refreshed from the exact table every 4 steps to bound
+// rounding.
+template <bool CONJ, bool ONE0>
+__device__ __forceinline__ void twiddle_run(double2 v[16], const double2 *__restrict__ tw, uint32_t e0, uint32_t de) {
+#pragma unroll
+	for (int q = 0; q < 4; ++q) {
+		double2 r = tw[(e0 + uint32_t(4 * q) * de) & 4095u];
+		const double2 w = tw[de & 4095u];
+		const double2 wc = CONJ ? cconj(w) : w;
+		if (CONJ) r = cconj(r);
+#pragma unroll
+		for (int j = 0; j < 4; ++j) {
+			const int idx = 4 * q + j;
+			if (idx || !ONE0) v[idx] = cmul(v[idx], r);
+			if (j < 3) r = cmul(r, wc);
+		}
+	}
+}
+
+struct Lds4096 {
+	double2 a[4352]; // 4096 for the cross-wave exchange; 4 x 1088 padded for the in-wave one
+};
+
+__device__ __forceinline__ void wave_sync() {
+	__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+	__builtin_amdgcn_wave_barrier();
+	__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Forward FFT.  In: thread t = x0 + 16 x1 holds psi[t + 256 x2] in v[x2].
+// Out: thread t = k0 + 16 k1' ... precisely, thread t with k1 = t & 15,
+// k0 = t >> 4 holds X[k0 + 16 k1 + 256 k2] in v[k2].
+__device__ __forceinline__ void fft4096_fwd(double2 v[16], Lds4096 &L, const double2 *__restrict__ tw) {
+	const uint32_t t = threadIdx.x, lo = t & 15, hi = t >> 4;
+	const uint32_t lane = t & 63, g = lane >> 4, wbase = (t >> 6) * 1088;
+	// stage 1: DFT over x2, twiddle W4096^{(x0+16x1) k0}
+	dft16<-1>(v);
+	twiddle_run<false, true>(v, tw, 0, t);
+	// exchange 1 (cross-wave): C'[x0,x1,k0] at x0 + 16 x1 + 256 k0
+	__syncthreads();
+#pragma unroll
+	for (int k0 = 0; k0 < 16; ++k0) L.a[t + 256 * k0] = v[k0];
+	__syncthreads();
+	// now thread (x0 = lo, k0 = hi) gathers x1
+#pragma unroll
+	for (int x1 = 0; x1 < 16; ++x1) v[x1] = L.a[lo + 16 * x1 + 256 * hi];
+	// stage 2: DFT over x1, twiddle W256^{x0 k1} = W4096^{16 x0 k1}
+	dft16<-1>(v);
+	twiddle_run<false, true>(v, tw, 0, 16 * lo);
+	// exchange 2 (within the wave; its region overlaps exchange 1 of other waves)
+	__syncthreads();
+#pragma unroll
+	for (int k1 = 0; k1 < 16; ++k1) L.a[wbase + g * 272 + lo + 17 * k1] = v[k1];
+	wave_sync();
+	// now thread (k1 = lo, k0 = hi) gathers x0
+#pragma unroll
+	for (int x0 = 0; x0 < 16; ++x0) v[x0] = L.a[wbase + g * 272 + x0 + 17 * lo];
+	// stage 3: DFT over x0
+	dft16<-1>(v);
+}
+
+// Inverse (unnormalised) FFT, exact reverse of fft4096_fwd.
+__device__ __forceinline__ void fft4096_inv(double2 v[16], Lds4096 &L, const double2 *__restrict__ tw) {
+	const uint32_t t = threadIdx.x, lo = t & 15, hi = t >> 4;
+	const uint32_t lane = t & 63, g = lane >> 4, wbase = (t >> 6) * 1088;
+	// stage 3': over k2 -> x0 (thread k1 = lo), twiddle W256^{-x0 k1}
+	dft16<+1>(v);
+	twiddle_run<true, true>(v, tw, 0, 16 * lo);
+	// exchange 2' (within the wave): thread (k1 = lo) has x0 in registers
+	wave_sync();
+#pragma unroll
+	for (int x0 = 0; x0 < 16; ++x0) L.a[wbase + g * 272 + x0 + 17 * lo] = v[x0];
+	wave_sync();
+#pragma unroll
+	for (int k1 = 0; k1 < 16; ++k1) v[k1] = L.a[wbase + g * 272 + lo + 17 * k1];
+	// stage 2': over k1 -> x1 (thread x0 = lo, k0 = hi), twiddle W4096^{-(x0 + 16 x1) k0}
+	dft16<+1>(v);
+	twiddle_run<true, false>(v, tw, lo * hi, 16 * hi);
+	// exchange 1' (cross-wave)
+	__syncthreads();
+#pragma unroll
+	for (int x1 = 0; x1 < 16; ++x1) L.a[lo + 16 * x1 + 256 * hi] = v[x1];
+	__syncthreads();
+#pragma unroll
+	for (int k0 = 0; k0 < 16; ++k0) v[k0] = L.a[t + 256 * k0];
+	// stage 1': over k0 -> x2
+	dft16<+1>(v);
+}
+
+// theta[k] = beta dt c_k (k < 7), theta[7] = beta dt c_7, theta[8] = beta dt 2 c_0.
+struct DnlsArgs {
+	double theta[9];
+};
+
+__global__ void __launch_bounds__(kT, 1)
+k_dnls_4096(double2 *__restrict__ state, uint32_t copies, uint32_t steps, DnlsArgs a, const double2 *__restrict__ tw,
+            const double2 *__restrict__ mult) {
+	__shared__ Lds4096 L;
+	const uint32_t chain = blockIdx.x, t = threadIdx.x;
+	double2 *psi = state + size_t(chain) * 4096;
+	double2 v[16];
+#pragma unroll
+	for (int j = 0; j < 16; ++j) v[j] = psi[t + 256 * j];
+	for (uint32_t s = 0; s < steps; ++s) {
+#pragma unroll 1
+		for (int k = 0; k < 7; ++k) {
+			const double th = (k == 0 && s) ? a.theta[8] : a.theta[k];
+#pragma unroll
+			for (int j = 0; j < 16; ++j) v[j] = nonlinear(v[j], th);
+			fft4096_fwd(v, L, tw);
+			// Fourier multiplier, stored per thread in the digit-reversed order
+			const double2 *m = mult + size_t(k < 4 ? k : 6 - k) * 4096 + t;
+#pragma unroll
+			for (int j = 0; j < 16; ++j) v[j] = cmul(v[j], m[256 * j]);
+			fft4096_inv(v, L, tw);
+		}
+	}
+#pragma unroll
+	for (int j = 0; j < 16; ++j) psi[t + 256 * j] = nonlinear(v[j], a.theta[7]);
+}
+
+// ------------------------------------------------------------ generic n
+
+constexpr int kMaxPer = 4; // radix-4 butterflies per thread per pass (n <= 4096)
+
+// In-place radix-4 (+ final radix-2) Stockham FFT, forward sign, natural order.
+__device__ void lds_fft_fwd(double2 *buf, uint32_t n, const double2 *__restrict__ tw) {
+	const uint32_t tid = threadIdx.x, quarter = n >> 2;
+	uint32_t ns = 1;
+	for (; ns * 4 <= n; ns *= 4) {
+		double2 v[kMaxPer][4];
+		const uint32_t span = n / (ns * 4);
+#pragma unroll
+		for (int q = 0; q < kMaxPer; ++q) {
+			const uint32_t j = tid + q * kT;
+			if (j < quarter)
+#pragma unroll
+				for (int r = 0; r < 4; ++r) v[q][r] = buf[j + r * quarter];
+		}
+		__syncthreads();
+#pragma unroll
+		for (int q = 0; q < kMaxPer; ++q) {
+			const uint32_t j = tid + q * kT;
+			if (j < quarter) {
+				const uint32_t k = j % ns;
+				double2 a0 = v[q][0], a1 = cmul(v[q][1], tw[k * span]), a2 = cmul(v[q][2], tw[2 * k * span]),
+				        a3 = cmul(v[q][3], tw[3 * k * span]);
+				dft4<-1>(a0, a1, a2, a3);
+				const uint32_t o = (j / ns) * ns * 4 + k;
+				buf[o] = a0, buf[o + ns] = a1, buf[o + 2 * ns] = a2, buf[o + 3 * ns] = a3;
+			}
+		}
+		__syncthreads();
+	}
+	if (ns < n) {
+		const uint32_t half = n >> 1;
+		double2 v[2 * kMaxPer][2];
+#pragma unroll
+		for (int q = 0; q < 2 * kMaxPer; ++q) {
+			const uint32_t j = tid + q * kT;
+			if (j < half) v[q][0] = buf[j], v[q][1] = buf[j + half];
+		}
+		__syncthreads();
+#pragma unroll
+		for (int q = 0; q < 2 * kMaxPer; ++q) {
+			const uint32_t j = tid + q * kT;
+			if (j < half) {
+				const uint32_t k = j % ns;
+				const double2 b = cmul(v[q][1], tw[k * (n / (2 * ns))]);
+				const uint32_t o = (j / ns) * ns * 2 + k;
+				buf[o] = cadd(v[q][0], b);
+				buf[o + ns] = csub(v[q][0], b);
+			}
+		}
+		__syncthreads();
+	}
+}
+
+// Chain in LDS.  Between substeps the buffer holds conj(psi): the inverse
+// transform is conj(FFT(conj(X T))).
+__global__ void __launch_bounds__(kT)
+k_dnls_lds(double2 *__restrict__ state, uint32_t n, uint32_t copies, uint32_t steps, DnlsArgs a,
+           const double2 *__restrict__ tw, const double2 *__restrict__ table) {
+	extern __shared__ double2 buf[];
+	const uint32_t chain = blockIdx.x, tid = threadIdx.x;
+	double2 *psi = state + size_t(chain) * n;
+	for (uint32_t x = tid; x < n; x += kT) buf[x] = psi[x];
+	bool conj_state = false;
+	__syncthreads();
+	for (uint32_t s = 0; s < steps; ++s) {
+		for (int k = 0; k < 7; ++k) {
+			const double th = (k == 0 && s) ? a.theta[8] : a.theta[k];
+			for (uint32_t x = tid; x < n; x += kT) {
+				const double2 p = conj_state ? cconj(buf[x]) : buf[x];
+				buf[x] = nonlinear(p, th);
+			}
+			__syncthreads();
+			lds_fft_fwd(buf, n, tw);
+			const double2 *tk = table + size_t(k) * n;
+			for (uint32_t j = tid; j < n; j += kT) buf[j] = cconj(cmul(buf[j], tk[j]));
+			__syncthreads();
+			lds_fft_fwd(buf, n, tw);
+			conj_state = true;
+		}
+	}
+	for (uint32_t x = tid; x < n; x += kT) {
+		const double2 p = conj_state ? cconj(buf[x]) : buf[x];
+		psi[x] = nonlinear(p, a.theta[7]);
+	}
+}
+
+} // namespace
+
+// Fourier multiplier tables (host side).  table[k][j] = T_k[j] for the generic
+// kernel; for n = 4096 the 4 distinct tables (d_k = d_{6-k}) are stored in the
+// order the radix-16 kernel reads them: entry [k][256 j + t] = T_k[b(t) + 256 j]
+// with b(t) = (t >> 4) + 16 (t & 15).
+bool dnls_fast_path(uint32_t n) { return n == 4096; }
+
+void dnls_permute_tables(uint32_t n, const double2 *table, double2 *out) {
+	for (int k = 0; k < 4; ++k)
+		for (uint32_t j = 0; j < 16; ++j)
+			for (uint32_t t = 0; t < 256; ++t) {
+				const uint32_t b = (t >> 4) + 16 * (t & 15);
+				out[size_t(k) * n + 256 * j + t] = table[size_t(k) * n + b + 256 * j];
+			}
+}
+
+hipError_t launch_dnls(hipStream_t st, double2 *state, uint32_t n, uint32_t copies, uint32_t steps,
+                       const double *theta9, const double2 *tw, const double2 *table) {
+	DnlsArgs a;
+	for (int i = 0; i < 9; ++i) a.theta[i] = theta9[i];
+	if (dnls_fast_path(n)) {
+		hipLaunchKernelGGL(k_dnls_4096, dim3(copies), dim3(kT), 0, st, state, copies, steps, a, tw, table);
+	} else {
+		const size_t lds = sizeof(double2) * n;
+		if (lds > 64 * 1024) {
+			hipError_t e = hipFuncSetAttribute((const void *) k_dnls_lds, hipFuncAttributeMaxDynamicSharedMemorySize,
+			                                   int(lds));
+			if (e != hipSuccess) return e;
+		}
+		hipLaunchKernelGGL(k_dnls_lds, dim3(copies), dim3(kT), lds, st, state, n, copies, steps, a, tw, table);
+	}
+	return hipGetLastError();
+}
+
+} // namespace nlc

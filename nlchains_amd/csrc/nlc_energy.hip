@@ -184,6 +184,26 @@ __global__ void k_zero(double *p, size_t count) {
 		p[i] = 0.0;
 }
 
+// First pass of the per-mode reduction: block (x, y) sums rows
+// [64y, 64y+64) of partial for modes [64x, 64x+64); 4 row slices per mode,
+// combined in LDS in a fixed order.
+__global__ void __launch_bounds__(256)
+k_reduce_rows(const double *__restrict__ partial, uint32_t rows, uint32_t n, double *__restrict__ chunk_sums) {
+	__shared__ double red[4][64];
+	const uint32_t j = threadIdx.x & 63, slice = threadIdx.x >> 6;
+	const uint32_t k = blockIdx.x * 64 + j;
+	const uint32_t r0 = blockIdx.y * 64;
+	double s = 0.0;
+	if (k < n) {
+#pragma unroll 4
+		for (uint32_t r = r0 + slice; r < min(r0 + 64, rows); r += 4) s += partial[size_t(r) * n + k];
+	}
+	red[slice][j] = s;
+	__syncthreads();
+	if (slice == 0 && k < n)
+		chunk_sums[size_t(blockIdx.y) * n + k] = (red[0][j] + red[1][j]) + (red[2][j] + red[3][j]);
+}
+
 __global__ void k_reduce_partial(const double *__restrict__ partial, uint32_t grid, uint32_t n,
                                  const double *__restrict__ scale, double *__restrict__ e_raw) {
 	const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -312,8 +332,16 @@ hipError_t launch_energy_dnls(hipStream_t st, const double2 *state, uint32_t n, 
 	return launch_energy_common(st, true, state, n, copies, tw, nullptr, partial, grid);
 }
 
+uint32_t reduce_scratch_rows(uint32_t grid) { return (grid + 63) / 64; }
+
 hipError_t launch_reduce_partial(hipStream_t st, const double *partial, uint32_t grid, uint32_t n, const double *scale,
-                                 double *e_raw) {
+                                 double *e_raw, double *scratch) {
+	if (grid > 64) {
+		const uint32_t chunks = reduce_scratch_rows(grid);
+		hipLaunchKernelGGL(k_reduce_rows, dim3((n + 63) / 64, chunks), dim3(256), 0, st, partial, grid, n, scratch);
+		partial = scratch;
+		grid = chunks;
+	}
 	hipLaunchKernelGGL(k_reduce_partial, dim3((n + 255) / 256), dim3(256), 0, st, partial, grid, n, scale, e_raw);
 	return hipGetLastError();
 }

@@ -51,9 +51,11 @@ hipError_t launch_energy_real(hipStream_t, const double2 *state, uint32_t n, uin
 // DNLS energies: partial[g][k] = sum |psi_hat_k|^2 (omega applied in the reduction).
 hipError_t launch_energy_dnls(hipStream_t, const double2 *state, uint32_t n, uint32_t copies, const double2 *tw,
                               double *partial, uint32_t grid);
-// e_raw[k] = scale_k * sum_g partial[g][k] (scale = omega for DNLS, else NULL = 1).
+// e_raw[k] = scale_k * sum_g partial[g][k] (scale = omega for DNLS, else NULL = 1),
+// fixed-order; scratch holds reduce_scratch_rows(grid) * n doubles.
+uint32_t reduce_scratch_rows(uint32_t grid);
 hipError_t launch_reduce_partial(hipStream_t, const double *partial, uint32_t grid, uint32_t n,
-                                 const double *scale, double *e_raw);
+                                 const double *scale, double *e_raw, double *scratch);
 // out[0..n) = normalised linenergies, out[n] = S_WT, out[n+1] = S_INF.
 hipError_t launch_results(hipStream_t, const double *e_raw, uint32_t n, double norm, int a0is0,
                           const uint32_t *idx, uint32_t nidx, double *out);
@@ -74,4 +76,15 @@ hipError_t launch_energy_ddnkg(hipStream_t, const double2 *state, uint32_t n, ui
 // Host eigensystem (nlc_host.cpp).
 int eigensystem(uint32_t n, const double *mp2, double *omegas, double *V);
 extern const double kYoshidaC[8], kYoshidaD[8];
+} // namespace nlc
+
+namespace nlc {
+// DNLS split step (nlc_dnls.hip).  theta9[k] = beta dt c_k (k <= 7),
+// theta9[8] = beta dt 2 c_0; table: 7 Fourier multipliers of n entries, or for
+// the radix-16 path (dnls_fast_path) the 4 distinct ones permuted by
+// dnls_permute_tables.
+bool dnls_fast_path(uint32_t n);
+void dnls_permute_tables(uint32_t n, const double2 *table, double2 *out);
+hipError_t launch_dnls(hipStream_t, double2 *state, uint32_t n, uint32_t copies, uint32_t steps,
+                       const double *theta9, const double2 *tw, const double2 *table);
 } // namespace nlc

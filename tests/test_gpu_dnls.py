@@ -1,0 +1,110 @@
+"""GPU parity of the DNLS split-step integrator (FP64) against the oracle.
+
+Bars: psi after 100 steps within 1e-10 relative (max-norm); normalised linear
+energies and entropies within 1e-9; norm conserved to 1e-12 relative.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def banner(msg):
+    print(f"[gpu-dnls] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def rel_err(got, want):
+    return np.abs(got - want).max() / np.abs(want).max()
+
+
+def run(N, n, copies, steps, dt, beta, s0):
+    with N.Ensemble(N.DNLS, n, copies, dt, beta=beta) as ens:
+        ens.upload(s0)
+        ens.move(steps)
+        got = ens.download()
+        desc = ens.describe()
+    return got, desc
+
+
+@pytest.mark.parametrize("n,copies,steps", [(4096, 6, 100), (2, 5, 100), (16, 5, 100), (64, 7, 100),
+                                            (256, 5, 100), (1024, 3, 100), (2048, 3, 60)])
+def test_dnls_move_parity(gpu, n, copies, steps):
+    N = gpu
+    dt, beta = 0.01, 1.0
+    s0 = N.ic_dnls(n, copies, max(1, min(n // 16, n // 2)), 1.0, 3)
+    banner(f"move n={n} copies={copies} steps={steps}")
+    got, desc = run(N, n, copies, steps, dt, beta, s0)
+    want = O.move_dnls(s0, steps, dt, beta)
+    err = rel_err(got, want)
+    banner(f"  {desc}: rel err {err:.3e}")
+    assert np.isfinite(got).all() and err <= 1e-10, (desc, err)
+    np.testing.assert_allclose(O.dnls_norm(got), O.dnls_norm(s0), rtol=1e-12)
+
+
+def test_dnls_large_angles_use_library_sincos(gpu):
+    """beta large enough that |theta| > pi/4 at many sites: the library sincos
+    branch of the nonlinear rotation is exercised."""
+    N = gpu
+    n, copies, dt, beta = 4096, 3, 0.02, 60.0
+    s0 = N.ic_dnls(n, copies, 64, 1.0, 11)
+    assert (beta * dt * 2 * 0.3922568 * (s0 ** 2).sum(-1)).max() > np.pi / 4
+    # strongly nonlinear, so round-off grows fast: few steps suffice to test the branch
+    got, _ = run(N, n, copies, 3, dt, beta, s0)
+    want = O.move_dnls(s0, 3, dt, beta)
+    assert rel_err(got, want) <= 1e-12
+
+
+def test_dnls_batches_compose(gpu):
+    N = gpu
+    s0 = N.ic_dnls(4096, 2, 256, 1.0, 5)
+    a, _ = run(N, 4096, 2, 30, 0.01, 1.0, s0)
+    with N.Ensemble(N.DNLS, 4096, 2, 0.01, beta=1.0) as ens:
+        ens.upload(s0)
+        ens.move(10)
+        ens.move(20)
+        b = ens.download()
+    assert rel_err(a, b) < 1e-13
+
+
+@pytest.mark.parametrize("n,copies", [(4096, 5), (256, 9), (64, 3)])
+def test_dnls_linenergies(gpu, n, copies):
+    N = gpu
+    s0 = N.ic_dnls(n, copies, max(1, n // 16), 1.0, 4)
+    s0 = s0 + 0.01 * np.random.default_rng(3).normal(size=s0.shape)
+    with N.Ensemble(N.DNLS, n, copies, 0.01, beta=1.0) as ens:
+        ens.upload(s0)
+        e_gpu, s_gpu = ens.linenergies()
+        E, norms = ens.chain_energies()
+    e = O.calc_linenergies(O.linenergies_dnls(s0), 1.0 / (n * copies), True)
+    wt, inf = O.entropies(e, True)
+    assert rel_err(e_gpu, e) <= 1e-9
+    assert abs(s_gpu[0] - wt) <= 1e-9 * abs(wt) and abs(s_gpu[1] - inf) <= 1e-9 * abs(inf)
+    np.testing.assert_allclose(E, O.chain_energy(O.DNLS, s0, beta=1.0), rtol=1e-12)
+    np.testing.assert_allclose(norms, O.dnls_norm(s0), rtol=1e-12)
+
+
+def test_dnls_full_size_norm_and_sampled_parity(gpu):
+    """4096 sites x 4096 copies: norm conservation (both substeps are unitary),
+    and a sample of chains re-run on the oracle."""
+    N = gpu
+    n, copies, steps, dt = 4096, 4096, 50, 0.01
+    banner("full-size DNLS 4096 x 4096")
+    s0 = N.ic_dnls(n, copies, 256, 1.0, 3)
+    with N.Ensemble(N.DNLS, n, copies, dt, beta=1.0) as ens:
+        ens.upload(s0)
+        E0, n0 = ens.chain_energies()
+        ens.move(steps)
+        got = ens.download()
+        E1, n1 = ens.chain_energies()
+    np.testing.assert_allclose(n1, n0, rtol=1e-12)
+    pick = np.random.default_rng(1).choice(copies, 3, replace=False)
+    want = O.move_dnls(s0[pick], steps, dt, 1.0)
+    assert rel_err(got[pick], want) <= 1e-10
+    E1o = O.chain_energy(O.DNLS, want, beta=1.0)
+    np.testing.assert_allclose(E1[pick], E1o, rtol=1e-11)
+    banner(f"  norm drift {np.abs(n1 / n0 - 1).max():.2e}, energy error {np.abs(E1 / E0 - 1).mean():.2e}")
