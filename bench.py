@@ -34,8 +34,9 @@ METRIC = "site-steps/s plain FP64 vs mixed/batched at stated energy drift; % of 
 FP64_PEAK_TFLOPS = 78.6
 HBM_PEAK_GBS = 8000.0
 
-# algorithmic FLOP per site-step (FMA = 2, exp = 0): 7 kicks + 7 drifts
-FLOP_PER_SITE_STEP = {"DNKG": 70, "dDNKG": 70, "FPUT": 119, "Toda": 56}
+# algorithmic FLOP per site-step (FMA = 2, exp/sincos = 0): 7 kicks + 7 drifts;
+# DNLS: 7 x (two complex FFTs at 5 N log2 N / N each + 16 pointwise), N = 4096.
+FLOP_PER_SITE_STEP = {"DNKG": 70, "dDNKG": 70, "FPUT": 119, "Toda": 56, "DNLS": 7 * (10 * 12 + 16)}
 
 WORKLOADS = {
     "dnkg": dict(model="DNKG", n=1024, copies=16384, dt=0.05, m=1.0, alpha=0.0, beta=1.0, k_lo=0, k_hi=128,
@@ -44,6 +45,9 @@ WORKLOADS = {
                  dens=0.5, seed=2, batch=1000),
     "toda": dict(model="Toda", n=1024, copies=16384, dt=0.05, m=0.0, alpha=1.0, beta=0.0, k_lo=1, k_hi=52,
                  dens=0.25, seed=4, batch=1000),
+    # DNLS beta=1, N=4096, 4096 copies, dt=0.01, norm density 1 on |k| < N/16
+    "dnls": dict(model="DNLS", n=4096, copies=4096, dt=0.01, m=0.0, alpha=0.0, beta=1.0, kmax=256, dens=1.0,
+                 seed=3, batch=1000),
 }
 
 
@@ -81,17 +85,26 @@ def cpu_baseline(w, s0, seconds_budget: float):
     from oracle import oracle as O
     lib = O.fast()
     cores = max(1, min(len(os.sched_getaffinity(0)), 16))
-    model = {"DNKG": O.DNKG, "FPUT": O.FPUT, "Toda": O.TODA}[w["model"]]
+    def move(state, steps):
+        if w["model"] == "DNLS":
+            return O.move_dnls(state, steps, w["dt"], w["beta"], threads=cores, lib=lib)
+        model = {"DNKG": O.DNKG, "FPUT": O.FPUT, "Toda": O.TODA}[w["model"]]
+        return O.move_real(model, state, steps, w["dt"], m=w["m"], alpha=w["alpha"], beta=w["beta"],
+                           threads=cores, lib=lib)
+
     # calibrate on a small slice, then size the sample to the time budget
-    probe_copies = min(w["copies"], 64 * cores)
+    probe_copies = min(w["copies"], 4 * cores if w["model"] == "DNLS" else 64 * cores)
     t = time.perf_counter()
-    O.move_real(model, s0[:probe_copies], 10, w["dt"], m=w["m"], alpha=w["alpha"], beta=w["beta"], threads=cores,
-                lib=lib)
-    rate = probe_copies * w["n"] * 10 / (time.perf_counter() - t)
+    move(s0[:probe_copies], 4)
+    rate = probe_copies * w["n"] * 4 / (time.perf_counter() - t)
     copies = w["copies"]
-    steps = int(max(10, min(w["batch"], seconds_budget * rate / (copies * w["n"]))))
+    steps = int(seconds_budget * rate / (copies * w["n"]))
+    if steps < 10:  # keep at least 10 steps: shrink the copy count instead
+        copies = max(cores, min(copies, int(seconds_budget * rate / (10 * w["n"]))))
+        steps = 10
+    steps = min(steps, w["batch"])
     t = time.perf_counter()
-    O.move_real(model, s0, steps, w["dt"], m=w["m"], alpha=w["alpha"], beta=w["beta"], threads=cores, lib=lib)
+    move(s0[:copies], steps)
     el = time.perf_counter() - t
     return {"value": copies * w["n"] * steps / el, "unit": "site-steps/s", "cores": cores, "kind": "port",
             "sample": f"oracle/liboracle_fast.so (-O3 -ffast-math, OpenMP over copies), {w['model']} "
@@ -147,7 +160,10 @@ def main():
         w["batch"] = args.batch
     model = N.MODELS[w["model"]]
     n, copies, b = w["n"], w["copies"], w["batch"]
-    s0 = N.ic_plane_waves(n, copies, w["m"], w["k_lo"], w["k_hi"], w["dens"], w["seed"])
+    if w["model"] == "DNLS":
+        s0 = N.ic_dnls(n, copies, w["kmax"], w["dens"], w["seed"])
+    else:
+        s0 = N.ic_plane_waves(n, copies, w["m"], w["k_lo"], w["k_hi"], w["dens"], w["seed"])
 
     ev = HipEvents()
     with N.Ensemble(model, n, copies, w["dt"], m=w["m"], alpha=w["alpha"], beta=w["beta"],
@@ -203,7 +219,8 @@ def main():
         "entropies_final": [float(ent[0]), float(ent[1])],
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
-                     "kernel": "k_step_wave (FP64 vector ALU bound)", "kernel_ms": k_ms,
+                     "kernel": ("k_dnls_4096" if w["model"] == "DNLS" else "k_step_wave") + " (FP64 vector ALU bound)",
+                     "kernel_ms": k_ms,
                      "flop_per_launch": flop},
     }
     if args.cpu_seconds > 0:

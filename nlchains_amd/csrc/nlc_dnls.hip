@@ -37,31 +37,58 @@ __device__ __forceinline__ double2 cconj(double2 a) { return make_double2(a.x, -
 __device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
 
-// (cos a, sin a).  Taylor series about 0 in x = a^2 for |a| <= pi/4: the first
-// omitted terms are below 5e-17 relative; otherwise the library sincos.
+// (cos r, sin r) for |r| <= pi/16: Taylor series about 0 in x = r^2 (degree
+// 12 and 11); the first omitted terms are below 1e-18 relative.
+__device__ __forceinline__ double2 expi_small(double r) {
+	const double x = r * r;
+	double s = -2.505210838544172e-08;         // -1/11!
+	s = fma(s, x, 2.7557319223985893e-06);     // 1/9!
+	s = fma(s, x, -0.0001984126984126984);     // -1/7!
+	s = fma(s, x, 0.008333333333333333);       // 1/5!
+	s = fma(s, x, -0.16666666666666666);       // -1/3!
+	double c = 2.08767569878681e-09;           // 1/12!
+	c = fma(c, x, -2.755731922398589e-07);     // -1/10!
+	c = fma(c, x, 2.48015873015873e-05);       // 1/8!
+	c = fma(c, x, -0.001388888888888889);      // -1/6!
+	c = fma(c, x, 0.041666666666666664);       // 1/4!
+	c = fma(c, x, -0.5);                       // -1/2!
+	return make_double2(fma(c, x, 1.0), fma(r * x, s, r));
+}
+
+// e^{i k pi/8}, k = 0..15
+__constant__ double2 kOctant[16] = {
+    {1.0, 0.0},
+    {0.9238795325112867, 0.3826834323650898},
+    {0.7071067811865476, 0.7071067811865476},
+    {0.3826834323650898, 0.9238795325112867},
+    {0.0, 1.0},
+    {-0.3826834323650898, 0.9238795325112867},
+    {-0.7071067811865476, 0.7071067811865476},
+    {-0.9238795325112867, 0.3826834323650898},
+    {-1.0, 0.0},
+    {-0.9238795325112867, -0.3826834323650898},
+    {-0.7071067811865476, -0.7071067811865476},
+    {-0.3826834323650898, -0.9238795325112867},
+    {0.0, -1.0},
+    {0.3826834323650898, -0.9238795325112867},
+    {0.7071067811865476, -0.7071067811865476},
+    {0.9238795325112867, -0.3826834323650898},
+};
+
+// (cos a, sin a).  |a| <= pi/16 directly (the nonlinear angles of a DNLS run
+// are far smaller, so the other branch is normally skipped by the whole
+// wavefront); otherwise Cody & Waite reduction a = k pi/8 + r, |r| <= pi/16,
+// with pi/8 split in three parts (33, 33 and 53 significant bits, so k*Q1 and
+// k*Q2 are exact for |k| < 2^20), and e^{ia} = e^{i k pi/8} e^{ir}.
 __device__ __forceinline__ double2 expi(double a) {
-	if (fabs(a) <= 0.78539816339744830962) {
-		const double x = a * a;
-		double s = -7.6471637318198164759e-13;          // -1/15!
-		s = fma(s, x, 1.6059043836821614599e-10);        // 1/13!
-		s = fma(s, x, -2.5052108385441718775e-08);       // -1/11!
-		s = fma(s, x, 2.7557319223985890653e-06);        // 1/9!
-		s = fma(s, x, -1.9841269841269841270e-04);       // -1/7!
-		s = fma(s, x, 8.3333333333333333333e-03);        // 1/5!
-		s = fma(s, x, -1.6666666666666666667e-01);       // -1/3!
-		double c = 4.7794773323873852974e-14;           // 1/16!
-		c = fma(c, x, -1.1470745597729724714e-11);       // -1/14!
-		c = fma(c, x, 2.0876756987868098979e-09);        // 1/12!
-		c = fma(c, x, -2.7557319223985890653e-07);       // -1/10!
-		c = fma(c, x, 2.4801587301587301587e-05);        // 1/8!
-		c = fma(c, x, -1.3888888888888888889e-03);       // -1/6!
-		c = fma(c, x, 4.1666666666666666667e-02);        // 1/4!
-		c = fma(c, x, -0.5);                             // -1/2!
-		return make_double2(fma(c, x, 1.0), fma(a * x, s, a));
-	}
-	double sn, cs;
-	sincos(a, &sn, &cs);
-	return make_double2(cs, sn);
+	if (fabs(a) <= 0.19634954084936207) return expi_small(a);
+	constexpr double kEightOverPi = 2.5464790894703255, kQ1 = 0x1.921fb54400000p-2, kQ2 = 0x1.0b4611a600000p-36,
+	                 kQ3 = 0x1.3198a2e037073p-71;
+	const double k = rint(a * kEightOverPi);
+	double r = fma(-k, kQ1, a);
+	r = fma(-k, kQ2, r);
+	r = fma(-k, kQ3, r);
+	return cmul(expi_small(r), kOctant[int(k) & 15]);
 }
 
 // psi * e^{-i theta |psi|^2} (DNLS.cu:11-17)
@@ -96,18 +123,46 @@ __device__ __forceinline__ double2 w16() {
 	else return make_double2(-c1, -SIGN * s1); // E == 9
 }
 
-// 16-point DFT in registers: v[k] <- sum_j v[j] exp(SIGN 2 pi i jk/16), natural
-// order in and out (j = 4a + b, k = c + 4d: DFT4 over a, twiddle W16^{bc},
-// DFT4 over b).
+// 16-point DFT in registers, in place, v[.] <- sum_j v[j] exp(SIGN 2 pi i jk/16),
+// as two layers of 4-point DFTs (j = 4a + b, k = c + 4d) around the twiddles
+// W16^{bc}.  Element k of a transform lives either at register k ("natural")
+// or at register pos(k) = 4 (k & 3) + (k >> 2) ("transposed"); pos is an
+// involution and every index is a compile-time constant, so the order is
+// tracked for free instead of copying registers.
+__host__ __device__ constexpr int pos(int k) { return 4 * (k & 3) + (k >> 2); }
+
+// natural in -> transposed out
 template <int SIGN>
-__device__ __forceinline__ void dft16(double2 v[16]) {
+__device__ __forceinline__ void dft16_n2t(double2 v[16]) {
 #pragma unroll
-	for (int b = 0; b < 4; ++b) dft4<SIGN>(v[b], v[4 + b], v[8 + b], v[12 + b]); // u[b][c] at v[4c + b]
+	for (int b = 0; b < 4; ++b) dft4<SIGN>(v[b], v[4 + b], v[8 + b], v[12 + b]); // u[b][c] at 4c + b
 	v[4 * 1 + 1] = cmul(v[4 * 1 + 1], w16<SIGN, 1>());
 	v[4 * 2 + 1] = cmul(v[4 * 2 + 1], w16<SIGN, 2>());
 	v[4 * 3 + 1] = cmul(v[4 * 3 + 1], w16<SIGN, 3>());
 	v[4 * 1 + 2] = cmul(v[4 * 1 + 2], w16<SIGN, 2>());
-	{ // W16^4 = SIGN i
+	{
+		const double2 t = v[4 * 2 + 2]; // W16^4 = SIGN i
+		v[4 * 2 + 2] = SIGN < 0 ? make_double2(t.y, -t.x) : make_double2(-t.y, t.x);
+	}
+	v[4 * 3 + 2] = cmul(v[4 * 3 + 2], w16<SIGN, 6>());
+	v[4 * 1 + 3] = cmul(v[4 * 1 + 3], w16<SIGN, 3>());
+	v[4 * 2 + 3] = cmul(v[4 * 2 + 3], w16<SIGN, 6>());
+	v[4 * 3 + 3] = cmul(v[4 * 3 + 3], w16<SIGN, 9>());
+#pragma unroll
+	for (int c = 0; c < 4; ++c) dft4<SIGN>(v[4 * c], v[4 * c + 1], v[4 * c + 2], v[4 * c + 3]); // X[c+4d] at 4c + d
+}
+
+// transposed in -> natural out: element k = c + 4d at register 4c + d; DFT4
+// over d (contiguous), twiddle W16^{c b'}, DFT4 over c gives j = 4a' + b'.
+template <int SIGN>
+__device__ __forceinline__ void dft16_t2n(double2 v[16]) {
+#pragma unroll
+	for (int c = 0; c < 4; ++c) dft4<SIGN>(v[4 * c], v[4 * c + 1], v[4 * c + 2], v[4 * c + 3]); // Y[c][b'] at 4c + b'
+	v[4 * 1 + 1] = cmul(v[4 * 1 + 1], w16<SIGN, 1>());
+	v[4 * 2 + 1] = cmul(v[4 * 2 + 1], w16<SIGN, 2>());
+	v[4 * 3 + 1] = cmul(v[4 * 3 + 1], w16<SIGN, 3>());
+	v[4 * 1 + 2] = cmul(v[4 * 1 + 2], w16<SIGN, 2>());
+	{
 		const double2 t = v[4 * 2 + 2];
 		v[4 * 2 + 2] = SIGN < 0 ? make_double2(t.y, -t.x) : make_double2(-t.y, t.x);
 	}
@@ -115,22 +170,15 @@ __device__ __forceinline__ void dft16(double2 v[16]) {
 	v[4 * 1 + 3] = cmul(v[4 * 1 + 3], w16<SIGN, 3>());
 	v[4 * 2 + 3] = cmul(v[4 * 2 + 3], w16<SIGN, 6>());
 	v[4 * 3 + 3] = cmul(v[4 * 3 + 3], w16<SIGN, 9>());
-	double2 o[16];
 #pragma unroll
-	for (int c = 0; c < 4; ++c) {
-		double2 a0 = v[4 * c], a1 = v[4 * c + 1], a2 = v[4 * c + 2], a3 = v[4 * c + 3];
-		dft4<SIGN>(a0, a1, a2, a3);
-		o[c] = a0, o[c + 4] = a1, o[c + 8] = a2, o[c + 12] = a3;
-	}
-#pragma unroll
-	for (int k = 0; k < 16; ++k) v[k] = o[k];
+	for (int b = 0; b < 4; ++b) dft4<SIGN>(v[b], v[4 + b], v[8 + b], v[12 + b]); // x[4a' + b'] at 4a' + b'
 }
 
 // v[j] *= W^{e0 + j de} for j = 0..15 with W = exp(-2 pi i/4096) = tw[1]
 // (conjugated when CONJ); ONE0 asserts e0 == 0, so v[0] is left alone.  The
 // running power is refreshed from the exact table every 4 steps to bound
 // rounding.
-template <bool CONJ, bool ONE0>
+template <bool CONJ, bool ONE0, bool TRANSPOSED>
 __device__ __forceinline__ void twiddle_run(double2 v[16], const double2 *__restrict__ tw, uint32_t e0, uint32_t de) {
 #pragma unroll
 	for (int q = 0; q < 4; ++q) {
@@ -141,7 +189,8 @@ __device__ __forceinline__ void twiddle_run(double2 v[16], const double2 *__rest
 #pragma unroll
 		for (int j = 0; j < 4; ++j) {
 			const int idx = 4 * q + j;
-			if (idx || !ONE0) v[idx] = cmul(v[idx], r);
+			const int at = TRANSPOSED ? pos(idx) : idx;
+			if (idx || !ONE0) v[at] = cmul(v[at], r);
 			if (j < 3) r = cmul(r, wc);
 		}
 	}
@@ -157,64 +206,65 @@ __device__ __forceinline__ void wave_sync() {
 	__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Forward FFT.  In: thread t = x0 + 16 x1 holds psi[t + 256 x2] in v[x2].
-// Out: thread t = k0 + 16 k1' ... precisely, thread t with k1 = t & 15,
-// k0 = t >> 4 holds X[k0 + 16 k1 + 256 k2] in v[k2].
+// Forward FFT.  In: thread t = x0 + 16 x1 holds psi[t + 256 x2] at v[pos(x2)].
+// Out: thread t with k1 = t & 15, k0 = t >> 4 holds X[k0 + 16 k1 + 256 k2] at
+// v[pos(k2)].
 __device__ __forceinline__ void fft4096_fwd(double2 v[16], Lds4096 &L, const double2 *__restrict__ tw) {
 	const uint32_t t = threadIdx.x, lo = t & 15, hi = t >> 4;
 	const uint32_t lane = t & 63, g = lane >> 4, wbase = (t >> 6) * 1088;
-	// stage 1: DFT over x2, twiddle W4096^{(x0+16x1) k0}
-	dft16<-1>(v);
-	twiddle_run<false, true>(v, tw, 0, t);
+	// stage 1: DFT over x2 (k0 natural), twiddle W4096^{(x0+16x1) k0}
+	dft16_t2n<-1>(v);
+	twiddle_run<false, true, false>(v, tw, 0, t);
 	// exchange 1 (cross-wave): C'[x0,x1,k0] at x0 + 16 x1 + 256 k0
 	__syncthreads();
 #pragma unroll
 	for (int k0 = 0; k0 < 16; ++k0) L.a[t + 256 * k0] = v[k0];
 	__syncthreads();
-	// now thread (x0 = lo, k0 = hi) gathers x1
+	// thread (x0 = lo, k0 = hi) gathers x1
 #pragma unroll
 	for (int x1 = 0; x1 < 16; ++x1) v[x1] = L.a[lo + 16 * x1 + 256 * hi];
-	// stage 2: DFT over x1, twiddle W256^{x0 k1} = W4096^{16 x0 k1}
-	dft16<-1>(v);
-	twiddle_run<false, true>(v, tw, 0, 16 * lo);
+	// stage 2: DFT over x1 (k1 transposed), twiddle W256^{x0 k1} = W4096^{16 x0 k1}
+	dft16_n2t<-1>(v);
+	twiddle_run<false, true, true>(v, tw, 0, 16 * lo);
 	// exchange 2 (within the wave; its region overlaps exchange 1 of other waves)
 	__syncthreads();
 #pragma unroll
-	for (int k1 = 0; k1 < 16; ++k1) L.a[wbase + g * 272 + lo + 17 * k1] = v[k1];
+	for (int k1 = 0; k1 < 16; ++k1) L.a[wbase + g * 272 + lo + 17 * k1] = v[pos(k1)];
 	wave_sync();
-	// now thread (k1 = lo, k0 = hi) gathers x0
+	// thread (k1 = lo, k0 = hi) gathers x0
 #pragma unroll
 	for (int x0 = 0; x0 < 16; ++x0) v[x0] = L.a[wbase + g * 272 + x0 + 17 * lo];
-	// stage 3: DFT over x0
-	dft16<-1>(v);
+	// stage 3: DFT over x0 (k2 transposed)
+	dft16_n2t<-1>(v);
 }
 
-// Inverse (unnormalised) FFT, exact reverse of fft4096_fwd.
+// Inverse (unnormalised) FFT, the exact reverse of fft4096_fwd: spectrum at
+// v[pos(k2)] in, psi[t + 256 x2] at v[pos(x2)] out.
 __device__ __forceinline__ void fft4096_inv(double2 v[16], Lds4096 &L, const double2 *__restrict__ tw) {
 	const uint32_t t = threadIdx.x, lo = t & 15, hi = t >> 4;
 	const uint32_t lane = t & 63, g = lane >> 4, wbase = (t >> 6) * 1088;
-	// stage 3': over k2 -> x0 (thread k1 = lo), twiddle W256^{-x0 k1}
-	dft16<+1>(v);
-	twiddle_run<true, true>(v, tw, 0, 16 * lo);
-	// exchange 2' (within the wave): thread (k1 = lo) has x0 in registers
+	// stage 3': over k2 -> x0 natural (thread k1 = lo), twiddle W256^{-x0 k1}
+	dft16_t2n<+1>(v);
+	twiddle_run<true, true, false>(v, tw, 0, 16 * lo);
+	// exchange 2' (within the wave)
 	wave_sync();
 #pragma unroll
 	for (int x0 = 0; x0 < 16; ++x0) L.a[wbase + g * 272 + x0 + 17 * lo] = v[x0];
 	wave_sync();
 #pragma unroll
 	for (int k1 = 0; k1 < 16; ++k1) v[k1] = L.a[wbase + g * 272 + lo + 17 * k1];
-	// stage 2': over k1 -> x1 (thread x0 = lo, k0 = hi), twiddle W4096^{-(x0 + 16 x1) k0}
-	dft16<+1>(v);
-	twiddle_run<true, false>(v, tw, lo * hi, 16 * hi);
+	// stage 2': over k1 -> x1 transposed (thread x0 = lo, k0 = hi), twiddle W4096^{-(x0 + 16 x1) k0}
+	dft16_n2t<+1>(v);
+	twiddle_run<true, false, true>(v, tw, lo * hi, 16 * hi);
 	// exchange 1' (cross-wave)
 	__syncthreads();
 #pragma unroll
-	for (int x1 = 0; x1 < 16; ++x1) L.a[lo + 16 * x1 + 256 * hi] = v[x1];
+	for (int x1 = 0; x1 < 16; ++x1) L.a[lo + 16 * x1 + 256 * hi] = v[pos(x1)];
 	__syncthreads();
 #pragma unroll
 	for (int k0 = 0; k0 < 16; ++k0) v[k0] = L.a[t + 256 * k0];
-	// stage 1': over k0 -> x2
-	dft16<+1>(v);
+	// stage 1': over k0 -> x2 transposed
+	dft16_n2t<+1>(v);
 }
 
 // theta[k] = beta dt c_k (k < 7), theta[7] = beta dt c_7, theta[8] = beta dt 2 c_0.
@@ -222,31 +272,35 @@ struct DnlsArgs {
 	double theta[9];
 };
 
-__global__ void __launch_bounds__(kT, 1)
+__global__ void __launch_bounds__(kT, 2)
 k_dnls_4096(double2 *__restrict__ state, uint32_t copies, uint32_t steps, DnlsArgs a, const double2 *__restrict__ tw,
             const double2 *__restrict__ mult) {
 	__shared__ Lds4096 L;
 	const uint32_t chain = blockIdx.x, t = threadIdx.x;
 	double2 *psi = state + size_t(chain) * 4096;
-	double2 v[16];
+	double2 v[16]; // psi[t + 256 j] at v[pos(j)]
 #pragma unroll
-	for (int j = 0; j < 16; ++j) v[j] = psi[t + 256 * j];
+	for (int j = 0; j < 16; ++j) v[pos(j)] = psi[t + 256 * j];
 	for (uint32_t s = 0; s < steps; ++s) {
 #pragma unroll 1
 		for (int k = 0; k < 7; ++k) {
 			const double th = (k == 0 && s) ? a.theta[8] : a.theta[k];
+			// the twiddles are re-read every substep (L1-resident) rather than held
+			// in registers across the loop: registers are the occupancy limit here
+			const double2 *twk = tw;
+			asm volatile("" : "+s"(twk));
 #pragma unroll
 			for (int j = 0; j < 16; ++j) v[j] = nonlinear(v[j], th);
-			fft4096_fwd(v, L, tw);
+			fft4096_fwd(v, L, twk);
 			// Fourier multiplier, stored per thread in the digit-reversed order
 			const double2 *m = mult + size_t(k < 4 ? k : 6 - k) * 4096 + t;
 #pragma unroll
-			for (int j = 0; j < 16; ++j) v[j] = cmul(v[j], m[256 * j]);
-			fft4096_inv(v, L, tw);
+			for (int j = 0; j < 16; ++j) v[pos(j)] = cmul(v[pos(j)], m[256 * j]);
+			fft4096_inv(v, L, twk);
 		}
 	}
 #pragma unroll
-	for (int j = 0; j < 16; ++j) psi[t + 256 * j] = nonlinear(v[j], a.theta[7]);
+	for (int j = 0; j < 16; ++j) psi[t + 256 * j] = nonlinear(v[pos(j)], a.theta[7]);
 }
 
 // ------------------------------------------------------------ generic n
