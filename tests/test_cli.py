@@ -1,0 +1,194 @@
+"""The nlchains command line: argument rules, file formats and batch cadence.
+
+CPU tests cover parsing and validation (common/main.cpp:53-134) and the
+error exits (common/main.cpp:137-176); GPU tests run whole simulations and
+check every file the reference writes (README.md:74-82) against the oracle.
+"""
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+ROOT = Path(__file__).resolve().parent.parent
+CLI = ROOT / "nlchains_amd" / "nlchains"
+
+
+def run(*args, check=None, timeout=300):
+    p = subprocess.run([str(CLI), *map(str, args)], capture_output=True, text=True, timeout=timeout)
+    if check is not None:
+        assert p.returncode == check, (p.returncode, p.stdout, p.stderr)
+    return p
+
+
+def common(tmp, n=64, c=8, dt=0.1, b=10, extra=()):
+    init = tmp / "init"
+    if not init.exists():
+        np.random.default_rng(0).normal(size=(c, n, 2)).tofile(init)
+    return ["-n", n, "-c", c, "--dt", dt, "-b", b, "-p", tmp / "out", "-i", init, *extra]
+
+
+def test_cli_lists_programs():
+    p = run(check=0)
+    assert "DNKG, DNLS, FPUT, Toda, dDNKG" in p.stdout
+
+
+@pytest.mark.parametrize("prog,opts", [("DNKG", ["-m", "--beta"]), ("FPUT", ["--alpha", "--beta"]),
+                                       ("Toda", ["--alpha"]), ("dDNKG", ["-m", "--beta"]),
+                                       ("DNLS", ["--beta", "--no_linear_callback"])])
+def test_cli_help_per_program(prog, opts):
+    p = run(prog, check=0)
+    assert f"Options for {prog}" in p.stdout
+    for o in ["--chain_length [ -n ]", "--kernel_batching [ -b ]", "--dump_interval", "--entropymask", *opts]:
+        assert o in p.stdout
+
+
+@pytest.mark.parametrize("extra,msg", [
+    (["-b", "0"], "--kernel_batching must be positive"),
+    (["--dt", "0"], "--dt and --kernel_batching must be positive"),
+    (["-n", "1"], "--chain_length must be >= 2"),
+    (["--dump_interval", "15"], "--dump_interval must be a multiple of --kernel_batching"),
+    (["--dump_interval", "0"], "--dump_interval must be > 0"),
+    (["-s", "25"], "--steps must be a multiple of --kernel_batching"),
+    (["-s", "20", "-o", "20"], "--time_offset must be less than --steps"),
+    (["--bogus", "1"], "unrecognised option '--bogus'"),
+    (["--dt", "abc"], "is invalid"),
+])
+def test_cli_argument_rules(tmp_path, extra, msg):
+    args = common(tmp_path)
+    for k, v in zip(extra[::2], extra[1::2]):  # override or append
+        if k in args:
+            args[args.index(k) + 1] = v
+        else:
+            args += [k, v]
+    p = run("FPUT", *args, "--alpha", 0, "--beta", 1, check=1)
+    assert "invalid argument" in p.stderr and msg.split(" must")[0] in p.stderr, p.stderr
+
+
+def test_cli_missing_required(tmp_path):
+    p = run("FPUT", *common(tmp_path), "--alpha", 0, check=1)
+    assert "the option '--beta' is required but missing" in p.stderr
+
+
+def test_cli_io_errors(tmp_path):
+    args = common(tmp_path)
+    args[args.index("-i") + 1] = tmp_path / "missing"
+    p = run("FPUT", *args, "--alpha", 0, "--beta", 1, check=1)
+    assert "I/O error, could not read initial state" in p.stderr
+    short = tmp_path / "short"
+    np.zeros(10).tofile(short)
+    args[args.index("-i") + 1] = short
+    assert "could not read initial state" in run("FPUT", *args, "--alpha", 0, "--beta", 1, check=1).stderr
+    mask = tmp_path / "mask"
+    np.zeros(64, np.uint8).tofile(mask)
+    p = run("FPUT", *common(tmp_path), "--alpha", 0, "--beta", 1, "--entropymask", mask, check=1)
+    assert "Too few modes for entropy calculation" in p.stderr
+    p = run("dDNKG", *common(tmp_path), "-m", tmp_path / "nomasses", "--beta", 1, check=1)
+    assert "could not read m file" in p.stderr
+
+
+def test_cli_without_gpu_fails_cleanly(tmp_path):
+    import ctypes
+    try:
+        hip = ctypes.CDLL("libamdhip64.so")
+        cnt = ctypes.c_int(0)
+        if hip.hipGetDeviceCount(ctypes.byref(cnt)) == 0 and cnt.value > 0:
+            pytest.skip("a GPU is present")
+    except OSError:
+        pass
+    p = run("FPUT", *common(tmp_path), "--alpha", 0, "--beta", 1, check=1)
+    assert "no HIP device available" in p.stderr
+
+
+# ------------------------------------------------------------------ GPU
+
+def read_entropy(prefix):
+    return np.fromfile(f"{prefix}-entropy").reshape(-1, 3)
+
+
+@pytest.mark.gpu
+def test_cli_fput_run_files_and_cadence(gpu, tmp_path):
+    n, c, dt, b = 64, 32, 0.1, 100
+    # every mode carries energy (zero-energy modes make log e, hence the entropies, meaningless)
+    s0 = gpu.ic_plane_waves(n, c, 0.0, 1, 5, 0.1, 1) + 0.01 * np.random.default_rng(4).normal(size=(c, n, 2))
+    init = tmp_path / "init"
+    s0.tofile(init)
+    pre = tmp_path / "run"
+    run("FPUT", "-n", n, "-c", c, "--dt", dt, "-b", b, "-s", 300, "--dump_interval", 200, "-p", pre, "-i", init,
+        "--alpha", 0, "--beta", 1, check=0)
+    files = sorted(p.name for p in tmp_path.iterdir() if p.name.startswith("run"))
+    assert files == ["run-0", "run-200", "run-300", "run-entropy", "run-linenergies-0", "run-linenergies-200",
+                     "run-linenergies-300"], files
+    ent = read_entropy(pre)
+    np.testing.assert_allclose(ent[:, 0], [0, 10, 20, 30])
+    states = {0: s0}
+    for t in (100, 200, 300):
+        states[t] = O.move_real(O.FPUT, states[t - 100], 100, dt, beta=1.0)
+    w = O.dispersion_real(n, 0.0)
+    for row, t in zip(ent, (0, 100, 200, 300)):
+        e = O.calc_linenergies(O.linenergies_real(states[t], w), 0.5 / c / n, True)
+        wt, inf = O.entropies(e, True)
+        assert abs(row[1] - wt) <= 1e-9 * abs(wt) and abs(row[2] - inf) <= 1e-9 * abs(inf)
+        if t in (0, 200, 300):
+            got = np.fromfile(f"{pre}-linenergies-{t}")
+            np.testing.assert_allclose(got, e, rtol=0, atol=1e-9 * np.abs(e).max())
+            dump = np.fromfile(f"{pre}-{t}").reshape(c, n, 2)
+            assert np.abs(dump - states[t]).max() <= 1e-10 * np.abs(states[t]).max()
+    # resume from the t=200 dump: -o 200 -s 300 reproduces the t=300 dump
+    pre2 = tmp_path / "resumed"
+    run("FPUT", "-n", n, "-c", c, "--dt", dt, "-b", b, "-s", 300, "-o", 200, "-p", pre2, "-i", f"{pre}-200",
+        "--alpha", 0, "--beta", 1, check=0)
+    assert np.array_equal(np.fromfile(f"{pre2}-300"), np.fromfile(f"{pre}-300"))
+    np.testing.assert_array_equal(read_entropy(pre2)[:, 0], [20, 30])
+
+
+@pytest.mark.gpu
+def test_cli_entropy_limit_stops_run(gpu, tmp_path):
+    n, c = 64, 8
+    gpu.ic_plane_waves(n, c, 0.0, 1, 5, 0.1, 1).tofile(tmp_path / "init")
+    pre = tmp_path / "lim"
+    run("FPUT", "-n", n, "-c", c, "--dt", 0.1, "-b", 10, "-p", pre, "-i", tmp_path / "init", "--alpha", 0, "--beta", 1,
+        "-e", 1e9, "--dump_interval", 20, check=0)
+    ent = read_entropy(pre)
+    assert ent.shape[0] == 1  # the entropy at t=0 is already below the limit
+    assert (tmp_path / "lim-0").exists() and not (tmp_path / "lim-10").exists()
+
+
+@pytest.mark.gpu
+def test_cli_ddnkg_eigensystem_files_and_run(gpu, tmp_path):
+    n, c = 48, 6
+    masses = gpu.ic_masses(n, 0.5, 1.5, 5)
+    masses.tofile(tmp_path / "m")
+    om, V = gpu.eigensystem(masses + 2)
+    s0 = gpu.ic_eigenmodes(n, c, om, V, 6, 0.5, 5)
+    s0.tofile(tmp_path / "init")
+    pre = tmp_path / "d"
+    run("dDNKG", "-n", n, "-c", c, "--dt", 0.05, "-b", 20, "-s", 40, "-p", pre, "-i", tmp_path / "init", "-m",
+        tmp_path / "m", "--beta", 1, check=0)
+    omegas = np.fromfile(f"{pre}-omegas")
+    A = np.diag(masses + 2) - np.eye(n, k=1) - np.eye(n, k=-1)
+    A[0, -1] = A[-1, 0] = -1
+    np.testing.assert_allclose(omegas, np.sqrt(np.linalg.eigvalsh(A)), rtol=1e-12)
+    Vf = np.fromfile(f"{pre}-eigenvectors").reshape(n, n).T  # column-major on disk
+    np.testing.assert_allclose(A @ Vf, Vf * omegas ** 2, atol=1e-11)
+    want = O.move_real(O.DDNKG, s0, 40, 0.05, beta=1.0, mp2=masses + 2)
+    got = np.fromfile(f"{pre}-40").reshape(c, n, 2)
+    assert np.abs(got - want).max() <= 1e-10 * np.abs(want).max()
+
+
+@pytest.mark.gpu
+def test_cli_dnls_run(gpu, tmp_path):
+    n, c = 256, 4
+    s0 = gpu.ic_dnls(n, c, 16, 1.0, 3)
+    s0.tofile(tmp_path / "init")
+    pre = tmp_path / "s"
+    run("DNLS", "-n", n, "-c", c, "--dt", 0.01, "-b", 50, "-s", 100, "-p", pre, "-i", tmp_path / "init", "--beta", 1,
+        check=0)
+    want = O.move_dnls(s0, 100, 0.01, 1.0)
+    got = np.fromfile(f"{pre}-100").reshape(c, n, 2)
+    assert np.abs(got - want).max() <= 1e-10 * np.abs(want).max()
+    e = O.calc_linenergies(O.linenergies_dnls(want), 1.0 / (n * c), True)
+    np.testing.assert_allclose(np.fromfile(f"{pre}-linenergies-100"), e, rtol=0, atol=1e-9 * e.max())
