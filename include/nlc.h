@@ -152,6 +152,35 @@ int nlc_linenergies_raw(nlc_ctx *ctx, double *e_raw);
  * For DNLS also norms[c] = sum |psi|^2 (norms may be NULL; ignored otherwise). */
 int nlc_chain_energies(nlc_ctx *ctx, double *energies, double *norms);
 
+/* ---- batched form ------------------------------------------------------ */
+
+/* Ragged ensemble: nchains chains of individual lengths chain_lengths[i]
+ * (DNKG, FPUT, Toda).  The reference runs one chain length per process
+ * (gconf.chain_length, common/configuration.hpp:24); here chains of all lengths
+ * share one context and one persistent launch per batch.  State layout for
+ * upload/download: the chains concatenated in the caller's order, each in the
+ * reference layout.  p->chain_length and p->copies are ignored.  Energies and
+ * entropies are computed per length class: nlc_linenergies writes, for every
+ * class in ascending length, that class's linear energies (sum of the class
+ * lengths doubles in total) and its two entropies (2 doubles per class). */
+int nlc_create_ragged(nlc_ctx **out, const nlc_params *p, const uint32_t *chain_lengths, uint32_t nchains);
+/* Length classes of a context (ascending length); a uniform context has one. */
+int nlc_ragged_classes(const nlc_ctx *ctx, uint32_t *nclasses, uint32_t *lengths, uint32_t *counts, uint32_t cap);
+
+/* Synchronous.  One dump interval: nlc_move(steps) followed by the energy pass,
+ * normalisation and entropies, replayed from a hipGraph captured on the first
+ * call (and recaptured when steps or norm change).  Outputs as nlc_linenergies.
+ * The batched counterpart of one iteration of DNKG_FPUT_Toda.cpp:108-143. */
+int nlc_batch(nlc_ctx *ctx, uint32_t steps, double norm, double *linenergies, double *entropies);
+
+/* Times every applicable step-kernel shape (wave shapes lanes x sites and the
+ * split kernel for uniform ensembles; sites per lane of the persistent kernel
+ * for ragged ones) on the context's own state for trial_steps steps, keeps the
+ * fastest, restores the state, and writes a one-line report.  Replaces the
+ * reference's compile-time optimized_chain_length choice (CMakeLists.txt:74-78)
+ * and its launch heuristic (common/utilities_cuda.cpp:46-72). */
+int nlc_autotune(nlc_ctx *ctx, uint32_t trial_steps, char *report, size_t len);
+
 /* Waits for all work enqueued on the context. */
 int nlc_sync(nlc_ctx *ctx);
 /* The context's hipStream_t (for events and graph capture by the caller). */
@@ -185,6 +214,14 @@ int nlc_ic_dnls(uint32_t n, uint32_t copies, uint32_t kmax, double norm_density,
  * e_density*n/modes each, uniform random phases. */
 int nlc_ic_eigenmodes(uint32_t n, uint32_t copies, const double *omegas, const double *eigenvectors, uint32_t modes,
                       double e_density, uint64_t seed, double *state);
+/* Ragged chain lengths: each drawn uniformly from choices[0..nchoices). */
+int nlc_ic_ragged_lengths(uint32_t nchains, const uint32_t *choices, uint32_t nchoices, uint64_t seed,
+                          uint32_t *lengths);
+/* Ragged running waves (m = 0 dispersion for FPUT/Toda, m > 0 for DNKG): chain
+ * i of length n_i gets energy density ~ U[e_lo, e_hi) spread equally over modes
+ * 1..max(1, floor(mode_fraction n_i)).  Chains concatenated in order. */
+int nlc_ic_ragged_plane_waves(const uint32_t *lengths, uint32_t nchains, double m, double mode_fraction, double e_lo,
+                              double e_hi, uint64_t seed, double *state);
 /* Per-site masses m_x ~ U[lo, hi) (dDNKG masses file content). */
 int nlc_ic_masses(uint32_t n, double lo, double hi, uint64_t seed, double *masses);
 

@@ -81,6 +81,14 @@ _sig("nlc_ic_plane_waves", C.c_int, C.c_uint32, C.c_uint32, C.c_double, C.c_uint
 _sig("nlc_ic_dnls", C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, C.c_double, C.c_uint64, _dp)
 _sig("nlc_ic_eigenmodes", C.c_int, C.c_uint32, C.c_uint32, _dp, _dp, C.c_uint32, C.c_double, C.c_uint64, _dp)
 _sig("nlc_ic_masses", C.c_int, C.c_uint32, C.c_double, C.c_double, C.c_uint64, _dp)
+_u32p = C.POINTER(C.c_uint32)
+_sig("nlc_create_ragged", C.c_int, C.POINTER(_ctx_p), C.POINTER(Params), _u32p, C.c_uint32)
+_sig("nlc_ragged_classes", C.c_int, _ctx_p, _u32p, _u32p, _u32p, C.c_uint32)
+_sig("nlc_batch", C.c_int, _ctx_p, C.c_uint32, C.c_double, _dp, _dp)
+_sig("nlc_autotune", C.c_int, _ctx_p, C.c_uint32, C.c_char_p, C.c_size_t)
+_sig("nlc_ic_ragged_lengths", C.c_int, C.c_uint32, _u32p, C.c_uint32, C.c_uint64, _u32p)
+_sig("nlc_ic_ragged_plane_waves", C.c_int, _u32p, C.c_uint32, C.c_double, C.c_double, C.c_double, C.c_double,
+     C.c_uint64, _dp)
 
 
 class NlcError(RuntimeError):
@@ -150,6 +158,22 @@ def ic_masses(n: int, lo: float, hi: float, seed: int) -> np.ndarray:
     return m
 
 
+def ic_ragged_lengths(nchains: int, choices, seed: int) -> np.ndarray:
+    ch = np.ascontiguousarray(choices, dtype=np.uint32)
+    out = np.zeros(nchains, dtype=np.uint32)
+    _check(lib.nlc_ic_ragged_lengths(nchains, ch.ctypes.data_as(_u32p), ch.size, seed, out.ctypes.data_as(_u32p)))
+    return out
+
+
+def ic_ragged_plane_waves(lengths, m: float, mode_fraction: float, e_lo: float, e_hi: float, seed: int) -> np.ndarray:
+    """Concatenated chains, float64[sum(lengths), 2]."""
+    ln = np.ascontiguousarray(lengths, dtype=np.uint32)
+    s = np.zeros((int(ln.sum()), 2))
+    _check(lib.nlc_ic_ragged_plane_waves(ln.ctypes.data_as(_u32p), ln.size, m, mode_fraction, e_lo, e_hi, seed,
+                                         _ptr(s)))
+    return s
+
+
 # ---- ensembles on the GPU -----------------------------------------------------
 
 class Ensemble:
@@ -170,6 +194,33 @@ class Ensemble:
         self.model, self.n, self.copies = model, n, copies
         self._ctx = _ctx_p()
         _check(lib.nlc_create(C.byref(self._ctx), C.byref(p)))
+        self.lengths = None
+        self._shape = (copies, n, 2)
+        self._nlin, self._nent = n, 2
+
+    @classmethod
+    def ragged(cls, model: int, lengths, dt: float, *, m: float = 0.0, alpha: float = 0.0, beta: float = 0.0,
+               precision: int = FP64):
+        """A ragged ensemble: chains of the given lengths in one context."""
+        self = cls.__new__(cls)
+        p = Params()
+        _check(lib.nlc_params_init(C.byref(p)))
+        p.model, p.precision, p.dt, p.m, p.alpha, p.beta = model, precision, dt, m, alpha, beta
+        self._masses = None
+        ln = np.ascontiguousarray(lengths, dtype=np.uint32)
+        self._ctx = _ctx_p()
+        _check(lib.nlc_create_ragged(C.byref(self._ctx), C.byref(p), ln.ctypes.data_as(_u32p), ln.size))
+        self.model, self.lengths = model, ln
+        self.n, self.copies = int(ln.max()), int(ln.size)
+        self._shape = (int(ln.sum()), 2)
+        nc = C.c_uint32()
+        _check(lib.nlc_ragged_classes(self._ctx, C.byref(nc), None, None, 0), self._ctx)
+        cl, ct = np.zeros(nc.value, np.uint32), np.zeros(nc.value, np.uint32)
+        _check(lib.nlc_ragged_classes(self._ctx, C.byref(nc), cl.ctypes.data_as(_u32p), ct.ctypes.data_as(_u32p),
+                                      nc.value), self._ctx)
+        self.class_lengths, self.class_counts = cl, ct
+        self._nlin, self._nent = int(cl.sum()), 2 * nc.value
+        return self
 
     def close(self):
         if self._ctx:
@@ -194,11 +245,11 @@ class Ensemble:
 
     def upload(self, state: np.ndarray) -> None:
         state = np.ascontiguousarray(state, dtype=np.float64)
-        assert state.shape == (self.copies, self.n, 2), state.shape
+        assert state.shape == self._shape, state.shape
         _check(lib.nlc_upload(self._ctx, state.ctypes.data_as(C.c_void_p)), self._ctx)
 
     def download(self) -> np.ndarray:
-        s = np.empty((self.copies, self.n, 2))
+        s = np.empty(self._shape)
         _check(lib.nlc_download(self._ctx, s.ctypes.data_as(C.c_void_p)), self._ctx)
         return s
 
@@ -217,12 +268,23 @@ class Ensemble:
             _check(lib.nlc_set_entropy_mask(self._ctx, mk.ctypes.data_as(C.c_void_p), int(a0is0)), self._ctx)
 
     def linenergies(self, norm: float = 0.0) -> tuple[np.ndarray, np.ndarray]:
-        e, s = np.zeros(self.n), np.zeros(2)
+        e, s = np.zeros(self._nlin), np.zeros(self._nent)
         _check(lib.nlc_linenergies(self._ctx, norm, _ptr(e), _ptr(s)), self._ctx)
         return e, s
 
+    def batch(self, steps: int, norm: float = 0.0) -> tuple[np.ndarray, np.ndarray]:
+        """One dump interval (move + energies + entropies) from a cached hipGraph."""
+        e, s = np.zeros(self._nlin), np.zeros(self._nent)
+        _check(lib.nlc_batch(self._ctx, steps, norm, _ptr(e), _ptr(s)), self._ctx)
+        return e, s
+
+    def autotune(self, trial_steps: int = 20) -> str:
+        buf = C.create_string_buffer(1024)
+        _check(lib.nlc_autotune(self._ctx, trial_steps, buf, 1024), self._ctx)
+        return buf.value.decode()
+
     def linenergies_raw(self) -> np.ndarray:
-        e = np.zeros(self.n)
+        e = np.zeros(self._nlin)
         _check(lib.nlc_linenergies_raw(self._ctx, _ptr(e)), self._ctx)
         return e
 

@@ -1,5 +1,6 @@
-// C-ABI layer of libnlchains_amd: context life cycle, kernel selection and the
-// entry points declared in include/nlc.h.  No exception crosses the ABI.
+// C-ABI layer of libnlchains_amd: context life cycle, kernel selection,
+// ragged ensembles, hipGraph batches, the autotuner and the entry points
+// declared in include/nlc.h.  No exception crosses the ABI.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -17,6 +18,19 @@ namespace {
 
 thread_local std::string g_create_error;
 
+// A length class: `count` chains of n sites, contiguous in device memory.
+struct Klass {
+	uint32_t n = 0, count = 0;
+	uint64_t site0 = 0;  // first site of the class in the device shard
+	uint64_t chain0 = 0; // first chain (internal order)
+	uint64_t tab0 = 0;   // offset of the class in the per-class tables (sum of smaller n)
+	uint32_t rows = 0;   // energy partial rows
+	// per-class step kernel when the ragged persistent kernel does not apply
+	nlc::StepLauncher launcher = nullptr;
+	int L = 0, S = 0;
+	bool exact = false;
+};
+
 } // namespace
 
 struct nlc_ctx {
@@ -24,17 +38,30 @@ struct nlc_ctx {
 	std::vector<double> masses; // dDNKG m_x
 	hipStream_t stream = nullptr;
 	nlc::StepArgs args{};
-	uint32_t n = 0, copies = 0;
+	uint32_t n = 0, copies = 0; // uniform: chain length and copies; ragged: max length and chain count
+	uint64_t total_sites = 0;
 	size_t state_bytes = 0;
+	std::vector<Klass> cls;
+	uint64_t table_len = 0; // sum of class lengths
+
+	// ragged ensembles (chains in the caller's order vs class-major on the device)
+	bool ragged = false;
+	std::vector<uint32_t> lengths;       // caller order
+	std::vector<uint64_t> caller_site0;  // caller order
+	std::vector<uint64_t> device_site0;  // caller order -> device offset
+	std::vector<uint32_t> device_chain;  // caller order -> internal chain index
+	nlc::RaggedItem *items = nullptr;
+	uint32_t nitems = 0;
+	int ragged_S = 0;
+	nlc::RaggedLauncher ragged_launcher = nullptr;
 
 	double2 *state = nullptr;
-	double *omega = nullptr;   // n
-	double2 *tw = nullptr;     // n roots of unity exp(-2 pi i j/n)
+	double *omega = nullptr;   // per-class dispersion, table_len
+	double2 *tw = nullptr;     // per-class roots of unity exp(-2 pi i j/n), table_len
 	double *partial = nullptr; // energy partial sums
 	double *reduce_scratch = nullptr;
-	uint32_t partial_rows = 0;
-	double *e_raw = nullptr;   // n
-	double *results = nullptr; // n + 2
+	double *e_raw = nullptr;   // table_len
+	double *results = nullptr; // per class: n energies + 2 entropies
 	uint32_t *modes = nullptr; // entropy mode indices
 	uint32_t nmodes = 0;
 	int a0is0 = 0;
@@ -45,12 +72,19 @@ struct nlc_ctx {
 	double *energy_d = nullptr, *norm_d = nullptr;
 	double2 *dnls_table = nullptr; // DNLS Fourier multipliers
 	double theta[9] = {};          // DNLS nonlinear angles
-	double *h_results = nullptr; // pinned, n + 2
+	double *h_results = nullptr;   // pinned copy of results
 
-	nlc::StepLauncher launcher = nullptr;
+	nlc::StepLauncher launcher = nullptr; // uniform wave kernel
 	int L = 0, S = 0;
 	bool exact = false, use_split = false;
-	std::string err;
+
+	// cached graph of one batch (nlc_batch)
+	hipGraph_t graph = nullptr;
+	hipGraphExec_t graph_exec = nullptr;
+	uint32_t graph_steps = 0;
+	double graph_norm = 0;
+
+	std::string err, tune_report;
 
 	int fail(int code, const char *fmt, ...) {
 		char buf[512];
@@ -64,6 +98,7 @@ struct nlc_ctx {
 	int hip_fail(hipError_t e, const char *what) {
 		return fail(NLC_EHIP, "%s: %s (%s)", what, hipGetErrorName(e), hipGetErrorString(e));
 	}
+	size_t results_len() const { return size_t(table_len) + 2 * cls.size(); }
 };
 
 #define NLC_HIP(ctx, call)                                                                                            \
@@ -71,10 +106,20 @@ struct nlc_ctx {
 		hipError_t e_ = (call);                                                                                        \
 		if (e_ != hipSuccess) return (ctx)->hip_fail(e_, #call);                                                       \
 	} while (0)
+#define NLC_TRY(call)                                                                                                 \
+	do {                                                                                                             \
+		int rc_ = (call);                                                                                              \
+		if (rc_) return rc_;                                                                                           \
+	} while (0)
 
 namespace {
 
-bool is_pow2(uint32_t x) { return x && !(x & (x - 1)); }
+bool is_pow2(uint64_t x) { return x && !(x & (x - 1)); }
+int log2u(uint32_t x) {
+	int l = 0;
+	while ((1u << l) < x) ++l;
+	return l;
+}
 
 nlc::StepLauncher lookup(int model, int L, int S, bool exact) {
 	switch (model) {
@@ -82,6 +127,15 @@ nlc::StepLauncher lookup(int model, int L, int S, bool exact) {
 	case NLC_FPUT: return nlc::lookup_step_fput(L, S, exact);
 	case NLC_TODA: return nlc::lookup_step_toda(L, S, exact);
 	case NLC_DDNKG: return nlc::lookup_step_ddnkg(L, S, exact);
+	}
+	return nullptr;
+}
+
+nlc::RaggedLauncher lookup_ragged(int model, int S) {
+	switch (model) {
+	case NLC_DNKG: return nlc::lookup_ragged_dnkg(S);
+	case NLC_FPUT: return nlc::lookup_ragged_fput(S);
+	case NLC_TODA: return nlc::lookup_ragged_toda(S);
 	}
 	return nullptr;
 }
@@ -104,12 +158,47 @@ bool default_shape(uint32_t n, int &L, int &S, bool &exact) {
 	return true;
 }
 
+// Every instantiated wave shape that holds a chain of n sites.
+struct Shape {
+	int L, S;
+	bool exact;
+};
+std::vector<Shape> wave_shapes(int model, uint32_t n) {
+	std::vector<Shape> out;
+	for (int L = 1; L <= 64; L *= 2) {
+		const int S = int((n + L - 1) / L);
+		if (S < 1 || S > 32) continue;
+		const bool exact = uint32_t(L * S) == n;
+		if (!exact && (L != 64 || S < 2)) continue;
+		if (lookup(model, L, S, exact)) out.push_back({L, S, exact});
+	}
+	return out;
+}
+
+// The ragged persistent kernel with S sites per lane needs every class to be a
+// power of two in [S, 64 S].
+bool ragged_fits(const std::vector<Klass> &cls, int S) {
+	for (const auto &k : cls)
+		if (!is_pow2(k.n) || k.n < uint32_t(S) || k.n > 64u * S) return false;
+	return true;
+}
+
+void drop_graph(nlc_ctx *c) {
+	if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
+	if (c->graph) hipGraphDestroy(c->graph);
+	c->graph_exec = nullptr;
+	c->graph = nullptr;
+	c->graph_steps = 0;
+}
+
 void free_all(nlc_ctx *c) {
 	if (c->stream) hipStreamSynchronize(c->stream);
-	for (void *ptr : {(void *) c->state, (void *) c->omega, (void *) c->tw, (void *) c->partial, (void *) c->reduce_scratch, (void *) c->e_raw,
-	                  (void *) c->results, (void *) c->modes, (void *) c->mp2, (void *) c->masses_d,
-	                  (void *) c->eigvec, (void *) c->split_q, (void *) c->split_p, (void *) c->energy_d,
-	                  (void *) c->norm_d})
+	drop_graph(c);
+	for (void *ptr :
+	     {(void *) c->state, (void *) c->omega, (void *) c->tw, (void *) c->partial, (void *) c->reduce_scratch,
+	      (void *) c->e_raw, (void *) c->results, (void *) c->modes, (void *) c->mp2, (void *) c->masses_d,
+	      (void *) c->eigvec, (void *) c->split_q, (void *) c->split_p, (void *) c->energy_d, (void *) c->norm_d,
+	      (void *) c->dnls_table, (void *) c->items})
 		if (ptr) hipFree(ptr);
 	if (c->h_results) hipHostFree(c->h_results);
 	if (c->stream) hipStreamDestroy(c->stream);
@@ -126,9 +215,9 @@ int dev_alloc(nlc_ctx *c, T **ptr, size_t count, const char *what) {
 	return NLC_OK;
 }
 
-int setup(nlc_ctx *c) {
+// Buffers and tables shared by uniform and ragged contexts.
+int setup_common(nlc_ctx *c) {
 	const nlc_params &p = c->p;
-	const uint32_t n = c->n;
 	// constants (DNKG_FPUT_Toda.cpp:76-84, dDNKG.cpp:93-97)
 	for (int i = 0; i < 8; ++i) {
 		c->args.dc[i] = nlc::kYoshidaC[i] * p.dt;
@@ -143,82 +232,91 @@ int setup(nlc_ctx *c) {
 	c->a0is0 = (p.model == NLC_DNLS) ? 1 : (p.model == NLC_DDNKG ? 0 : (p.m == 0.0));
 
 	NLC_HIP(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-	int rc;
-	if ((rc = dev_alloc(c, &c->state, size_t(n) * c->copies, "shard"))) return rc;
-	if ((rc = dev_alloc(c, &c->omega, n, "omega"))) return rc;
-	if ((rc = dev_alloc(c, &c->tw, n, "twiddles"))) return rc;
-	if ((rc = dev_alloc(c, &c->e_raw, n, "linenergies"))) return rc;
-	if ((rc = dev_alloc(c, &c->results, n + 2, "results"))) return rc;
-	if ((rc = dev_alloc(c, &c->modes, n, "entropy modes"))) return rc;
-	if ((rc = dev_alloc(c, &c->energy_d, c->copies, "chain energies"))) return rc;
-	if ((rc = dev_alloc(c, &c->norm_d, c->copies, "chain norms"))) return rc;
+	c->state_bytes = sizeof(double2) * c->total_sites;
+	NLC_TRY(dev_alloc(c, &c->state, c->total_sites, "shard"));
+	NLC_TRY(dev_alloc(c, &c->omega, c->table_len, "omega"));
+	NLC_TRY(dev_alloc(c, &c->tw, c->table_len, "twiddles"));
+	NLC_TRY(dev_alloc(c, &c->e_raw, c->table_len, "linenergies"));
+	NLC_TRY(dev_alloc(c, &c->results, c->results_len(), "results"));
+	NLC_TRY(dev_alloc(c, &c->modes, c->n, "entropy modes"));
+	NLC_TRY(dev_alloc(c, &c->energy_d, c->copies, "chain energies"));
+	NLC_TRY(dev_alloc(c, &c->norm_d, c->copies, "chain norms"));
 	{
-		hipError_t e = hipHostMalloc((void **) &c->h_results, sizeof(double) * (n + 2), hipHostMallocDefault);
+		hipError_t e = hipHostMalloc((void **) &c->h_results, sizeof(double) * c->results_len(), hipHostMallocDefault);
 		if (e != hipSuccess) return c->fail(NLC_ENOMEM, "hipHostMalloc(results): %s", hipGetErrorString(e));
 	}
-
-	// twiddles and dispersion, computed on the host in FP64
-	std::vector<double2> tw(n);
-	for (uint32_t j = 0; j < n; ++j) {
-		const double ang = 2.0 * M_PI * double(j) / double(n);
-		tw[j] = make_double2(std::cos(ang), -std::sin(ang));
+	// per-class twiddles and dispersion, computed on the host in FP64
+	std::vector<double2> tw(c->table_len);
+	std::vector<double> omega(c->table_len);
+	size_t partial_len = 1, scratch_len = 1;
+	for (auto &k : c->cls) {
+		for (uint32_t j = 0; j < k.n; ++j) {
+			const double ang = 2.0 * M_PI * double(j) / double(k.n);
+			tw[k.tab0 + j] = make_double2(std::cos(ang), -std::sin(ang));
+		}
+		if (p.model != NLC_DDNKG) nlc_dispersion(p.model, k.n, p.m, omega.data() + k.tab0);
+		k.rows = p.model == NLC_DDNKG ? nlc::ddnkg_partial_rows(k.count) : std::min<uint32_t>(k.count, 1024);
+		partial_len = std::max(partial_len, size_t(k.rows) * k.n);
+		scratch_len = std::max(scratch_len, size_t(nlc::reduce_scratch_rows(k.rows)) * k.n);
 	}
-	NLC_HIP(c, hipMemcpy(c->tw, tw.data(), sizeof(double2) * n, hipMemcpyHostToDevice));
-	std::vector<double> omega(n);
-
+	NLC_HIP(c, hipMemcpy(c->tw, tw.data(), sizeof(double2) * c->table_len, hipMemcpyHostToDevice));
 	if (p.model == NLC_DDNKG) {
+		const uint32_t n = c->n;
 		std::vector<double> mp2(n), V(size_t(n) * n);
 		for (uint32_t i = 0; i < n; ++i) mp2[i] = c->masses[i] + 2; // dDNKG.cpp:47
-		rc = nlc::eigensystem(n, mp2.data(), omega.data(), V.data());
+		const int rc = nlc::eigensystem(n, mp2.data(), omega.data(), V.data());
 		if (rc) return c->fail(rc, "eigensystem of the dDNKG linear operator did not converge");
-		if ((rc = dev_alloc(c, &c->mp2, n, "mp2"))) return rc;
-		if ((rc = dev_alloc(c, &c->masses_d, n, "masses"))) return rc;
-		if ((rc = dev_alloc(c, &c->eigvec, size_t(n) * n, "eigenvectors"))) return rc;
+		NLC_TRY(dev_alloc(c, &c->mp2, n, "mp2"));
+		NLC_TRY(dev_alloc(c, &c->masses_d, n, "masses"));
+		NLC_TRY(dev_alloc(c, &c->eigvec, size_t(n) * n, "eigenvectors"));
 		NLC_HIP(c, hipMemcpy(c->mp2, mp2.data(), sizeof(double) * n, hipMemcpyHostToDevice));
 		NLC_HIP(c, hipMemcpy(c->masses_d, c->masses.data(), sizeof(double) * n, hipMemcpyHostToDevice));
 		NLC_HIP(c, hipMemcpy(c->eigvec, V.data(), sizeof(double) * n * n, hipMemcpyHostToDevice));
-		c->partial_rows = nlc::ddnkg_partial_rows(c->copies);
-	} else {
-		nlc_dispersion(p.model, n, p.m, omega.data());
-		c->partial_rows = std::min<uint32_t>(c->copies, 1024);
 	}
-	NLC_HIP(c, hipMemcpy(c->omega, omega.data(), sizeof(double) * n, hipMemcpyHostToDevice));
-	if ((rc = dev_alloc(c, &c->partial, size_t(c->partial_rows) * n, "energy partials"))) return rc;
-	if ((rc = dev_alloc(c, &c->reduce_scratch, size_t(nlc::reduce_scratch_rows(c->partial_rows)) * n,
-	                    "reduction scratch")))
-		return rc;
+	NLC_HIP(c, hipMemcpy(c->omega, omega.data(), sizeof(double) * c->table_len, hipMemcpyHostToDevice));
+	NLC_TRY(dev_alloc(c, &c->partial, partial_len, "energy partials"));
+	NLC_TRY(dev_alloc(c, &c->reduce_scratch, scratch_len, "reduction scratch"));
+	return NLC_OK;
+}
 
-	if (p.model == NLC_DNLS) {
-		// DNLS/dispersion.cpp:19-26 (multipliers) and DNLS.cpp:96-100,161-175 (angles)
-		if (!is_pow2(n) || n > nlc::kEnergyFftMax)
-			return c->fail(NLC_EUNSUPPORTED, "DNLS needs a power-of-two chain_length <= %u in this build",
-			               nlc::kEnergyFftMax);
-		const double bdt = p.beta * p.dt;
-		for (int k = 0; k < 8; ++k) c->theta[k] = (bdt * 1.) * nlc::kYoshidaC[k];
-		c->theta[8] = (bdt * 2.) * nlc::kYoshidaC[0];
-		std::vector<double2> tab(size_t(7) * n), perm(size_t(7) * n);
-		const double nrm = 1.0 / n;
-		for (int k = 0; k < 7; ++k)
-			for (uint32_t j = 0; j < n; ++j) {
-				const double ang = ((-p.dt) * nlc::kYoshidaD[k]) * omega[j];
-				tab[size_t(k) * n + j] = make_double2(std::cos(ang) * nrm, std::sin(ang) * nrm);
-			}
-		const double2 *src = tab.data();
-		if (nlc::dnls_fast_path(n)) {
-			nlc::dnls_permute_tables(n, tab.data(), perm.data());
-			src = perm.data();
+// DNLS tables: DNLS/dispersion.cpp:19-26 (multipliers), DNLS.cpp:96-100,161-175 (angles)
+int setup_dnls(nlc_ctx *c) {
+	const uint32_t n = c->n;
+	const nlc_params &p = c->p;
+	if (!is_pow2(n) || n > nlc::kEnergyFftMax)
+		return c->fail(NLC_EUNSUPPORTED, "DNLS needs a power-of-two chain_length <= %u in this build", nlc::kEnergyFftMax);
+	const double bdt = p.beta * p.dt;
+	for (int k = 0; k < 8; ++k) c->theta[k] = (bdt * 1.) * nlc::kYoshidaC[k];
+	c->theta[8] = (bdt * 2.) * nlc::kYoshidaC[0];
+	std::vector<double> omega(n);
+	nlc_dispersion(NLC_DNLS, n, 0, omega.data());
+	std::vector<double2> tab(size_t(7) * n), perm(size_t(7) * n);
+	const double nrm = 1.0 / n;
+	for (int k = 0; k < 7; ++k)
+		for (uint32_t j = 0; j < n; ++j) {
+			const double ang = ((-p.dt) * nlc::kYoshidaD[k]) * omega[j];
+			tab[size_t(k) * n + j] = make_double2(std::cos(ang) * nrm, std::sin(ang) * nrm);
 		}
-		if ((rc = dev_alloc(c, &c->dnls_table, size_t(7) * n, "DNLS multipliers"))) return rc;
-		NLC_HIP(c, hipMemcpy(c->dnls_table, src, sizeof(double2) * 7 * n, hipMemcpyHostToDevice));
-		return nlc_set_entropy_mask(c, nullptr, c->a0is0);
+	const double2 *src = tab.data();
+	if (nlc::dnls_fast_path(n)) {
+		nlc::dnls_permute_tables(n, tab.data(), perm.data());
+		src = perm.data();
 	}
+	NLC_TRY(dev_alloc(c, &c->dnls_table, size_t(7) * n, "DNLS multipliers"));
+	NLC_HIP(c, hipMemcpy(c->dnls_table, src, sizeof(double2) * 7 * n, hipMemcpyHostToDevice));
+	return NLC_OK;
+}
 
-	// step kernel selection (the reference's move<> dispatch, DNKG_FPUT_Toda.cu:262-297)
+// Step kernel of a uniform real-model context (the reference's move<>
+// dispatch, DNKG_FPUT_Toda.cu:262-297).
+int select_uniform(nlc_ctx *c) {
+	const nlc_params &p = c->p;
+	const uint32_t n = c->n;
 	bool want_split = p.kernel == NLC_KERNEL_SPLIT;
+	c->use_split = false;
 	if (!want_split) {
-		int L, S;
-		bool exact;
-		bool have;
+		int L = 0, S = 0;
+		bool exact = false, have;
 		if (p.lanes_per_chain || p.sites_per_lane) {
 			L = int(p.lanes_per_chain);
 			S = int(p.sites_per_lane);
@@ -240,10 +338,158 @@ int setup(nlc_ctx *c) {
 	}
 	if (want_split) {
 		c->use_split = true;
-		if ((rc = dev_alloc(c, &c->split_q, size_t(n) * c->copies, "split q"))) return rc;
-		if ((rc = dev_alloc(c, &c->split_p, size_t(n) * c->copies, "split p"))) return rc;
+		if (!c->split_q) NLC_TRY(dev_alloc(c, &c->split_q, size_t(n) * c->copies, "split q"));
+		if (!c->split_p) NLC_TRY(dev_alloc(c, &c->split_p, size_t(n) * c->copies, "split p"));
 	}
-	return nlc_set_entropy_mask(c, nullptr, c->a0is0);
+	return NLC_OK;
+}
+
+// Ragged: the persistent kernel with S sites per lane (items built here), or
+// one uniform kernel per class when no S fits every class.
+int select_ragged(nlc_ctx *c, int S) {
+	if (c->items) {
+		hipFree(c->items);
+		c->items = nullptr;
+	}
+	c->ragged_launcher = S ? lookup_ragged(c->p.model, S) : nullptr;
+	if (c->ragged_launcher) {
+		std::vector<nlc::RaggedItem> items;
+		for (const auto &k : c->cls) {
+			const uint32_t L = k.n / uint32_t(S), cpw = 64 / L;
+			for (uint32_t i = 0; i < k.count; i += cpw)
+				items.push_back({k.site0 + uint64_t(i) * k.n, k.n, uint16_t(std::min(cpw, k.count - i)),
+				                 uint16_t(log2u(L))});
+		}
+		c->nitems = uint32_t(items.size());
+		NLC_TRY(dev_alloc(c, &c->items, items.size(), "ragged items"));
+		NLC_HIP(c, hipMemcpy(c->items, items.data(), sizeof(nlc::RaggedItem) * items.size(), hipMemcpyHostToDevice));
+		c->ragged_S = S;
+		return NLC_OK;
+	}
+	c->ragged_S = 0;
+	c->nitems = 0;
+	for (auto &k : c->cls) {
+		if (!default_shape(k.n, k.L, k.S, k.exact) || !(k.launcher = lookup(c->p.model, k.L, k.S, k.exact)))
+			return c->fail(NLC_EUNSUPPORTED, "ragged ensembles need chain lengths with a wave shape (got %u)", k.n);
+	}
+	return NLC_OK;
+}
+
+int enqueue_move(nlc_ctx *c, uint32_t steps) {
+	if (!steps) return c->fail(NLC_EARG, "--kernel_batching must be a positive number");
+	if (c->p.model == NLC_DNLS) {
+		NLC_HIP(c, nlc::launch_dnls(c->stream, c->state, c->n, c->copies, steps, c->theta, c->tw, c->dnls_table));
+	} else if (c->ragged) {
+		if (c->ragged_launcher) {
+			NLC_HIP(c, c->ragged_launcher(c->stream, c->state, c->items, c->nitems, steps, c->args, 0));
+		} else {
+			for (const auto &k : c->cls)
+				NLC_HIP(c, k.launcher(c->stream, c->state + k.site0, k.count, k.n, steps, c->args, nullptr));
+		}
+	} else if (c->use_split) {
+		NLC_HIP(c, nlc::launch_planar_to_split(c->stream, c->state, c->split_q, c->split_p, c->copies, c->n));
+		NLC_HIP(c, nlc::launch_step_split(c->stream, c->p.model, c->split_q, c->split_p, c->copies, c->n, steps,
+		                                  c->args, c->mp2));
+		NLC_HIP(c, nlc::launch_split_to_planar(c->stream, c->split_q, c->split_p, c->state, c->copies, c->n));
+	} else {
+		NLC_HIP(c, c->launcher(c->stream, c->state, c->copies, c->n, steps, c->args, c->mp2));
+	}
+	return NLC_OK;
+}
+
+// Energy pass of every class into e_raw (raw sums over copies).
+int enqueue_energy(nlc_ctx *c) {
+	for (const auto &k : c->cls) {
+		const double2 *st = c->state + k.site0;
+		double *er = c->e_raw + k.tab0;
+		switch (c->p.model) {
+		case NLC_DNKG:
+		case NLC_FPUT:
+		case NLC_TODA:
+			NLC_HIP(c, nlc::launch_energy_real(c->stream, st, k.n, k.count, c->tw + k.tab0, c->omega + k.tab0,
+			                                   c->partial, k.rows));
+			NLC_HIP(c, nlc::launch_reduce_partial(c->stream, c->partial, k.rows, k.n, nullptr, er, c->reduce_scratch));
+			break;
+		case NLC_DDNKG:
+			NLC_HIP(c, nlc::launch_energy_ddnkg(c->stream, st, k.n, k.count, c->eigvec, c->omega, c->partial));
+			NLC_HIP(c, nlc::launch_reduce_partial(c->stream, c->partial, k.rows, k.n, nullptr, er, c->reduce_scratch));
+			break;
+		case NLC_DNLS:
+			NLC_HIP(c, nlc::launch_energy_dnls(c->stream, st, k.n, k.count, c->tw + k.tab0, c->partial, k.rows));
+			NLC_HIP(c, nlc::launch_reduce_partial(c->stream, c->partial, k.rows, k.n, c->omega + k.tab0, er,
+			                                      c->reduce_scratch));
+			break;
+		default: return c->fail(NLC_EUNSUPPORTED, "no energy pass for this model");
+		}
+	}
+	return NLC_OK;
+}
+
+// results::calc_linenergies normalisation of the drivers for a class
+double default_norm(int model, const Klass &k) {
+	switch (model) {
+	case NLC_DDNKG: return 0.5 / k.count;                // dDNKG.cpp:141
+	case NLC_DNLS: return 1.0 / (double(k.n) * k.count); // DNLS.cpp:150
+	default: return (0.5 / k.count) / k.n;               // DNKG_FPUT_Toda.cpp:129
+	}
+}
+
+// Normalisation, entropies and the copy of all results to pinned memory.
+int enqueue_results(nlc_ctx *c, double norm) {
+	for (size_t i = 0; i < c->cls.size(); ++i) {
+		const Klass &k = c->cls[i];
+		const double nm = norm > 0 ? norm : default_norm(c->p.model, k);
+		NLC_HIP(c, nlc::launch_results(c->stream, c->e_raw + k.tab0, k.n, nm, c->a0is0, c->modes,
+		                               c->ragged ? 0 : c->nmodes, c->results + k.tab0 + 2 * i));
+	}
+	NLC_HIP(c, hipMemcpyAsync(c->h_results, c->results, sizeof(double) * c->results_len(), hipMemcpyDeviceToHost,
+	                          c->stream));
+	return NLC_OK;
+}
+
+void unpack_results(const nlc_ctx *c, double *linenergies, double *entropies) {
+	for (size_t i = 0; i < c->cls.size(); ++i) {
+		const Klass &k = c->cls[i];
+		const double *r = c->h_results + k.tab0 + 2 * i;
+		if (linenergies) std::memcpy(linenergies + k.tab0, r, sizeof(double) * k.n);
+		if (entropies) entropies[2 * i] = r[k.n], entropies[2 * i + 1] = r[k.n + 1];
+	}
+}
+
+int reject(int code, const char *msg) {
+	g_create_error = msg;
+	return code;
+}
+
+int validate_params(const nlc_params *p, bool ragged) {
+	if (!p || p->struct_size != sizeof(nlc_params)) return reject(NLC_EARG, "params NULL or struct_size mismatch");
+	if (p->model < NLC_DNKG || p->model > NLC_DNLS) return reject(NLC_EARG, "unknown model");
+	if (!ragged) {
+		if (p->chain_length < 2) return reject(NLC_EARG, "--chain_length must be >= 2");
+		if (!p->copies) return reject(NLC_EARG, "--copies must be a positive number");
+		if (uint64_t(p->chain_length) * p->copies >= 0x40000000ULL)
+			return reject(NLC_EARG, "(--chain_length * --copies) must be < 2^30");
+	}
+	if (!(p->dt > 0)) return reject(NLC_EARG, "--dt must be a positive number");
+	if (p->model == NLC_TODA && p->alpha == 0) return reject(NLC_EARG, "Toda needs --alpha != 0");
+	if (p->model == NLC_DDNKG && !p->masses) return reject(NLC_EARG, "dDNKG needs the masses (-m file content)");
+	if (p->precision != NLC_FP64) return reject(NLC_EUNSUPPORTED, "only FP64 precision is available in this build");
+	if (p->kernel < NLC_KERNEL_AUTO || p->kernel > NLC_KERNEL_SPLIT) return reject(NLC_EARG, "unknown kernel family");
+	int ndev = 0;
+	if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+		return reject(NLC_EHIP, "no HIP device available (the hot path has no CPU fallback)");
+	return NLC_OK;
+}
+
+int finish_create(nlc_ctx *c, nlc_ctx **out, int rc) {
+	if (rc) {
+		g_create_error = c->err;
+		free_all(c);
+		delete c;
+		return rc;
+	}
+	*out = c;
+	return NLC_OK;
 }
 
 } // namespace
@@ -278,45 +524,105 @@ int nlc_params_init(nlc_params *p) {
 
 int nlc_create(nlc_ctx **out, const nlc_params *p) {
 	g_create_error.clear();
-	auto reject = [](int code, const char *msg) {
-		g_create_error = msg;
-		return code;
-	};
 	if (!out) return reject(NLC_EARG, "out is NULL");
 	*out = nullptr;
-	if (!p || p->struct_size != sizeof(nlc_params)) return reject(NLC_EARG, "params NULL or struct_size mismatch");
-	// argument rules of common/main.cpp:86-87,101-103 and the model options
-	if (p->model < NLC_DNKG || p->model > NLC_DNLS) return reject(NLC_EARG, "unknown model");
-	if (p->chain_length < 2) return reject(NLC_EARG, "--chain_length must be >= 2");
-	if (!p->copies) return reject(NLC_EARG, "--copies must be a positive number");
-	if (!(p->dt > 0)) return reject(NLC_EARG, "--dt must be a positive number");
-	if (uint64_t(p->chain_length) * p->copies >= 0x40000000ULL)
-		return reject(NLC_EARG, "(--chain_length * --copies) must be < 2^30");
-	if (p->model == NLC_TODA && p->alpha == 0) return reject(NLC_EARG, "Toda needs --alpha != 0");
-	if (p->model == NLC_DDNKG && !p->masses) return reject(NLC_EARG, "dDNKG needs the masses (-m file content)");
-	if (p->precision != NLC_FP64) return reject(NLC_EUNSUPPORTED, "only FP64 precision is available in this build");
-	if (p->kernel < NLC_KERNEL_AUTO || p->kernel > NLC_KERNEL_SPLIT) return reject(NLC_EARG, "unknown kernel family");
-
-	int ndev = 0;
-	if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
-		return reject(NLC_EHIP, "no HIP device available (the hot path has no CPU fallback)");
-
+	if (int rc = validate_params(p, false)) return rc;
 	nlc_ctx *c = new (std::nothrow) nlc_ctx;
 	if (!c) return reject(NLC_ENOMEM, "context allocation failed");
 	c->p = *p;
 	c->n = p->chain_length;
 	c->copies = p->copies;
-	c->state_bytes = sizeof(double2) * size_t(c->n) * c->copies;
+	c->total_sites = uint64_t(c->n) * c->copies;
+	c->table_len = c->n;
+	Klass k;
+	k.n = c->n;
+	k.count = c->copies;
+	c->cls.push_back(k);
 	if (p->masses) c->masses.assign(p->masses, p->masses + c->n);
 	c->p.masses = nullptr;
-	int rc = setup(c);
-	if (rc) {
-		g_create_error = c->err;
-		free_all(c);
-		delete c;
-		return rc;
+	int rc = setup_common(c);
+	if (!rc) rc = p->model == NLC_DNLS ? setup_dnls(c) : select_uniform(c);
+	if (!rc) rc = nlc_set_entropy_mask(c, nullptr, c->a0is0);
+	return finish_create(c, out, rc);
+}
+
+int nlc_create_ragged(nlc_ctx **out, const nlc_params *p, const uint32_t *chain_lengths, uint32_t nchains) {
+	g_create_error.clear();
+	if (!out) return reject(NLC_EARG, "out is NULL");
+	*out = nullptr;
+	if (!chain_lengths || !nchains) return reject(NLC_EARG, "no chains");
+	if (int rc = validate_params(p, true)) return rc;
+	if (p->model != NLC_DNKG && p->model != NLC_FPUT && p->model != NLC_TODA)
+		return reject(NLC_EUNSUPPORTED, "ragged ensembles are available for DNKG, FPUT and Toda");
+	uint64_t total = 0;
+	for (uint32_t i = 0; i < nchains; ++i) {
+		if (chain_lengths[i] < 2) return reject(NLC_EARG, "every chain length must be >= 2");
+		total += chain_lengths[i];
 	}
-	*out = c;
+	if (total >= 0x40000000ULL) return reject(NLC_EARG, "total number of sites must be < 2^30");
+	nlc_ctx *c = new (std::nothrow) nlc_ctx;
+	if (!c) return reject(NLC_ENOMEM, "context allocation failed");
+	c->p = *p;
+	c->p.masses = nullptr;
+	c->ragged = true;
+	c->lengths.assign(chain_lengths, chain_lengths + nchains);
+	c->copies = nchains;
+	c->total_sites = total;
+	// classes in ascending length; inside a class the caller's order
+	std::vector<uint32_t> distinct(c->lengths);
+	std::sort(distinct.begin(), distinct.end());
+	distinct.erase(std::unique(distinct.begin(), distinct.end()), distinct.end());
+	c->n = distinct.back();
+	for (uint32_t len : distinct) {
+		Klass k;
+		k.n = len;
+		k.tab0 = c->table_len;
+		c->table_len += len;
+		c->cls.push_back(k);
+	}
+	std::vector<size_t> class_of(nchains);
+	for (uint32_t i = 0; i < nchains; ++i) {
+		class_of[i] = std::lower_bound(distinct.begin(), distinct.end(), c->lengths[i]) - distinct.begin();
+		c->cls[class_of[i]].count++;
+	}
+	uint64_t site = 0, chain = 0;
+	for (auto &k : c->cls) {
+		k.site0 = site;
+		k.chain0 = chain;
+		site += uint64_t(k.n) * k.count;
+		chain += k.count;
+	}
+	c->caller_site0.resize(nchains);
+	c->device_site0.resize(nchains);
+	c->device_chain.resize(nchains);
+	std::vector<uint32_t> filled(c->cls.size(), 0);
+	uint64_t caller = 0;
+	for (uint32_t i = 0; i < nchains; ++i) {
+		Klass &k = c->cls[class_of[i]];
+		const uint32_t f = filled[class_of[i]]++;
+		c->caller_site0[i] = caller;
+		c->device_site0[i] = k.site0 + uint64_t(f) * k.n;
+		c->device_chain[i] = uint32_t(k.chain0 + f);
+		caller += c->lengths[i];
+	}
+	int rc = setup_common(c);
+	if (!rc) {
+		int S = 0;
+		for (int s : {32, 16, 8})
+			if (ragged_fits(c->cls, s)) S = s; // smallest S that fits: most chains per wave
+		rc = select_ragged(c, S);
+	}
+	if (!rc) rc = nlc_set_entropy_mask(c, nullptr, c->a0is0);
+	return finish_create(c, out, rc);
+}
+
+int nlc_ragged_classes(const nlc_ctx *c, uint32_t *nclasses, uint32_t *lengths, uint32_t *counts, uint32_t cap) {
+	if (!c || !nclasses) return NLC_EARG;
+	*nclasses = uint32_t(c->cls.size());
+	for (uint32_t i = 0; i < std::min<uint32_t>(cap, *nclasses); ++i) {
+		if (lengths) lengths[i] = c->cls[i].n;
+		if (counts) counts[i] = c->cls[i].count;
+	}
 	return NLC_OK;
 }
 
@@ -328,15 +634,33 @@ void nlc_destroy(nlc_ctx *ctx) {
 
 int nlc_upload(nlc_ctx *c, const void *state) {
 	if (!c || !state) return NLC_EARG;
-	NLC_HIP(c, hipMemcpyAsync(c->state, state, c->state_bytes, hipMemcpyHostToDevice, c->stream));
+	if (!c->ragged) {
+		NLC_HIP(c, hipMemcpyAsync(c->state, state, c->state_bytes, hipMemcpyHostToDevice, c->stream));
+		NLC_HIP(c, hipStreamSynchronize(c->stream));
+		return NLC_OK;
+	}
+	std::vector<double2> stage(c->total_sites);
+	const double2 *src = static_cast<const double2 *>(state);
+	for (size_t i = 0; i < c->lengths.size(); ++i)
+		std::memcpy(&stage[c->device_site0[i]], src + c->caller_site0[i], sizeof(double2) * c->lengths[i]);
+	NLC_HIP(c, hipMemcpyAsync(c->state, stage.data(), c->state_bytes, hipMemcpyHostToDevice, c->stream));
 	NLC_HIP(c, hipStreamSynchronize(c->stream));
 	return NLC_OK;
 }
 
 int nlc_download(nlc_ctx *c, void *state) {
 	if (!c || !state) return NLC_EARG;
-	NLC_HIP(c, hipMemcpyAsync(state, c->state, c->state_bytes, hipMemcpyDeviceToHost, c->stream));
+	if (!c->ragged) {
+		NLC_HIP(c, hipMemcpyAsync(state, c->state, c->state_bytes, hipMemcpyDeviceToHost, c->stream));
+		NLC_HIP(c, hipStreamSynchronize(c->stream));
+		return NLC_OK;
+	}
+	std::vector<double2> stage(c->total_sites);
+	NLC_HIP(c, hipMemcpyAsync(stage.data(), c->state, c->state_bytes, hipMemcpyDeviceToHost, c->stream));
 	NLC_HIP(c, hipStreamSynchronize(c->stream));
+	double2 *dst = static_cast<double2 *>(state);
+	for (size_t i = 0; i < c->lengths.size(); ++i)
+		std::memcpy(dst + c->caller_site0[i], &stage[c->device_site0[i]], sizeof(double2) * c->lengths[i]);
 	return NLC_OK;
 }
 
@@ -349,26 +673,14 @@ int nlc_device_state(nlc_ctx *c, void **dptr, size_t *bytes) {
 
 int nlc_move(nlc_ctx *c, uint32_t steps) {
 	if (!c) return NLC_EARG;
-	if (!steps) return c->fail(NLC_EARG, "--kernel_batching must be a positive number");
-	if (c->p.model == NLC_DNLS) {
-		NLC_HIP(c, nlc::launch_dnls(c->stream, c->state, c->n, c->copies, steps, c->theta, c->tw, c->dnls_table));
-		return NLC_OK;
-	}
-	if (c->use_split) {
-		NLC_HIP(c, nlc::launch_planar_to_split(c->stream, c->state, c->split_q, c->split_p, c->copies, c->n));
-		NLC_HIP(c, nlc::launch_step_split(c->stream, c->p.model, c->split_q, c->split_p, c->copies, c->n, steps,
-		                                  c->args, c->mp2));
-		NLC_HIP(c, nlc::launch_split_to_planar(c->stream, c->split_q, c->split_p, c->state, c->copies, c->n));
-		return NLC_OK;
-	}
-	NLC_HIP(c, c->launcher(c->stream, c->state, c->copies, c->n, steps, c->args, c->mp2));
-	return NLC_OK;
+	return enqueue_move(c, steps);
 }
 
 int nlc_set_entropy_mask(nlc_ctx *c, const uint8_t *mask, int a0is0) {
 	if (!c) return NLC_EARG;
 	std::vector<uint32_t> idx;
 	if (mask) {
+		if (c->ragged) return c->fail(NLC_EUNSUPPORTED, "entropy masks apply to uniform ensembles only");
 		// common/main.cpp:119-128 then common/results.cpp:8-15
 		for (uint32_t k = 0; k < c->n; ++k)
 			if (mask[k]) idx.push_back(k);
@@ -383,74 +695,151 @@ int nlc_set_entropy_mask(nlc_ctx *c, const uint8_t *mask, int a0is0) {
 	}
 	c->nmodes = uint32_t(idx.size());
 	c->a0is0 = a0is0 ? 1 : 0;
-	return NLC_OK;
-}
-
-static int energy_pass(nlc_ctx *c) {
-	const uint32_t n = c->n;
-	switch (c->p.model) {
-	case NLC_DNKG:
-	case NLC_FPUT:
-	case NLC_TODA:
-		NLC_HIP(c, nlc::launch_energy_real(c->stream, c->state, n, c->copies, c->tw, c->omega, c->partial,
-		                                   c->partial_rows));
-		NLC_HIP(c, nlc::launch_reduce_partial(c->stream, c->partial, c->partial_rows, n, nullptr, c->e_raw,
-		                                      c->reduce_scratch));
-		break;
-	case NLC_DDNKG:
-		NLC_HIP(c, nlc::launch_energy_ddnkg(c->stream, c->state, n, c->copies, c->eigvec, c->omega, c->partial));
-		NLC_HIP(c, nlc::launch_reduce_partial(c->stream, c->partial, c->partial_rows, n, nullptr, c->e_raw,
-		                                      c->reduce_scratch));
-		break;
-	case NLC_DNLS:
-		NLC_HIP(c, nlc::launch_energy_dnls(c->stream, c->state, n, c->copies, c->tw, c->partial, c->partial_rows));
-		NLC_HIP(c, nlc::launch_reduce_partial(c->stream, c->partial, c->partial_rows, n, c->omega, c->e_raw,
-		                                      c->reduce_scratch));
-		break;
-	default: return c->fail(NLC_EUNSUPPORTED, "no energy pass for this model");
-	}
+	drop_graph(c);
 	return NLC_OK;
 }
 
 int nlc_linenergies_raw(nlc_ctx *c, double *e_raw) {
 	if (!c || !e_raw) return NLC_EARG;
-	int rc = energy_pass(c);
-	if (rc) return rc;
-	NLC_HIP(c, hipMemcpyAsync(c->h_results, c->e_raw, sizeof(double) * c->n, hipMemcpyDeviceToHost, c->stream));
+	NLC_TRY(enqueue_energy(c));
+	NLC_HIP(c, hipMemcpyAsync(c->h_results, c->e_raw, sizeof(double) * c->table_len, hipMemcpyDeviceToHost, c->stream));
 	NLC_HIP(c, hipStreamSynchronize(c->stream));
-	std::memcpy(e_raw, c->h_results, sizeof(double) * c->n);
+	std::memcpy(e_raw, c->h_results, sizeof(double) * c->table_len);
 	return NLC_OK;
 }
 
 int nlc_linenergies(nlc_ctx *c, double norm, double *linenergies, double *entropies) {
 	if (!c) return NLC_EARG;
-	const uint32_t n = c->n;
-	if (!(norm > 0)) {
-		// results::calc_linenergies normalisations of the drivers
-		switch (c->p.model) {
-		case NLC_DDNKG: norm = 0.5 / c->copies; break;                      // dDNKG.cpp:141
-		case NLC_DNLS: norm = 1.0 / (double(n) * c->copies); break;         // DNLS.cpp:150
-		default: norm = (0.5 / c->copies) / n; break;                       // DNKG_FPUT_Toda.cpp:129
-		}
-	}
-	int rc = energy_pass(c);
-	if (rc) return rc;
-	NLC_HIP(c, nlc::launch_results(c->stream, c->e_raw, n, norm, c->a0is0, c->modes, c->nmodes, c->results));
-	NLC_HIP(c, hipMemcpyAsync(c->h_results, c->results, sizeof(double) * (n + 2), hipMemcpyDeviceToHost, c->stream));
+	NLC_TRY(enqueue_energy(c));
+	NLC_TRY(enqueue_results(c, norm));
 	NLC_HIP(c, hipStreamSynchronize(c->stream));
-	if (linenergies) std::memcpy(linenergies, c->h_results, sizeof(double) * n);
-	if (entropies) entropies[0] = c->h_results[n], entropies[1] = c->h_results[n + 1];
+	unpack_results(c, linenergies, entropies);
+	return NLC_OK;
+}
+
+int nlc_batch(nlc_ctx *c, uint32_t steps, double norm, double *linenergies, double *entropies) {
+	if (!c) return NLC_EARG;
+	if (!steps) return c->fail(NLC_EARG, "--kernel_batching must be a positive number");
+	if (!c->graph_exec || c->graph_steps != steps || c->graph_norm != norm) {
+		drop_graph(c);
+		NLC_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
+		int rc = enqueue_move(c, steps);
+		if (!rc) rc = enqueue_energy(c);
+		if (!rc) rc = enqueue_results(c, norm);
+		hipGraph_t g = nullptr;
+		const hipError_t e = hipStreamEndCapture(c->stream, &g);
+		if (rc) {
+			if (g) hipGraphDestroy(g);
+			return rc;
+		}
+		if (e != hipSuccess) return c->hip_fail(e, "hipStreamEndCapture");
+		c->graph = g;
+		NLC_HIP(c, hipGraphInstantiate(&c->graph_exec, c->graph, nullptr, nullptr, 0));
+		c->graph_steps = steps;
+		c->graph_norm = norm;
+	}
+	NLC_HIP(c, hipGraphLaunch(c->graph_exec, c->stream));
+	NLC_HIP(c, hipStreamSynchronize(c->stream));
+	unpack_results(c, linenergies, entropies);
 	return NLC_OK;
 }
 
 int nlc_chain_energies(nlc_ctx *c, double *energies, double *norms) {
 	if (!c || !energies) return NLC_EARG;
-	NLC_HIP(c, nlc::launch_chain_energy(c->stream, c->p.model, c->state, c->n, c->copies, c->args, c->masses_d,
-	                                    c->energy_d, c->norm_d));
-	NLC_HIP(c, hipMemcpyAsync(energies, c->energy_d, sizeof(double) * c->copies, hipMemcpyDeviceToHost, c->stream));
-	if (norms && c->p.model == NLC_DNLS)
-		NLC_HIP(c, hipMemcpyAsync(norms, c->norm_d, sizeof(double) * c->copies, hipMemcpyDeviceToHost, c->stream));
+	for (const auto &k : c->cls)
+		NLC_HIP(c, nlc::launch_chain_energy(c->stream, c->p.model, c->state + k.site0, k.n, k.count, c->args,
+		                                    c->masses_d, c->energy_d + k.chain0, c->norm_d + k.chain0));
+	std::vector<double> e(c->copies), nr(c->copies);
+	NLC_HIP(c, hipMemcpyAsync(e.data(), c->energy_d, sizeof(double) * c->copies, hipMemcpyDeviceToHost, c->stream));
+	NLC_HIP(c, hipMemcpyAsync(nr.data(), c->norm_d, sizeof(double) * c->copies, hipMemcpyDeviceToHost, c->stream));
 	NLC_HIP(c, hipStreamSynchronize(c->stream));
+	for (uint32_t i = 0; i < c->copies; ++i) {
+		const uint32_t j = c->ragged ? c->device_chain[i] : i;
+		energies[i] = e[j];
+		if (norms && c->p.model == NLC_DNLS) norms[i] = nr[j];
+	}
+	return NLC_OK;
+}
+
+int nlc_autotune(nlc_ctx *c, uint32_t trial_steps, char *report, size_t len) {
+	if (!c || !trial_steps) return NLC_EARG;
+	struct Cand {
+		int L, S;
+		bool exact, split;
+		int ragged_S;
+		float ms;
+	};
+	std::vector<Cand> cands;
+	if (c->p.model == NLC_DNLS) {
+		c->tune_report = "DNLS: a single kernel, nothing to tune";
+	} else if (c->ragged) {
+		for (int S : {8, 16, 32})
+			if (ragged_fits(c->cls, S) && lookup_ragged(c->p.model, S)) cands.push_back({0, 0, false, false, S, 0});
+		if (cands.empty()) c->tune_report = "ragged: no persistent shape fits these lengths";
+	} else {
+		for (const Shape &s : wave_shapes(c->p.model, c->n)) cands.push_back({s.L, s.S, s.exact, false, 0, 0});
+		cands.push_back({0, 0, false, true, 0, 0});
+	}
+	if (!cands.empty()) {
+		drop_graph(c);
+		double2 *backup = nullptr;
+		NLC_TRY(dev_alloc(c, &backup, c->total_sites, "autotune backup"));
+		hipEvent_t e0 = nullptr, e1 = nullptr;
+		int rc = NLC_OK;
+		if (hipEventCreate(&e0) || hipEventCreate(&e1) ||
+		    hipMemcpyAsync(backup, c->state, c->state_bytes, hipMemcpyDeviceToDevice, c->stream))
+			rc = c->fail(NLC_EHIP, "autotune setup failed");
+		const nlc_params saved = c->p;
+		for (auto &cd : cands) {
+			if (rc) break;
+			if (c->ragged) {
+				rc = select_ragged(c, cd.ragged_S);
+			} else {
+				c->p.kernel = cd.split ? NLC_KERNEL_SPLIT : NLC_KERNEL_WAVE;
+				c->p.lanes_per_chain = cd.split ? 0 : uint32_t(cd.L);
+				c->p.sites_per_lane = cd.split ? 0 : uint32_t(cd.S);
+				rc = select_uniform(c);
+			}
+			for (int rep = 0; rep < 2 && !rc; ++rep) { // the first run warms up
+				rc = enqueue_move(c, trial_steps);
+				if (!rc && (hipEventRecord(e0, c->stream) != hipSuccess)) rc = c->fail(NLC_EHIP, "event record");
+				if (!rc) rc = enqueue_move(c, trial_steps);
+				if (!rc && (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+				            hipEventElapsedTime(&cd.ms, e0, e1) != hipSuccess))
+					rc = c->fail(NLC_EHIP, "autotune timing failed");
+			}
+			if (hipMemcpyAsync(c->state, backup, c->state_bytes, hipMemcpyDeviceToDevice, c->stream) && !rc)
+				rc = c->fail(NLC_EHIP, "autotune restore failed");
+		}
+		hipStreamSynchronize(c->stream);
+		if (e0) hipEventDestroy(e0);
+		if (e1) hipEventDestroy(e1);
+		hipFree(backup);
+		c->p = saved;
+		if (rc) return rc;
+		const Cand *best = &cands[0];
+		for (const auto &cd : cands)
+			if (cd.ms < best->ms) best = &cd;
+		std::string rep = c->ragged ? "ragged sites-per-lane:" : "wave shapes (lanes x sites):";
+		char buf[96];
+		for (const auto &cd : cands) {
+			const char *mark = &cd == best ? "*" : "";
+			if (c->ragged) snprintf(buf, sizeof buf, " S=%d %.3f ms%s", cd.ragged_S, cd.ms, mark);
+			else if (cd.split) snprintf(buf, sizeof buf, " split %.3f ms%s", cd.ms, mark);
+			else snprintf(buf, sizeof buf, " %dx%d%s %.3f ms%s", cd.L, cd.S, cd.exact ? "" : "r", cd.ms, mark);
+			rep += buf;
+		}
+		c->tune_report = rep;
+		if (c->ragged) {
+			NLC_TRY(select_ragged(c, best->ragged_S));
+		} else {
+			c->p.kernel = best->split ? NLC_KERNEL_SPLIT : NLC_KERNEL_WAVE;
+			c->p.lanes_per_chain = best->split ? 0 : uint32_t(best->L);
+			c->p.sites_per_lane = best->split ? 0 : uint32_t(best->S);
+			NLC_TRY(select_uniform(c));
+		}
+	}
+	if (report && len) snprintf(report, len, "%s", c->tune_report.c_str());
 	return NLC_OK;
 }
 
@@ -468,6 +857,10 @@ int nlc_describe(const nlc_ctx *c, char *buf, size_t len) {
 	if (c->p.model == NLC_DNLS)
 		snprintf(buf, len, "model=DNLS n=%u copies=%u kernel=%s", c->n, c->copies,
 		         nlc::dnls_fast_path(c->n) ? "dnls_radix16_registers" : "dnls_lds_stockham");
+	else if (c->ragged)
+		snprintf(buf, len, "model=%s ragged chains=%u classes=%zu sites=%llu kernel=%s sites_per_lane=%d items=%u",
+		         names[c->p.model], c->copies, c->cls.size(), (unsigned long long) c->total_sites,
+		         c->ragged_launcher ? "ragged_persistent" : "per_class_wave", c->ragged_S, c->nitems);
 	else if (c->use_split)
 		snprintf(buf, len, "model=%s n=%u copies=%u kernel=split", names[c->p.model], c->n, c->copies);
 	else

@@ -314,6 +314,45 @@ int nlc_ic_eigenmodes(uint32_t n, uint32_t copies, const double *omegas, const d
 	return NLC_OK;
 }
 
+int nlc_ic_ragged_lengths(uint32_t nchains, const uint32_t *choices, uint32_t nchoices, uint64_t seed,
+                          uint32_t *lengths) {
+	if (!nchains || !choices || !nchoices || !lengths) return NLC_EARG;
+	SplitMix64 rng = copy_stream(seed, 0xFFFFFFFEULL);
+	for (uint32_t i = 0; i < nchains; ++i) lengths[i] = choices[rng.next() % nchoices];
+	return NLC_OK;
+}
+
+int nlc_ic_ragged_plane_waves(const uint32_t *lengths, uint32_t nchains, double m, double mode_fraction, double e_lo,
+                              double e_hi, uint64_t seed, double *state) {
+	if (!lengths || !nchains || !state || !(e_hi >= e_lo) || e_lo < 0 || mode_fraction < 0) return NLC_EARG;
+	std::vector<uint64_t> offset(nchains + 1, 0);
+	for (uint32_t i = 0; i < nchains; ++i) {
+		if (lengths[i] < 2) return NLC_EARG;
+		offset[i + 1] = offset[i] + lengths[i];
+	}
+	parallel_for(nchains, [&](uint32_t c) {
+		const uint32_t n = lengths[c];
+		SplitMix64 rng = copy_stream(seed, c);
+		const double dens = e_lo + (e_hi - e_lo) * rng.uniform();
+		const uint32_t kmax = std::max<uint32_t>(1, uint32_t(std::floor(mode_fraction * n)));
+		double *chain = state + offset[c] * 2;
+		for (uint32_t x = 0; x < 2 * n; ++x) chain[x] = 0.0;
+		const double e_mode = dens * n / double(kmax);
+		for (uint32_t k = 1; k <= kmax && k < n; ++k) {
+			const double phi = 2.0 * M_PI * rng.uniform();
+			const double s2 = 2 * std::sin(k * M_PI / n), w = std::sqrt(m + s2 * s2);
+			if (!(w > 0)) continue;
+			const double amp = std::sqrt(2.0 * e_mode / (n * w * w));
+			for (uint32_t x = 0; x < n; ++x) {
+				const double th = 2.0 * M_PI * double((uint64_t(k) * x) % n) / double(n) + phi;
+				chain[2 * x] += amp * std::cos(th);
+				chain[2 * x + 1] += amp * w * std::sin(th);
+			}
+		}
+	});
+	return NLC_OK;
+}
+
 int nlc_ic_masses(uint32_t n, double lo, double hi, uint64_t seed, double *masses) {
 	if (n < 1 || !masses || !(hi >= lo)) return NLC_EARG;
 	SplitMix64 rng = copy_stream(seed, 0xFFFFFFFFULL);

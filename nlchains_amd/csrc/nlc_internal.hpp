@@ -35,6 +35,21 @@ StepLauncher lookup_step_fput(int L, int S, bool exact);
 StepLauncher lookup_step_toda(int L, int S, bool exact);
 StepLauncher lookup_step_ddnkg(int L, int S, bool exact);
 
+// Ragged ensembles: an item is one wavefront's worth (nchains <= 64/L) of
+// consecutive chains of n = L*S sites starting at site index site0.
+struct RaggedItem {
+	uint64_t site0;
+	uint32_t n;
+	uint16_t nchains;
+	uint16_t log2_lanes;
+};
+using RaggedLauncher = hipError_t (*)(hipStream_t, double2 *state, const RaggedItem *items, uint32_t nitems,
+                                      uint32_t steps, const StepArgs &args, uint32_t max_blocks);
+// persistent ragged kernels, S in {8, 16, 32}
+RaggedLauncher lookup_ragged_dnkg(int S);
+RaggedLauncher lookup_ragged_fput(int S);
+RaggedLauncher lookup_ragged_toda(int S);
+
 // HBM-streaming step over the split layout q[n][copies], p[n][copies].
 hipError_t launch_step_split(hipStream_t, int model, double *q, double *p, uint32_t copies, uint32_t n,
                              uint32_t steps, const StepArgs &args, const double *mp2);

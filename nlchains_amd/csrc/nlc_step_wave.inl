@@ -21,6 +21,8 @@
 //  * the 7 substeps are unrolled with their weights in SGPRs; the c7 drift of a
 //    step is fused with the c0 drift of the next (the reference's 2*c0 merge).
 //  * HBM is touched once per batch: 16 B/site in, 16 B/site out.
+#include <algorithm>
+
 #include "nlc_internal.hpp"
 
 namespace nlc {
@@ -118,40 +120,41 @@ struct Chain {
 			}
 			if (!EXACT && !full) p[S - 1] = 0.0;
 		} else {
-			double b[S];
-			b[0] = bond_force<MODEL>(q[0] - hl, a);
-#pragma unroll
-			for (int j = 1; j < S; ++j) b[j] = bond_force<MODEL>(q[j] - q[j - 1], a);
-			const double bs = from_right<L>(b[0], lane);
-			if (!EXACT && !full) b[S - 1] = bs; // the dummy slot gets zero force
+			// bond terms rolled through the chain: only two are live at a time
+			const double b0 = bond_force<MODEL>(q[0] - hl, a);
+			const double bs = from_right<L>(b0, lane); // right bond of the lane's last site
+			double bl = b0;
 #pragma unroll
 			for (int j = 0; j < S; ++j) {
-				const double br = (j == S - 1) ? bs : b[j + 1];
-				p[j] = fma(d, br - b[j], p[j]);
+				double br;
+				if (j == S - 1) br = bs;
+				else if (!EXACT && j == S - 2) br = full ? bond_force<MODEL>(q[j + 1] - q[j], a) : bs;
+				else br = bond_force<MODEL>(q[j + 1] - q[j], a);
+				p[j] = fma(d, br - bl, p[j]);
+				bl = br;
 			}
+			if (!EXACT && !full) p[S - 1] = 0.0; // the dummy slot of a short lane
 		}
 	}
 };
 
+// One wavefront integrates `live_chains` (<= 64/L) consecutive chains of
+// length n starting at chain0 for `steps` steps.  All 64 lanes take part in
+// the halo exchange even when the wave is only partly filled.
 template <int MODEL, int L, int S, bool EXACT>
-__global__ void __launch_bounds__(256)
-k_step_wave(double2 *__restrict__ state, uint32_t copies, uint32_t n, uint32_t steps, StepArgs a,
-            const double *__restrict__ mp2g) {
-	constexpr int CPW = 64 / L; // chains per wavefront
+__device__ __forceinline__ void run_wave(double2 *__restrict__ chain0, uint32_t n, uint32_t live_chains, uint32_t steps,
+                                         const StepArgs &a, const double *__restrict__ mp2g) {
 	const int lane = threadIdx.x & 63;
-	const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-	const uint32_t first_chain = wave * CPW;
-	if (first_chain >= copies) return; // whole wavefront idle: uniform exit
 	const int l = lane & (L - 1);
-	const uint32_t chain = first_chain + lane / L;
-	const bool live = chain < copies; // lanes of a missing chain still take part in the halo exchange
+	const uint32_t sub = uint32_t(lane / L);
+	const bool live = sub < live_chains;
 
 	Chain<MODEL, L, S, EXACT> ch;
 	ch.lane = lane;
 	const int full_lanes = EXACT ? L : int(n) - (S - 1) * L;
 	ch.full = EXACT || l < full_lanes;
 	const uint32_t off = ch.full ? uint32_t(l * S) : uint32_t(full_lanes * S + (l - full_lanes) * (S - 1));
-	double2 *base = state + size_t(live ? chain : 0) * n + off;
+	double2 *base = chain0 + size_t(live ? sub : 0) * n + off;
 
 #pragma unroll
 	for (int j = 0; j < S; ++j) {
@@ -191,6 +194,17 @@ k_step_wave(double2 *__restrict__ state, uint32_t copies, uint32_t n, uint32_t s
 }
 
 template <int MODEL, int L, int S, bool EXACT>
+__global__ void __launch_bounds__(256)
+k_step_wave(double2 *__restrict__ state, uint32_t copies, uint32_t n, uint32_t steps, StepArgs a,
+            const double *__restrict__ mp2g) {
+	constexpr uint32_t CPW = 64 / L; // chains per wavefront
+	const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+	const uint32_t first_chain = wave * CPW;
+	if (first_chain >= copies) return; // whole wavefront idle: uniform exit
+	run_wave<MODEL, L, S, EXACT>(state + size_t(first_chain) * n, n, min(CPW, copies - first_chain), steps, a, mp2g);
+}
+
+template <int MODEL, int L, int S, bool EXACT>
 hipError_t launch_wave(hipStream_t st, double2 *state, uint32_t copies, uint32_t n, uint32_t steps,
                        const StepArgs &a, const double *mp2) {
 	constexpr uint32_t CPW = 64 / L;
@@ -201,7 +215,64 @@ hipError_t launch_wave(hipStream_t st, double2 *state, uint32_t copies, uint32_t
 	return hipGetLastError();
 }
 
+#ifdef NLC_STEP_RAGGED
+// Persistent step over a ragged ensemble: every item is one wavefront's worth
+// of chains of one length class (64 S / n chains of n = L S sites), so all
+// items cost the same; waves take items round-robin until none is left (no
+// atomics, deterministic, every wave reaches the end of the list).
+template <int MODEL, int S>
+__global__ void __launch_bounds__(256)
+k_step_ragged(double2 *__restrict__ state, const RaggedItem *__restrict__ items, uint32_t nitems, uint32_t steps,
+              StepArgs a) {
+	const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+	const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
+	for (uint32_t i = wave; i < nitems; i += nwaves) {
+		const RaggedItem it = items[i];
+		double2 *c0 = state + it.site0;
+		switch (it.log2_lanes) {
+		case 0: run_wave<MODEL, 1, S, true>(c0, it.n, it.nchains, steps, a, nullptr); break;
+		case 1: run_wave<MODEL, 2, S, true>(c0, it.n, it.nchains, steps, a, nullptr); break;
+		case 2: run_wave<MODEL, 4, S, true>(c0, it.n, it.nchains, steps, a, nullptr); break;
+		case 3: run_wave<MODEL, 8, S, true>(c0, it.n, it.nchains, steps, a, nullptr); break;
+		case 4: run_wave<MODEL, 16, S, true>(c0, it.n, it.nchains, steps, a, nullptr); break;
+		case 5: run_wave<MODEL, 32, S, true>(c0, it.n, it.nchains, steps, a, nullptr); break;
+		default: run_wave<MODEL, 64, S, true>(c0, it.n, it.nchains, steps, a, nullptr); break;
+		}
+	}
+}
+
+template <int MODEL, int S>
+hipError_t launch_ragged(hipStream_t st, double2 *state, const RaggedItem *items, uint32_t nitems, uint32_t steps,
+                         const StepArgs &a, uint32_t max_blocks) {
+	int per_cu = 0, cus = 0, dev = 0;
+	hipError_t e = hipGetDevice(&dev);
+	if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+	if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_step_ragged<MODEL, S>, 256, 0);
+	if (e != hipSuccess) return e;
+	// as many waves as fit at once, but no more rounds than the item count needs
+	const uint64_t slots = uint64_t(std::max(per_cu, 1)) * std::max(cus, 1) * 4;
+	const uint64_t rounds = (nitems + slots - 1) / slots;
+	uint64_t waves = rounds ? (nitems + rounds - 1) / rounds : 1;
+	uint64_t blocks = (waves + 3) / 4;
+	if (max_blocks) blocks = std::min<uint64_t>(blocks, max_blocks);
+	hipLaunchKernelGGL((k_step_ragged<MODEL, S>), dim3(uint32_t(std::max<uint64_t>(blocks, 1))), dim3(256), 0, st,
+	                   state, items, nitems, steps, a);
+	return hipGetLastError();
+}
+#endif
+
 } // namespace
+
+#ifdef NLC_STEP_RAGGED
+RaggedLauncher NLC_STEP_RAGGED(int S) {
+	switch (S) {
+	case 8: return &launch_ragged<NLC_STEP_MODEL, 8>;
+	case 16: return &launch_ragged<NLC_STEP_MODEL, 16>;
+	case 32: return &launch_ragged<NLC_STEP_MODEL, 32>;
+	}
+	return nullptr;
+}
+#endif
 
 // Shapes instantiated: every power-of-two (L, S) with 2 <= L*S <= 2048, S <= 32
 // (exact fit), plus L = 64 with S = 2..32 for any n in (64(S-1), 64S].

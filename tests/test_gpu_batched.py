@@ -1,0 +1,163 @@
+"""The batched form: ragged ensembles in one persistent launch, hipGraph
+batches and the autotuner, all checked against the oracle (FP64)."""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+ORACLE_MODEL = {0: O.DNKG, 1: O.FPUT, 2: O.TODA}
+
+
+def banner(msg):
+    print(f"[gpu-batched] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def split_chains(flat, lengths):
+    out, at = [], 0
+    for n in lengths:
+        out.append(flat[at:at + n])
+        at += n
+    return out
+
+
+def oracle_ragged(model, lengths, s0, steps, dt, kw):
+    """Advance every chain with the oracle, grouped by length."""
+    chains = split_chains(s0, lengths)
+    res = [None] * len(lengths)
+    for n in sorted(set(lengths)):
+        idx = [i for i, l in enumerate(lengths) if l == n]
+        out = O.move_real(ORACLE_MODEL[model], np.stack([chains[i] for i in idx]), steps, dt, **kw)
+        for j, i in enumerate(idx):
+            res[i] = out[j]
+    return np.concatenate(res)
+
+
+@pytest.mark.parametrize("model,kw,choices", [
+    (2, dict(alpha=1.0), [32, 64, 128, 256, 512]),
+    (0, dict(m=1.0, beta=1.0), [32, 128, 1024, 2048]),
+    (1, dict(alpha=0.25, beta=1.0), [64, 256]),
+    (0, dict(m=1.0, beta=1.0), [8, 16, 32, 64]),
+    (2, dict(alpha=1.0), [70, 100, 300]),  # no persistent shape: one uniform kernel per class
+])
+def test_ragged_parity(gpu, model, kw, choices):
+    N = gpu
+    lengths = N.ic_ragged_lengths(301, choices, 4)
+    s0 = N.ic_ragged_plane_waves(lengths, kw.get("m", 0.0), 0.05, 0.05, 0.5, 4)
+    with N.Ensemble.ragged(model, lengths, 0.05, **kw) as ens:
+        desc = ens.describe()
+        banner(desc)
+        ens.upload(s0)
+        assert np.array_equal(ens.download(), s0)  # upload/download permutation round trip
+        ens.move(40)
+        got = ens.download()
+        E, _ = ens.chain_energies()
+    want = oracle_ragged(model, list(lengths), s0, 40, 0.05, kw)
+    err = np.abs(got - want).max() / np.abs(want).max()
+    banner(f"  rel err {err:.3e}")
+    assert err <= 1e-10
+    Ew = np.concatenate([O.chain_energy(ORACLE_MODEL[model], c[None], **kw) for c in split_chains(want, lengths)])
+    np.testing.assert_allclose(E, Ew, rtol=1e-11)
+
+
+def test_ragged_linenergies_per_class(gpu):
+    N = gpu
+    lengths = N.ic_ragged_lengths(200, [32, 64, 256], 7)
+    s0 = N.ic_ragged_plane_waves(lengths, 0.0, 0.05, 0.05, 0.5, 7)
+    s0 = s0 + 0.01 * np.random.default_rng(0).normal(size=s0.shape)
+    with N.Ensemble.ragged(N.TODA, lengths, 0.05, alpha=1.0) as ens:
+        ens.upload(s0)
+        e_gpu, s_gpu = ens.linenergies()
+        cl = list(ens.class_lengths)
+    chains = split_chains(s0, lengths)
+    at = 0
+    for ci, n in enumerate(cl):
+        group = np.stack([c for c, l in zip(chains, lengths) if l == n])
+        e = O.calc_linenergies(O.linenergies_real(group, O.dispersion_real(n, 0.0)), 0.5 / len(group) / n, True)
+        wt, inf = O.entropies(e, True)
+        np.testing.assert_allclose(e_gpu[at:at + n], e, rtol=0, atol=1e-9 * e.max())
+        assert abs(s_gpu[2 * ci] - wt) <= 1e-9 * abs(wt) and abs(s_gpu[2 * ci + 1] - inf) <= 1e-9 * abs(inf)
+        at += n
+
+
+@pytest.mark.parametrize("ragged", [False, True])
+def test_graph_batches_match_eager(gpu, ragged):
+    """nlc_batch replays a captured graph; every replay (with eager work and
+    synchronisations in between) equals the eager move + energy pass."""
+    N = gpu
+    if ragged:
+        lengths = N.ic_ragged_lengths(120, [32, 64, 128], 2)
+        s0 = N.ic_ragged_plane_waves(lengths, 1.0, 0.05, 0.1, 0.3, 2)
+        mk = lambda: N.Ensemble.ragged(N.DNKG, lengths, 0.05, m=1.0, beta=1.0)  # noqa: E731
+    else:
+        s0 = N.ic_plane_waves(1024, 64, 1.0, 0, 128, 0.5, 2)
+        mk = lambda: N.Ensemble(N.DNKG, 1024, 64, 0.05, m=1.0, beta=1.0)  # noqa: E731
+    with mk() as g, mk() as e:
+        g.upload(s0)
+        e.upload(s0)
+        for rep in range(4):
+            eg, sg = g.batch(25)
+            e.move(25)
+            ee, se = e.linenergies()
+            assert np.array_equal(g.download(), e.download()), rep
+            assert np.array_equal(eg, ee) and np.array_equal(sg, se), rep
+            e.chain_energies()  # unrelated eager work between replays
+
+
+def test_autotune_keeps_state_and_parity(gpu):
+    N = gpu
+    s0 = N.ic_plane_waves(64, 500, 0.0, 1, 8, 0.3, 5)
+    with N.Ensemble(N.FPUT, 64, 500, 0.05, alpha=0.25, beta=1.0) as ens:
+        ens.upload(s0)
+        report = ens.autotune(10)
+        banner(f"autotune: {report} -> {ens.describe()}")
+        assert "*" in report
+        assert np.array_equal(ens.download(), s0)
+        ens.move(50)
+        got = ens.download()
+    want = O.move_real(O.FPUT, s0, 50, 0.05, alpha=0.25, beta=1.0)
+    assert np.abs(got - want).max() <= 1e-10 * np.abs(want).max()
+
+
+def test_autotune_ragged(gpu):
+    N = gpu
+    lengths = N.ic_ragged_lengths(400, [64, 128, 256], 9)
+    s0 = N.ic_ragged_plane_waves(lengths, 1.0, 0.05, 0.1, 0.3, 9)
+    with N.Ensemble.ragged(N.DNKG, lengths, 0.05, m=1.0, beta=1.0) as ens:
+        ens.upload(s0)
+        report = ens.autotune(5)
+        banner(f"autotune ragged: {report}")
+        assert all(f"S={s}" in report for s in (8, 16, 32))  # S <= n_min = 64 and 64 S >= n_max = 256
+        assert np.array_equal(ens.download(), s0)
+        ens.move(20)
+        got = ens.download()
+    want = oracle_ragged(0, list(lengths), s0, 20, 0.05, dict(m=1.0, beta=1.0))
+    assert np.abs(got - want).max() <= 1e-10 * np.abs(want).max()
+
+
+def test_ragged_toda_full_size_sampled(gpu):
+    """The ragged Toda workload: 65536 chains, lengths uniform over
+    {32, ..., 2048}; a sample of each length re-run on the oracle."""
+    N = gpu
+    choices = [32, 64, 128, 256, 512, 1024, 2048]
+    lengths = N.ic_ragged_lengths(65536, choices, 4)
+    s0 = N.ic_ragged_plane_waves(lengths, 0.0, 0.05, 0.05, 0.5, 4)
+    steps = 20
+    with N.Ensemble.ragged(N.TODA, lengths, 0.05, alpha=1.0) as ens:
+        banner(ens.describe())
+        ens.upload(s0)
+        e, s = ens.batch(steps)
+        got = ens.download()
+    assert np.isfinite(e).all() and np.isfinite(s).all()
+    offs = np.concatenate([[0], np.cumsum(lengths.astype(np.int64))])
+    rng = np.random.default_rng(1)
+    for n in choices:
+        idx = rng.choice(np.nonzero(lengths == n)[0], 3, replace=False)
+        chains = np.stack([s0[offs[i]:offs[i + 1]] for i in idx])
+        want = O.move_real(O.TODA, chains, steps, 0.05, alpha=1.0)
+        g = np.stack([got[offs[i]:offs[i + 1]] for i in idx])
+        assert np.abs(g - want).max() <= 1e-10 * np.abs(want).max(), n
