@@ -45,6 +45,13 @@ WORKLOADS = {
                  dens=0.5, seed=2, batch=1000),
     "toda": dict(model="Toda", n=1024, copies=16384, dt=0.05, m=0.0, alpha=1.0, beta=0.0, k_lo=1, k_hi=52,
                  dens=0.25, seed=4, batch=1000),
+    # ragged Toda alpha=1: 65536 chains, lengths uniform over {32,...,2048}, harmonic energy
+    # density U[0.05,0.25] in the lowest 5% of modes (at 0.5 the longest chains reach bond
+    # stretches near 3 and leave the dt=0.05 stability region, in the oracle as on the GPU);
+    # autotuned kernel, one hipGraph (move + energy pass + entropies) per batch
+    "toda_ragged": dict(model="Toda", ragged=True, nchains=65536, choices=[32, 64, 128, 256, 512, 1024, 2048],
+                        dt=0.05, m=0.0, alpha=1.0, beta=0.0, mode_fraction=0.05, e_lo=0.05, e_hi=0.25, seed=4,
+                        batch=1000),
     # DNLS beta=1, N=4096, 4096 copies, dt=0.01, norm density 1 on |k| < N/16
     "dnls": dict(model="DNLS", n=4096, copies=4096, dt=0.01, m=0.0, alpha=0.0, beta=1.0, kmax=256, dens=1.0,
                  seed=3, batch=1000),
@@ -132,6 +139,45 @@ def copy_calibration(mib: int) -> None:
     print(json.dumps({"copy_calibration_bytes": nbytes, "copies": 3}), flush=True)
 
 
+def traffic_bytes(desc: str):
+    """HBM bytes per step launch for a kernel shape, from the calibrated
+    rocprofv3 FETCH_SIZE/WRITE_SIZE runs summarised in profiles/traffic.json."""
+    f = ROOT / "profiles" / "traffic.json"
+    if not f.exists():
+        return None
+    t = json.loads(f.read_text()).get(desc)
+    if not t:
+        return None
+    return (t["fetch_kib"] * t["fetch_scale"] + t["write_kib"] * t["write_scale"]) * 1024.0
+
+
+def cpu_baseline_ragged(w, lengths, s0, seconds_budget: float):
+    """Oracle release build on a sample of the ragged ensemble: the first chains
+    of every length class, each class run as one oracle call."""
+    from oracle import oracle as O
+    lib = O.fast()
+    cores = max(1, min(len(os.sched_getaffinity(0)), 16))
+    offs = np.concatenate([[0], np.cumsum(lengths.astype(np.int64))])
+    per_class = max(cores, 64)
+    groups = []
+    for n in sorted(set(int(x) for x in lengths)):
+        idx = np.nonzero(lengths == n)[0][:per_class]
+        groups.append(np.stack([s0[offs[i]:offs[i + 1]] for i in idx]))
+    sites = sum(g.shape[0] * g.shape[1] for g in groups)
+    t = time.perf_counter()
+    for g in groups:
+        O.move_real(O.TODA, g, 2, w["dt"], alpha=w["alpha"], threads=cores, lib=lib)
+    rate = sites * 2 / (time.perf_counter() - t)
+    steps = int(max(4, min(w["batch"], seconds_budget * rate / sites)))
+    t = time.perf_counter()
+    for g in groups:
+        O.move_real(O.TODA, g, steps, w["dt"], alpha=w["alpha"], threads=cores, lib=lib)
+    el = time.perf_counter() - t
+    return {"value": sites * steps / el, "unit": "site-steps/s", "cores": cores, "kind": "port",
+            "sample": f"oracle/liboracle_fast.so (-O3 -ffast-math, OpenMP over copies), Toda, first {per_class} "
+                      f"chains of each length class ({sites} sites) x {steps} steps, {el:.1f} s"}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -139,6 +185,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--batch", type=int, default=None, help="Yoshida steps per bench step")
     ap.add_argument("--workload", default="dnkg", choices=sorted(WORKLOADS))
+    ap.add_argument("--mode", default="plain", choices=["plain", "batched"],
+                    help="plain: move + energy pass per batch; batched: autotuned shape, one hipGraph per batch")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU-baseline time budget (0 = skip)")
     ap.add_argument("--lanes", type=int, default=0)
     ap.add_argument("--sites", type=int, default=0)
@@ -159,46 +207,71 @@ def main():
     if args.batch:
         w["batch"] = args.batch
     model = N.MODELS[w["model"]]
-    n, copies, b = w["n"], w["copies"], w["batch"]
-    if w["model"] == "DNLS":
-        s0 = N.ic_dnls(n, copies, w["kmax"], w["dens"], w["seed"])
+    b = w["batch"]
+    ragged = w.get("ragged", False)
+    mode = "batched" if ragged else args.mode
+    lengths = None
+    if ragged:
+        lengths = N.ic_ragged_lengths(w["nchains"], w["choices"], w["seed"])
+        s0 = N.ic_ragged_plane_waves(lengths, w["m"], w["mode_fraction"], w["e_lo"], w["e_hi"], w["seed"])
+        sites = float(lengths.sum())
+        ens = N.Ensemble.ragged(model, lengths, w["dt"], m=w["m"], alpha=w["alpha"], beta=w["beta"])
+        shape = f"{w['nchains']} chains, lengths uniform over {w['choices']}"
     else:
-        s0 = N.ic_plane_waves(n, copies, w["m"], w["k_lo"], w["k_hi"], w["dens"], w["seed"])
+        n, copies = w["n"], w["copies"]
+        if w["model"] == "DNLS":
+            s0 = N.ic_dnls(n, copies, w["kmax"], w["dens"], w["seed"])
+        else:
+            s0 = N.ic_plane_waves(n, copies, w["m"], w["k_lo"], w["k_hi"], w["dens"], w["seed"])
+        sites = float(n) * copies
+        ens = N.Ensemble(model, n, copies, w["dt"], m=w["m"], alpha=w["alpha"], beta=w["beta"],
+                         lanes_per_chain=args.lanes, sites_per_lane=args.sites)
+        shape = f"N={n} copies={copies}"
 
     ev = HipEvents()
-    with N.Ensemble(model, n, copies, w["dt"], m=w["m"], alpha=w["alpha"], beta=w["beta"],
-                    lanes_per_chain=args.lanes, sites_per_lane=args.sites) as ens:
+    tune = None
+    with ens:
         ens.upload(s0)
+        if mode == "batched" and w["model"] != "DNLS":
+            tune = ens.autotune(max(1, b // 20))
         desc = ens.describe()
         stream = ens.stream()
         E0, _ = ens.chain_energies()
+
+        def bench_step():
+            if mode == "batched":
+                return ens.batch(b)  # one hipGraph: move + energy pass + entropies
+            ens.move(b)
+            return ens.linenergies()  # energy pass + entropies, synchronous
+
         for _ in range(args.warmup):
-            ens.move(b)
-            ens.linenergies()
+            bench_step()
         ens.sync()
-        starts = [ev.create() for _ in range(args.steps)]
-        stops = [ev.create() for _ in range(args.steps)]
         t0 = time.perf_counter()
-        for i in range(args.steps):
-            ev.record(starts[i], stream)
-            ens.move(b)
-            ev.record(stops[i], stream)
-            ens.linenergies()  # energy pass + entropies, synchronous (results reach the host every batch)
+        for _ in range(args.steps):
+            e_lin, ent = bench_step()
         ens.sync()
         elapsed = time.perf_counter() - t0
-        kernel_ms = [ev.elapsed_ms(a, z) for a, z in zip(starts, stops)]
+        E1, _ = ens.chain_energies()
+        # step-kernel duration from HIP events on the context stream (separate launches)
+        starts = [ev.create() for _ in range(3)]
+        stops = [ev.create() for _ in range(3)]
+        for a_, z_ in zip(starts, stops):
+            ev.record(a_, stream)
+            ens.move(b)
+            ev.record(z_, stream)
+        kernel_ms = [ev.elapsed_ms(a_, z_) for a_, z_ in zip(starts, stops)]
         for e_ in starts + stops:
             ev.destroy(e_)
-        E1, _ = ens.chain_energies()
-        e_lin, ent = ens.linenergies()
 
     total_steps = (args.warmup + args.steps) * b
-    drift = float(np.mean(np.abs(E1 - E0) / np.abs(E0)))
-    site_steps = float(n) * copies * b * args.steps
-    value = site_steps / elapsed
+    rel = np.abs(E1 - E0) / np.abs(E0)
+    drift = float(np.mean(rel[np.isfinite(rel)])) if np.isfinite(rel).any() else float("nan")
+    value = sites * b * args.steps / elapsed
     k_ms = float(np.mean(kernel_ms))
-    flop = FLOP_PER_SITE_STEP[w["model"]] * float(n) * copies * b
+    flop = FLOP_PER_SITE_STEP[w["model"]] * sites * b
     achieved = flop / (k_ms * 1e-3) / 1e12
+    kname = "k_dnls_4096" if w["model"] == "DNLS" else ("k_step_ragged" if ragged else "k_step_wave")
     out = {
         "metric": METRIC,
         "value": value,
@@ -212,19 +285,21 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded plane-wave initial conditions)",
-        "config": {"workload": f"{w['model']} m={w['m']} alpha={w['alpha']} beta={w['beta']} N={n} copies={copies} "
-                               f"dt={w['dt']}, plain FP64, {b} steps + energy pass + entropies per bench step",
-                   "kernel": desc, "yoshida_steps_per_bench_step": b},
-        "energy_drift": {"mean_rel": drift, "over_steps": total_steps},
-        "entropies_final": [float(ent[0]), float(ent[1])],
+        "config": {"workload": f"{w['model']} m={w['m']} alpha={w['alpha']} beta={w['beta']} {shape} dt={w['dt']}, "
+                               f"{mode} FP64, {b} steps + energy pass + entropies per bench step",
+                   "kernel": desc, "yoshida_steps_per_bench_step": b, "mode": mode},
+        "energy_drift": {"mean_rel": drift, "over_steps": total_steps,
+                         "nonfinite_chains": int((~np.isfinite(rel)).sum())},
+        "entropies_final": [float(x) for x in ent[:2]],
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
-                     "kernel": ("k_dnls_4096" if w["model"] == "DNLS" else "k_step_wave") + " (FP64 vector ALU bound)",
-                     "kernel_ms": k_ms,
-                     "flop_per_launch": flop},
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic_bytes(desc),
+                     "kernel": kname + " (FP64 vector ALU bound)", "kernel_ms": k_ms, "flop_per_launch": flop},
     }
+    if tune:
+        out["config"]["autotune"] = tune
     if args.cpu_seconds > 0:
-        out["cpu_baseline"] = cpu_baseline(w, s0, args.cpu_seconds)
+        out["cpu_baseline"] = (cpu_baseline_ragged(w, lengths, s0, args.cpu_seconds) if ragged
+                               else cpu_baseline(w, s0, args.cpu_seconds))
     print(json.dumps(out), flush=True)
 
 

@@ -775,7 +775,13 @@ int nlc_autotune(nlc_ctx *c, uint32_t trial_steps, char *report, size_t len) {
 	} else if (c->ragged) {
 		for (int S : {8, 16, 32})
 			if (ragged_fits(c->cls, S) && lookup_ragged(c->p.model, S)) cands.push_back({0, 0, false, false, S, 0});
-		if (cands.empty()) c->tune_report = "ragged: no persistent shape fits these lengths";
+		bool per_class = true; // one uniform kernel per length class, launched in sequence
+		for (const auto &k : c->cls) {
+			int L, S;
+			bool exact;
+			per_class = per_class && default_shape(k.n, L, S, exact) && lookup(c->p.model, L, S, exact);
+		}
+		if (per_class) cands.push_back({0, 0, false, false, 0, 0});
 	} else {
 		for (const Shape &s : wave_shapes(c->p.model, c->n)) cands.push_back({s.L, s.S, s.exact, false, 0, 0});
 		cands.push_back({0, 0, false, true, 0, 0});
@@ -824,7 +830,8 @@ int nlc_autotune(nlc_ctx *c, uint32_t trial_steps, char *report, size_t len) {
 		char buf[96];
 		for (const auto &cd : cands) {
 			const char *mark = &cd == best ? "*" : "";
-			if (c->ragged) snprintf(buf, sizeof buf, " S=%d %.3f ms%s", cd.ragged_S, cd.ms, mark);
+			if (c->ragged && cd.ragged_S) snprintf(buf, sizeof buf, " S=%d %.3f ms%s", cd.ragged_S, cd.ms, mark);
+			else if (c->ragged) snprintf(buf, sizeof buf, " per-class %.3f ms%s", cd.ms, mark);
 			else if (cd.split) snprintf(buf, sizeof buf, " split %.3f ms%s", cd.ms, mark);
 			else snprintf(buf, sizeof buf, " %dx%d%s %.3f ms%s", cd.L, cd.S, cd.exact ? "" : "r", cd.ms, mark);
 			rep += buf;
