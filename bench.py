@@ -52,6 +52,10 @@ WORKLOADS = {
     "toda_ragged": dict(model="Toda", ragged=True, nchains=65536, choices=[32, 64, 128, 256, 512, 1024, 2048],
                         dt=0.05, m=0.0, alpha=1.0, beta=0.0, mode_fraction=0.05, e_lo=0.05, e_hi=0.25, seed=4,
                         batch=1000),
+    # dDNKG beta=1, masses U[0.5,1.5] (seed 5), N=512, 8192 copies, dt=0.05, energy density 0.5 in
+    # the lowest N/8 eigenmodes; energy pass = fused FP64 projection on the eigenbasis per batch
+    "ddnkg": dict(model="dDNKG", n=512, copies=8192, dt=0.05, m=0.0, alpha=0.0, beta=1.0, m_lo=0.5, m_hi=1.5,
+                  modes=64, dens=0.5, seed=5, batch=500),
     # DNLS beta=1, N=4096, 4096 copies, dt=0.01, norm density 1 on |k| < N/16
     "dnls": dict(model="DNLS", n=4096, copies=4096, dt=0.01, m=0.0, alpha=0.0, beta=1.0, kmax=256, dens=1.0,
                  seed=3, batch=1000),
@@ -95,9 +99,9 @@ def cpu_baseline(w, s0, seconds_budget: float):
     def move(state, steps):
         if w["model"] == "DNLS":
             return O.move_dnls(state, steps, w["dt"], w["beta"], threads=cores, lib=lib)
-        model = {"DNKG": O.DNKG, "FPUT": O.FPUT, "Toda": O.TODA}[w["model"]]
+        model = {"DNKG": O.DNKG, "FPUT": O.FPUT, "Toda": O.TODA, "dDNKG": O.DDNKG}[w["model"]]
         return O.move_real(model, state, steps, w["dt"], m=w["m"], alpha=w["alpha"], beta=w["beta"],
-                           threads=cores, lib=lib)
+                           mp2=w.get("mp2"), threads=cores, lib=lib)
 
     # calibrate on a small slice, then size the sample to the time budget
     probe_copies = min(w["copies"], 4 * cores if w["model"] == "DNLS" else 64 * cores)
@@ -219,14 +223,22 @@ def main():
         shape = f"{w['nchains']} chains, lengths uniform over {w['choices']}"
     else:
         n, copies = w["n"], w["copies"]
+        masses = None
         if w["model"] == "DNLS":
             s0 = N.ic_dnls(n, copies, w["kmax"], w["dens"], w["seed"])
+        elif w["model"] == "dDNKG":
+            masses = N.ic_masses(n, w["m_lo"], w["m_hi"], w["seed"])
+            w["mp2"] = masses + 2.0
+            om, V = N.eigensystem(w["mp2"])
+            s0 = N.ic_eigenmodes(n, copies, om, V, w["modes"], w["dens"], w["seed"])
         else:
             s0 = N.ic_plane_waves(n, copies, w["m"], w["k_lo"], w["k_hi"], w["dens"], w["seed"])
         sites = float(n) * copies
-        ens = N.Ensemble(model, n, copies, w["dt"], m=w["m"], alpha=w["alpha"], beta=w["beta"],
+        ens = N.Ensemble(model, n, copies, w["dt"], m=w["m"], alpha=w["alpha"], beta=w["beta"], masses=masses,
                          lanes_per_chain=args.lanes, sites_per_lane=args.sites)
         shape = f"N={n} copies={copies}"
+        if w["model"] == "dDNKG":
+            shape += f" masses U[{w['m_lo']},{w['m_hi']}] (seed {w['seed']}), {w['modes']} lowest eigenmodes excited"
 
     ev = HipEvents()
     tune = None
@@ -261,8 +273,16 @@ def main():
             ens.move(b)
             ev.record(z_, stream)
         kernel_ms = [ev.elapsed_ms(a_, z_) for a_, z_ in zip(starts, stops)]
+        # energy pass alone (projection or FFT, fixed-order reduction, D2H of the sums)
+        energy_ms = []
+        for a_, z_ in zip(starts, stops):
+            ev.record(a_, stream)
+            ens.linenergies_raw()
+            ev.record(z_, stream)
+            energy_ms.append(ev.elapsed_ms(a_, z_))
         for e_ in starts + stops:
             ev.destroy(e_)
+        desc = ens.describe()
 
     total_steps = (args.warmup + args.steps) * b
     rel = np.abs(E1 - E0) / np.abs(E0)
@@ -284,7 +304,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (seeded plane-wave initial conditions)",
+        "data": "synthetic (seeded initial conditions: " + ("random-phase Fourier modes" if w["model"] == "DNLS" else
+                 "eigenmodes" if w["model"] == "dDNKG" else "plane waves") + ")",
         "config": {"workload": f"{w['model']} m={w['m']} alpha={w['alpha']} beta={w['beta']} {shape} dt={w['dt']}, "
                                f"{mode} FP64, {b} steps + energy pass + entropies per bench step",
                    "kernel": desc, "yoshida_steps_per_bench_step": b, "mode": mode},
@@ -295,6 +316,13 @@ def main():
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic_bytes(desc),
                      "kernel": kname + " (FP64 vector ALU bound)", "kernel_ms": k_ms, "flop_per_launch": flop},
     }
+    e_ms = float(np.mean(energy_ms))
+    out["energy_pass"] = {"ms": e_ms, "per_bench_step_fraction": e_ms / (elapsed / args.steps * 1e3)}
+    if w["model"] == "dDNKG":
+        # two FP64 products (copies x n) . (n x n) per pass, FMA = 2
+        pflop = 4.0 * w["n"] ** 2 * w["copies"]
+        out["energy_pass"].update({"bound": "mfma", "flop": pflop, "achieved_tflops": pflop / (e_ms * 1e-3) / 1e12,
+                                   "peak_tflops": 78.6, "frac": pflop / (e_ms * 1e-3) / 1e12 / 78.6})
     if tune:
         out["config"]["autotune"] = tune
     if args.cpu_seconds > 0:

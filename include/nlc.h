@@ -59,6 +59,11 @@ typedef enum { NLC_FP64 = 0, NLC_FP32C = 1 } nlc_precision;
  * DNKG_FPUT_Toda.cu:262-297). */
 typedef enum { NLC_KERNEL_AUTO = 0, NLC_KERNEL_WAVE = 1, NLC_KERNEL_SPLIT = 2 } nlc_kernel;
 
+/* dDNKG energy projection (the two cublasDgemm of dDNKG.cpp:118-135 and the
+ * reduction of dDNKG.cu:237-265, fused).  NLC_PROJ_AUTO = FP64 matrix cores
+ * until nlc_autotune times both forms; every form is FP64 throughout. */
+typedef enum { NLC_PROJ_AUTO = 0, NLC_PROJ_MFMA_F64 = 1, NLC_PROJ_VALU_F64 = 2 } nlc_projection;
+
 /* Model and ensemble parameters: the reference's common options (gconf,
  * common/configuration.hpp:20-39) and model options (DNKG_FPUT_Toda.cpp:26-43,
  * dDNKG.cpp:31-34, DNLS.cpp:34-37). */
@@ -76,6 +81,7 @@ typedef struct nlc_params {
 	const double *masses;     /* dDNKG: m_x, chain_length doubles (content of the -m file); copied */
 	uint32_t lanes_per_chain; /* wave shape override (0 = default); used by the autotuner and tests */
 	uint32_t sites_per_lane;  /* wave shape override (0 = default) */
+	int32_t projection;       /* nlc_projection (dDNKG only) */
 } nlc_params;
 
 typedef struct nlc_ctx nlc_ctx;

@@ -23,6 +23,7 @@ DNKG, FPUT, TODA, DDNKG, DNLS = range(5)
 MODELS = {"DNKG": DNKG, "FPUT": FPUT, "Toda": TODA, "dDNKG": DDNKG, "DNLS": DNLS}
 FP64, FP32C = 0, 1
 KERNEL_AUTO, KERNEL_WAVE, KERNEL_SPLIT = 0, 1, 2
+PROJ_AUTO, PROJ_MFMA_F64, PROJ_VALU_F64 = 0, 1, 2
 
 NLC_OK, NLC_EARG, NLC_EHIP, NLC_ENOMEM, NLC_EUNSUPPORTED, NLC_ESTATE = 0, -1, -2, -3, -4, -5
 
@@ -42,6 +43,7 @@ class Params(C.Structure):
         ("masses", C.POINTER(C.c_double)),
         ("lanes_per_chain", C.c_uint32),
         ("sites_per_lane", C.c_uint32),
+        ("projection", C.c_int32),
     ]
 
 
@@ -181,12 +183,13 @@ class Ensemble:
 
     def __init__(self, model: int, n: int, copies: int, dt: float, *, m: float = 0.0, alpha: float = 0.0,
                  beta: float = 0.0, masses: np.ndarray | None = None, precision: int = FP64,
-                 kernel: int = KERNEL_AUTO, lanes_per_chain: int = 0, sites_per_lane: int = 0):
+                 kernel: int = KERNEL_AUTO, lanes_per_chain: int = 0, sites_per_lane: int = 0,
+                 projection: int = 0):
         p = Params()
         _check(lib.nlc_params_init(C.byref(p)))
         p.model, p.precision, p.kernel = model, precision, kernel
         p.chain_length, p.copies, p.dt, p.m, p.alpha, p.beta = n, copies, dt, m, alpha, beta
-        p.lanes_per_chain, p.sites_per_lane = lanes_per_chain, sites_per_lane
+        p.lanes_per_chain, p.sites_per_lane, p.projection = lanes_per_chain, sites_per_lane, projection
         self._masses = None
         if masses is not None:
             self._masses = np.ascontiguousarray(masses, dtype=np.float64)

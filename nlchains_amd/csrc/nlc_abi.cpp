@@ -68,6 +68,7 @@ struct nlc_ctx {
 	double *mp2 = nullptr;     // dDNKG m_x + 2
 	double *masses_d = nullptr;
 	double *eigvec = nullptr;  // dDNKG V, column-major
+	int proj_variant = nlc::kProjMfma; // dDNKG projection: FP64 matrix cores or vector ALU
 	double *split_q = nullptr, *split_p = nullptr;
 	double *energy_d = nullptr, *norm_d = nullptr;
 	double2 *dnls_table = nullptr; // DNLS Fourier multipliers
@@ -411,7 +412,8 @@ int enqueue_energy(nlc_ctx *c) {
 			NLC_HIP(c, nlc::launch_reduce_partial(c->stream, c->partial, k.rows, k.n, nullptr, er, c->reduce_scratch));
 			break;
 		case NLC_DDNKG:
-			NLC_HIP(c, nlc::launch_energy_ddnkg(c->stream, st, k.n, k.count, c->eigvec, c->omega, c->partial));
+			NLC_HIP(c, nlc::launch_energy_ddnkg(c->stream, st, k.n, k.count, c->eigvec, c->omega, c->partial,
+			                                    c->proj_variant));
 			NLC_HIP(c, nlc::launch_reduce_partial(c->stream, c->partial, k.rows, k.n, nullptr, er, c->reduce_scratch));
 			break;
 		case NLC_DNLS:
@@ -475,6 +477,8 @@ int validate_params(const nlc_params *p, bool ragged) {
 	if (p->model == NLC_DDNKG && !p->masses) return reject(NLC_EARG, "dDNKG needs the masses (-m file content)");
 	if (p->precision != NLC_FP64) return reject(NLC_EUNSUPPORTED, "only FP64 precision is available in this build");
 	if (p->kernel < NLC_KERNEL_AUTO || p->kernel > NLC_KERNEL_SPLIT) return reject(NLC_EARG, "unknown kernel family");
+	if (p->projection < NLC_PROJ_AUTO || p->projection > NLC_PROJ_VALU_F64)
+		return reject(NLC_EARG, "unknown projection form");
 	int ndev = 0;
 	if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
 		return reject(NLC_EHIP, "no HIP device available (the hot path has no CPU fallback)");
@@ -530,6 +534,7 @@ int nlc_create(nlc_ctx **out, const nlc_params *p) {
 	nlc_ctx *c = new (std::nothrow) nlc_ctx;
 	if (!c) return reject(NLC_ENOMEM, "context allocation failed");
 	c->p = *p;
+	c->proj_variant = p->projection == NLC_PROJ_VALU_F64 ? nlc::kProjValu : nlc::kProjMfma;
 	c->n = p->chain_length;
 	c->copies = p->copies;
 	c->total_sites = uint64_t(c->n) * c->copies;
@@ -761,6 +766,37 @@ int nlc_chain_energies(nlc_ctx *c, double *energies, double *norms) {
 	return NLC_OK;
 }
 
+// dDNKG: time the energy pass with each projection form (it only reads the
+// state) and keep the faster one.
+static int tune_projection(nlc_ctx *c) {
+	hipEvent_t e0 = nullptr, e1 = nullptr;
+	int rc = NLC_OK;
+	float ms[2] = {0, 0};
+	if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+		rc = c->fail(NLC_EHIP, "autotune: event creation failed");
+	for (int v : {nlc::kProjMfma, nlc::kProjValu}) {
+		if (rc) break;
+		c->proj_variant = v;
+		for (int rep = 0; rep < 2 && !rc; ++rep) { // the first run warms up
+			rc = enqueue_energy(c);
+			if (!rc && hipEventRecord(e0, c->stream) != hipSuccess) rc = c->fail(NLC_EHIP, "event record");
+			for (int i = 0; i < 3 && !rc; ++i) rc = enqueue_energy(c);
+			if (!rc && (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+			            hipEventElapsedTime(&ms[v], e0, e1) != hipSuccess))
+				rc = c->fail(NLC_EHIP, "autotune timing failed");
+		}
+	}
+	if (e0) hipEventDestroy(e0);
+	if (e1) hipEventDestroy(e1);
+	if (rc) return rc;
+	c->proj_variant = ms[nlc::kProjValu] < ms[nlc::kProjMfma] ? nlc::kProjValu : nlc::kProjMfma;
+	char buf[128];
+	snprintf(buf, sizeof buf, "; projection: mfma %.3f ms%s valu %.3f ms%s", ms[0] / 3,
+	         c->proj_variant == nlc::kProjMfma ? "*" : "", ms[1] / 3, c->proj_variant == nlc::kProjValu ? "*" : "");
+	c->tune_report += buf;
+	return NLC_OK;
+}
+
 int nlc_autotune(nlc_ctx *c, uint32_t trial_steps, char *report, size_t len) {
 	if (!c || !trial_steps) return NLC_EARG;
 	struct Cand {
@@ -846,6 +882,7 @@ int nlc_autotune(nlc_ctx *c, uint32_t trial_steps, char *report, size_t len) {
 			NLC_TRY(select_uniform(c));
 		}
 	}
+	if (c->p.model == NLC_DDNKG && c->p.projection == NLC_PROJ_AUTO) NLC_TRY(tune_projection(c));
 	if (report && len) snprintf(report, len, "%s", c->tune_report.c_str());
 	return NLC_OK;
 }
@@ -873,6 +910,11 @@ int nlc_describe(const nlc_ctx *c, char *buf, size_t len) {
 	else
 		snprintf(buf, len, "model=%s n=%u copies=%u kernel=wave lanes_per_chain=%d sites_per_lane=%d exact=%d",
 		         names[c->p.model], c->n, c->copies, c->L, c->S, int(c->exact));
+	if (c->p.model == NLC_DDNKG) {
+		const size_t at = strlen(buf);
+		if (at + 1 < len)
+			snprintf(buf + at, len - at, " projection=%s", c->proj_variant == nlc::kProjMfma ? "mfma_f64" : "valu_f64");
+	}
 	return NLC_OK;
 }
 

@@ -7,10 +7,18 @@
 // dDNKG/cpu-implementation/dDNKG.cpp:58-72.
 //
 // Design: one fused kernel reads the planar shard directly (no transpose, no
-// projection matrices in HBM): a workgroup computes a 64-copy x 64-mode tile of
-// both projections (q.V and p.V share the V tile), staged through LDS, and its
-// epilogue squares, applies omega and reduces the 64 copies into one partial
-// row; the per-tile rows are reduced in a fixed order afterwards.
+// projection matrices in HBM) and its epilogue squares, applies omega and
+// reduces 64 copies into one partial row; the per-tile rows are reduced in a
+// fixed order afterwards.  Two forms of the product:
+//  * k_proj_mfma (default): FP64 matrix cores, v_mfma_f64_16x16x4f64.  A
+//    wavefront owns 64 copies x 32 modes; the K index of the instruction runs
+//    over sites in a lane-private order (lane group g = lane/16 takes sites
+//    4g..4g+3 of each 16-site chunk, one per instruction), the same order for
+//    the state operand and for V, so every lane loads 64 contiguous bytes of its
+//    copy's (q,p) pairs and 32 contiguous bytes of its mode's eigenvector, with
+//    no LDS staging and no transpose;
+//  * k_proj_energy: the FP64 vector-ALU form (64x64 tile staged through LDS),
+//    kept as the autotuner's alternative.
 #include "nlc_internal.hpp"
 
 namespace nlc {
@@ -83,14 +91,109 @@ k_proj_energy(const double2 *__restrict__ state, uint32_t n, uint32_t copies, co
 	}
 }
 
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kMfRows = 4;               // 16-copy row blocks per wavefront (64 copies)
+constexpr int kMfCols = 2;               // 16-mode column blocks per wavefront (32 modes)
+constexpr int kMfWaves = 4;              // wavefronts per workgroup, side by side over modes
+constexpr int kMfModes = kMfWaves * 16 * kMfCols; // 128 modes per workgroup
+
+// Workgroup (x, y): copies [64 y, 64 y + 64), modes [128 x, 128 x + 128); wave w
+// of the group takes modes 128 x + 32 w + [0, 32).  Output: partial[y][j].
+// Operand layout of v_mfma_f64_16x16x4f64 (A 16x4, B 4x16, D 16x16): lane l
+// supplies A[l % 16][l / 16] and B[l / 16][l % 16] and holds D[4 (l / 16) + r][l % 16]
+// in accumulator element r.
+__global__ void __launch_bounds__(256, 2)
+k_proj_mfma(const double2 *__restrict__ state, uint32_t n, uint32_t copies, const double *__restrict__ V,
+            const double *__restrict__ omega, double *__restrict__ partial) {
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+	const int r16 = lane & 15, g = lane >> 4;
+	const uint32_t c_base = blockIdx.y * 64u + r16;
+	const uint32_t j_base = blockIdx.x * uint32_t(kMfModes) + uint32_t(wave) * 32u + r16;
+
+	f64x4 acc_q[kMfRows][kMfCols], acc_p[kMfRows][kMfCols];
+#pragma unroll
+	for (int rb = 0; rb < kMfRows; ++rb)
+#pragma unroll
+		for (int cb = 0; cb < kMfCols; ++cb) acc_q[rb][cb] = acc_p[rb][cb] = f64x4{0.0, 0.0, 0.0, 0.0};
+
+	const bool full_rows = blockIdx.y * 64u + 64u <= copies;
+	const bool full_cols = blockIdx.x * uint32_t(kMfModes) + uint32_t(kMfModes) <= n;
+	for (uint32_t k0 = 0; k0 < n; k0 += 16) {
+		const uint32_t site = k0 + 4u * uint32_t(g);
+		const bool full_k = k0 + 16u <= n;
+		double2 a[kMfRows][4];
+		double b[kMfCols][4];
+#pragma unroll
+		for (int rb = 0; rb < kMfRows; ++rb) {
+			const uint32_t c = c_base + 16u * uint32_t(rb);
+			const double2 *src = state + size_t(c) * n + site;
+			if (full_k && full_rows) {
+#pragma unroll
+				for (int t = 0; t < 4; ++t) a[rb][t] = src[t];
+			} else {
+#pragma unroll
+				for (int t = 0; t < 4; ++t)
+					a[rb][t] = (c < copies && site + t < n) ? src[t] : make_double2(0.0, 0.0);
+			}
+		}
+#pragma unroll
+		for (int cb = 0; cb < kMfCols; ++cb) {
+			const uint32_t j = j_base + 16u * uint32_t(cb);
+			const double *src = V + size_t(j) * n + site;
+			if (full_k && full_cols) {
+#pragma unroll
+				for (int t = 0; t < 4; ++t) b[cb][t] = src[t];
+			} else {
+#pragma unroll
+				for (int t = 0; t < 4; ++t) b[cb][t] = (j < n && site + t < n) ? src[t] : 0.0;
+			}
+		}
+#pragma unroll
+		for (int t = 0; t < 4; ++t)
+#pragma unroll
+			for (int rb = 0; rb < kMfRows; ++rb)
+#pragma unroll
+				for (int cb = 0; cb < kMfCols; ++cb) {
+					acc_q[rb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rb][t].x, b[cb][t], acc_q[rb][cb], 0, 0, 0);
+					acc_p[rb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rb][t].y, b[cb][t], acc_p[rb][cb], 0, 0, 0);
+				}
+	}
+
+	// epilogue: lane holds modes j_base + 16 cb and copies 16 rb + 4 g + r of the tile
+#pragma unroll
+	for (int cb = 0; cb < kMfCols; ++cb) {
+		const uint32_t j = j_base + 16u * uint32_t(cb);
+		const double w = j < n ? omega[j] : 0.0;
+		double sum = 0.0;
+#pragma unroll
+		for (int rb = 0; rb < kMfRows; ++rb)
+#pragma unroll
+			for (int r = 0; r < 4; ++r) {
+				const double wq = w * acc_q[rb][cb][r], pp = acc_p[rb][cb][r];
+				sum += fma(wq, wq, pp * pp);
+			}
+		// add the four lane groups (copy rows 4g..4g+3 of each block); a + b == b + a
+		// exactly, so every lane of a group ends with the same value
+		sum += __shfl_xor(sum, 16, 64);
+		sum += __shfl_xor(sum, 32, 64);
+		if (g == 0 && j < n) partial[size_t(blockIdx.y) * n + j] = sum;
+	}
+}
+
 } // namespace
 
 uint32_t ddnkg_partial_rows(uint32_t copies) { return (copies + kTc - 1) / kTc; }
 
 hipError_t launch_energy_ddnkg(hipStream_t st, const double2 *state, uint32_t n, uint32_t copies, const double *V,
-                               const double *omega, double *partial) {
-	const dim3 grid((n + kTj - 1) / kTj, ddnkg_partial_rows(copies));
-	hipLaunchKernelGGL(k_proj_energy, grid, dim3(256), 0, st, state, n, copies, V, omega, partial);
+                               const double *omega, double *partial, int variant) {
+	if (variant == kProjValu) {
+		const dim3 grid((n + kTj - 1) / kTj, ddnkg_partial_rows(copies));
+		hipLaunchKernelGGL(k_proj_energy, grid, dim3(256), 0, st, state, n, copies, V, omega, partial);
+	} else {
+		const dim3 grid((n + kMfModes - 1) / kMfModes, ddnkg_partial_rows(copies));
+		hipLaunchKernelGGL(k_proj_mfma, grid, dim3(256), 0, st, state, n, copies, V, omega, partial);
+	}
 	return hipGetLastError();
 }
 

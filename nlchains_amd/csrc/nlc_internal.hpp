@@ -84,10 +84,12 @@ constexpr uint32_t kEnergyFftMax = 4096;
 } // namespace nlc
 
 namespace nlc {
-// dDNKG projection energies: partial[rows][n] with rows = ddnkg_partial_rows(copies).
+// dDNKG projection energies: partial[rows][n] with rows = ddnkg_partial_rows(copies),
+// on FP64 matrix cores (kProjMfma) or the FP64 vector ALU (kProjValu).
+constexpr int kProjMfma = 0, kProjValu = 1;
 uint32_t ddnkg_partial_rows(uint32_t copies);
 hipError_t launch_energy_ddnkg(hipStream_t, const double2 *state, uint32_t n, uint32_t copies, const double *V,
-                               const double *omega, double *partial);
+                               const double *omega, double *partial, int variant);
 // Host eigensystem (nlc_host.cpp).
 int eigensystem(uint32_t n, const double *mp2, double *omegas, double *V);
 extern const double kYoshidaC[8], kYoshidaD[8];

@@ -224,3 +224,62 @@ def test_full_size_dnkg_sampled_parity_and_drift(gpu):
     np.testing.assert_allclose((E1[pick] - E0[pick]) / E0[pick], (E1_o - E0_o) / E0_o, rtol=0, atol=1e-12)
     assert drift < 1e-6
     assert np.isfinite(e_lin).all() and np.isfinite(ent).all()
+
+
+# ------------------------------------------------ dDNKG projection forms
+
+@pytest.mark.parametrize("proj", ["mfma", "valu"])
+@pytest.mark.parametrize("n,copies", [(512, 70), (100, 9), (130, 129), (48, 200), (16, 64)])
+def test_ddnkg_projection_forms(gpu, proj, n, copies):
+    """Both forms of the fused projection (FP64 matrix cores, FP64 vector ALU)
+    against the oracle's O(copies n^2) projection, including chain lengths that
+    are not multiples of the tile sizes and partial copy tiles."""
+    N = gpu
+    banner(f"dDNKG projection {proj} n={n} copies={copies}")
+    model, kw, masses, s0 = make_case(N, "dDNKG", n, copies, 11)
+    s0 = s0 + 0.01 * np.random.default_rng(3).normal(size=s0.shape)
+    p = N.PROJ_MFMA_F64 if proj == "mfma" else N.PROJ_VALU_F64
+    with N.Ensemble(model, n, copies, 0.05, masses=masses, projection=p, **kw) as ens:
+        ens.upload(s0)
+        raw_gpu = ens.linenergies_raw()
+        assert f"projection={proj}_f64" in ens.describe()
+    om, V = N.eigensystem(masses + 2)
+    raw = O.linenergies_ddnkg(s0, V, om)
+    assert rel_err(raw_gpu, raw) <= ENERGY_TOL
+
+
+def test_ddnkg_projection_full_size(gpu):
+    """The dDNKG workload's shape (512 sites x 8192 copies): the two forms agree,
+    and the modal sum equals the harmonic energy sum_c (p.p + q.A.q) with
+    A = diag(m_x + 2) - periodic adjacency, since V is orthonormal."""
+    N = gpu
+    n, copies = 512, 8192
+    masses = N.ic_masses(n, 0.5, 1.5, 5)
+    om, V = N.eigensystem(masses + 2)
+    s0 = N.ic_eigenmodes(n, copies, om, V, n // 8, 0.5, 5)
+    s0 = s0 + 0.01 * np.random.default_rng(4).normal(size=s0.shape)
+    raws = {}
+    for name, p in (("mfma", N.PROJ_MFMA_F64), ("valu", N.PROJ_VALU_F64)):
+        with N.Ensemble(N.DDNKG, n, copies, 0.05, masses=masses, beta=1.0, projection=p) as ens:
+            ens.upload(s0)
+            raws[name] = ens.linenergies_raw()
+    q, pm = s0[..., 0], s0[..., 1]
+    harmonic = (pm * pm).sum() + ((masses + 2) * q * q).sum() - 2.0 * (q * np.roll(q, -1, axis=1)).sum()
+    banner(f"  mfma vs valu {rel_err(raws['mfma'], raws['valu']):.3e}; "
+           f"sum {raws['mfma'].sum():.15g} vs harmonic {harmonic:.15g}")
+    assert rel_err(raws["mfma"], raws["valu"]) <= 1e-12
+    assert abs(raws["mfma"].sum() - harmonic) <= 1e-11 * harmonic
+
+
+def test_ddnkg_autotune_reports_projection(gpu):
+    N = gpu
+    model, kw, masses, s0 = make_case(N, "dDNKG", 256, 300, 12)
+    with N.Ensemble(model, 256, 300, 0.05, masses=masses, **kw) as ens:
+        ens.upload(s0)
+        before = ens.linenergies_raw()
+        report = ens.autotune(5)
+        banner(f"autotune: {report}")
+        assert "projection: mfma" in report and "valu" in report
+        after = ens.linenergies_raw()
+        assert np.array_equal(ens.download(), s0)
+    assert rel_err(after, before) <= 1e-13
