@@ -291,7 +291,8 @@ def main():
     k_ms = float(np.mean(kernel_ms))
     flop = FLOP_PER_SITE_STEP[w["model"]] * sites * b
     achieved = flop / (k_ms * 1e-3) / 1e12
-    kname = "k_dnls_4096" if w["model"] == "DNLS" else ("k_step_ragged" if ragged else "k_step_wave")
+    kname = ("k_dnls_4096" if w["model"] == "DNLS" else "k_step_ragged" if "ragged_persistent" in desc
+             else "k_step_split" if "kernel=split" in desc else "k_step_wave")
     out = {
         "metric": METRIC,
         "value": value,

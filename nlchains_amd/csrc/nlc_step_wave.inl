@@ -24,9 +24,23 @@
 #include <algorithm>
 
 #include "nlc_internal.hpp"
+// after the HIP runtime header: nlc_exp.hpp uses its __device__ qualifiers
+#include "nlc_exp.hpp"
 
 namespace nlc {
 namespace {
+
+__constant__ double kExp2Tab[kExpTabSize] = NLC_EXP2_TABLE;
+
+// Toda kernels: copy the exp table into LDS (every thread of the workgroup
+// reaches the barrier).
+template <int MODEL>
+__device__ __forceinline__ void load_exp_table(double *lds) {
+	if constexpr (MODEL == TODA) {
+		if (threadIdx.x < kExpTabSize) lds[threadIdx.x] = kExp2Tab[threadIdx.x];
+		__syncthreads();
+	}
+}
 
 template <int CTRL>
 __device__ __forceinline__ double dpp_f64(double v) {
@@ -80,13 +94,13 @@ __device__ __forceinline__ double from_right(double v, int lane) {
 // site i is bond(x_{i+1}) - bond(x_i).
 //   FPUT: V'(x) = x + alpha x^2 + beta x^3  (Horner), the same force as
 //         (l+r-2c)(1+alpha(r-l)) + beta(d_r^3-d_l^3) of DNKG_FPUT_Toda.cu:41-58;
-//   Toda: e^{2 alpha x}/(2 alpha)  (DNKG_FPUT_Toda.cu:62-74; the constant
-//         -1/(2 alpha) of V' cancels in the difference).  FP64 exp of ROCm's
-//         device math library.
+//   Toda: e^{2 alpha x}, the force of DNKG_FPUT_Toda.cu:62-74 without its factor
+//         1/(2 alpha), which the kick folds into its weight (the constant
+//         -1/(2 alpha) of V' cancels in the difference).  FP64 table-driven exp.
 template <int MODEL>
-__device__ __forceinline__ double bond_force(double x, const StepArgs &a) {
+__device__ __forceinline__ double bond_force(double x, const StepArgs &a, const double *etab) {
 	if constexpr (MODEL == FPUT) return x * fma(x, fma(a.beta, x, a.alpha), 1.0);
-	else return a.a2i * exp(a.a2 * x);
+	else return exp_tab(a.a2 * x, etab);
 }
 
 template <int MODEL, int L, int S, bool EXACT>
@@ -95,6 +109,7 @@ struct Chain {
 	double mp[MODEL == DDNKG ? S : 1];
 	bool full; // lane holds S real sites (else S-1 and a dummy last slot)
 	int lane;
+	const double *etab; // Toda: exp table in LDS
 
 	__device__ __forceinline__ void drift(double c) {
 #pragma unroll
@@ -121,16 +136,17 @@ struct Chain {
 			if (!EXACT && !full) p[S - 1] = 0.0;
 		} else {
 			// bond terms rolled through the chain: only two are live at a time
-			const double b0 = bond_force<MODEL>(q[0] - hl, a);
+			const double w = (MODEL == TODA) ? d * a.a2i : d;
+			const double b0 = bond_force<MODEL>(q[0] - hl, a, etab);
 			const double bs = from_right<L>(b0, lane); // right bond of the lane's last site
 			double bl = b0;
 #pragma unroll
 			for (int j = 0; j < S; ++j) {
 				double br;
 				if (j == S - 1) br = bs;
-				else if (!EXACT && j == S - 2) br = full ? bond_force<MODEL>(q[j + 1] - q[j], a) : bs;
-				else br = bond_force<MODEL>(q[j + 1] - q[j], a);
-				p[j] = fma(d, br - bl, p[j]);
+				else if (!EXACT && j == S - 2) br = full ? bond_force<MODEL>(q[j + 1] - q[j], a, etab) : bs;
+				else br = bond_force<MODEL>(q[j + 1] - q[j], a, etab);
+				p[j] = fma(w, br - bl, p[j]);
 				bl = br;
 			}
 			if (!EXACT && !full) p[S - 1] = 0.0; // the dummy slot of a short lane
@@ -143,7 +159,7 @@ struct Chain {
 // the halo exchange even when the wave is only partly filled.
 template <int MODEL, int L, int S, bool EXACT>
 __device__ __forceinline__ void run_wave(double2 *__restrict__ chain0, uint32_t n, uint32_t live_chains, uint32_t steps,
-                                         const StepArgs &a, const double *__restrict__ mp2g) {
+                                         const StepArgs &a, const double *__restrict__ mp2g, const double *etab) {
 	const int lane = threadIdx.x & 63;
 	const int l = lane & (L - 1);
 	const uint32_t sub = uint32_t(lane / L);
@@ -151,6 +167,7 @@ __device__ __forceinline__ void run_wave(double2 *__restrict__ chain0, uint32_t 
 
 	Chain<MODEL, L, S, EXACT> ch;
 	ch.lane = lane;
+	ch.etab = etab;
 	const int full_lanes = EXACT ? L : int(n) - (S - 1) * L;
 	ch.full = EXACT || l < full_lanes;
 	const uint32_t off = ch.full ? uint32_t(l * S) : uint32_t(full_lanes * S + (l - full_lanes) * (S - 1));
@@ -198,10 +215,13 @@ __global__ void __launch_bounds__(256)
 k_step_wave(double2 *__restrict__ state, uint32_t copies, uint32_t n, uint32_t steps, StepArgs a,
             const double *__restrict__ mp2g) {
 	constexpr uint32_t CPW = 64 / L; // chains per wavefront
+	__shared__ double etab[MODEL == TODA ? kExpTabSize : 1];
+	load_exp_table<MODEL>(etab);
 	const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
 	const uint32_t first_chain = wave * CPW;
 	if (first_chain >= copies) return; // whole wavefront idle: uniform exit
-	run_wave<MODEL, L, S, EXACT>(state + size_t(first_chain) * n, n, min(CPW, copies - first_chain), steps, a, mp2g);
+	run_wave<MODEL, L, S, EXACT>(state + size_t(first_chain) * n, n, min(CPW, copies - first_chain), steps, a, mp2g,
+	                             etab);
 }
 
 template <int MODEL, int L, int S, bool EXACT>
@@ -224,19 +244,21 @@ template <int MODEL, int S>
 __global__ void __launch_bounds__(256)
 k_step_ragged(double2 *__restrict__ state, const RaggedItem *__restrict__ items, uint32_t nitems, uint32_t steps,
               StepArgs a) {
+	__shared__ double etab[MODEL == TODA ? kExpTabSize : 1];
+	load_exp_table<MODEL>(etab);
 	const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
 	const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
 	for (uint32_t i = wave; i < nitems; i += nwaves) {
 		const RaggedItem it = items[i];
 		double2 *c0 = state + it.site0;
 		switch (it.log2_lanes) {
-		case 0: run_wave<MODEL, 1, S, true>(c0, it.n, it.nchains, steps, a, nullptr); break;
-		case 1: run_wave<MODEL, 2, S, true>(c0, it.n, it.nchains, steps, a, nullptr); break;
-		case 2: run_wave<MODEL, 4, S, true>(c0, it.n, it.nchains, steps, a, nullptr); break;
-		case 3: run_wave<MODEL, 8, S, true>(c0, it.n, it.nchains, steps, a, nullptr); break;
-		case 4: run_wave<MODEL, 16, S, true>(c0, it.n, it.nchains, steps, a, nullptr); break;
-		case 5: run_wave<MODEL, 32, S, true>(c0, it.n, it.nchains, steps, a, nullptr); break;
-		default: run_wave<MODEL, 64, S, true>(c0, it.n, it.nchains, steps, a, nullptr); break;
+		case 0: run_wave<MODEL, 1, S, true>(c0, it.n, it.nchains, steps, a, nullptr, etab); break;
+		case 1: run_wave<MODEL, 2, S, true>(c0, it.n, it.nchains, steps, a, nullptr, etab); break;
+		case 2: run_wave<MODEL, 4, S, true>(c0, it.n, it.nchains, steps, a, nullptr, etab); break;
+		case 3: run_wave<MODEL, 8, S, true>(c0, it.n, it.nchains, steps, a, nullptr, etab); break;
+		case 4: run_wave<MODEL, 16, S, true>(c0, it.n, it.nchains, steps, a, nullptr, etab); break;
+		case 5: run_wave<MODEL, 32, S, true>(c0, it.n, it.nchains, steps, a, nullptr, etab); break;
+		default: run_wave<MODEL, 64, S, true>(c0, it.n, it.nchains, steps, a, nullptr, etab); break;
 		}
 	}
 }
