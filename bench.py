@@ -144,15 +144,14 @@ def copy_calibration(mib: int) -> None:
 
 
 def traffic_bytes(desc: str):
-    """HBM bytes per step launch for a kernel shape, from the calibrated
-    rocprofv3 FETCH_SIZE/WRITE_SIZE runs summarised in profiles/traffic.json."""
+    """HBM bytes per launch of the step kernel(s) for this kernel configuration:
+    rocprofv3 FETCH_SIZE and WRITE_SIZE counts scaled by a copy kernel of known
+    size, as recorded in profiles/traffic.json (None for an unprofiled shape)."""
     f = ROOT / "profiles" / "traffic.json"
     if not f.exists():
         return None
-    t = json.loads(f.read_text()).get(desc)
-    if not t:
-        return None
-    return (t["fetch_kib"] * t["fetch_scale"] + t["write_kib"] * t["write_scale"]) * 1024.0
+    t = json.loads(f.read_text())["kernels"].get(desc)
+    return t["hbm_bytes"] if t else None
 
 
 def cpu_baseline_ragged(w, lengths, s0, seconds_budget: float):

@@ -98,18 +98,27 @@ constexpr int kMfCols = 2;               // 16-mode column blocks per wavefront 
 constexpr int kMfWaves = 4;              // wavefronts per workgroup, side by side over modes
 constexpr int kMfModes = kMfWaves * 16 * kMfCols; // 128 modes per workgroup
 
-// Workgroup (x, y): copies [64 y, 64 y + 64), modes [128 x, 128 x + 128); wave w
-// of the group takes modes 128 x + 32 w + [0, 32).  Output: partial[y][j].
+// Workgroup b covers copies [64 y, 64 y + 64) and modes [128 x, 128 x + 128),
+// wave w of the group modes 128 x + 32 w + [0, 32); output partial[y][j].
+// XCD-aware numbering: workgroups go to the 8 XCDs round-robin (b mod 8), so
+// b = 8 (mode_blocks u + x) + c with y = 8 u + c puts every mode block of copy
+// block y on the same XCD at about the same time, and the copies' (q, p) rows
+// come from that XCD's L2 after the first read instead of once per mode block
+// from HBM.
 // Operand layout of v_mfma_f64_16x16x4f64 (A 16x4, B 4x16, D 16x16): lane l
 // supplies A[l % 16][l / 16] and B[l / 16][l % 16] and holds D[4 (l / 16) + r][l % 16]
 // in accumulator element r.
 __global__ void __launch_bounds__(256, 2)
 k_proj_mfma(const double2 *__restrict__ state, uint32_t n, uint32_t copies, const double *__restrict__ V,
             const double *__restrict__ omega, double *__restrict__ partial) {
+	const uint32_t mode_blocks = (n + kMfModes - 1) / kMfModes;
+	const uint32_t xcd = blockIdx.x & 7, rest = blockIdx.x >> 3;
+	const uint32_t bx = rest % mode_blocks, by = (rest / mode_blocks) * 8u + xcd;
+	if (by * 64u >= copies) return; // padding workgroup of the last group of 8 copy blocks
 	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 	const int r16 = lane & 15, g = lane >> 4;
-	const uint32_t c_base = blockIdx.y * 64u + r16;
-	const uint32_t j_base = blockIdx.x * uint32_t(kMfModes) + uint32_t(wave) * 32u + r16;
+	const uint32_t c_base = by * 64u + r16;
+	const uint32_t j_base = bx * uint32_t(kMfModes) + uint32_t(wave) * 32u + r16;
 
 	f64x4 acc_q[kMfRows][kMfCols], acc_p[kMfRows][kMfCols];
 #pragma unroll
@@ -117,8 +126,8 @@ k_proj_mfma(const double2 *__restrict__ state, uint32_t n, uint32_t copies, cons
 #pragma unroll
 		for (int cb = 0; cb < kMfCols; ++cb) acc_q[rb][cb] = acc_p[rb][cb] = f64x4{0.0, 0.0, 0.0, 0.0};
 
-	const bool full_rows = blockIdx.y * 64u + 64u <= copies;
-	const bool full_cols = blockIdx.x * uint32_t(kMfModes) + uint32_t(kMfModes) <= n;
+	const bool full_rows = by * 64u + 64u <= copies;
+	const bool full_cols = bx * uint32_t(kMfModes) + uint32_t(kMfModes) <= n;
 	for (uint32_t k0 = 0; k0 < n; k0 += 16) {
 		const uint32_t site = k0 + 4u * uint32_t(g);
 		const bool full_k = k0 + 16u <= n;
@@ -177,7 +186,7 @@ k_proj_mfma(const double2 *__restrict__ state, uint32_t n, uint32_t copies, cons
 		// exactly, so every lane of a group ends with the same value
 		sum += __shfl_xor(sum, 16, 64);
 		sum += __shfl_xor(sum, 32, 64);
-		if (g == 0 && j < n) partial[size_t(blockIdx.y) * n + j] = sum;
+		if (g == 0 && j < n) partial[size_t(by) * n + j] = sum;
 	}
 }
 
@@ -191,8 +200,10 @@ hipError_t launch_energy_ddnkg(hipStream_t st, const double2 *state, uint32_t n,
 		const dim3 grid((n + kTj - 1) / kTj, ddnkg_partial_rows(copies));
 		hipLaunchKernelGGL(k_proj_energy, grid, dim3(256), 0, st, state, n, copies, V, omega, partial);
 	} else {
-		const dim3 grid((n + kMfModes - 1) / kMfModes, ddnkg_partial_rows(copies));
-		hipLaunchKernelGGL(k_proj_mfma, grid, dim3(256), 0, st, state, n, copies, V, omega, partial);
+		const uint32_t mode_blocks = (n + kMfModes - 1) / kMfModes;
+		const uint32_t groups = (ddnkg_partial_rows(copies) + 7) / 8; // of 8 copy blocks, one per XCD
+		hipLaunchKernelGGL(k_proj_mfma, dim3(groups * 8 * mode_blocks), dim3(256), 0, st, state, n, copies, V, omega,
+		                   partial);
 	}
 	return hipGetLastError();
 }

@@ -12,12 +12,15 @@
 //  * k_dnls_4096: 256 threads x 16 complex values in VGPRs.  The 4096-point FFT
 //    is three radix-16 passes computed in registers (decimation in frequency);
 //    between passes the data moves through LDS: one cross-wavefront exchange
-//    (workgroup barrier) and one exchange within each wavefront.  The forward
-//    transform leaves the spectrum in digit-reversed order, the Fourier
-//    multiplier is read from a table stored in that order, and the inverse
-//    transform (decimation in time, conjugate twiddles) returns natural order,
-//    so no reordering pass is ever made.  Padded LDS layouts keep both
-//    exchanges free of bank conflicts.
+//    and one exchange within each wavefront.  The forward transform leaves the
+//    spectrum in digit-reversed order, the Fourier multiplier is read from a
+//    table stored in that order, and the inverse transform (decimation in
+//    time, conjugate twiddles) returns natural order, so no reordering pass is
+//    ever made.  LDS holds exactly the chain (64 KB): the in-wave exchange is
+//    a 16 x 16 transpose per 16-lane group inside the 1024 entries that the
+//    wavefront alone reads in the cross-wave exchange, XOR-swizzled so that
+//    neither exchange has bank conflicts, and a substep needs two workgroup
+//    barriers (one per cross-wave exchange).
 //  * k_dnls_lds: any power-of-two n <= 4096, chain in LDS, radix-4 Stockham
 //    passes (the inverse transform as conj(FFT(conj(.)))).
 //  * the nonlinear phase e^{-i theta} uses Taylor polynomials of sin and cos
@@ -197,8 +200,17 @@ __device__ __forceinline__ void twiddle_run(double2 v[16], const double2 *__rest
 }
 
 struct Lds4096 {
-	double2 a[4352]; // 4096 for the cross-wave exchange; 4 x 1088 padded for the in-wave one
+	double2 a[4096];
 };
+
+// In-wave exchange slot of element (row, col), row and col in 0..15, of the
+// 16 x 16 block of 16-lane group hi = t >> 4: region [256 hi, 256 hi + 256)
+// (wavefront w owns [1024 w, 1024 w + 1024), which is exactly what its lanes
+// read in the cross-wave exchange).  Entry index mod 16 = col ^ row: the 16
+// lanes of a group always touch 16 distinct bank quads (an entry spans 4 banks).
+__device__ __forceinline__ uint32_t xslot(uint32_t hi, uint32_t row, uint32_t col) {
+	return 256u * hi + 16u * row + (col ^ row);
+}
 
 __device__ __forceinline__ void wave_sync() {
 	__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -209,14 +221,19 @@ __device__ __forceinline__ void wave_sync() {
 // Forward FFT.  In: thread t = x0 + 16 x1 holds psi[t + 256 x2] at v[pos(x2)].
 // Out: thread t with k1 = t & 15, k0 = t >> 4 holds X[k0 + 16 k1 + 256 k2] at
 // v[pos(k2)].
+//
+// LDS hazards: exchange 1 writes only the slots t + 256 k0 that the same
+// thread read last (in exchange 1' of the previous inverse transform), and
+// every other write goes to the wavefront's own region, last read by that
+// wavefront; LDS operations of one wavefront complete in order.  So the only
+// workgroup barriers are the ones between the writes and the reads of the
+// cross-wave exchanges.
 __device__ __forceinline__ void fft4096_fwd(double2 v[16], Lds4096 &L, const double2 *__restrict__ tw) {
 	const uint32_t t = threadIdx.x, lo = t & 15, hi = t >> 4;
-	const uint32_t lane = t & 63, g = lane >> 4, wbase = (t >> 6) * 1088;
 	// stage 1: DFT over x2 (k0 natural), twiddle W4096^{(x0+16x1) k0}
 	dft16_t2n<-1>(v);
 	twiddle_run<false, true, false>(v, tw, 0, t);
 	// exchange 1 (cross-wave): C'[x0,x1,k0] at x0 + 16 x1 + 256 k0
-	__syncthreads();
 #pragma unroll
 	for (int k0 = 0; k0 < 16; ++k0) L.a[t + 256 * k0] = v[k0];
 	__syncthreads();
@@ -226,14 +243,14 @@ __device__ __forceinline__ void fft4096_fwd(double2 v[16], Lds4096 &L, const dou
 	// stage 2: DFT over x1 (k1 transposed), twiddle W256^{x0 k1} = W4096^{16 x0 k1}
 	dft16_n2t<-1>(v);
 	twiddle_run<false, true, true>(v, tw, 0, 16 * lo);
-	// exchange 2 (within the wave; its region overlaps exchange 1 of other waves)
-	__syncthreads();
+	// exchange 2 (within the 16-lane group): element (x0 = lo, k1)
+	wave_sync();
 #pragma unroll
-	for (int k1 = 0; k1 < 16; ++k1) L.a[wbase + g * 272 + lo + 17 * k1] = v[pos(k1)];
+	for (int k1 = 0; k1 < 16; ++k1) L.a[xslot(hi, lo, k1)] = v[pos(k1)];
 	wave_sync();
 	// thread (k1 = lo, k0 = hi) gathers x0
 #pragma unroll
-	for (int x0 = 0; x0 < 16; ++x0) v[x0] = L.a[wbase + g * 272 + x0 + 17 * lo];
+	for (int x0 = 0; x0 < 16; ++x0) v[x0] = L.a[xslot(hi, x0, lo)];
 	// stage 3: DFT over x0 (k2 transposed)
 	dft16_n2t<-1>(v);
 }
@@ -242,22 +259,21 @@ __device__ __forceinline__ void fft4096_fwd(double2 v[16], Lds4096 &L, const dou
 // v[pos(k2)] in, psi[t + 256 x2] at v[pos(x2)] out.
 __device__ __forceinline__ void fft4096_inv(double2 v[16], Lds4096 &L, const double2 *__restrict__ tw) {
 	const uint32_t t = threadIdx.x, lo = t & 15, hi = t >> 4;
-	const uint32_t lane = t & 63, g = lane >> 4, wbase = (t >> 6) * 1088;
 	// stage 3': over k2 -> x0 natural (thread k1 = lo), twiddle W256^{-x0 k1}
 	dft16_t2n<+1>(v);
 	twiddle_run<true, true, false>(v, tw, 0, 16 * lo);
-	// exchange 2' (within the wave)
+	// exchange 2' (within the 16-lane group): element (x0, k1 = lo)
 	wave_sync();
 #pragma unroll
-	for (int x0 = 0; x0 < 16; ++x0) L.a[wbase + g * 272 + x0 + 17 * lo] = v[x0];
+	for (int x0 = 0; x0 < 16; ++x0) L.a[xslot(hi, x0, lo)] = v[x0];
 	wave_sync();
 #pragma unroll
-	for (int k1 = 0; k1 < 16; ++k1) v[k1] = L.a[wbase + g * 272 + lo + 17 * k1];
+	for (int k1 = 0; k1 < 16; ++k1) v[k1] = L.a[xslot(hi, lo, k1)];
 	// stage 2': over k1 -> x1 transposed (thread x0 = lo, k0 = hi), twiddle W4096^{-(x0 + 16 x1) k0}
 	dft16_n2t<+1>(v);
 	twiddle_run<true, false, true>(v, tw, lo * hi, 16 * hi);
-	// exchange 1' (cross-wave)
-	__syncthreads();
+	// exchange 1' (cross-wave), written into the wavefront's own region
+	wave_sync();
 #pragma unroll
 	for (int x1 = 0; x1 < 16; ++x1) L.a[lo + 16 * x1 + 256 * hi] = v[pos(x1)];
 	__syncthreads();
