@@ -35,8 +35,16 @@ FP64_PEAK_TFLOPS = 78.6
 HBM_PEAK_GBS = 8000.0
 
 # algorithmic FLOP per site-step (FMA = 2, exp/sincos = 0): 7 kicks + 7 drifts;
-# DNLS: 7 x (two complex FFTs at 5 N log2 N / N each + 16 pointwise), N = 4096.
+# FPUT: 17 FLOP per substep in the form DNKG_FPUT_Toda.cu:33-50 writes, 13 for its
+# beta-only form when alpha = 0; DNLS: 7 x (two complex FFTs at 5 N log2 N / N
+# each + 16 pointwise), N = 4096.
 FLOP_PER_SITE_STEP = {"DNKG": 70, "dDNKG": 70, "FPUT": 119, "Toda": 56, "DNLS": 7 * (10 * 12 + 16)}
+
+
+def flop_per_site_step(w) -> int:
+    if w["model"] == "FPUT" and w["alpha"] == 0.0:
+        return 91
+    return FLOP_PER_SITE_STEP[w["model"]]
 
 WORKLOADS = {
     "dnkg": dict(model="DNKG", n=1024, copies=16384, dt=0.05, m=1.0, alpha=0.0, beta=1.0, k_lo=0, k_hi=128,
@@ -282,13 +290,21 @@ def main():
         for e_ in starts + stops:
             ev.destroy(e_)
         desc = ens.describe()
+        # host transfers of the whole shard over PCIe (a dump, an upload), for the
+        # PCIe-inclusive rate of a run that writes the shard after every batch
+        t = time.perf_counter()
+        host_state = ens.download()
+        download_ms = (time.perf_counter() - t) * 1e3
+        t = time.perf_counter()
+        ens.upload(host_state)
+        upload_ms = (time.perf_counter() - t) * 1e3
 
     total_steps = (args.warmup + args.steps) * b
     rel = np.abs(E1 - E0) / np.abs(E0)
     drift = float(np.mean(rel[np.isfinite(rel)])) if np.isfinite(rel).any() else float("nan")
     value = sites * b * args.steps / elapsed
     k_ms = float(np.mean(kernel_ms))
-    flop = FLOP_PER_SITE_STEP[w["model"]] * sites * b
+    flop = flop_per_site_step(w) * sites * b
     achieved = flop / (k_ms * 1e-3) / 1e12
     kname = ("k_dnls_4096" if w["model"] == "DNLS" else "k_step_ragged" if "ragged_persistent" in desc
              else "k_step_split" if "kernel=split" in desc else "k_step_wave")
@@ -316,6 +332,9 @@ def main():
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic_bytes(desc),
                      "kernel": kname + " (FP64 vector ALU bound)", "kernel_ms": k_ms, "flop_per_launch": flop},
     }
+    out["host_transfer"] = {"shard_bytes": int(host_state.nbytes), "download_ms": download_ms, "upload_ms": upload_ms,
+                            "value_with_shard_download_every_bench_step":
+                                sites * b / ((elapsed / args.steps * 1e3 + download_ms) * 1e-3)}
     e_ms = float(np.mean(energy_ms))
     out["energy_pass"] = {"ms": e_ms, "per_bench_step_fraction": e_ms / (elapsed / args.steps * 1e3)}
     if w["model"] == "dDNKG":
