@@ -69,4 +69,38 @@ NLC_EXP_QUAL double exp_tab(double y, const double *tab) {
 	return __builtin_ldexp(__builtin_fma(T, q, T), n >> 5);
 }
 
+// Table-free variant: y = N ln2 + r, |r| <= ln2/2, e^r by its degree-13 Taylor
+// polynomial (truncation below 5e-18 relative), 2^N by ldexp.  18 VALU and no
+// memory access: with the table, the scheduler hoists the table loads of a
+// whole unrolled run of bonds to hide their latency and the kernel loses
+// occupancy; this form keeps a chain of bonds in few registers.
+constexpr double kExpInvLn2 = 0x1.71547652b82fep+0; // 1/ln2
+constexpr double kExpLn2Hi = 0x1.62e42fefa39efp-1;  // ln2 rounded
+constexpr double kExpLn2Lo = 0x1.abc9e3b39803fp-56; // ln2 - kExpLn2Hi, rounded
+
+NLC_EXP_QUAL double exp_poly(double y) {
+	const double s = __builtin_fma(y, kExpInvLn2, kExpShifter);
+	const double nd = s - kExpShifter;
+	union {
+		double d;
+		unsigned long long u;
+	} bits = {s};
+	const int n = int(unsigned(bits.u));
+	double r = __builtin_fma(-nd, kExpLn2Hi, y);
+	r = __builtin_fma(-nd, kExpLn2Lo, r);
+	double t = __builtin_fma(r, 0x1.6124613a86d09p-33, 0x1.1eed8eff8d898p-29); // 1/13!, 1/12!
+	t = __builtin_fma(t, r, 0x1.ae64567f544e4p-26);                            // 1/11!
+	t = __builtin_fma(t, r, 0x1.27e4fb7789f5cp-22);                            // 1/10!
+	t = __builtin_fma(t, r, 0x1.71de3a556c734p-19);                            // 1/9!
+	t = __builtin_fma(t, r, 0x1.a01a01a01a01ap-16);                            // 1/8!
+	t = __builtin_fma(t, r, 0x1.a01a01a01a01ap-13);                            // 1/7!
+	t = __builtin_fma(t, r, 0x1.6c16c16c16c17p-10);                            // 1/6!
+	t = __builtin_fma(t, r, 0x1.1111111111111p-7);                             // 1/5!
+	t = __builtin_fma(t, r, 0x1.5555555555555p-5);                             // 1/4!
+	t = __builtin_fma(t, r, 0x1.5555555555555p-3);                             // 1/3!
+	t = __builtin_fma(t, r, 0.5);
+	t = __builtin_fma(t, r, 1.0);
+	return __builtin_ldexp(__builtin_fma(t, r, 1.0), n);
+}
+
 } // namespace nlc

@@ -96,10 +96,14 @@ __device__ __forceinline__ double from_right(double v, int lane) {
 //         (l+r-2c)(1+alpha(r-l)) + beta(d_r^3-d_l^3) of DNKG_FPUT_Toda.cu:41-58;
 //   Toda: e^{2 alpha x}, the force of DNKG_FPUT_Toda.cu:62-74 without its factor
 //         1/(2 alpha), which the kick folds into its weight (the constant
-//         -1/(2 alpha) of V' cancels in the difference).  FP64 table-driven exp.
-template <int MODEL>
+//         -1/(2 alpha) of V' cancels in the difference).  FP64 table-driven exp
+//         (fewest instructions); for S >= 25 sites per lane, where the state
+//         takes 100+ VGPRs and hoisted table loads would leave one wave
+//         per SIMD, the table-free form.
+template <int MODEL, int S>
 __device__ __forceinline__ double bond_force(double x, const StepArgs &a, const double *etab) {
 	if constexpr (MODEL == FPUT) return x * fma(x, fma(a.beta, x, a.alpha), 1.0);
+	else if constexpr (S >= 25) return exp_poly(a.a2 * x);
 	else return exp_tab(a.a2 * x, etab);
 }
 
@@ -137,15 +141,15 @@ struct Chain {
 		} else {
 			// bond terms rolled through the chain: only two are live at a time
 			const double w = (MODEL == TODA) ? d * a.a2i : d;
-			const double b0 = bond_force<MODEL>(q[0] - hl, a, etab);
+			const double b0 = bond_force<MODEL, S>(q[0] - hl, a, etab);
 			const double bs = from_right<L>(b0, lane); // right bond of the lane's last site
 			double bl = b0;
 #pragma unroll
 			for (int j = 0; j < S; ++j) {
 				double br;
 				if (j == S - 1) br = bs;
-				else if (!EXACT && j == S - 2) br = full ? bond_force<MODEL>(q[j + 1] - q[j], a, etab) : bs;
-				else br = bond_force<MODEL>(q[j + 1] - q[j], a, etab);
+				else if (!EXACT && j == S - 2) br = full ? bond_force<MODEL, S>(q[j + 1] - q[j], a, etab) : bs;
+				else br = bond_force<MODEL, S>(q[j + 1] - q[j], a, etab);
 				p[j] = fma(w, br - bl, p[j]);
 				bl = br;
 			}
