@@ -103,8 +103,8 @@ __device__ __forceinline__ double from_right(double v, int lane) {
 template <int MODEL, int S>
 __device__ __forceinline__ double bond_force(double x, const StepArgs &a, const double *etab) {
 	if constexpr (MODEL == FPUT) return x * fma(x, fma(a.beta, x, a.alpha), 1.0);
-	else if constexpr (S >= 25) return exp_poly(a.a2 * x);
-	else return exp_tab(a.a2 * x, etab);
+	else if constexpr (S >= 25) return exp_poly(x); // Toda: x is already 2 alpha (q_i - q_{i-1})
+	else return exp_tab(x, etab);
 }
 
 template <int MODEL, int L, int S, bool EXACT>
@@ -160,7 +160,9 @@ struct Chain {
 
 // One wavefront integrates `live_chains` (<= 64/L) consecutive chains of
 // length n starting at chain0 for `steps` steps.  All 64 lanes take part in
-// the halo exchange even when the wave is only partly filled.
+// the halo exchange even when the wave is only partly filled.  Toda keeps
+// u = 2 alpha q in registers (the exponent of its bond term is then a plain
+// difference) and drifts u with 2 alpha c; q = u / (2 alpha) is stored back.
 template <int MODEL, int L, int S, bool EXACT>
 __device__ __forceinline__ void run_wave(double2 *__restrict__ chain0, uint32_t n, uint32_t live_chains, uint32_t steps,
                                          const StepArgs &a, const double *__restrict__ mp2g, const double *etab) {
@@ -182,21 +184,23 @@ __device__ __forceinline__ void run_wave(double2 *__restrict__ chain0, uint32_t 
 		const bool real_site = EXACT || j < S - 1 || ch.full;
 		double2 v = make_double2(0.0, 0.0);
 		if (live && real_site) v = base[j];
-		ch.q[j] = v.x;
+		ch.q[j] = (MODEL == TODA) ? a.a2 * v.x : v.x;
 		ch.p[j] = v.y;
 		if constexpr (MODEL == DDNKG) ch.mp[j] = real_site ? mp2g[off + j] : 0.0;
 	}
 
-	const double dc0 = a.dc[0];
+	// Toda drifts u = 2 alpha q: its drift weights carry the factor 2 alpha
+	const double cs = (MODEL == TODA) ? a.a2 : 1.0;
+	const double dc0 = cs * a.dc[0];
 	ch.drift(dc0);
 	for (uint32_t s = 0; s < steps; ++s) {
-		const double c_last = (s + 1 < steps) ? 2.0 * dc0 : a.dc[7];
+		const double c_last = (s + 1 < steps) ? 2.0 * dc0 : cs * a.dc[7];
 		if constexpr (MODEL == TODA) {
 			// exp is long: keep the substep loop rolled to bound code size
 #pragma unroll 1
 			for (int k = 0; k < 7; ++k) {
 				ch.kick(a.dd[k], a);
-				ch.drift(k < 6 ? a.dc[k + 1] : c_last);
+				ch.drift(k < 6 ? cs * a.dc[k + 1] : c_last);
 			}
 		} else {
 #pragma unroll
@@ -210,7 +214,8 @@ __device__ __forceinline__ void run_wave(double2 *__restrict__ chain0, uint32_t 
 	if (live) {
 #pragma unroll
 		for (int j = 0; j < S; ++j)
-			if (EXACT || j < S - 1 || ch.full) base[j] = make_double2(ch.q[j], ch.p[j]);
+			if (EXACT || j < S - 1 || ch.full)
+				base[j] = make_double2((MODEL == TODA) ? ch.q[j] / a.a2 : ch.q[j], ch.p[j]);
 	}
 }
 
