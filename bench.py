@@ -251,7 +251,7 @@ def main():
     tune = None
     with ens:
         ens.upload(s0)
-        if mode == "batched" and w["model"] != "DNLS":
+        if mode == "batched":
             tune = ens.autotune(max(1, b // 20))
         desc = ens.describe()
         stream = ens.stream()
@@ -306,7 +306,8 @@ def main():
     k_ms = float(np.mean(kernel_ms))
     flop = flop_per_site_step(w) * sites * b
     achieved = flop / (k_ms * 1e-3) / 1e12
-    kname = ("k_dnls_4096" if w["model"] == "DNLS" else "k_step_ragged" if "ragged_persistent" in desc
+    kname = ("k_dnls_4096_r8" if "radix8" in desc else "k_dnls_4096" if w["model"] == "DNLS"
+             else "k_step_ragged" if "ragged_persistent" in desc
              else "k_step_split" if "kernel=split" in desc else "k_step_wave")
     out = {
         "metric": METRIC,

@@ -80,7 +80,8 @@ typedef struct nlc_params {
 	double beta;              /* --beta (DNKG, FPUT, dDNKG, DNLS) */
 	const double *masses;     /* dDNKG: m_x, chain_length doubles (content of the -m file); copied */
 	uint32_t lanes_per_chain; /* wave shape override (0 = default); used by the autotuner and tests */
-	uint32_t sites_per_lane;  /* wave shape override (0 = default) */
+	uint32_t sites_per_lane;  /* wave shape override (0 = default); DNLS with chain_length 4096:
+	                             16 or 8 complex values per thread (radix-16 or radix-8 kernel) */
 	int32_t projection;       /* nlc_projection (dDNKG only) */
 } nlc_params;
 

@@ -72,6 +72,7 @@ struct nlc_ctx {
 	double *split_q = nullptr, *split_p = nullptr;
 	double *energy_d = nullptr, *norm_d = nullptr;
 	double2 *dnls_table = nullptr; // DNLS Fourier multipliers
+	int dnls_variant = nlc::kDnlsRadix16; // n = 4096: register kernel, radix 16 or radix 8
 	double theta[9] = {};          // DNLS nonlinear angles
 	double *h_results = nullptr;   // pinned copy of results
 
@@ -298,13 +299,20 @@ int setup_dnls(nlc_ctx *c) {
 			const double ang = ((-p.dt) * nlc::kYoshidaD[k]) * omega[j];
 			tab[size_t(k) * n + j] = make_double2(std::cos(ang) * nrm, std::sin(ang) * nrm);
 		}
-	const double2 *src = tab.data();
+	// generic kernel: the 7 tables; n = 4096: the 4 distinct ones in the order of
+	// each register kernel, radix 16 at [0, 4n), radix 8 at [4n, 8n)
 	if (nlc::dnls_fast_path(n)) {
-		nlc::dnls_permute_tables(n, tab.data(), perm.data());
-		src = perm.data();
+		perm.resize(size_t(8) * n);
+		nlc::dnls_permute_tables(n, nlc::kDnlsRadix16, tab.data(), perm.data());
+		nlc::dnls_permute_tables(n, nlc::kDnlsRadix8, tab.data(), perm.data() + size_t(4) * n);
+		tab.swap(perm);
 	}
-	NLC_TRY(dev_alloc(c, &c->dnls_table, size_t(7) * n, "DNLS multipliers"));
-	NLC_HIP(c, hipMemcpy(c->dnls_table, src, sizeof(double2) * 7 * n, hipMemcpyHostToDevice));
+	// sites_per_lane selects the register kernel: values per thread, 16 or 8
+	if (p.sites_per_lane == 8 && nlc::dnls_fast_path(n)) c->dnls_variant = nlc::kDnlsRadix8;
+	else if (p.sites_per_lane == 0 || (p.sites_per_lane == 16 && nlc::dnls_fast_path(n))) c->dnls_variant = nlc::kDnlsRadix16;
+	else return c->fail(NLC_EUNSUPPORTED, "DNLS: sites_per_lane %u has no kernel for chain_length %u", p.sites_per_lane, n);
+	NLC_TRY(dev_alloc(c, &c->dnls_table, tab.size(), "DNLS multipliers"));
+	NLC_HIP(c, hipMemcpy(c->dnls_table, tab.data(), sizeof(double2) * tab.size(), hipMemcpyHostToDevice));
 	return NLC_OK;
 }
 
@@ -379,7 +387,9 @@ int select_ragged(nlc_ctx *c, int S) {
 int enqueue_move(nlc_ctx *c, uint32_t steps) {
 	if (!steps) return c->fail(NLC_EARG, "--kernel_batching must be a positive number");
 	if (c->p.model == NLC_DNLS) {
-		NLC_HIP(c, nlc::launch_dnls(c->stream, c->state, c->n, c->copies, steps, c->theta, c->tw, c->dnls_table));
+		const bool r8 = nlc::dnls_fast_path(c->n) && c->dnls_variant == nlc::kDnlsRadix8;
+		NLC_HIP(c, nlc::launch_dnls(c->stream, c->state, c->n, c->copies, steps, c->theta, c->tw,
+		                            c->dnls_table + (r8 ? size_t(4) * c->n : 0), c->dnls_variant));
 	} else if (c->ragged) {
 		if (c->ragged_launcher) {
 			NLC_HIP(c, c->ragged_launcher(c->stream, c->state, c->items, c->nitems, steps, c->args, 0));
@@ -804,23 +814,27 @@ int nlc_autotune(nlc_ctx *c, uint32_t trial_steps, char *report, size_t len) {
 		bool exact, split;
 		int ragged_S;
 		float ms;
+		int dnls;
 	};
 	std::vector<Cand> cands;
 	if (c->p.model == NLC_DNLS) {
-		c->tune_report = "DNLS: a single kernel, nothing to tune";
+		if (nlc::dnls_fast_path(c->n))
+			for (int v : {nlc::kDnlsRadix16, nlc::kDnlsRadix8}) cands.push_back({0, 0, false, false, 0, 0, v});
+		else
+			c->tune_report = "DNLS: a single kernel for this chain length, nothing to tune";
 	} else if (c->ragged) {
 		for (int S : {8, 16, 32})
-			if (ragged_fits(c->cls, S) && lookup_ragged(c->p.model, S)) cands.push_back({0, 0, false, false, S, 0});
+			if (ragged_fits(c->cls, S) && lookup_ragged(c->p.model, S)) cands.push_back({0, 0, false, false, S, 0, 0});
 		bool per_class = true; // one uniform kernel per length class, launched in sequence
 		for (const auto &k : c->cls) {
 			int L, S;
 			bool exact;
 			per_class = per_class && default_shape(k.n, L, S, exact) && lookup(c->p.model, L, S, exact);
 		}
-		if (per_class) cands.push_back({0, 0, false, false, 0, 0});
+		if (per_class) cands.push_back({0, 0, false, false, 0, 0, 0});
 	} else {
-		for (const Shape &s : wave_shapes(c->p.model, c->n)) cands.push_back({s.L, s.S, s.exact, false, 0, 0});
-		cands.push_back({0, 0, false, true, 0, 0});
+		for (const Shape &s : wave_shapes(c->p.model, c->n)) cands.push_back({s.L, s.S, s.exact, false, 0, 0, 0});
+		cands.push_back({0, 0, false, true, 0, 0, 0});
 	}
 	if (!cands.empty()) {
 		drop_graph(c);
@@ -834,7 +848,9 @@ int nlc_autotune(nlc_ctx *c, uint32_t trial_steps, char *report, size_t len) {
 		const nlc_params saved = c->p;
 		for (auto &cd : cands) {
 			if (rc) break;
-			if (c->ragged) {
+			if (c->p.model == NLC_DNLS) {
+				c->dnls_variant = cd.dnls;
+			} else if (c->ragged) {
 				rc = select_ragged(c, cd.ragged_S);
 			} else {
 				c->p.kernel = cd.split ? NLC_KERNEL_SPLIT : NLC_KERNEL_WAVE;
@@ -862,18 +878,25 @@ int nlc_autotune(nlc_ctx *c, uint32_t trial_steps, char *report, size_t len) {
 		const Cand *best = &cands[0];
 		for (const auto &cd : cands)
 			if (cd.ms < best->ms) best = &cd;
-		std::string rep = c->ragged ? "ragged sites-per-lane:" : "wave shapes (lanes x sites):";
+		std::string rep = c->p.model == NLC_DNLS ? "DNLS 4096-point kernels:"
+		                  : c->ragged            ? "ragged sites-per-lane:"
+		                                         : "wave shapes (lanes x sites):";
 		char buf[96];
 		for (const auto &cd : cands) {
 			const char *mark = &cd == best ? "*" : "";
-			if (c->ragged && cd.ragged_S) snprintf(buf, sizeof buf, " S=%d %.3f ms%s", cd.ragged_S, cd.ms, mark);
+			if (c->p.model == NLC_DNLS)
+				snprintf(buf, sizeof buf, " %s %.3f ms%s", cd.dnls == nlc::kDnlsRadix8 ? "radix8x512" : "radix16x256", cd.ms,
+				         mark);
+			else if (c->ragged && cd.ragged_S) snprintf(buf, sizeof buf, " S=%d %.3f ms%s", cd.ragged_S, cd.ms, mark);
 			else if (c->ragged) snprintf(buf, sizeof buf, " per-class %.3f ms%s", cd.ms, mark);
 			else if (cd.split) snprintf(buf, sizeof buf, " split %.3f ms%s", cd.ms, mark);
 			else snprintf(buf, sizeof buf, " %dx%d%s %.3f ms%s", cd.L, cd.S, cd.exact ? "" : "r", cd.ms, mark);
 			rep += buf;
 		}
 		c->tune_report = rep;
-		if (c->ragged) {
+		if (c->p.model == NLC_DNLS) {
+			c->dnls_variant = best->dnls;
+		} else if (c->ragged) {
 			NLC_TRY(select_ragged(c, best->ragged_S));
 		} else {
 			c->p.kernel = best->split ? NLC_KERNEL_SPLIT : NLC_KERNEL_WAVE;
@@ -900,7 +923,9 @@ int nlc_describe(const nlc_ctx *c, char *buf, size_t len) {
 	static const char *names[] = {"DNKG", "FPUT", "Toda", "dDNKG", "DNLS"};
 	if (c->p.model == NLC_DNLS)
 		snprintf(buf, len, "model=DNLS n=%u copies=%u kernel=%s", c->n, c->copies,
-		         nlc::dnls_fast_path(c->n) ? "dnls_radix16_registers" : "dnls_lds_stockham");
+		         !nlc::dnls_fast_path(c->n)                ? "dnls_lds_stockham"
+		         : c->dnls_variant == nlc::kDnlsRadix8 ? "dnls_radix8_registers"
+		                                                : "dnls_radix16_registers");
 	else if (c->ragged)
 		snprintf(buf, len, "model=%s ragged chains=%u classes=%zu sites=%llu kernel=%s sites_per_lane=%d items=%u",
 		         names[c->p.model], c->copies, c->cls.size(), (unsigned long long) c->total_sites,

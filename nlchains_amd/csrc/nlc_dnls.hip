@@ -319,6 +319,225 @@ k_dnls_4096(double2 *__restrict__ state, uint32_t copies, uint32_t steps, DnlsAr
 	for (int j = 0; j < 16; ++j) psi[t + 256 * j] = nonlinear(v[pos(j)], a.theta[7]);
 }
 
+// ---------------------------------------------------------------- radix 8
+//
+// k_dnls_4096_r8: 512 threads x 8 complex values, 4096 = 8^4 in four radix-8
+// passes.  Same DFT and twiddle work per point as the radix-16 kernel, half
+// the values per thread, so about half the VGPRs and twice the wavefronts per
+// SIMD to hide L2 and LDS latency, for one more LDS exchange per transform.
+// Digits: x = x0 + 8 x1 + 64 x2 + 512 x3, k = k0 + 8 k1 + 64 k2 + 512 k3;
+// thread t = lane + 64 w.
+
+constexpr int kT8 = 512;
+
+// position of element k of an 8-point transform in "transposed" order
+__host__ __device__ constexpr int pos8(int k) { return 4 * (k & 1) + (k >> 1); }
+
+// z * exp(SIGN 2 pi i j / 8), j = 0..3
+template <int SIGN, int J>
+__device__ __forceinline__ double2 w8mul(double2 z) {
+	constexpr double h = 0.70710678118654752440;
+	if constexpr (J == 0) return z;
+	else if constexpr (J == 2) return SIGN < 0 ? make_double2(z.y, -z.x) : make_double2(-z.y, z.x);
+	else if constexpr (J == 1) {
+		return SIGN < 0 ? make_double2(h * (z.x + z.y), h * (z.y - z.x)) : make_double2(h * (z.x - z.y), h * (z.x + z.y));
+	} else {
+		return SIGN < 0 ? make_double2(h * (z.y - z.x), -h * (z.x + z.y)) : make_double2(-h * (z.x + z.y), h * (z.x - z.y));
+	}
+}
+
+// 8-point DFT, natural in -> transposed out: radix-2 butterflies (j, j + 4)
+// with twiddles W8^j, then DFT4 of the sums (even outputs, registers 0..3)
+// and of the differences (odd outputs, registers 4..7).
+template <int SIGN>
+__device__ __forceinline__ void dft8_n2t(double2 v[8]) {
+	double2 d[4];
+#pragma unroll
+	for (int j = 0; j < 4; ++j) {
+		d[j] = csub(v[j], v[j + 4]);
+		v[j] = cadd(v[j], v[j + 4]);
+	}
+	v[4] = d[0];
+	v[5] = w8mul<SIGN, 1>(d[1]);
+	v[6] = w8mul<SIGN, 2>(d[2]);
+	v[7] = w8mul<SIGN, 3>(d[3]);
+	dft4<SIGN>(v[0], v[1], v[2], v[3]);
+	dft4<SIGN>(v[4], v[5], v[6], v[7]);
+}
+
+// transposed in -> natural out (decimation in time): DFT4 of the even and of
+// the odd elements, twiddle the odd ones by W8^j, butterflies (j, j + 4).
+template <int SIGN>
+__device__ __forceinline__ void dft8_t2n(double2 v[8]) {
+	dft4<SIGN>(v[0], v[1], v[2], v[3]);
+	dft4<SIGN>(v[4], v[5], v[6], v[7]);
+	const double2 o[4] = {v[4], w8mul<SIGN, 1>(v[5]), w8mul<SIGN, 2>(v[6]), w8mul<SIGN, 3>(v[7])};
+#pragma unroll
+	for (int j = 0; j < 4; ++j) {
+		v[j + 4] = csub(v[j], o[j]);
+		v[j] = cadd(v[j], o[j]);
+	}
+}
+
+// v[at(j)] *= W^{e0 + j de}, j = 0..7, W = exp(-2 pi i/4096) = tw[1] (conjugated
+// when CONJ), at(j) = pos8(j) when TRANSPOSED; powers refreshed from the table
+// at j = 0 and 4.  ONE0 asserts e0 == 0.
+template <bool CONJ, bool ONE0, bool TRANSPOSED>
+__device__ __forceinline__ void twiddle_run8(double2 v[8], const double2 *__restrict__ tw, uint32_t e0, uint32_t de) {
+	const double2 w = tw[de & 4095u];
+	const double2 wc = CONJ ? cconj(w) : w;
+#pragma unroll
+	for (int q = 0; q < 2; ++q) {
+		double2 r = tw[(e0 + uint32_t(4 * q) * de) & 4095u];
+		if (CONJ) r = cconj(r);
+#pragma unroll
+		for (int j = 0; j < 4; ++j) {
+			const int idx = 4 * q + j;
+			const int at = TRANSPOSED ? pos8(idx) : idx;
+			if (idx || !ONE0) v[at] = cmul(v[at], r);
+			if (j < 3) r = cmul(r, wc);
+		}
+	}
+}
+
+// LDS layout of the radix-8 kernel: wavefront w owns the 640 entries from
+// kR8Region w (dynamic LDS, 8 x 640 x 16 B = 80 KB per chain, two chains per
+// CU).  The cross-wave exchange puts element (t, k0) at kR8Region k0 + t, so
+// the wavefront w = k0 that gathers it reads only its own region; the two
+// in-wave exchanges are 8 x 8 transposes per group of 8 lanes through padded
+// rows (stride 9) inside the same region.  Every slot is a per-lane base plus
+// a compile-time offset (no per-register address arithmetic), and 16
+// consecutive lanes touch distinct bank quads except for one pair in the
+// read of exchange 2.
+//   exchange 2: element (x0, x1, k1) at 9 (x0 + 8 x1) + k1, written by lane
+//               x0 + 8 x1 (offset k1), read by lane x0 + 8 k1 (offset 72 x1)
+//   exchange 3: element (x0, k1, k2) at 72 k1 + 9 x0 + k2, written by lane
+//               x0 + 8 k1 (offset k2), read by lane k2 + 8 k1 (offset 9 x0)
+constexpr uint32_t kR8Region = 640;
+constexpr uint32_t kR8Lds = 8 * kR8Region * sizeof(double2);
+
+// Forward FFT.  In: thread t holds psi[t + 512 x3] at v[pos8(x3)].  Out: thread
+// t = k2 + 8 k1 + 64 k0 holds X[k0 + 8 k1 + 64 k2 + 512 k3] at v[pos8(k3)].
+// LDS hazards as in fft4096_fwd: exchange 1 writes only slots the same thread
+// read last, every other write lands in the wavefront's own region.
+__device__ __forceinline__ void fft4096_fwd8(double2 v[8], double2 *L, const double2 *__restrict__ tw) {
+	const uint32_t t = threadIdx.x, lane = t & 63, w = t >> 6, lo = lane & 7, hi = lane >> 3;
+	double2 *own = L + kR8Region * w;
+	// A: over x3 -> k0 (natural), twiddle W^{t k0}
+	dft8_t2n<-1>(v);
+	twiddle_run8<false, true, false>(v, tw, 0, t);
+	// exchange 1 (cross-wave); thread (lane, w = k0) gathers x2
+	double2 *x1w = L + t;
+#pragma unroll
+	for (int k0 = 0; k0 < 8; ++k0) x1w[kR8Region * k0] = v[k0];
+	__syncthreads();
+	const double2 *x1r = own + lane;
+#pragma unroll
+	for (int x2 = 0; x2 < 8; ++x2) v[x2] = x1r[64 * x2];
+	// B: over x2 -> k1 (transposed), twiddle W^{8 (x0 + 8 x1) k1}, x0 + 8 x1 = lane
+	dft8_n2t<-1>(v);
+	twiddle_run8<false, true, true>(v, tw, 0, 8 * lane);
+	// exchange 2 (in-wave): lane (x0 = lo, x1 = hi) -> lane (x0 = lo, k1 = hi)
+	wave_sync();
+	double2 *x2w = own + 9 * lane;
+#pragma unroll
+	for (int k1 = 0; k1 < 8; ++k1) x2w[k1] = v[pos8(k1)];
+	wave_sync();
+	const double2 *x2r = own + 9 * lo + hi;
+#pragma unroll
+	for (int x1 = 0; x1 < 8; ++x1) v[x1] = x2r[72 * x1];
+	// C: over x1 -> k2 (transposed), twiddle W^{64 x0 k2}
+	dft8_n2t<-1>(v);
+	twiddle_run8<false, true, true>(v, tw, 0, 64 * lo);
+	// exchange 3 (in-wave): lane (x0 = lo, k1 = hi) -> lane (k2 = lo, k1 = hi)
+	wave_sync();
+	double2 *x3w = own + 72 * hi + 9 * lo;
+#pragma unroll
+	for (int k2 = 0; k2 < 8; ++k2) x3w[k2] = v[pos8(k2)];
+	wave_sync();
+	const double2 *x3r = own + 72 * hi + lo;
+#pragma unroll
+	for (int x0 = 0; x0 < 8; ++x0) v[x0] = x3r[9 * x0];
+	// D: over x0 -> k3 (transposed)
+	dft8_n2t<-1>(v);
+}
+
+// Inverse (unnormalised) FFT, the exact reverse of fft4096_fwd8.
+__device__ __forceinline__ void fft4096_inv8(double2 v[8], double2 *L, const double2 *__restrict__ tw) {
+	const uint32_t t = threadIdx.x, lane = t & 63, w = t >> 6, lo = lane & 7, hi = lane >> 3;
+	double2 *own = L + kR8Region * w;
+	// D': over k3 -> x0 (natural), twiddle W^{-64 x0 k2}, k2 = lo
+	dft8_t2n<+1>(v);
+	twiddle_run8<true, true, false>(v, tw, 0, 64 * lo);
+	// exchange 3': lane (k2 = lo, k1 = hi) -> lane (x0 = lo, k1 = hi)
+	wave_sync();
+	double2 *x3w = own + 72 * hi + lo;
+#pragma unroll
+	for (int x0 = 0; x0 < 8; ++x0) x3w[9 * x0] = v[x0];
+	wave_sync();
+	const double2 *x3r = own + 72 * hi + 9 * lo;
+#pragma unroll
+	for (int k2 = 0; k2 < 8; ++k2) v[pos8(k2)] = x3r[k2];
+	// C': over k2 -> x1 (natural), twiddle W^{-8 (x0 + 8 x1) k1}, x0 = lo, k1 = hi
+	dft8_t2n<+1>(v);
+	twiddle_run8<true, false, false>(v, tw, 8 * lo * hi, 64 * hi);
+	// exchange 2': lane (x0 = lo, k1 = hi) -> lane (x0 = lo, x1 = hi)
+	wave_sync();
+	double2 *x2w = own + 9 * lo + hi;
+#pragma unroll
+	for (int x1 = 0; x1 < 8; ++x1) x2w[72 * x1] = v[x1];
+	wave_sync();
+	const double2 *x2r = own + 9 * lane;
+#pragma unroll
+	for (int k1 = 0; k1 < 8; ++k1) v[pos8(k1)] = x2r[k1];
+	// B': over k1 -> x2 (natural), twiddle W^{-(lane + 64 x2) k0}, k0 = w
+	dft8_t2n<+1>(v);
+	twiddle_run8<true, false, false>(v, tw, lane * w, 64 * w);
+	// exchange 1' (cross-wave), written into the wavefront's own region
+	wave_sync();
+	double2 *x1w = own + lane;
+#pragma unroll
+	for (int x2 = 0; x2 < 8; ++x2) x1w[64 * x2] = v[x2];
+	__syncthreads();
+	const double2 *x1r = L + t;
+#pragma unroll
+	for (int k0 = 0; k0 < 8; ++k0) v[k0] = x1r[kR8Region * k0];
+	// A': over k0 -> x3 (transposed)
+	dft8_n2t<+1>(v);
+}
+
+__global__ void __launch_bounds__(kT8, 4)
+k_dnls_4096_r8(double2 *__restrict__ state, uint32_t copies, uint32_t steps, DnlsArgs a,
+               const double2 *__restrict__ tw, const double2 *__restrict__ mult) {
+	extern __shared__ double2 lds8[]; // kR8Lds bytes
+	const uint32_t chain = blockIdx.x, t = threadIdx.x;
+	double2 *psi = state + size_t(chain) * 4096;
+	double2 v[8]; // psi[t + 512 j] at v[pos8(j)]
+#pragma unroll
+	for (int j = 0; j < 8; ++j) v[pos8(j)] = psi[t + 512 * j];
+	for (uint32_t s = 0; s < steps; ++s) {
+#pragma unroll 1
+		for (int k = 0; k < 7; ++k) {
+			const double th = (k == 0 && s) ? a.theta[8] : a.theta[k];
+			const double2 *twk = tw;
+			asm volatile("" : "+s"(twk));
+#pragma unroll
+			for (int j = 0; j < 8; ++j) v[j] = nonlinear(v[j], th);
+			fft4096_fwd8(v, lds8, twk);
+			// Fourier multiplier, stored per thread in the digit-reversed order;
+			// the fence keeps its loads out of the transform (4 wavefronts per
+			// SIMD hide their latency, registers are the scarce resource here)
+			asm volatile("" ::: "memory");
+			const double2 *m = mult + size_t(k < 4 ? k : 6 - k) * 4096 + t;
+#pragma unroll
+			for (int j = 0; j < 8; ++j) v[pos8(j)] = cmul(v[pos8(j)], m[512 * j]);
+			fft4096_inv8(v, lds8, twk);
+		}
+	}
+#pragma unroll
+	for (int j = 0; j < 8; ++j) psi[t + 512 * j] = nonlinear(v[pos8(j)], a.theta[7]);
+}
+
 // ------------------------------------------------------------ generic n
 
 constexpr int kMaxPer = 4; // radix-4 butterflies per thread per pass (n <= 4096)
@@ -413,24 +632,36 @@ k_dnls_lds(double2 *__restrict__ state, uint32_t n, uint32_t copies, uint32_t st
 
 // Fourier multiplier tables (host side).  table[k][j] = T_k[j] for the generic
 // kernel; for n = 4096 the 4 distinct tables (d_k = d_{6-k}) are stored in the
-// order the radix-16 kernel reads them: entry [k][256 j + t] = T_k[b(t) + 256 j]
-// with b(t) = (t >> 4) + 16 (t & 15).
+// order the register kernels read them:
+//   radix 16: entry [k][256 j + t] = T_k[(t >> 4) + 16 (t & 15) + 256 j],
+//   radix 8:  entry [k][512 j + t] = T_k[(t >> 6) + 8 ((t >> 3) & 7) + 64 (t & 7) + 512 j].
 bool dnls_fast_path(uint32_t n) { return n == 4096; }
 
-void dnls_permute_tables(uint32_t n, const double2 *table, double2 *out) {
+void dnls_permute_tables(uint32_t n, int variant, const double2 *table, double2 *out) {
 	for (int k = 0; k < 4; ++k)
-		for (uint32_t j = 0; j < 16; ++j)
-			for (uint32_t t = 0; t < 256; ++t) {
-				const uint32_t b = (t >> 4) + 16 * (t & 15);
-				out[size_t(k) * n + 256 * j + t] = table[size_t(k) * n + b + 256 * j];
+		for (uint32_t e = 0; e < n; ++e) {
+			uint32_t src;
+			if (variant == kDnlsRadix8) {
+				const uint32_t j = e >> 9, t = e & 511;
+				src = (t >> 6) + 8 * ((t >> 3) & 7) + 64 * (t & 7) + 512 * j;
+			} else {
+				const uint32_t j = e >> 8, t = e & 255;
+				src = (t >> 4) + 16 * (t & 15) + 256 * j;
 			}
+			out[size_t(k) * n + e] = table[size_t(k) * n + src];
+		}
 }
 
 hipError_t launch_dnls(hipStream_t st, double2 *state, uint32_t n, uint32_t copies, uint32_t steps,
-                       const double *theta9, const double2 *tw, const double2 *table) {
+                       const double *theta9, const double2 *tw, const double2 *table, int variant) {
 	DnlsArgs a;
 	for (int i = 0; i < 9; ++i) a.theta[i] = theta9[i];
-	if (dnls_fast_path(n)) {
+	if (dnls_fast_path(n) && variant == kDnlsRadix8) {
+		hipError_t e = hipFuncSetAttribute((const void *) k_dnls_4096_r8, hipFuncAttributeMaxDynamicSharedMemorySize,
+		                                   int(kR8Lds));
+		if (e != hipSuccess) return e;
+		hipLaunchKernelGGL(k_dnls_4096_r8, dim3(copies), dim3(kT8), kR8Lds, st, state, copies, steps, a, tw, table);
+	} else if (dnls_fast_path(n)) {
 		hipLaunchKernelGGL(k_dnls_4096, dim3(copies), dim3(kT), 0, st, state, copies, steps, a, tw, table);
 	} else {
 		const size_t lds = sizeof(double2) * n;

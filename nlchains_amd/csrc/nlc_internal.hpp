@@ -98,10 +98,12 @@ extern const double kYoshidaC[8], kYoshidaD[8];
 namespace nlc {
 // DNLS split step (nlc_dnls.hip).  theta9[k] = beta dt c_k (k <= 7),
 // theta9[8] = beta dt 2 c_0; table: 7 Fourier multipliers of n entries, or for
-// the radix-16 path (dnls_fast_path) the 4 distinct ones permuted by
-// dnls_permute_tables.
+// the register kernels (dnls_fast_path: n = 4096, radix 16 with 256 threads or
+// radix 8 with 512 threads per chain) the 4 distinct ones permuted by
+// dnls_permute_tables for that variant.
+constexpr int kDnlsRadix16 = 0, kDnlsRadix8 = 1;
 bool dnls_fast_path(uint32_t n);
-void dnls_permute_tables(uint32_t n, const double2 *table, double2 *out);
+void dnls_permute_tables(uint32_t n, int variant, const double2 *table, double2 *out);
 hipError_t launch_dnls(hipStream_t, double2 *state, uint32_t n, uint32_t copies, uint32_t steps,
-                       const double *theta9, const double2 *tw, const double2 *table);
+                       const double *theta9, const double2 *tw, const double2 *table, int variant);
 } // namespace nlc

@@ -22,8 +22,9 @@ def rel_err(got, want):
     return np.abs(got - want).max() / np.abs(want).max()
 
 
-def run(N, n, copies, steps, dt, beta, s0):
-    with N.Ensemble(N.DNLS, n, copies, dt, beta=beta) as ens:
+def run(N, n, copies, steps, dt, beta, s0, spl=0):
+    """spl: values per thread of the 4096-point register kernel (16 or 8; 0 = default)."""
+    with N.Ensemble(N.DNLS, n, copies, dt, beta=beta, sites_per_lane=spl) as ens:
         ens.upload(s0)
         ens.move(steps)
         got = ens.download()
@@ -46,7 +47,25 @@ def test_dnls_move_parity(gpu, n, copies, steps):
     np.testing.assert_allclose(O.dnls_norm(got), O.dnls_norm(s0), rtol=1e-12)
 
 
-def test_dnls_large_angles_use_library_sincos(gpu):
+@pytest.mark.parametrize("spl", [16, 8])
+def test_dnls_4096_register_kernels(gpu, spl):
+    """Both 4096-point register kernels (radix 16 x 256 threads, radix 8 x 512
+    threads) against the oracle, and against each other."""
+    N = gpu
+    n, copies, steps, dt, beta = 4096, 5, 100, 0.01, 1.0
+    s0 = N.ic_dnls(n, copies, 256, 1.0, 7)
+    got, desc = run(N, n, copies, steps, dt, beta, s0, spl)
+    banner(f"  {desc}")
+    assert ("radix8" in desc) == (spl == 8)
+    want = O.move_dnls(s0, steps, dt, beta)
+    assert np.isfinite(got).all() and rel_err(got, want) <= 1e-10
+    np.testing.assert_allclose(O.dnls_norm(got), O.dnls_norm(s0), rtol=1e-12)
+    other, _ = run(N, n, copies, steps, dt, beta, s0, 24 - spl)
+    assert rel_err(got, other) <= 1e-12
+
+
+@pytest.mark.parametrize("spl", [16, 8])
+def test_dnls_large_angles_use_library_sincos(gpu, spl):
     """beta large enough that |theta| > pi/4 at many sites: the library sincos
     branch of the nonlinear rotation is exercised."""
     N = gpu
@@ -54,16 +73,17 @@ def test_dnls_large_angles_use_library_sincos(gpu):
     s0 = N.ic_dnls(n, copies, 64, 1.0, 11)
     assert (beta * dt * 2 * 0.3922568 * (s0 ** 2).sum(-1)).max() > np.pi / 4
     # strongly nonlinear, so round-off grows fast: few steps suffice to test the branch
-    got, _ = run(N, n, copies, 3, dt, beta, s0)
+    got, _ = run(N, n, copies, 3, dt, beta, s0, spl)
     want = O.move_dnls(s0, 3, dt, beta)
     assert rel_err(got, want) <= 1e-12
 
 
-def test_dnls_batches_compose(gpu):
+@pytest.mark.parametrize("spl", [16, 8])
+def test_dnls_batches_compose(gpu, spl):
     N = gpu
     s0 = N.ic_dnls(4096, 2, 256, 1.0, 5)
-    a, _ = run(N, 4096, 2, 30, 0.01, 1.0, s0)
-    with N.Ensemble(N.DNLS, 4096, 2, 0.01, beta=1.0) as ens:
+    a, _ = run(N, 4096, 2, 30, 0.01, 1.0, s0, spl)
+    with N.Ensemble(N.DNLS, 4096, 2, 0.01, beta=1.0, sites_per_lane=spl) as ens:
         ens.upload(s0)
         ens.move(10)
         ens.move(20)
@@ -88,14 +108,15 @@ def test_dnls_linenergies(gpu, n, copies):
     np.testing.assert_allclose(norms, O.dnls_norm(s0), rtol=1e-12)
 
 
-def test_dnls_full_size_norm_and_sampled_parity(gpu):
+@pytest.mark.parametrize("spl", [16, 8])
+def test_dnls_full_size_norm_and_sampled_parity(gpu, spl):
     """4096 sites x 4096 copies: norm conservation (both substeps are unitary),
     and a sample of chains re-run on the oracle."""
     N = gpu
     n, copies, steps, dt = 4096, 4096, 50, 0.01
-    banner("full-size DNLS 4096 x 4096")
+    banner(f"full-size DNLS 4096 x 4096, {spl} values per thread")
     s0 = N.ic_dnls(n, copies, 256, 1.0, 3)
-    with N.Ensemble(N.DNLS, n, copies, dt, beta=1.0) as ens:
+    with N.Ensemble(N.DNLS, n, copies, dt, beta=1.0, sites_per_lane=spl) as ens:
         ens.upload(s0)
         E0, n0 = ens.chain_energies()
         ens.move(steps)
@@ -108,3 +129,18 @@ def test_dnls_full_size_norm_and_sampled_parity(gpu):
     E1o = O.chain_energy(O.DNLS, want, beta=1.0)
     np.testing.assert_allclose(E1[pick], E1o, rtol=1e-11)
     banner(f"  norm drift {np.abs(n1 / n0 - 1).max():.2e}, energy error {np.abs(E1 / E0 - 1).mean():.2e}")
+
+
+def test_dnls_autotune_and_unsupported_shapes(gpu):
+    N = gpu
+    s0 = N.ic_dnls(4096, 64, 256, 1.0, 2)
+    with N.Ensemble(N.DNLS, 4096, 64, 0.01, beta=1.0) as ens:
+        ens.upload(s0)
+        report = ens.autotune(5)
+        banner(f"autotune: {report} -> {ens.describe()}")
+        assert "radix16x256" in report and "radix8x512" in report and "*" in report
+        assert np.array_equal(ens.download(), s0)
+    with pytest.raises(N.NlcError):
+        N.Ensemble(N.DNLS, 4096, 4, 0.01, beta=1.0, sites_per_lane=4)
+    with pytest.raises(N.NlcError):
+        N.Ensemble(N.DNLS, 1024, 4, 0.01, beta=1.0, sites_per_lane=8)
