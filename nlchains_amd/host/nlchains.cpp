@@ -11,6 +11,7 @@
 // operation goes through the C ABI of include/nlc.h.
 #include <unistd.h>
 
+#include <algorithm>
 #include <atomic>
 #include <cmath>
 #include <csignal>
@@ -54,7 +55,7 @@ struct help_quit {
 
 // Common options (common/main.cpp:53-70) and the parsed configuration.
 struct Config {
-	bool verbose = false;
+	bool verbose = false, autotune = false;
 	std::string initial, entropymask, prefix;
 	uint32_t n = 0, copies = 0, batch = 0, dump_interval = 0;
 	double dt = 0;
@@ -81,7 +82,8 @@ void add_common(Options &o) {
 	    .add("dump_interval", 0, true, false,
 	         "number of steps between full state dumps (defaults to same value as --batch, does not affect time "
 	         "granularity of entropy)")
-	    .add("kernel_shape", 0, true, false, "wave shape LANESxSITES of the step kernel (default: automatic)");
+	    .add("kernel_shape", 0, true, false, "wave shape LANESxSITES of the step kernel (default: automatic)")
+	    .add("autotune", 0, false, false, "time the step kernel forms on the initial state and keep the fastest");
 }
 
 // Validation and input reading: common/main.cpp:72-134 (single process, so
@@ -91,6 +93,7 @@ void run_common(Options &o, int argc, char **argv, Config &c) {
 	try {
 		o.parse(argc, argv);
 		c.verbose = o.count("verbose");
+		c.autotune = o.count("autotune");
 		c.n = o.get<uint32_t>("chain_length");
 		c.copies = o.get<uint32_t>("copies");
 		c.dt = o.get<double>("dt");
@@ -236,16 +239,23 @@ int run_loop(Config &c, nlc_params p, bool split_kernel, const std::string &shap
 		ctx.check(nlc_download(ctx.ctx, shard.data()), "download");
 		res.write_shard(tt, std::move(shard));
 	};
+	if (c.autotune) {
+		char report[1024];
+		ctx.check(nlc_autotune(ctx.ctx, std::max<uint32_t>(1, c.batch / 10), report, sizeof report), "autotune");
+		if (c.verbose) std::cerr << process_ident() << ": autotune: " << report << std::endl;
+	}
 	bool synched = false;
+	// energies at the start; afterwards every batch is one replayed hipGraph
+	// (move, energy pass, entropies, result copy)
+	ctx.check(nlc_linenergies(ctx.ctx, 0.0, e.data(), s), "linenergies");
 	while (true) {
-		ctx.check(nlc_linenergies(ctx.ctx, 0.0, e.data(), s), "linenergies");
 		res.write_entropy(t, s);
 		res.check_entropy(s);
 		if (t % c.dump_interval == 0) dump(t);
 		// loop_control::break_now: on a quit request the end step becomes the current one
 		if (quit_requested && !synched) c.steps = t, synched = true;
 		if (c.steps == t && (synched || c.steps)) break;
-		ctx.check(nlc_move(ctx.ctx, c.batch), "move");
+		ctx.check(nlc_batch(ctx.ctx, c.batch, 0.0, e.data(), s), "batch");
 		t += c.batch;
 	}
 	if (t % c.dump_interval != 0) dump(t);

@@ -146,6 +146,23 @@ def test_cli_fput_run_files_and_cadence(gpu, tmp_path):
 
 
 @pytest.mark.gpu
+def test_cli_autotune_run_matches_default(gpu, tmp_path):
+    """--autotune times the kernel forms on the initial state (restored after)
+    and may pick another wave shape: same trajectory to rounding level."""
+    n, c, dt, b = 256, 40, 0.05, 50
+    s0 = gpu.ic_plane_waves(n, c, 1.0, 0, 32, 0.5, 3) + 0.01 * np.random.default_rng(5).normal(size=(c, n, 2))
+    init = tmp_path / "init"
+    s0.tofile(init)
+    common = ["-n", n, "-c", c, "--dt", dt, "-b", b, "-s", 100, "-i", init, "-m", 1, "--beta", 1]
+    run("DNKG", *common, "-p", tmp_path / "plain", check=0)
+    p = run("DNKG", *common, "-p", tmp_path / "tuned", "--autotune", "-v", check=0)
+    assert "autotune:" in p.stderr and "*" in p.stderr
+    a, t = np.fromfile(tmp_path / "plain-100"), np.fromfile(tmp_path / "tuned-100")
+    assert np.abs(a - t).max() <= 1e-12 * np.abs(a).max()
+    np.testing.assert_allclose(read_entropy(tmp_path / "tuned"), read_entropy(tmp_path / "plain"), rtol=1e-12)
+
+
+@pytest.mark.gpu
 def test_cli_entropy_limit_stops_run(gpu, tmp_path):
     n, c = 64, 8
     gpu.ic_plane_waves(n, c, 0.0, 1, 5, 0.1, 1).tofile(tmp_path / "init")

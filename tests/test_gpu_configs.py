@@ -1,0 +1,110 @@
+"""Benchmark-sized runs against the oracle.
+
+FPUT-beta, 64 sites x 1024 copies, dt 0.1, 1e4 steps with energies and
+entropies every 100 steps, runs whole: the trajectories are chaotic, so the
+state is compared where rounding differences are still at the 1e-13 level
+(t = 1000; two IEEE-correct orderings of the same arithmetic measured 1.9e-13
+apart there and 1e-6 apart at t = 5000), the entropies to 1e-9 up to t = 1000
+and as ensemble statistics (1e-2) over the whole run; the same for the mode
+energies.
+dDNKG, 512 sites x 8192 copies, 500 steps per dump: one dump interval at full
+size, sampled chains and the full projection against the oracle.  The DNKG
+1024 x 16384, DNLS 4096 x 4096 and ragged Toda runs have their full-size tests
+in test_gpu_parity.py, test_gpu_dnls.py and test_gpu_batched.py.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def banner(msg):
+    print(f"[gpu-configs] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def host_threads():
+    return max(1, min(len(os.sched_getaffinity(0)), 16))
+
+
+def test_fput_64x1024_full_run(gpu):
+    N = gpu
+    n, copies, dt, b, steps = 64, 1024, 0.1, 100, 10000
+    s0 = N.ic_plane_waves(n, copies, 0.0, 1, 5, 0.1, 1)  # energy density 0.1 in k = 1..4, seed 1
+    w = O.dispersion_real(n, 0.0)
+    norm = 0.5 / copies / n
+    banner("FPUT 64 x 1024, 1e4 steps in batches of 100 (graph replays)")
+
+    def oracle_energies(st):
+        e = O.calc_linenergies(O.linenergies_real(st, w), norm, True)
+        return e, np.array(O.entropies(e, True))
+
+    gpu_e, gpu_s, orc_e, orc_s = [], [], [], []
+    st = s0
+    with N.Ensemble(N.FPUT, n, copies, dt, beta=1.0) as ens:
+        ens.upload(s0)
+        e, s = ens.linenergies()
+        for t in range(0, steps + 1, b):
+            if t:
+                e, s = ens.batch(b)
+                st = O.move_real(O.FPUT, st, b, dt, beta=1.0, threads=host_threads())
+            eo, so = oracle_energies(st)
+            gpu_e.append(e.copy()), gpu_s.append(s[:2].copy()), orc_e.append(eo), orc_s.append(so)
+            if t == 1000:
+                got = ens.download()
+                err = np.abs(got - st).max() / np.abs(st).max()
+                banner(f"  t=1000 state rel err {err:.2e}")
+                assert err <= 1e-10
+        E_gpu, _ = ens.chain_energies()
+    ge, oe = np.array(gpu_e), np.array(orc_e)
+    de = np.abs(ge - oe).max(axis=1) / oe.max(axis=1)
+    banner(f"  mode energies, max diff / max energy: t<=1000 {de[:11].max():.2e}, whole run {de.max():.2e}")
+    assert de[:11].max() <= 1e-9
+    assert de.max() <= 1e-2
+    # entropies: the initial state excites only k = 1..4, so early on some mode
+    # energies are zero or at rounding level and the entropies are not finite
+    # (which modes come out exactly zero depends on the FFT's operation order)
+    gs, os_ = np.array(gpu_s), np.array(orc_s)
+    fin = np.isfinite(gs) & np.isfinite(os_)
+    rel = np.where(fin, np.abs(gs - os_) / np.where(fin, np.abs(os_), 1.0), 0.0)
+    banner(f"  finite entropies: {fin.sum()} of {fin.size}; rel diff t<=1000 {rel[:11].max():.2e}, "
+           f"whole run {rel.max():.2e}")
+    assert fin[-1].all()
+    assert rel[:11].max() <= 1e-9
+    assert rel.max() <= 1e-2
+    # energy conservation of the whole run, chain by chain, matches the oracle's in size
+    E0 = O.chain_energy(O.FPUT, s0, beta=1.0)
+    drift_gpu = np.mean(np.abs(E_gpu - E0) / E0)
+    drift_orc = np.mean(np.abs(O.chain_energy(O.FPUT, st, beta=1.0) - E0) / E0)
+    banner(f"  mean relative energy error after 1e4 steps: GPU {drift_gpu:.2e}, oracle {drift_orc:.2e}")
+    assert drift_gpu <= 2 * drift_orc + 1e-12
+
+
+def test_ddnkg_512x8192_dump_interval(gpu):
+    N = gpu
+    n, copies, dt, b = 512, 8192, 0.05, 500
+    masses = N.ic_masses(n, 0.5, 1.5, 5)
+    om, V = N.eigensystem(masses + 2)
+    s0 = N.ic_eigenmodes(n, copies, om, V, n // 8, 0.5, 5)
+    banner("dDNKG 512 x 8192, one 500-step dump interval")
+    with N.Ensemble(N.DDNKG, n, copies, dt, masses=masses, beta=1.0) as ens:
+        ens.upload(s0)
+        raw, ent = ens.batch(b)
+        raw_sums = ens.linenergies_raw()
+        got = ens.download()
+    pick = np.random.default_rng(2).choice(copies, 6, replace=False)
+    want = O.move_real(O.DDNKG, s0[pick], b, dt, beta=1.0, mp2=masses + 2, threads=host_threads())
+    err = np.abs(got[pick] - want).max() / np.abs(want).max()
+    banner(f"  sampled chains after 500 steps: rel err {err:.2e}")
+    assert err <= 1e-10
+    # the full projection of the GPU state, against the oracle's O(copies n^2) projection
+    ref = O.linenergies_ddnkg(got, V, om)
+    assert np.abs(raw_sums - ref).max() <= 1e-9 * np.abs(ref).max()
+    e = O.calc_linenergies(ref, 0.5 / copies, False)
+    wt, inf = O.entropies(e, False)
+    assert abs(ent[0] - wt) <= 1e-9 * abs(wt) and abs(ent[1] - inf) <= 1e-9 * abs(inf)
