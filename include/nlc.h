@@ -61,8 +61,17 @@ typedef enum { NLC_KERNEL_AUTO = 0, NLC_KERNEL_WAVE = 1, NLC_KERNEL_SPLIT = 2 } 
 
 /* dDNKG energy projection (the two cublasDgemm of dDNKG.cpp:118-135 and the
  * reduction of dDNKG.cu:237-265, fused).  NLC_PROJ_AUTO = FP64 matrix cores
- * until nlc_autotune times both forms; every form is FP64 throughout. */
-typedef enum { NLC_PROJ_AUTO = 0, NLC_PROJ_MFMA_F64 = 1, NLC_PROJ_VALU_F64 = 2 } nlc_projection;
+ * until nlc_autotune times the two FP64 forms.  NLC_PROJ_SPLIT_BF16 (chain
+ * lengths divisible by 32, chosen explicitly, never by the autotuner) runs the
+ * products on BF16 matrix cores with three-term splits of both operands and
+ * FP32 accumulation: mode energies differ from the FP64 forms at the 1e-7
+ * level of the largest mode energy. */
+typedef enum {
+	NLC_PROJ_AUTO = 0,
+	NLC_PROJ_MFMA_F64 = 1,
+	NLC_PROJ_VALU_F64 = 2,
+	NLC_PROJ_SPLIT_BF16 = 3
+} nlc_projection;
 
 /* Model and ensemble parameters: the reference's common options (gconf,
  * common/configuration.hpp:20-39) and model options (DNKG_FPUT_Toda.cpp:26-43,

@@ -68,7 +68,8 @@ struct nlc_ctx {
 	double *mp2 = nullptr;     // dDNKG m_x + 2
 	double *masses_d = nullptr;
 	double *eigvec = nullptr;  // dDNKG V, column-major
-	int proj_variant = nlc::kProjMfma; // dDNKG projection: FP64 matrix cores or vector ALU
+	int proj_variant = nlc::kProjMfma; // dDNKG projection: FP64 matrix cores, vector ALU or split BF16
+	uint16_t *vplanes = nullptr, *splanes = nullptr; // split BF16 form: V and state splits
 	double *split_q = nullptr, *split_p = nullptr;
 	double *energy_d = nullptr, *norm_d = nullptr;
 	double2 *dnls_table = nullptr; // DNLS Fourier multipliers
@@ -200,7 +201,7 @@ void free_all(nlc_ctx *c) {
 	     {(void *) c->state, (void *) c->omega, (void *) c->tw, (void *) c->partial, (void *) c->reduce_scratch,
 	      (void *) c->e_raw, (void *) c->results, (void *) c->modes, (void *) c->mp2, (void *) c->masses_d,
 	      (void *) c->eigvec, (void *) c->split_q, (void *) c->split_p, (void *) c->energy_d, (void *) c->norm_d,
-	      (void *) c->dnls_table, (void *) c->items})
+	      (void *) c->dnls_table, (void *) c->items, (void *) c->vplanes, (void *) c->splanes})
 		if (ptr) hipFree(ptr);
 	if (c->h_results) hipHostFree(c->h_results);
 	if (c->stream) hipStreamDestroy(c->stream);
@@ -274,6 +275,15 @@ int setup_common(nlc_ctx *c) {
 		NLC_HIP(c, hipMemcpy(c->mp2, mp2.data(), sizeof(double) * n, hipMemcpyHostToDevice));
 		NLC_HIP(c, hipMemcpy(c->masses_d, c->masses.data(), sizeof(double) * n, hipMemcpyHostToDevice));
 		NLC_HIP(c, hipMemcpy(c->eigvec, V.data(), sizeof(double) * n * n, hipMemcpyHostToDevice));
+		if (c->proj_variant == nlc::kProjBf16) {
+			if (!nlc::ddnkg_bf16_supported(n))
+				return c->fail(NLC_EUNSUPPORTED, "the split BF16 projection needs chain_length divisible by 32");
+			std::vector<uint16_t> vp(size_t(3) * n * n);
+			nlc::ddnkg_split_host(V.data(), size_t(n) * n, vp.data());
+			NLC_TRY(dev_alloc(c, &c->vplanes, vp.size(), "eigenvector splits"));
+			NLC_TRY(dev_alloc(c, &c->splanes, size_t(6) * n * c->copies, "state splits"));
+			NLC_HIP(c, hipMemcpy(c->vplanes, vp.data(), sizeof(uint16_t) * vp.size(), hipMemcpyHostToDevice));
+		}
 	}
 	NLC_HIP(c, hipMemcpy(c->omega, omega.data(), sizeof(double) * c->table_len, hipMemcpyHostToDevice));
 	NLC_TRY(dev_alloc(c, &c->partial, partial_len, "energy partials"));
@@ -423,7 +433,7 @@ int enqueue_energy(nlc_ctx *c) {
 			break;
 		case NLC_DDNKG:
 			NLC_HIP(c, nlc::launch_energy_ddnkg(c->stream, st, k.n, k.count, c->eigvec, c->omega, c->partial,
-			                                    c->proj_variant));
+			                                    c->proj_variant, c->vplanes, c->splanes));
 			NLC_HIP(c, nlc::launch_reduce_partial(c->stream, c->partial, k.rows, k.n, nullptr, er, c->reduce_scratch));
 			break;
 		case NLC_DNLS:
@@ -487,7 +497,7 @@ int validate_params(const nlc_params *p, bool ragged) {
 	if (p->model == NLC_DDNKG && !p->masses) return reject(NLC_EARG, "dDNKG needs the masses (-m file content)");
 	if (p->precision != NLC_FP64) return reject(NLC_EUNSUPPORTED, "only FP64 precision is available in this build");
 	if (p->kernel < NLC_KERNEL_AUTO || p->kernel > NLC_KERNEL_SPLIT) return reject(NLC_EARG, "unknown kernel family");
-	if (p->projection < NLC_PROJ_AUTO || p->projection > NLC_PROJ_VALU_F64)
+	if (p->projection < NLC_PROJ_AUTO || p->projection > NLC_PROJ_SPLIT_BF16)
 		return reject(NLC_EARG, "unknown projection form");
 	int ndev = 0;
 	if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
@@ -544,7 +554,9 @@ int nlc_create(nlc_ctx **out, const nlc_params *p) {
 	nlc_ctx *c = new (std::nothrow) nlc_ctx;
 	if (!c) return reject(NLC_ENOMEM, "context allocation failed");
 	c->p = *p;
-	c->proj_variant = p->projection == NLC_PROJ_VALU_F64 ? nlc::kProjValu : nlc::kProjMfma;
+	c->proj_variant = p->projection == NLC_PROJ_VALU_F64     ? nlc::kProjValu
+	                  : p->projection == NLC_PROJ_SPLIT_BF16 ? nlc::kProjBf16
+	                                                         : nlc::kProjMfma;
 	c->n = p->chain_length;
 	c->copies = p->copies;
 	c->total_sites = uint64_t(c->n) * c->copies;
@@ -938,7 +950,10 @@ int nlc_describe(const nlc_ctx *c, char *buf, size_t len) {
 	if (c->p.model == NLC_DDNKG) {
 		const size_t at = strlen(buf);
 		if (at + 1 < len)
-			snprintf(buf + at, len - at, " projection=%s", c->proj_variant == nlc::kProjMfma ? "mfma_f64" : "valu_f64");
+			snprintf(buf + at, len - at, " projection=%s",
+			         c->proj_variant == nlc::kProjMfma   ? "mfma_f64"
+			         : c->proj_variant == nlc::kProjBf16 ? "split_bf16"
+			                                             : "valu_f64");
 	}
 	return NLC_OK;
 }

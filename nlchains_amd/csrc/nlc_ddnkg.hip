@@ -19,6 +19,8 @@
 //    no LDS staging and no transpose;
 //  * k_proj_energy: the FP64 vector-ALU form (64x64 tile staged through LDS),
 //    kept as the autotuner's alternative.
+#include <cstring>
+
 #include "nlc_internal.hpp"
 
 namespace nlc {
@@ -190,15 +192,170 @@ k_proj_mfma(const double2 *__restrict__ state, uint32_t n, uint32_t copies, cons
 	}
 }
 
+// ------------------------------------------------------- split BF16 form
+//
+// Each FP64 operand x is split into three BF16 terms, x = h + m + l (each the
+// nearest BF16 of the remainder, through FP32), which carries x to about
+// 2^-24 relative.  The product keeps the six significant cross terms,
+// hh + (hm + mh + hl + lh + mm), on v_mfma_f32_16x16x32_bf16 with the leading
+// term and the corrections in separate FP32 accumulators (the corrections are
+// 2^-8 smaller, so their rounding is negligible), summed in FP64 at the end.
+// The state is split once per pass by k_split_state into six planes
+// [qh qm ql ph pm pl][copy][site]; the eigenvectors once on the host.  Requires
+// n % 32 == 0.  Workgroup: 64 copies x 64 modes, wavefront (wr, wc) owns copies
+// 32 wr + [0, 32) and modes 32 wc + [0, 32); operand layout of the instruction:
+// lane l supplies A[l % 16][8 (l / 16) + e] and B[8 (l / 16) + e][l % 16],
+// e = 0..7, and holds D[4 (l / 16) + r][l % 16] in accumulator element r.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ void split3(double x, __bf16 &h, __bf16 &m, __bf16 &l) {
+	h = __bf16(float(x));
+	const double r1 = x - double(float(h));
+	m = __bf16(float(r1));
+	l = __bf16(float(r1 - double(float(m))));
+}
+
+__global__ void __launch_bounds__(256)
+k_split_state(const double2 *__restrict__ state, size_t sites, __bf16 *__restrict__ planes) {
+	const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+	if (i >= sites) return;
+	const double2 v = state[i];
+	__bf16 h, m, l;
+	split3(v.x, h, m, l);
+	planes[i] = h, planes[sites + i] = m, planes[2 * sites + i] = l;
+	split3(v.y, h, m, l);
+	planes[3 * sites + i] = h, planes[4 * sites + i] = m, planes[5 * sites + i] = l;
+}
+
+__global__ void __launch_bounds__(256)
+k_proj_bf16x3(const __bf16 *__restrict__ planes, uint32_t n, uint32_t copies, const __bf16 *__restrict__ vplanes,
+              const double *__restrict__ omega, double *__restrict__ partial) {
+	__shared__ double red[2][64];
+	const uint32_t mode_blocks = (n + 63) / 64;
+	const uint32_t xcd = blockIdx.x & 7, rest = blockIdx.x >> 3;
+	const uint32_t bx = rest % mode_blocks, by = (rest / mode_blocks) * 8u + xcd;
+	if (by * 64u >= copies) return; // padding workgroup (uniform over the group)
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, wr = wave >> 1, wc = wave & 1;
+	const int r16 = lane & 15, g = lane >> 4;
+	const size_t sites = size_t(n) * copies, vsz = size_t(n) * n;
+	const uint32_t c_base = by * 64u + 32u * wr + r16, j_base = bx * 64u + 32u * wc + r16;
+
+	f32x4 hq[2][2], cq[2][2], hp[2][2], cp[2][2];
+#pragma unroll
+	for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+		for (int cb = 0; cb < 2; ++cb)
+			hq[rb][cb] = cq[rb][cb] = hp[rb][cb] = cp[rb][cb] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+	const bf16x8 zero = bf16x8{};
+	for (uint32_t k0 = 0; k0 < n; k0 += 32) {
+		const uint32_t site = k0 + 8u * uint32_t(g);
+		bf16x8 a[2][6], b[2][3];
+#pragma unroll
+		for (int rb = 0; rb < 2; ++rb) {
+			const uint32_t c = c_base + 16u * rb;
+#pragma unroll
+			for (int pl = 0; pl < 6; ++pl)
+				a[rb][pl] = c < copies ? *(const bf16x8 *) (planes + pl * sites + size_t(c) * n + site) : zero;
+		}
+#pragma unroll
+		for (int cb = 0; cb < 2; ++cb) {
+			const uint32_t j = j_base + 16u * cb;
+#pragma unroll
+			for (int pl = 0; pl < 3; ++pl)
+				b[cb][pl] = j < n ? *(const bf16x8 *) (vplanes + pl * vsz + size_t(j) * n + site) : zero;
+		}
+#pragma unroll
+		for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+			for (int cb = 0; cb < 2; ++cb) {
+				const bf16x8 *q = a[rb], *p = a[rb] + 3, *v = b[cb];
+				hq[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(q[0], v[0], hq[rb][cb], 0, 0, 0);
+				hp[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(p[0], v[0], hp[rb][cb], 0, 0, 0);
+				f32x4 x = cq[rb][cb], y = cp[rb][cb];
+				x = __builtin_amdgcn_mfma_f32_16x16x32_bf16(q[0], v[1], x, 0, 0, 0);
+				y = __builtin_amdgcn_mfma_f32_16x16x32_bf16(p[0], v[1], y, 0, 0, 0);
+				x = __builtin_amdgcn_mfma_f32_16x16x32_bf16(q[1], v[0], x, 0, 0, 0);
+				y = __builtin_amdgcn_mfma_f32_16x16x32_bf16(p[1], v[0], y, 0, 0, 0);
+				x = __builtin_amdgcn_mfma_f32_16x16x32_bf16(q[0], v[2], x, 0, 0, 0);
+				y = __builtin_amdgcn_mfma_f32_16x16x32_bf16(p[0], v[2], y, 0, 0, 0);
+				x = __builtin_amdgcn_mfma_f32_16x16x32_bf16(q[2], v[0], x, 0, 0, 0);
+				y = __builtin_amdgcn_mfma_f32_16x16x32_bf16(p[2], v[0], y, 0, 0, 0);
+				x = __builtin_amdgcn_mfma_f32_16x16x32_bf16(q[1], v[1], x, 0, 0, 0);
+				y = __builtin_amdgcn_mfma_f32_16x16x32_bf16(p[1], v[1], y, 0, 0, 0);
+				cq[rb][cb] = x, cp[rb][cb] = y;
+			}
+	}
+
+	// epilogue: e(c, j) = (omega_j Q)^2 + P^2 over the wave's 32 copies, then the two copy halves
+#pragma unroll
+	for (int cb = 0; cb < 2; ++cb) {
+		const uint32_t j = j_base + 16u * cb;
+		const double w = j < n ? omega[j] : 0.0;
+		double sum = 0.0;
+#pragma unroll
+		for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+			for (int r = 0; r < 4; ++r) {
+				const double wq = w * (double(hq[rb][cb][r]) + double(cq[rb][cb][r]));
+				const double pp = double(hp[rb][cb][r]) + double(cp[rb][cb][r]);
+				sum += fma(wq, wq, pp * pp);
+			}
+		sum += __shfl_xor(sum, 16, 64);
+		sum += __shfl_xor(sum, 32, 64);
+		if (g == 0) red[wr][32 * wc + 16 * cb + r16] = sum;
+	}
+	__syncthreads();
+	if (threadIdx.x < 64) {
+		const uint32_t j = bx * 64u + threadIdx.x;
+		if (j < n) partial[size_t(by) * n + j] = red[0][threadIdx.x] + red[1][threadIdx.x];
+	}
+}
+
 } // namespace
 
 uint32_t ddnkg_partial_rows(uint32_t copies) { return (copies + kTc - 1) / kTc; }
 
+bool ddnkg_bf16_supported(uint32_t n) { return n % 32 == 0; }
+
+void ddnkg_split_host(const double *x, size_t count, uint16_t *planes) {
+	// the device split3 on the host: nearest BF16 of the FP32 nearest, remainders in FP64
+	auto bf16 = [](float f) -> uint16_t {
+		uint32_t u;
+		std::memcpy(&u, &f, 4);
+		u += 0x7FFFu + ((u >> 16) & 1u);
+		return uint16_t(u >> 16);
+	};
+	auto back = [](uint16_t h) -> double {
+		const uint32_t u = uint32_t(h) << 16;
+		float f;
+		std::memcpy(&f, &u, 4);
+		return double(f);
+	};
+	for (size_t i = 0; i < count; ++i) {
+		const uint16_t h = bf16(float(x[i]));
+		const double r1 = x[i] - back(h);
+		const uint16_t m = bf16(float(r1));
+		const uint16_t l = bf16(float(r1 - back(m)));
+		planes[i] = h, planes[count + i] = m, planes[2 * count + i] = l;
+	}
+}
+
 hipError_t launch_energy_ddnkg(hipStream_t st, const double2 *state, uint32_t n, uint32_t copies, const double *V,
-                               const double *omega, double *partial, int variant) {
+                               const double *omega, double *partial, int variant, const uint16_t *vplanes,
+                               uint16_t *planes) {
 	if (variant == kProjValu) {
 		const dim3 grid((n + kTj - 1) / kTj, ddnkg_partial_rows(copies));
 		hipLaunchKernelGGL(k_proj_energy, grid, dim3(256), 0, st, state, n, copies, V, omega, partial);
+	} else if (variant == kProjBf16) {
+		if (!ddnkg_bf16_supported(n) || !vplanes || !planes) return hipErrorInvalidValue;
+		const size_t sites = size_t(n) * copies;
+		hipLaunchKernelGGL(k_split_state, dim3(uint32_t((sites + 255) / 256)), dim3(256), 0, st, state, sites,
+		                   (__bf16 *) planes);
+		const uint32_t mode_blocks = (n + 63) / 64, groups = (ddnkg_partial_rows(copies) + 7) / 8;
+		hipLaunchKernelGGL(k_proj_bf16x3, dim3(groups * 8 * mode_blocks), dim3(256), 0, st, (const __bf16 *) planes, n,
+		                   copies, (const __bf16 *) vplanes, omega, partial);
 	} else {
 		const uint32_t mode_blocks = (n + kMfModes - 1) / kMfModes;
 		const uint32_t groups = (ddnkg_partial_rows(copies) + 7) / 8; // of 8 copy blocks, one per XCD

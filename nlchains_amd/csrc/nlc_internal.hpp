@@ -85,11 +85,16 @@ constexpr uint32_t kEnergyFftMax = 4096;
 
 namespace nlc {
 // dDNKG projection energies: partial[rows][n] with rows = ddnkg_partial_rows(copies),
-// on FP64 matrix cores (kProjMfma) or the FP64 vector ALU (kProjValu).
-constexpr int kProjMfma = 0, kProjValu = 1;
+// on FP64 matrix cores (kProjMfma), the FP64 vector ALU (kProjValu) or BF16
+// matrix cores with three-term operand splits (kProjBf16: n % 32 == 0; vplanes =
+// ddnkg_split_host of V, 3 n^2 values; planes = scratch for 6 n copies values).
+constexpr int kProjMfma = 0, kProjValu = 1, kProjBf16 = 2;
 uint32_t ddnkg_partial_rows(uint32_t copies);
+bool ddnkg_bf16_supported(uint32_t n);
+void ddnkg_split_host(const double *x, size_t count, uint16_t *planes);
 hipError_t launch_energy_ddnkg(hipStream_t, const double2 *state, uint32_t n, uint32_t copies, const double *V,
-                               const double *omega, double *partial, int variant);
+                               const double *omega, double *partial, int variant, const uint16_t *vplanes,
+                               uint16_t *planes);
 // Host eigensystem (nlc_host.cpp).
 int eigensystem(uint32_t n, const double *mp2, double *omegas, double *V);
 extern const double kYoshidaC[8], kYoshidaD[8];

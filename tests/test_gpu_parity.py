@@ -271,6 +271,38 @@ def test_ddnkg_projection_full_size(gpu):
     assert abs(raws["mfma"].sum() - harmonic) <= 1e-11 * harmonic
 
 
+@pytest.mark.parametrize("n,copies", [(512, 70), (96, 130), (64, 8192), (512, 8192)])
+def test_ddnkg_projection_split_bf16(gpu, n, copies):
+    """BF16 matrix cores with three-term operand splits: mode energies within
+    1e-6 of the largest mode energy of the FP64 result (measured: about 1e-7),
+    modes above 1e-3 of the largest within 1e-5 relative, entropies within 1e-6."""
+    N = gpu
+    model, kw, masses, s0 = make_case(N, "dDNKG", n, copies, 13)
+    s0 = s0 + 0.01 * np.random.default_rng(6).normal(size=s0.shape)
+    with N.Ensemble(model, n, copies, 0.05, masses=masses, projection=N.PROJ_SPLIT_BF16, **kw) as ens:
+        assert "projection=split_bf16" in ens.describe()
+        ens.upload(s0)
+        raw = ens.linenergies_raw()
+        e_gpu, s_gpu = ens.linenergies()
+    om, V = N.eigensystem(masses + 2)
+    ref = O.linenergies_ddnkg(s0, V, om)
+    abs_err = np.abs(raw - ref).max() / ref.max()
+    big = ref > 1e-3 * ref.max()
+    rel_big = (np.abs(raw - ref)[big] / ref[big]).max()
+    e = O.calc_linenergies(ref, 0.5 / copies, False)
+    wt, inf = O.entropies(e, False)
+    banner(f"  split BF16 n={n} copies={copies}: max err / max energy {abs_err:.2e}, "
+           f"rel err of modes > 1e-3 max {rel_big:.2e}, S_WT {abs(s_gpu[0] - wt) / abs(wt):.2e}")
+    assert abs_err <= 1e-6 and rel_big <= 1e-5
+    assert abs(s_gpu[0] - wt) <= 1e-6 * abs(wt) and abs(s_gpu[1] - inf) <= 1e-6 * abs(inf)
+
+
+def test_ddnkg_split_bf16_needs_multiple_of_32(gpu):
+    N = gpu
+    with pytest.raises(N.NlcError):
+        N.Ensemble(N.DDNKG, 100, 4, 0.05, masses=np.ones(100), beta=1.0, projection=N.PROJ_SPLIT_BF16)
+
+
 def test_ddnkg_autotune_reports_projection(gpu):
     N = gpu
     model, kw, masses, s0 = make_case(N, "dDNKG", 256, 300, 12)
