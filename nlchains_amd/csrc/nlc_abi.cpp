@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/nlc.h"
@@ -76,6 +77,10 @@ struct nlc_ctx {
 	int dnls_variant = nlc::kDnlsRadix16; // n = 4096: register kernel, radix 16 or radix 8
 	double theta[9] = {};          // DNLS nonlinear angles
 	double *h_results = nullptr;   // pinned copy of results
+	// host transfers of the shard: pinned double buffer and the ragged chains in device order
+	double2 *pinned[2] = {nullptr, nullptr};
+	hipEvent_t pinned_ev[2] = {nullptr, nullptr};
+	std::vector<uint32_t> device_order;
 
 	nlc::StepLauncher launcher = nullptr; // uniform wave kernel
 	int L = 0, S = 0;
@@ -204,6 +209,10 @@ void free_all(nlc_ctx *c) {
 	      (void *) c->dnls_table, (void *) c->items, (void *) c->vplanes, (void *) c->splanes})
 		if (ptr) hipFree(ptr);
 	if (c->h_results) hipHostFree(c->h_results);
+	for (int i = 0; i < 2; ++i) {
+		if (c->pinned[i]) hipHostFree(c->pinned[i]);
+		if (c->pinned_ev[i]) hipEventDestroy(c->pinned_ev[i]);
+	}
 	if (c->stream) hipStreamDestroy(c->stream);
 }
 
@@ -659,35 +668,101 @@ void nlc_destroy(nlc_ctx *ctx) {
 	delete ctx;
 }
 
+// Host <-> device copies of the shard.  Chunks of kXferSites sites alternate
+// between two pinned buffers: the DMA of one chunk overlaps the host-side copy
+// (several threads; for ragged ensembles the permutation between the caller's
+// chain order and the device's class-major order) of the other.
+constexpr size_t kXferSites = size_t(1) << 19; // 8 MiB of double2
+
+static int ensure_pinned(nlc_ctx *c) {
+	for (int i = 0; i < 2; ++i) {
+		if (!c->pinned[i] && hipHostMalloc((void **) &c->pinned[i], sizeof(double2) * kXferSites) != hipSuccess)
+			return c->fail(NLC_ENOMEM, "pinned staging buffer allocation failed");
+		if (!c->pinned_ev[i] && hipEventCreateWithFlags(&c->pinned_ev[i], hipEventDisableTiming) != hipSuccess)
+			return c->fail(NLC_EHIP, "event creation failed");
+	}
+	if (c->ragged && c->device_order.empty()) {
+		c->device_order.resize(c->lengths.size());
+		for (uint32_t i = 0; i < c->device_order.size(); ++i) c->device_order[i] = i;
+		std::sort(c->device_order.begin(), c->device_order.end(),
+		          [c](uint32_t a, uint32_t b) { return c->device_site0[a] < c->device_site0[b]; });
+	}
+	return NLC_OK;
+}
+
+// Copies the device-order sites [d0, d0 + len) between the pinned chunk and the
+// caller's buffer (to_host: pinned -> caller), on up to 8 threads.
+static void host_chunk(const nlc_ctx *c, double2 *pinned, double2 *caller, uint64_t d0, size_t len, bool to_host) {
+	auto copy = [&](uint64_t a, uint64_t b) { // device-order sub-range [a, b)
+		if (!c->ragged) {
+			if (to_host) std::memcpy(caller + a, pinned + (a - d0), sizeof(double2) * (b - a));
+			else std::memcpy(pinned + (a - d0), caller + a, sizeof(double2) * (b - a));
+			return;
+		}
+		// chains overlapping [a, b), in device order
+		auto it = std::upper_bound(c->device_order.begin(), c->device_order.end(), a,
+		                           [c](uint64_t x, uint32_t i) { return x < c->device_site0[i]; });
+		if (it != c->device_order.begin()) --it;
+		for (; it != c->device_order.end() && c->device_site0[*it] < b; ++it) {
+			const uint32_t i = *it;
+			const uint64_t s0 = std::max<uint64_t>(a, c->device_site0[i]);
+			const uint64_t s1 = std::min<uint64_t>(b, c->device_site0[i] + c->lengths[i]);
+			if (s0 >= s1) continue;
+			double2 *u = caller + c->caller_site0[i] + (s0 - c->device_site0[i]);
+			if (to_host) std::memcpy(u, pinned + (s0 - d0), sizeof(double2) * (s1 - s0));
+			else std::memcpy(pinned + (s0 - d0), u, sizeof(double2) * (s1 - s0));
+		}
+	};
+	const unsigned nt = len >= (size_t(1) << 16) ? std::max(1u, std::min(8u, std::thread::hardware_concurrency())) : 1u;
+	if (nt == 1) {
+		copy(d0, d0 + len);
+		return;
+	}
+	std::vector<std::thread> pool;
+	for (unsigned t = 0; t < nt; ++t)
+		pool.emplace_back([&, t] { copy(d0 + len * t / nt, d0 + len * (t + 1) / nt); });
+	for (auto &th : pool) th.join();
+}
+
 int nlc_upload(nlc_ctx *c, const void *state) {
 	if (!c || !state) return NLC_EARG;
-	if (!c->ragged) {
-		NLC_HIP(c, hipMemcpyAsync(c->state, state, c->state_bytes, hipMemcpyHostToDevice, c->stream));
-		NLC_HIP(c, hipStreamSynchronize(c->stream));
-		return NLC_OK;
+	NLC_TRY(ensure_pinned(c));
+	double2 *src = const_cast<double2 *>(static_cast<const double2 *>(state));
+	const size_t chunks = (c->total_sites + kXferSites - 1) / kXferSites;
+	for (size_t k = 0; k < chunks; ++k) {
+		const int b = int(k & 1);
+		const uint64_t d0 = k * kXferSites;
+		const size_t len = std::min<uint64_t>(kXferSites, c->total_sites - d0);
+		if (k >= 2) NLC_HIP(c, hipEventSynchronize(c->pinned_ev[b])); // the DMA of chunk k - 2 has read it
+		host_chunk(c, c->pinned[b], src, d0, len, false);
+		NLC_HIP(c, hipMemcpyAsync(c->state + d0, c->pinned[b], sizeof(double2) * len, hipMemcpyHostToDevice, c->stream));
+		NLC_HIP(c, hipEventRecord(c->pinned_ev[b], c->stream));
 	}
-	std::vector<double2> stage(c->total_sites);
-	const double2 *src = static_cast<const double2 *>(state);
-	for (size_t i = 0; i < c->lengths.size(); ++i)
-		std::memcpy(&stage[c->device_site0[i]], src + c->caller_site0[i], sizeof(double2) * c->lengths[i]);
-	NLC_HIP(c, hipMemcpyAsync(c->state, stage.data(), c->state_bytes, hipMemcpyHostToDevice, c->stream));
 	NLC_HIP(c, hipStreamSynchronize(c->stream));
 	return NLC_OK;
 }
 
 int nlc_download(nlc_ctx *c, void *state) {
 	if (!c || !state) return NLC_EARG;
-	if (!c->ragged) {
-		NLC_HIP(c, hipMemcpyAsync(state, c->state, c->state_bytes, hipMemcpyDeviceToHost, c->stream));
-		NLC_HIP(c, hipStreamSynchronize(c->stream));
-		return NLC_OK;
-	}
-	std::vector<double2> stage(c->total_sites);
-	NLC_HIP(c, hipMemcpyAsync(stage.data(), c->state, c->state_bytes, hipMemcpyDeviceToHost, c->stream));
-	NLC_HIP(c, hipStreamSynchronize(c->stream));
+	NLC_TRY(ensure_pinned(c));
 	double2 *dst = static_cast<double2 *>(state);
-	for (size_t i = 0; i < c->lengths.size(); ++i)
-		std::memcpy(dst + c->caller_site0[i], &stage[c->device_site0[i]], sizeof(double2) * c->lengths[i]);
+	const size_t chunks = (c->total_sites + kXferSites - 1) / kXferSites;
+	auto drain = [&](size_t k) -> int { // host side of chunk k, once its DMA is done
+		const int b = int(k & 1);
+		const uint64_t d0 = k * kXferSites;
+		NLC_HIP(c, hipEventSynchronize(c->pinned_ev[b]));
+		host_chunk(c, c->pinned[b], dst, d0, std::min<uint64_t>(kXferSites, c->total_sites - d0), true);
+		return NLC_OK;
+	};
+	for (size_t k = 0; k < chunks; ++k) {
+		const int b = int(k & 1);
+		const uint64_t d0 = k * kXferSites;
+		const size_t len = std::min<uint64_t>(kXferSites, c->total_sites - d0);
+		NLC_HIP(c, hipMemcpyAsync(c->pinned[b], c->state + d0, sizeof(double2) * len, hipMemcpyDeviceToHost, c->stream));
+		NLC_HIP(c, hipEventRecord(c->pinned_ev[b], c->stream));
+		if (k >= 1) NLC_TRY(drain(k - 1)); // overlaps the DMA of chunk k
+	}
+	if (chunks) NLC_TRY(drain(chunks - 1));
 	return NLC_OK;
 }
 
