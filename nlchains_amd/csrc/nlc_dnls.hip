@@ -199,23 +199,31 @@ __device__ __forceinline__ void twiddle_run(double2 v[16], const double2 *__rest
 	}
 }
 
-struct Lds4096 {
-	double2 a[4096];
+// LDS of the radix-16 kernel: 4 regions of 1088 entries, one per wavefront.
+// Cross-wave exchange: element a in [0, 4096) of the natural layout at
+// a + 64 (a >> 10), so wavefront w reads exactly [1088 w, 1088 w + 1024) of
+// its own region.  In-wave exchange: 16 x 16 transposes with rows of 17
+// entries (bank-conflict free) at 1088 w + 272 g, g = 16-lane group in the
+// wave, inside the same region.  Every slot is a per-lane base plus a
+// compile-time offset.
+struct Lds4352 {
+	double2 a[4352];
 };
 
-// In-wave exchange slot of element (row, col), row and col in 0..15, of the
-// 16 x 16 block of 16-lane group hi = t >> 4: region [256 hi, 256 hi + 256)
-// (wavefront w owns [1024 w, 1024 w + 1024), which is exactly what its lanes
-// read in the cross-wave exchange).  Entry index mod 16 = col ^ row: the 16
-// lanes of a group always touch 16 distinct bank quads (an entry spans 4 banks).
-__device__ __forceinline__ uint32_t xslot(uint32_t hi, uint32_t row, uint32_t col) {
-	return 256u * hi + 16u * row + (col ^ row);
+// Barriers of the LDS exchanges.  The fences are restricted to LDS: a plain
+// __syncthreads() also drains every global load in flight (s_waitcnt vmcnt(0)),
+// which would stall on the Fourier multipliers and twiddles read ahead of
+// their use; the exchanges only need the LDS writes of the other lanes.
+__device__ __forceinline__ void wave_sync() {
+	__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+	__builtin_amdgcn_wave_barrier();
+	__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
 }
 
-__device__ __forceinline__ void wave_sync() {
-	__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-	__builtin_amdgcn_wave_barrier();
-	__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+__device__ __forceinline__ void lds_barrier() {
+	__builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+	__builtin_amdgcn_s_barrier();
+	__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
 // Forward FFT.  In: thread t = x0 + 16 x1 holds psi[t + 256 x2] at v[pos(x2)].
@@ -228,57 +236,77 @@ __device__ __forceinline__ void wave_sync() {
 // wavefront; LDS operations of one wavefront complete in order.  So the only
 // workgroup barriers are the ones between the writes and the reads of the
 // cross-wave exchanges.
-__device__ __forceinline__ void fft4096_fwd(double2 v[16], Lds4096 &L, const double2 *__restrict__ tw) {
+__device__ __forceinline__ void fft4096_fwd(double2 v[16], Lds4352 &L, const double2 *__restrict__ tw) {
 	const uint32_t t = threadIdx.x, lo = t & 15, hi = t >> 4;
+	const uint32_t w = t >> 6, g = hi & 3;
 	// stage 1: DFT over x2 (k0 natural), twiddle W4096^{(x0+16x1) k0}
 	dft16_t2n<-1>(v);
 	twiddle_run<false, true, false>(v, tw, 0, t);
-	// exchange 1 (cross-wave): C'[x0,x1,k0] at x0 + 16 x1 + 256 k0
+	// exchange 1 (cross-wave): C'[x0,x1,k0] at x0 + 16 x1 + 256 k0 (+ region padding)
+	double2 *e1w = L.a + t;
 #pragma unroll
-	for (int k0 = 0; k0 < 16; ++k0) L.a[t + 256 * k0] = v[k0];
-	__syncthreads();
+	for (int k0 = 0; k0 < 16; ++k0) e1w[256 * k0 + 64 * (k0 >> 2)] = v[k0];
+	lds_barrier();
 	// thread (x0 = lo, k0 = hi) gathers x1
+	const double2 *e1r = L.a + lo + 256 * hi + 64 * w;
 #pragma unroll
-	for (int x1 = 0; x1 < 16; ++x1) v[x1] = L.a[lo + 16 * x1 + 256 * hi];
+	for (int i = 0; i < 16; ++i) { // in the order the first DFT-4 layer consumes them
+		const int x1 = 4 * (i & 3) + (i >> 2);
+		v[x1] = e1r[16 * x1];
+	}
 	// stage 2: DFT over x1 (k1 transposed), twiddle W256^{x0 k1} = W4096^{16 x0 k1}
 	dft16_n2t<-1>(v);
 	twiddle_run<false, true, true>(v, tw, 0, 16 * lo);
 	// exchange 2 (within the 16-lane group): element (x0 = lo, k1)
+	double2 *grp = L.a + 1088 * w + 272 * g;
 	wave_sync();
 #pragma unroll
-	for (int k1 = 0; k1 < 16; ++k1) L.a[xslot(hi, lo, k1)] = v[pos(k1)];
+	for (int k1 = 0; k1 < 16; ++k1) grp[lo + 17 * k1] = v[pos(k1)];
 	wave_sync();
 	// thread (k1 = lo, k0 = hi) gathers x0
 #pragma unroll
-	for (int x0 = 0; x0 < 16; ++x0) v[x0] = L.a[xslot(hi, x0, lo)];
+	for (int i = 0; i < 16; ++i) {
+		const int x0 = 4 * (i & 3) + (i >> 2);
+		v[x0] = grp[x0 + 17 * lo];
+	}
 	// stage 3: DFT over x0 (k2 transposed)
 	dft16_n2t<-1>(v);
 }
 
 // Inverse (unnormalised) FFT, the exact reverse of fft4096_fwd: spectrum at
 // v[pos(k2)] in, psi[t + 256 x2] at v[pos(x2)] out.
-__device__ __forceinline__ void fft4096_inv(double2 v[16], Lds4096 &L, const double2 *__restrict__ tw) {
+__device__ __forceinline__ void fft4096_inv(double2 v[16], Lds4352 &L, const double2 *__restrict__ tw) {
 	const uint32_t t = threadIdx.x, lo = t & 15, hi = t >> 4;
+	const uint32_t w = t >> 6, g = hi & 3;
 	// stage 3': over k2 -> x0 natural (thread k1 = lo), twiddle W256^{-x0 k1}
 	dft16_t2n<+1>(v);
 	twiddle_run<true, true, false>(v, tw, 0, 16 * lo);
 	// exchange 2' (within the 16-lane group): element (x0, k1 = lo)
+	double2 *grp = L.a + 1088 * w + 272 * g;
 	wave_sync();
 #pragma unroll
-	for (int x0 = 0; x0 < 16; ++x0) L.a[xslot(hi, x0, lo)] = v[x0];
+	for (int x0 = 0; x0 < 16; ++x0) grp[x0 + 17 * lo] = v[x0];
 	wave_sync();
 #pragma unroll
-	for (int k1 = 0; k1 < 16; ++k1) v[k1] = L.a[xslot(hi, lo, k1)];
+	for (int i = 0; i < 16; ++i) {
+		const int k1 = 4 * (i & 3) + (i >> 2);
+		v[k1] = grp[lo + 17 * k1];
+	}
 	// stage 2': over k1 -> x1 transposed (thread x0 = lo, k0 = hi), twiddle W4096^{-(x0 + 16 x1) k0}
 	dft16_n2t<+1>(v);
 	twiddle_run<true, false, true>(v, tw, lo * hi, 16 * hi);
 	// exchange 1' (cross-wave), written into the wavefront's own region
+	double2 *e1w = L.a + lo + 256 * hi + 64 * w;
 	wave_sync();
 #pragma unroll
-	for (int x1 = 0; x1 < 16; ++x1) L.a[lo + 16 * x1 + 256 * hi] = v[pos(x1)];
-	__syncthreads();
+	for (int x1 = 0; x1 < 16; ++x1) e1w[16 * x1] = v[pos(x1)];
+	lds_barrier();
+	const double2 *e1r = L.a + t;
 #pragma unroll
-	for (int k0 = 0; k0 < 16; ++k0) v[k0] = L.a[t + 256 * k0];
+	for (int i = 0; i < 16; ++i) {
+		const int k0 = 4 * (i & 3) + (i >> 2);
+		v[k0] = e1r[256 * k0 + 64 * (k0 >> 2)];
+	}
 	// stage 1': over k0 -> x2 transposed
 	dft16_n2t<+1>(v);
 }
@@ -291,7 +319,7 @@ struct DnlsArgs {
 __global__ void __launch_bounds__(kT, 2)
 k_dnls_4096(double2 *__restrict__ state, uint32_t copies, uint32_t steps, DnlsArgs a, const double2 *__restrict__ tw,
             const double2 *__restrict__ mult) {
-	__shared__ Lds4096 L;
+	__shared__ Lds4352 L;
 	const uint32_t chain = blockIdx.x, t = threadIdx.x;
 	double2 *psi = state + size_t(chain) * 4096;
 	double2 v[16]; // psi[t + 256 j] at v[pos(j)]
@@ -430,7 +458,7 @@ __device__ __forceinline__ void fft4096_fwd8(double2 v[8], double2 *L, const dou
 	double2 *x1w = L + t;
 #pragma unroll
 	for (int k0 = 0; k0 < 8; ++k0) x1w[kR8Region * k0] = v[k0];
-	__syncthreads();
+	lds_barrier();
 	const double2 *x1r = own + lane;
 #pragma unroll
 	for (int x2 = 0; x2 < 8; ++x2) v[x2] = x1r[64 * x2];
@@ -498,7 +526,7 @@ __device__ __forceinline__ void fft4096_inv8(double2 v[8], double2 *L, const dou
 	double2 *x1w = own + lane;
 #pragma unroll
 	for (int x2 = 0; x2 < 8; ++x2) x1w[64 * x2] = v[x2];
-	__syncthreads();
+	lds_barrier();
 	const double2 *x1r = L + t;
 #pragma unroll
 	for (int k0 = 0; k0 < 8; ++k0) v[k0] = x1r[kR8Region * k0];
