@@ -47,7 +47,11 @@ extern "C" {
  * dDNKG.cpp:168-170, DNLS.cpp:193-195). */
 typedef enum { NLC_DNKG = 0, NLC_FPUT = 1, NLC_TODA = 2, NLC_DDNKG = 3, NLC_DNLS = 4 } nlc_model;
 
-/* Arithmetic of the step.  NLC_FP64 is the reference's all-double method. */
+/* Arithmetic of the step.  NLC_FP64 is the reference's all-double method.
+ * NLC_FP32C (FP32 arithmetic with compensated double-single accumulation) is
+ * reserved: nlc_create rejects it with NLC_EUNSUPPORTED, because on gfx950
+ * without packed FP32 an FP32 instruction issues at the FP64 rate and the
+ * compensated form needs more instructions per site than the FP64 step. */
 typedef enum { NLC_FP64 = 0, NLC_FP32C = 1 } nlc_precision;
 
 /* Step-kernel family.  NLC_KERNEL_SPLIT is the reference's --split_kernel

@@ -16,16 +16,18 @@
 //    spectrum in digit-reversed order, the Fourier multiplier is read from a
 //    table stored in that order, and the inverse transform (decimation in
 //    time, conjugate twiddles) returns natural order, so no reordering pass is
-//    ever made.  LDS holds exactly the chain (64 KB): the in-wave exchange is
-//    a 16 x 16 transpose per 16-lane group inside the 1024 entries that the
-//    wavefront alone reads in the cross-wave exchange, XOR-swizzled so that
-//    neither exchange has bank conflicts, and a substep needs two workgroup
-//    barriers (one per cross-wave exchange).
+//    ever made.  LDS: one padded region per wavefront holding what that
+//    wavefront reads in the cross-wave exchange plus the rows of its in-wave
+//    16 x 16 transposes, so a substep needs two workgroup barriers (one per
+//    cross-wave exchange), every slot is a per-lane base plus a compile-time
+//    offset, and neither exchange has bank conflicts.
+//  * k_dnls_4096_r8: the same transform as four radix-8 passes on 512 threads
+//    x 8 values (more wavefronts per SIMD, one more LDS exchange).
 //  * k_dnls_lds: any power-of-two n <= 4096, chain in LDS, radix-4 Stockham
 //    passes (the inverse transform as conj(FFT(conj(.)))).
-//  * the nonlinear phase e^{-i theta} uses Taylor polynomials of sin and cos
-//    (degree 15 and 16, error below 1e-16 for |theta| <= pi/4) and ROCm's
-//    device sincos for larger angles.
+//  * the nonlinear phase e^{-i theta}: Taylor polynomials of sin and cos on
+//    |r| <= pi/16 (degree 11 and 12), after a Cody & Waite reduction by pi/8
+//    for larger angles.
 #include "nlc_internal.hpp"
 
 namespace nlc {

@@ -17,7 +17,8 @@
 //  * the kick of DNKG/dDNKG is 5 VALU ops per site, the drift 1 FMA;
 //    FPUT and Toda exchange the bond force of the lane's first bond instead of
 //    recomputing it, so each bond term (a cubic or an FP64 exp) is evaluated
-//    once per substep;
+//    once per substep; Toda works on u = 2 alpha q with the exp of
+//    nlc_exp.hpp;
 //  * the 7 substeps are unrolled with their weights in SGPRs; the c7 drift of a
 //    step is fused with the c0 drift of the next (the reference's 2*c0 merge).
 //  * HBM is touched once per batch: 16 B/site in, 16 B/site out.
