@@ -85,8 +85,11 @@ __constant__ double2 kOctant[16] = {
 // wavefront); otherwise Cody & Waite reduction a = k pi/8 + r, |r| <= pi/16,
 // with pi/8 split in three parts (33, 33 and 53 significant bits, so k*Q1 and
 // k*Q2 are exact for |k| < 2^20), and e^{ia} = e^{i k pi/8} e^{ir}.
-__device__ __forceinline__ double2 expi(double a) {
-	if (fabs(a) <= 0.19634954084936207) return expi_small(a);
+constexpr double kExpiSmall = 0.19634954084936207; // pi/16
+
+// The reduced form, valid for every angle; for |a| <= pi/16 it gives exactly
+// expi_small(a) (k = 0, r = a, and the octant factor is 1 + 0i).
+__device__ __forceinline__ double2 expi_reduced(double a) {
 	constexpr double kEightOverPi = 2.5464790894703255, kQ1 = 0x1.921fb54400000p-2, kQ2 = 0x1.0b4611a600000p-36,
 	                 kQ3 = 0x1.3198a2e037073p-71;
 	const double k = rint(a * kEightOverPi);
@@ -96,11 +99,40 @@ __device__ __forceinline__ double2 expi(double a) {
 	return cmul(expi_small(r), kOctant[int(k) & 15]);
 }
 
+__device__ __forceinline__ double2 expi(double a) {
+	return fabs(a) <= kExpiSmall ? expi_small(a) : expi_reduced(a);
+}
+
+__device__ __forceinline__ double2 rotate(double2 p, double2 e) {
+	return make_double2(fma(p.x, e.x, -p.y * e.y), fma(p.x, e.y, p.y * e.x));
+}
+
 // psi * e^{-i theta |psi|^2} (DNLS.cu:11-17)
 __device__ __forceinline__ double2 nonlinear(double2 p, double theta) {
-	const double a = -theta * fma(p.x, p.x, p.y * p.y);
-	const double2 e = expi(a);
-	return make_double2(fma(p.x, e.x, -p.y * e.y), fma(p.x, e.y, p.y * e.x));
+	return rotate(p, expi(-theta * fma(p.x, p.x, p.y * p.y)));
+}
+
+// The nonlinear step on all N values of a thread.  The branch between the
+// short and the reduced form is taken once for the whole wavefront (the
+// reduced form only when some angle of some lane exceeds pi/16), so each path
+// is straight-line code over the N values: their polynomial chains interleave
+// instead of running one after another behind a per-value branch.
+template <int N>
+__device__ __forceinline__ void nonlinear_all(double2 (&v)[N], double theta) {
+	double a[N];
+	bool small = true;
+#pragma unroll
+	for (int j = 0; j < N; ++j) {
+		a[j] = -theta * fma(v[j].x, v[j].x, v[j].y * v[j].y);
+		small = small && fabs(a[j]) <= kExpiSmall;
+	}
+	if (__all(small)) {
+#pragma unroll
+		for (int j = 0; j < N; ++j) v[j] = rotate(v[j], expi_small(a[j]));
+	} else {
+#pragma unroll
+		for (int j = 0; j < N; ++j) v[j] = rotate(v[j], expi_reduced(a[j]));
+	}
 }
 
 // ---------------------------------------------------------------- radix 16
@@ -335,8 +367,7 @@ k_dnls_4096(double2 *__restrict__ state, uint32_t copies, uint32_t steps, DnlsAr
 			// in registers across the loop: registers are the occupancy limit here
 			const double2 *twk = tw;
 			asm volatile("" : "+s"(twk));
-#pragma unroll
-			for (int j = 0; j < 16; ++j) v[j] = nonlinear(v[j], th);
+			nonlinear_all(v, th);
 			fft4096_fwd(v, L, twk);
 			// Fourier multiplier, stored per thread in the digit-reversed order
 			const double2 *m = mult + size_t(k < 4 ? k : 6 - k) * 4096 + t;
@@ -551,8 +582,7 @@ k_dnls_4096_r8(double2 *__restrict__ state, uint32_t copies, uint32_t steps, Dnl
 			const double th = (k == 0 && s) ? a.theta[8] : a.theta[k];
 			const double2 *twk = tw;
 			asm volatile("" : "+s"(twk));
-#pragma unroll
-			for (int j = 0; j < 8; ++j) v[j] = nonlinear(v[j], th);
+			nonlinear_all(v, th);
 			fft4096_fwd8(v, lds8, twk);
 			// Fourier multiplier, stored per thread in the digit-reversed order;
 			// the fence keeps its loads out of the transform (4 wavefronts per
