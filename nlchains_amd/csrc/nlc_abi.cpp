@@ -267,7 +267,9 @@ int setup_common(nlc_ctx *c) {
 			tw[k.tab0 + j] = make_double2(std::cos(ang), -std::sin(ang));
 		}
 		if (p.model != NLC_DDNKG) nlc_dispersion(p.model, k.n, p.m, omega.data() + k.tab0);
-		k.rows = p.model == NLC_DDNKG ? nlc::ddnkg_partial_rows(k.count) : std::min<uint32_t>(k.count, 1024);
+		// one partial row per workgroup; long chains get fewer rows (at most 256 MiB of partials)
+		k.rows = p.model == NLC_DDNKG ? nlc::ddnkg_partial_rows(k.count)
+		                              : std::min<uint32_t>({k.count, 1024u, std::max<uint32_t>(1u, (1u << 25) / k.n)});
 		partial_len = std::max(partial_len, size_t(k.rows) * k.n);
 		scratch_len = std::max(scratch_len, size_t(nlc::reduce_scratch_rows(k.rows)) * k.n);
 	}

@@ -26,7 +26,9 @@ namespace nlc {
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kMaxPerThread = kEnergyFftMax / 4 / kThreads; // radix-4 butterflies per thread per pass (n <= 4096)
+// The LDS FFT is instantiated for n <= MAXN with MAXN = 4096 (the common
+// sizes, 64 KB of LDS) and MAXN = 8192 (128 KB, one workgroup per CU).
+constexpr uint32_t kEnergyFftLarge = 8192;
 
 __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
 	return make_double2(fma(a.x, b.x, -a.y * b.y), fma(a.x, b.y, a.y * b.x));
@@ -35,7 +37,9 @@ __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
 // In-place radix-4 (plus one final radix-2 pass when log2 n is odd) Stockham
 // FFT of buf[0..n), forward sign, natural-order output.  tw[j] = exp(-2 pi i j/n).
 // Each pass reads all inputs into registers, synchronises, then writes.
+template <uint32_t MAXN>
 __device__ void lds_fft(double2 *buf, uint32_t n, const double2 *__restrict__ tw) {
+	constexpr int kMaxPerThread = MAXN / 4 / kThreads; // radix-4 butterflies per thread per pass
 	const uint32_t tid = threadIdx.x;
 	uint32_t ns = 1;
 	const uint32_t quarter = n >> 2;
@@ -97,8 +101,8 @@ __device__ void lds_fft(double2 *buf, uint32_t n, const double2 *__restrict__ tw
 	}
 }
 
-// Direct DFT into buf2 (any n; used when n is not a power of two or exceeds
-// the LDS FFT): Z_k = sum_x z_x W^{kx mod n}.
+// Direct DFT (chains that are not a power of two and whose 2n entries fit in
+// LDS): Z_k = sum_x z_x W^{kx mod n}.
 __device__ void lds_dft(const double2 *buf, double2 *out_k, uint32_t k, uint32_t n, const double2 *__restrict__ tw) {
 	double sr = 0, si = 0;
 	uint32_t idx = 0;
@@ -112,24 +116,24 @@ __device__ void lds_dft(const double2 *buf, double2 *out_k, uint32_t k, uint32_t
 	*out_k = make_double2(sr, si);
 }
 
-template <bool DNLS_MODE>
+template <bool DNLS_MODE, uint32_t MAXN>
 __global__ void __launch_bounds__(kThreads)
 k_energy(const double2 *__restrict__ state, uint32_t n, uint32_t copies, const double2 *__restrict__ tw,
          const double *__restrict__ omega, double *__restrict__ partial, int use_fft) {
 	extern __shared__ double2 lds[]; // n (FFT) or 2n (DFT) entries
-	constexpr int kMaxModes = kEnergyFftMax / kThreads;
+	constexpr int kMaxModes = MAXN / kThreads;
 	double acc[kMaxModes];
 #pragma unroll
 	for (int t = 0; t < kMaxModes; ++t) acc[t] = 0.0;
 	const uint32_t tid = threadIdx.x;
-	// modes owned by this thread: k = tid + t*kThreads; the DFT path (n > 4096
-	// allowed) accumulates directly into partial instead.
+	// modes owned by this thread: k = tid + t*kThreads; the DFT path
+	// accumulates directly into partial instead.
 	for (uint32_t c = blockIdx.x; c < copies; c += gridDim.x) {
 		const double2 *chain = state + size_t(c) * n;
 		for (uint32_t x = tid; x < n; x += kThreads) lds[x] = chain[x];
 		__syncthreads();
 		if (use_fft) {
-			lds_fft(lds, n, tw);
+			lds_fft<MAXN>(lds, n, tw);
 #pragma unroll
 			for (int t = 0; t < kMaxModes; ++t) {
 				const uint32_t k = tid + t * kThreads;
@@ -175,6 +179,44 @@ k_energy(const double2 *__restrict__ state, uint32_t n, uint32_t copies, const d
 		for (int t = 0; t < kMaxModes; ++t) {
 			const uint32_t k = tid + t * kThreads;
 			if (k < n) partial[size_t(blockIdx.x) * n + k] = acc[t];
+		}
+	}
+}
+
+// Chains too long for LDS (not a power of two above 5120 sites, or a power of
+// two above 8192): direct DFT with the chain read from L1/L2.  Thread k
+// accumulates Z_k = sum_x z_x W^{kx} and Z_{n-k} = sum_x z_x conj(W^{kx}) in the
+// same pass; all threads read the same z_x at a time.  O(n^2) per chain.
+template <bool DNLS_MODE>
+__global__ void __launch_bounds__(kThreads)
+k_energy_gdft(const double2 *__restrict__ state, uint32_t n, uint32_t copies, const double2 *__restrict__ tw,
+              const double *__restrict__ omega, double *__restrict__ partial) {
+	for (uint32_t c = blockIdx.x; c < copies; c += gridDim.x) {
+		const double2 *chain = state + size_t(c) * n;
+		for (uint32_t k = threadIdx.x; k < n; k += kThreads) {
+			double zr = 0, zi = 0, mr = 0, mi = 0;
+			uint32_t idx = 0;
+			for (uint32_t x = 0; x < n; ++x) {
+				const double2 w = tw[idx], z = chain[x];
+				zr = fma(z.x, w.x, fma(-z.y, w.y, zr)); // z W
+				zi = fma(z.x, w.y, fma(z.y, w.x, zi));
+				if constexpr (!DNLS_MODE) {
+					mr = fma(z.x, w.x, fma(z.y, w.y, mr)); // z conj(W)
+					mi = fma(z.y, w.x, fma(-z.x, w.y, mi));
+				}
+				idx += k;
+				if (idx >= n) idx -= n;
+			}
+			double e;
+			if constexpr (DNLS_MODE) {
+				e = zr * zr + zi * zi;
+			} else {
+				const double w = omega[k], a = 1.0 + w, b = 1.0 - w;
+				const double er = fma(a, zr, -b * mr);
+				const double ei = fma(a, zi, b * mi);
+				e = 0.25 * fma(er, er, ei * ei);
+			}
+			partial[size_t(blockIdx.x) * n + k] += e;
 		}
 	}
 }
@@ -295,31 +337,44 @@ k_chain_energy(int model, const double2 *__restrict__ state, uint32_t n, StepArg
 	}
 }
 
-bool fft_ok(uint32_t n) { return n >= 2 && (n & (n - 1)) == 0 && n <= kEnergyFftMax; }
+bool is_pow2(uint32_t n) { return n >= 2 && (n & (n - 1)) == 0; }
+
+template <bool DNLS_MODE, uint32_t MAXN>
+hipError_t launch_lds_energy(hipStream_t st, const double2 *state, uint32_t n, uint32_t copies, const double2 *tw,
+                             const double *omega, double *partial, uint32_t grid, int use_fft) {
+	const size_t lds = sizeof(double2) * n * (use_fft ? 1 : 2);
+	if (lds > 64 * 1024) {
+		hipError_t e = hipFuncSetAttribute((const void *) k_energy<DNLS_MODE, MAXN>,
+		                                   hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+		if (e != hipSuccess) return e;
+	}
+	hipLaunchKernelGGL((k_energy<DNLS_MODE, MAXN>), dim3(grid), dim3(kThreads), lds, st, state, n, copies, tw, omega,
+	                   partial, use_fft);
+	return hipGetLastError();
+}
+
+template <bool DNLS_MODE>
+hipError_t launch_energy_mode(hipStream_t st, const double2 *state, uint32_t n, uint32_t copies, const double2 *tw,
+                              const double *omega, double *partial, uint32_t grid) {
+	if (is_pow2(n) && n <= kEnergyFftMax)
+		return launch_lds_energy<DNLS_MODE, kEnergyFftMax>(st, state, n, copies, tw, omega, partial, grid, 1);
+	if (is_pow2(n) && n <= kEnergyFftLarge)
+		return launch_lds_energy<DNLS_MODE, kEnergyFftLarge>(st, state, n, copies, tw, omega, partial, grid, 1);
+	// the DFT paths accumulate into partial over the copies of a workgroup
+	hipLaunchKernelGGL(k_zero, dim3(1024), dim3(256), 0, st, partial, size_t(grid) * n);
+	if (2 * sizeof(double2) * n <= 160 * 1024)
+		return launch_lds_energy<DNLS_MODE, kEnergyFftMax>(st, state, n, copies, tw, omega, partial, grid, 0);
+	hipLaunchKernelGGL(k_energy_gdft<DNLS_MODE>, dim3(grid), dim3(kThreads), 0, st, state, n, copies, tw, omega,
+	                   partial);
+	return hipGetLastError();
+}
 
 } // namespace
 
 static hipError_t launch_energy_common(hipStream_t st, bool dnls, const double2 *state, uint32_t n, uint32_t copies,
                                        const double2 *tw, const double *omega, double *partial, uint32_t grid) {
-	const int use_fft = fft_ok(n) ? 1 : 0;
-	const size_t lds = sizeof(double2) * n * (use_fft ? 1 : 2);
-	if (lds > 160 * 1024) return hipErrorInvalidValue;
-	if (!use_fft) {
-		const size_t cnt = size_t(grid) * n;
-		hipLaunchKernelGGL(k_zero, dim3(1024), dim3(256), 0, st, partial, cnt);
-	}
-	if (lds > 64 * 1024) {
-		hipError_t e = hipFuncSetAttribute(dnls ? (const void *) k_energy<true> : (const void *) k_energy<false>,
-		                                   hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
-		if (e != hipSuccess) return e;
-	}
-	if (dnls)
-		hipLaunchKernelGGL(k_energy<true>, dim3(grid), dim3(kThreads), lds, st, state, n, copies, tw, omega, partial,
-		                   use_fft);
-	else
-		hipLaunchKernelGGL(k_energy<false>, dim3(grid), dim3(kThreads), lds, st, state, n, copies, tw, omega, partial,
-		                   use_fft);
-	return hipGetLastError();
+	return dnls ? launch_energy_mode<true>(st, state, n, copies, tw, omega, partial, grid)
+	            : launch_energy_mode<false>(st, state, n, copies, tw, omega, partial, grid);
 }
 
 hipError_t launch_energy_real(hipStream_t st, const double2 *state, uint32_t n, uint32_t copies, const double2 *tw,

@@ -138,7 +138,9 @@ def test_move_is_deterministic(gpu):
 
 @pytest.mark.parametrize("model_name,n,copies", [("DNKG", 64, 50), ("DNKG", 1024, 64), ("FPUT", 256, 33),
                                                  ("Toda", 128, 20), ("dDNKG", 512, 70), ("dDNKG", 100, 9),
-                                                 ("DNKG", 3000, 3), ("FPUT", 96, 5)])
+                                                 ("DNKG", 3000, 3), ("FPUT", 96, 5),
+                                                 # 8192-point LDS FFT; DFT in 160 KB of LDS; DFT from L1/L2
+                                                 ("DNKG", 8192, 5), ("Toda", 5120, 3), ("FPUT", 6000, 3)])
 def test_linenergies_and_entropy(gpu, model_name, n, copies):
     N = gpu
     banner(f"linenergies {model_name} n={n} copies={copies}")
