@@ -85,7 +85,8 @@ typedef struct nlc_params {
 	int32_t model;            /* nlc_model */
 	int32_t precision;        /* nlc_precision */
 	int32_t kernel;           /* nlc_kernel */
-	uint32_t chain_length;    /* -n, >= 2 */
+	uint32_t chain_length;    /* -n, >= 2; DNLS: at most 4096, or a power of two up to 8192
+	                             (NLC_EUNSUPPORTED otherwise) */
 	uint32_t copies;          /* -c, >= 1 */
 	double dt;                /* --dt, > 0 */
 	double m;                 /* -m (DNKG linear parameter) */

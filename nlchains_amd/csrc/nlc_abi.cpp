@@ -306,8 +306,9 @@ int setup_common(nlc_ctx *c) {
 int setup_dnls(nlc_ctx *c) {
 	const uint32_t n = c->n;
 	const nlc_params &p = c->p;
-	if (!is_pow2(n) || n > nlc::kEnergyFftMax)
-		return c->fail(NLC_EUNSUPPORTED, "DNLS needs a power-of-two chain_length <= %u in this build", nlc::kEnergyFftMax);
+	if (!nlc::dnls_supported(n))
+		return c->fail(NLC_EUNSUPPORTED, "DNLS needs a chain_length of 2 to %u, or a power of two up to %u", 4096u,
+		               nlc::kDnlsLdsMax);
 	const double bdt = p.beta * p.dt;
 	for (int k = 0; k < 8; ++k) c->theta[k] = (bdt * 1.) * nlc::kYoshidaC[k];
 	c->theta[8] = (bdt * 2.) * nlc::kYoshidaC[0];
@@ -321,8 +322,12 @@ int setup_dnls(nlc_ctx *c) {
 			tab[size_t(k) * n + j] = make_double2(std::cos(ang) * nrm, std::sin(ang) * nrm);
 		}
 	// generic kernel: the 7 tables; n = 4096: the 4 distinct ones in the order of
-	// each register kernel, radix 16 at [0, 4n), radix 8 at [4n, 8n)
-	if (nlc::dnls_fast_path(n)) {
+	// each register kernel, radix 16 at [0, 4n), radix 8 at [4n, 8n); lengths
+	// that are not a power of two: the 7 tables and the Bluestein tables
+	if (const uint32_t M = nlc::dnls_bluestein_size(n)) {
+		tab.resize(size_t(7) * n + size_t(2) * M + n);
+		nlc::dnls_bluestein_tables(n, tab.data() + size_t(7) * n);
+	} else if (nlc::dnls_fast_path(n)) {
 		perm.resize(size_t(8) * n);
 		nlc::dnls_permute_tables(n, nlc::kDnlsRadix16, tab.data(), perm.data());
 		nlc::dnls_permute_tables(n, nlc::kDnlsRadix8, tab.data(), perm.data() + size_t(4) * n);
@@ -1012,7 +1017,8 @@ int nlc_describe(const nlc_ctx *c, char *buf, size_t len) {
 	static const char *names[] = {"DNKG", "FPUT", "Toda", "dDNKG", "DNLS"};
 	if (c->p.model == NLC_DNLS)
 		snprintf(buf, len, "model=DNLS n=%u copies=%u kernel=%s", c->n, c->copies,
-		         !nlc::dnls_fast_path(c->n)                ? "dnls_lds_stockham"
+		         nlc::dnls_bluestein_size(c->n)            ? "dnls_lds_bluestein"
+		         : !nlc::dnls_fast_path(c->n)              ? "dnls_lds_stockham"
 		         : c->dnls_variant == nlc::kDnlsRadix8 ? "dnls_radix8_registers"
 		                                                : "dnls_radix16_registers");
 	else if (c->ragged)

@@ -23,12 +23,20 @@
 //    offset, and neither exchange has bank conflicts.
 //  * k_dnls_4096_r8: the same transform as four radix-8 passes on 512 threads
 //    x 8 values (more wavefronts per SIMD, one more LDS exchange).
-//  * k_dnls_lds: any power-of-two n <= 4096, chain in LDS, radix-4 Stockham
+//  * k_dnls_lds: any power-of-two n <= 8192, chain in LDS, radix-4 Stockham
 //    passes (the inverse transform as conj(FFT(conj(.)))).
+//  * k_dnls_bluestein: any other n <= 4096.  Each n-point DFT is Bluestein's
+//    chirp-z convolution (L. Bluestein, 1970): with w_j = e^{-i pi j^2/n},
+//    X_k = w_k sum_x (w_x psi_x) conj(w_{k-x}), a cyclic convolution of length
+//    M >= 2n - 1 (a power of two) done as two M-point FFTs in LDS.
 //  * the nonlinear phase e^{-i theta}: Taylor polynomials of sin and cos on
 //    |r| <= pi/16 (degree 11 and 12), after a Cody & Waite reduction by pi/8
 //    for larger angles.
 #include "nlc_internal.hpp"
+
+#include <cmath>
+#include <utility>
+#include <vector>
 
 namespace nlc {
 namespace {
@@ -600,10 +608,11 @@ k_dnls_4096_r8(double2 *__restrict__ state, uint32_t copies, uint32_t steps, Dnl
 
 // ------------------------------------------------------------ generic n
 
-constexpr int kMaxPer = 4; // radix-4 butterflies per thread per pass (n <= 4096)
-
-// In-place radix-4 (+ final radix-2) Stockham FFT, forward sign, natural order.
+// In-place radix-4 (+ final radix-2) Stockham FFT of n <= MAXN points, forward
+// sign, natural order; tw[j] = e^{-2 pi i j/n}.
+template <uint32_t MAXN>
 __device__ void lds_fft_fwd(double2 *buf, uint32_t n, const double2 *__restrict__ tw) {
+	constexpr int kMaxPer = MAXN / 4 / kT; // radix-4 butterflies per thread per pass
 	const uint32_t tid = threadIdx.x, quarter = n >> 2;
 	uint32_t ns = 1;
 	for (; ns * 4 <= n; ns *= 4) {
@@ -657,6 +666,7 @@ __device__ void lds_fft_fwd(double2 *buf, uint32_t n, const double2 *__restrict_
 
 // Chain in LDS.  Between substeps the buffer holds conj(psi): the inverse
 // transform is conj(FFT(conj(X T))).
+template <uint32_t MAXN>
 __global__ void __launch_bounds__(kT)
 k_dnls_lds(double2 *__restrict__ state, uint32_t n, uint32_t copies, uint32_t steps, DnlsArgs a,
            const double2 *__restrict__ tw, const double2 *__restrict__ table) {
@@ -674,11 +684,11 @@ k_dnls_lds(double2 *__restrict__ state, uint32_t n, uint32_t copies, uint32_t st
 				buf[x] = nonlinear(p, th);
 			}
 			__syncthreads();
-			lds_fft_fwd(buf, n, tw);
+			lds_fft_fwd<MAXN>(buf, n, tw);
 			const double2 *tk = table + size_t(k) * n;
 			for (uint32_t j = tid; j < n; j += kT) buf[j] = cconj(cmul(buf[j], tk[j]));
 			__syncthreads();
-			lds_fft_fwd(buf, n, tw);
+			lds_fft_fwd<MAXN>(buf, n, tw);
 			conj_state = true;
 		}
 	}
@@ -688,7 +698,112 @@ k_dnls_lds(double2 *__restrict__ state, uint32_t n, uint32_t copies, uint32_t st
 	}
 }
 
+// Cyclic convolution of buf[0..M) with b_j = conj(w_j) (|j| < n, zero
+// elsewhere), returned conjugated: buf <- conj(IFFT(FFT(buf) bhat)) with
+// bhat = FFT(b)/M, as FFT(conj(FFT(buf) bhat)).
+template <uint32_t MAXN>
+__device__ void bluestein_conv(double2 *buf, uint32_t M, const double2 *__restrict__ twm,
+                               const double2 *__restrict__ bhat) {
+	lds_fft_fwd<MAXN>(buf, M, twm);
+	for (uint32_t j = threadIdx.x; j < M; j += kT) buf[j] = cconj(cmul(buf[j], bhat[j]));
+	__syncthreads();
+	lds_fft_fwd<MAXN>(buf, M, twm);
+}
+
+// Chain in LDS as u_x = w_x psi_x (x < n), M entries.  One substep:
+//   nonlinear: |u_x| = |psi_x|, so u_x <- u_x e^{-i theta |u_x|^2};
+//   forward DFT: conv(u) holds conj(C_k), X_k = w_k C_k;
+//   backward DFT of Y = X T as conj(DFT(conj Y)): its Bluestein input
+//   conj(Y_k) w_k = conj(C_k) conj(T_k), and the second conv(.) leaves
+//   conj(C'_x) = w_x psi'_x, the same representation: the chirps cancel
+//   between substeps and are applied only at load and store.
+// table: 7 multipliers of n entries, then bhat (M), the M-point twiddles (M),
+// the chirp w (n).
+template <uint32_t MAXN>
+__global__ void __launch_bounds__(kT)
+k_dnls_bluestein(double2 *__restrict__ state, uint32_t n, uint32_t M, uint32_t copies, uint32_t steps, DnlsArgs a,
+                 const double2 *__restrict__ table) {
+	extern __shared__ double2 buf[];
+	const double2 *bhat = table + size_t(7) * n, *twm = bhat + M, *chirp = twm + M;
+	const uint32_t chain = blockIdx.x, tid = threadIdx.x;
+	const double2 zero = make_double2(0.0, 0.0);
+	double2 *psi = state + size_t(chain) * n;
+	for (uint32_t x = tid; x < n; x += kT) buf[x] = cmul(psi[x], chirp[x]);
+	for (uint32_t s = 0; s < steps; ++s) {
+		for (int k = 0; k < 7; ++k) {
+			const double th = (k == 0 && s) ? a.theta[8] : a.theta[k];
+			for (uint32_t x = tid; x < M; x += kT) buf[x] = x < n ? nonlinear(buf[x], th) : zero;
+			__syncthreads();
+			bluestein_conv<MAXN>(buf, M, twm, bhat);
+			const double2 *tk = table + size_t(k) * n;
+			for (uint32_t j = tid; j < M; j += kT) buf[j] = j < n ? cmul(buf[j], cconj(tk[j])) : zero;
+			__syncthreads();
+			bluestein_conv<MAXN>(buf, M, twm, bhat);
+		}
+	}
+	for (uint32_t x = tid; x < n; x += kT) psi[x] = nonlinear(cmul(buf[x], cconj(chirp[x])), a.theta[7]);
+}
+
+template <typename K>
+hipError_t launch_lds_chain_kernel(K kernel, size_t lds) {
+	if (lds > 64 * 1024)
+		return hipFuncSetAttribute((const void *) kernel, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+	return hipSuccess;
+}
+
+bool pow2(uint32_t n) { return n >= 2 && (n & (n - 1)) == 0; }
+
+// In-place radix-2 FFT on the host (forward sign, exact twiddles), for bhat.
+void host_fft(std::vector<double2> &v) {
+	const size_t n = v.size();
+	for (size_t i = 1, j = 0; i < n; ++i) {
+		size_t bit = n >> 1;
+		for (; j & bit; bit >>= 1) j ^= bit;
+		j ^= bit;
+		if (i < j) std::swap(v[i], v[j]);
+	}
+	for (size_t len = 2; len <= n; len <<= 1)
+		for (size_t j = 0; j < len / 2; ++j) {
+			const double ang = -2.0 * M_PI * double(j) / double(len);
+			const double wr = std::cos(ang), wi = std::sin(ang);
+			for (size_t b = 0; b < n; b += len) {
+				double2 &u = v[b + j], &t = v[b + j + len / 2];
+				const double tr = t.x * wr - t.y * wi, ti = t.x * wi + t.y * wr;
+				t = make_double2(u.x - tr, u.y - ti);
+				u = make_double2(u.x + tr, u.y + ti);
+			}
+		}
+}
+
 } // namespace
+
+uint32_t dnls_bluestein_size(uint32_t n) {
+	if (n < 2 || pow2(n)) return 0;
+	uint32_t M = 1;
+	while (M < 2 * n - 1) M <<= 1;
+	return M <= kDnlsBluesteinMax ? M : 0;
+}
+
+bool dnls_supported(uint32_t n) { return (pow2(n) && n <= kDnlsLdsMax) || dnls_bluestein_size(n) != 0; }
+
+void dnls_bluestein_tables(uint32_t n, double2 *out) {
+	const uint32_t M = dnls_bluestein_size(n);
+	std::vector<double2> b(M, make_double2(0.0, 0.0));
+	// w_j = e^{-i pi j^2/n}, with j^2 reduced mod 2n so the angle stays in [0, 2 pi)
+	auto chirp_angle = [n](uint32_t j) { return M_PI * double((uint64_t(j) * j) % (2 * uint64_t(n))) / double(n); };
+	for (uint32_t j = 0; j < n; ++j) {
+		const double ang = chirp_angle(j);
+		b[j] = make_double2(std::cos(ang), std::sin(ang)); // conj(w_j)
+		if (j) b[M - j] = b[j];
+		out[size_t(2) * M + j] = make_double2(std::cos(ang), -std::sin(ang)); // w_j
+	}
+	host_fft(b);
+	for (uint32_t j = 0; j < M; ++j) {
+		out[j] = make_double2(b[j].x / M, b[j].y / M);
+		const double ang = 2.0 * M_PI * double(j) / double(M);
+		out[M + j] = make_double2(std::cos(ang), -std::sin(ang));
+	}
+}
 
 // Fourier multiplier tables (host side).  table[k][j] = T_k[j] for the generic
 // kernel; for n = 4096 the 4 distinct tables (d_k = d_{6-k}) are stored in the
@@ -723,14 +838,29 @@ hipError_t launch_dnls(hipStream_t st, double2 *state, uint32_t n, uint32_t copi
 		hipLaunchKernelGGL(k_dnls_4096_r8, dim3(copies), dim3(kT8), kR8Lds, st, state, copies, steps, a, tw, table);
 	} else if (dnls_fast_path(n)) {
 		hipLaunchKernelGGL(k_dnls_4096, dim3(copies), dim3(kT), 0, st, state, copies, steps, a, tw, table);
-	} else {
+	} else if (pow2(n) && n <= kDnlsLdsMax) {
 		const size_t lds = sizeof(double2) * n;
-		if (lds > 64 * 1024) {
-			hipError_t e = hipFuncSetAttribute((const void *) k_dnls_lds, hipFuncAttributeMaxDynamicSharedMemorySize,
-			                                   int(lds));
+		if (n <= 4096) {
+			hipLaunchKernelGGL(k_dnls_lds<4096>, dim3(copies), dim3(kT), lds, st, state, n, copies, steps, a, tw, table);
+		} else {
+			hipError_t e = launch_lds_chain_kernel(k_dnls_lds<kDnlsLdsMax>, lds);
 			if (e != hipSuccess) return e;
+			hipLaunchKernelGGL(k_dnls_lds<kDnlsLdsMax>, dim3(copies), dim3(kT), lds, st, state, n, copies, steps, a, tw,
+			                   table);
 		}
-		hipLaunchKernelGGL(k_dnls_lds, dim3(copies), dim3(kT), lds, st, state, n, copies, steps, a, tw, table);
+	} else if (const uint32_t M = dnls_bluestein_size(n)) {
+		const size_t lds = sizeof(double2) * M;
+		if (M <= 4096) {
+			hipLaunchKernelGGL(k_dnls_bluestein<4096>, dim3(copies), dim3(kT), lds, st, state, n, M, copies, steps, a,
+			                   table);
+		} else {
+			hipError_t e = launch_lds_chain_kernel(k_dnls_bluestein<kDnlsBluesteinMax>, lds);
+			if (e != hipSuccess) return e;
+			hipLaunchKernelGGL(k_dnls_bluestein<kDnlsBluesteinMax>, dim3(copies), dim3(kT), lds, st, state, n, M,
+			                   copies, steps, a, table);
+		}
+	} else {
+		return hipErrorInvalidValue;
 	}
 	return hipGetLastError();
 }

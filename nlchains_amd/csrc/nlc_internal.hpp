@@ -105,8 +105,16 @@ namespace nlc {
 // theta9[8] = beta dt 2 c_0; table: 7 Fourier multipliers of n entries, or for
 // the register kernels (dnls_fast_path: n = 4096, radix 16 with 256 threads or
 // radix 8 with 512 threads per chain) the 4 distinct ones permuted by
-// dnls_permute_tables for that variant.
+// dnls_permute_tables for that variant.  Chain lengths: powers of two up to
+// kDnlsLdsMax; any other length whose Bluestein convolution size
+// (dnls_bluestein_size, the power of two >= 2n - 1) is at most
+// kDnlsBluesteinMax, with dnls_bluestein_tables' 2M + n entries after the 7
+// multipliers.
 constexpr int kDnlsRadix16 = 0, kDnlsRadix8 = 1;
+constexpr uint32_t kDnlsLdsMax = 8192, kDnlsBluesteinMax = 8192;
+bool dnls_supported(uint32_t n);
+uint32_t dnls_bluestein_size(uint32_t n);
+void dnls_bluestein_tables(uint32_t n, double2 *out);
 bool dnls_fast_path(uint32_t n);
 void dnls_permute_tables(uint32_t n, int variant, const double2 *table, double2 *out);
 hipError_t launch_dnls(hipStream_t, double2 *state, uint32_t n, uint32_t copies, uint32_t steps,

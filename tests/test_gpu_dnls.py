@@ -33,7 +33,10 @@ def run(N, n, copies, steps, dt, beta, s0, spl=0):
 
 
 @pytest.mark.parametrize("n,copies,steps", [(4096, 6, 100), (2, 5, 100), (16, 5, 100), (64, 7, 100),
-                                            (256, 5, 100), (1024, 3, 100), (2048, 3, 60)])
+                                            (256, 5, 100), (1024, 3, 100), (2048, 3, 60), (8192, 3, 20),
+                                            # Bluestein: convolution sizes 8, 16, 256, 2048, 8192, 8192
+                                            (3, 5, 100), (5, 4, 100), (100, 5, 100), (1000, 3, 30),
+                                            (3000, 2, 5), (4095, 2, 3)])
 def test_dnls_move_parity(gpu, n, copies, steps):
     N = gpu
     dt, beta = 0.01, 1.0
@@ -91,7 +94,7 @@ def test_dnls_batches_compose(gpu, spl):
     assert rel_err(a, b) < 1e-13
 
 
-@pytest.mark.parametrize("n,copies", [(4096, 5), (256, 9), (64, 3)])
+@pytest.mark.parametrize("n,copies", [(4096, 5), (256, 9), (64, 3), (100, 7), (8192, 2)])
 def test_dnls_linenergies(gpu, n, copies):
     N = gpu
     s0 = N.ic_dnls(n, copies, max(1, n // 16), 1.0, 4)
@@ -144,3 +147,26 @@ def test_dnls_autotune_and_unsupported_shapes(gpu):
         N.Ensemble(N.DNLS, 4096, 4, 0.01, beta=1.0, sites_per_lane=4)
     with pytest.raises(N.NlcError):
         N.Ensemble(N.DNLS, 1024, 4, 0.01, beta=1.0, sites_per_lane=8)
+    for n in (1, 4097, 16384):  # beyond the LDS kernels
+        with pytest.raises(N.NlcError):
+            N.Ensemble(N.DNLS, n, 4, 0.01, beta=1.0)
+
+
+@pytest.mark.parametrize("n", [100, 8192])
+def test_dnls_generic_kernels_describe_and_compose(gpu, n):
+    """The Bluestein and 8192-point LDS kernels: kernel names, and two batches
+    of 10 steps equal one of 20 to rounding."""
+    N = gpu
+    s0 = N.ic_dnls(n, 3, max(1, n // 16), 1.0, 5)
+    with N.Ensemble(N.DNLS, n, 3, 0.01, beta=1.0) as ens:
+        desc = ens.describe()
+        ens.upload(s0)
+        ens.move(10)
+        ens.move(10)
+        a = ens.download()
+        ens.upload(s0)
+        ens.move(20)
+        b = ens.download()
+    banner(f"  {desc}")
+    assert ("bluestein" in desc) == (n == 100) and ("stockham" in desc) == (n == 8192)
+    assert rel_err(a, b) < 1e-12
