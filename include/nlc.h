@@ -194,6 +194,20 @@ int nlc_ragged_classes(const nlc_ctx *ctx, uint32_t *nclasses, uint32_t *lengths
  * The batched counterpart of one iteration of DNKG_FPUT_Toda.cpp:108-143. */
 int nlc_batch(nlc_ctx *ctx, uint32_t steps, double norm, double *linenergies, double *entropies);
 
+/* Pipelined batches.  nlc_batch_submit queues one batch as nlc_batch does and
+ * returns without waiting: its results are copied to one of NLC_BATCH_SLOTS
+ * pinned slots.  nlc_batch_collect waits for the oldest queued batch and
+ * returns its outputs (as nlc_batch's), so results come back in submission
+ * order.  At most NLC_BATCH_SLOTS batches may be queued (NLC_EARG beyond,
+ * or for a collect with none queued).  Every other call is ordered after the
+ * queued batches on the context's stream: nlc_download, nlc_linenergies and
+ * nlc_chain_energies see the state after all of them.
+ * Replaces the reference's batch throttle (at most two batches in flight) and
+ * its result callbacks: common/loop_control.hpp:40-57, DNKG_FPUT_Toda.cpp:127-143. */
+#define NLC_BATCH_SLOTS 4
+int nlc_batch_submit(nlc_ctx *ctx, uint32_t steps, double norm);
+int nlc_batch_collect(nlc_ctx *ctx, double *linenergies, double *entropies);
+
 /* Times every applicable step-kernel shape (wave shapes lanes x sites and the
  * split kernel for uniform ensembles; sites per lane of the persistent kernel
  * for ragged ones) on the context's own state for trial_steps steps, keeps the

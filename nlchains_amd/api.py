@@ -87,6 +87,9 @@ _u32p = C.POINTER(C.c_uint32)
 _sig("nlc_create_ragged", C.c_int, C.POINTER(_ctx_p), C.POINTER(Params), _u32p, C.c_uint32)
 _sig("nlc_ragged_classes", C.c_int, _ctx_p, _u32p, _u32p, _u32p, C.c_uint32)
 _sig("nlc_batch", C.c_int, _ctx_p, C.c_uint32, C.c_double, _dp, _dp)
+_sig("nlc_batch_submit", C.c_int, _ctx_p, C.c_uint32, C.c_double)
+_sig("nlc_batch_collect", C.c_int, _ctx_p, _dp, _dp)
+BATCH_SLOTS = 4  # NLC_BATCH_SLOTS
 _sig("nlc_autotune", C.c_int, _ctx_p, C.c_uint32, C.c_char_p, C.c_size_t)
 _sig("nlc_ic_ragged_lengths", C.c_int, C.c_uint32, _u32p, C.c_uint32, C.c_uint64, _u32p)
 _sig("nlc_ic_ragged_plane_waves", C.c_int, _u32p, C.c_uint32, C.c_double, C.c_double, C.c_double, C.c_double,
@@ -279,6 +282,16 @@ class Ensemble:
         """One dump interval (move + energies + entropies) from a cached hipGraph."""
         e, s = np.zeros(self._nlin), np.zeros(self._nent)
         _check(lib.nlc_batch(self._ctx, steps, norm, _ptr(e), _ptr(s)), self._ctx)
+        return e, s
+
+    def batch_submit(self, steps: int, norm: float = 0.0) -> None:
+        """Queue one batch without waiting (at most BATCH_SLOTS queued)."""
+        _check(lib.nlc_batch_submit(self._ctx, steps, norm), self._ctx)
+
+    def batch_collect(self) -> tuple[np.ndarray, np.ndarray]:
+        """Wait for the oldest queued batch; its energies and entropies."""
+        e, s = np.zeros(self._nlin), np.zeros(self._nent)
+        _check(lib.nlc_batch_collect(self._ctx, _ptr(e), _ptr(s)), self._ctx)
         return e, s
 
     def autotune(self, trial_steps: int = 20) -> str:

@@ -62,7 +62,6 @@ struct nlc_ctx {
 	double *partial = nullptr; // energy partial sums
 	double *reduce_scratch = nullptr;
 	double *e_raw = nullptr;   // table_len
-	double *results = nullptr; // per class: n energies + 2 entropies
 	uint32_t *modes = nullptr; // entropy mode indices
 	uint32_t nmodes = 0;
 	int a0is0 = 0;
@@ -76,7 +75,13 @@ struct nlc_ctx {
 	double2 *dnls_table = nullptr; // DNLS Fourier multipliers
 	int dnls_variant = nlc::kDnlsRadix16; // n = 4096: register kernel, radix 16 or radix 8
 	double theta[9] = {};          // DNLS nonlinear angles
-	double *h_results = nullptr;   // pinned copy of results
+	// results (per class: n energies + 2 entropies) are written by the results
+	// kernel straight into pinned host memory: h_results for the synchronous
+	// calls, one of the slots h_slots for queued batches (nlc_batch_submit)
+	double *h_results = nullptr;
+	double *h_slots = nullptr;
+	hipEvent_t slot_ev[NLC_BATCH_SLOTS] = {};
+	uint32_t slot_head = 0, slot_count = 0;
 	// host transfers of the shard: pinned double buffer and the ragged chains in device order
 	double2 *pinned[2] = {nullptr, nullptr};
 	hipEvent_t pinned_ev[2] = {nullptr, nullptr};
@@ -86,11 +91,14 @@ struct nlc_ctx {
 	int L = 0, S = 0;
 	bool exact = false, use_split = false;
 
-	// cached graph of one batch (nlc_batch)
-	hipGraph_t graph = nullptr;
-	hipGraphExec_t graph_exec = nullptr;
-	uint32_t graph_steps = 0;
-	double graph_norm = 0;
+	// cached graphs of one batch, one per result destination: [0, NLC_BATCH_SLOTS)
+	// for the slots of queued batches, NLC_BATCH_SLOTS for nlc_batch
+	struct BatchGraph {
+		hipGraph_t graph = nullptr;
+		hipGraphExec_t exec = nullptr;
+		uint32_t steps = 0;
+		double norm = 0;
+	} graphs[NLC_BATCH_SLOTS + 1];
 
 	std::string err, tune_report;
 
@@ -192,11 +200,13 @@ bool ragged_fits(const std::vector<Klass> &cls, int S) {
 }
 
 void drop_graph(nlc_ctx *c) {
-	if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
-	if (c->graph) hipGraphDestroy(c->graph);
-	c->graph_exec = nullptr;
-	c->graph = nullptr;
-	c->graph_steps = 0;
+	// a queued replay may still be running
+	if (c->slot_count) hipStreamSynchronize(c->stream);
+	for (auto &g : c->graphs) {
+		if (g.exec) hipGraphExecDestroy(g.exec);
+		if (g.graph) hipGraphDestroy(g.graph);
+		g = nlc_ctx::BatchGraph();
+	}
 }
 
 void free_all(nlc_ctx *c) {
@@ -204,11 +214,14 @@ void free_all(nlc_ctx *c) {
 	drop_graph(c);
 	for (void *ptr :
 	     {(void *) c->state, (void *) c->omega, (void *) c->tw, (void *) c->partial, (void *) c->reduce_scratch,
-	      (void *) c->e_raw, (void *) c->results, (void *) c->modes, (void *) c->mp2, (void *) c->masses_d,
+	      (void *) c->e_raw, (void *) c->modes, (void *) c->mp2, (void *) c->masses_d,
 	      (void *) c->eigvec, (void *) c->split_q, (void *) c->split_p, (void *) c->energy_d, (void *) c->norm_d,
 	      (void *) c->dnls_table, (void *) c->items, (void *) c->vplanes, (void *) c->splanes})
 		if (ptr) hipFree(ptr);
 	if (c->h_results) hipHostFree(c->h_results);
+	if (c->h_slots) hipHostFree(c->h_slots);
+	for (hipEvent_t ev : c->slot_ev)
+		if (ev) hipEventDestroy(ev);
 	for (int i = 0; i < 2; ++i) {
 		if (c->pinned[i]) hipHostFree(c->pinned[i]);
 		if (c->pinned_ev[i]) hipEventDestroy(c->pinned_ev[i]);
@@ -249,12 +262,11 @@ int setup_common(nlc_ctx *c) {
 	NLC_TRY(dev_alloc(c, &c->omega, c->table_len, "omega"));
 	NLC_TRY(dev_alloc(c, &c->tw, c->table_len, "twiddles"));
 	NLC_TRY(dev_alloc(c, &c->e_raw, c->table_len, "linenergies"));
-	NLC_TRY(dev_alloc(c, &c->results, c->results_len(), "results"));
 	NLC_TRY(dev_alloc(c, &c->modes, c->n, "entropy modes"));
 	NLC_TRY(dev_alloc(c, &c->energy_d, c->copies, "chain energies"));
 	NLC_TRY(dev_alloc(c, &c->norm_d, c->copies, "chain norms"));
 	{
-		hipError_t e = hipHostMalloc((void **) &c->h_results, sizeof(double) * c->results_len(), hipHostMallocDefault);
+		hipError_t e = hipHostMalloc((void **) &c->h_results, sizeof(double) * c->results_len(), hipHostMallocCoherent);
 		if (e != hipSuccess) return c->fail(NLC_ENOMEM, "hipHostMalloc(results): %s", hipGetErrorString(e));
 	}
 	// per-class twiddles and dispersion, computed on the host in FP64
@@ -472,23 +484,21 @@ double default_norm(int model, const Klass &k) {
 	}
 }
 
-// Normalisation, entropies and the copy of all results to pinned memory.
-int enqueue_results(nlc_ctx *c, double norm) {
+// Normalisation and entropies, written to dst (pinned host memory, results_len doubles).
+int enqueue_results(nlc_ctx *c, double norm, double *dst) {
 	for (size_t i = 0; i < c->cls.size(); ++i) {
 		const Klass &k = c->cls[i];
 		const double nm = norm > 0 ? norm : default_norm(c->p.model, k);
 		NLC_HIP(c, nlc::launch_results(c->stream, c->e_raw + k.tab0, k.n, nm, c->a0is0, c->modes,
-		                               c->ragged ? 0 : c->nmodes, c->results + k.tab0 + 2 * i));
+		                               c->ragged ? 0 : c->nmodes, dst + k.tab0 + 2 * i));
 	}
-	NLC_HIP(c, hipMemcpyAsync(c->h_results, c->results, sizeof(double) * c->results_len(), hipMemcpyDeviceToHost,
-	                          c->stream));
 	return NLC_OK;
 }
 
-void unpack_results(const nlc_ctx *c, double *linenergies, double *entropies) {
+void unpack_results(const nlc_ctx *c, const double *src, double *linenergies, double *entropies) {
 	for (size_t i = 0; i < c->cls.size(); ++i) {
 		const Klass &k = c->cls[i];
-		const double *r = c->h_results + k.tab0 + 2 * i;
+		const double *r = src + k.tab0 + 2 * i;
 		if (linenergies) std::memcpy(linenergies + k.tab0, r, sizeof(double) * k.n);
 		if (entropies) entropies[2 * i] = r[k.n], entropies[2 * i + 1] = r[k.n + 1];
 	}
@@ -532,6 +542,37 @@ int finish_create(nlc_ctx *c, nlc_ctx **out, int rc) {
 	return NLC_OK;
 }
 
+
+// The cached graph of one batch for result destination `which`: move, energy
+// pass, normalisation and entropies.
+int ensure_batch_graph(nlc_ctx *c, uint32_t which, uint32_t steps, double norm, double *dst) {
+	if (!steps) return c->fail(NLC_EARG, "--kernel_batching must be a positive number");
+	nlc_ctx::BatchGraph &bg = c->graphs[which];
+	if (!bg.exec || bg.steps != steps || bg.norm != norm) {
+		if (bg.exec) {
+			if (c->slot_count) NLC_HIP(c, hipStreamSynchronize(c->stream));
+			hipGraphExecDestroy(bg.exec);
+			hipGraphDestroy(bg.graph);
+			bg = nlc_ctx::BatchGraph();
+		}
+		NLC_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
+		int rc = enqueue_move(c, steps);
+		if (!rc) rc = enqueue_energy(c);
+		if (!rc) rc = enqueue_results(c, norm, dst);
+		hipGraph_t g = nullptr;
+		const hipError_t e = hipStreamEndCapture(c->stream, &g);
+		if (rc) {
+			if (g) hipGraphDestroy(g);
+			return rc;
+		}
+		if (e != hipSuccess) return c->hip_fail(e, "hipStreamEndCapture");
+		bg.graph = g;
+		NLC_HIP(c, hipGraphInstantiate(&bg.exec, bg.graph, nullptr, nullptr, 0));
+		bg.steps = steps;
+		bg.norm = norm;
+	}
+	return NLC_OK;
+}
 } // namespace
 
 extern "C" {
@@ -820,36 +861,51 @@ int nlc_linenergies_raw(nlc_ctx *c, double *e_raw) {
 int nlc_linenergies(nlc_ctx *c, double norm, double *linenergies, double *entropies) {
 	if (!c) return NLC_EARG;
 	NLC_TRY(enqueue_energy(c));
-	NLC_TRY(enqueue_results(c, norm));
+	NLC_TRY(enqueue_results(c, norm, c->h_results));
 	NLC_HIP(c, hipStreamSynchronize(c->stream));
-	unpack_results(c, linenergies, entropies);
+	unpack_results(c, c->h_results, linenergies, entropies);
 	return NLC_OK;
 }
 
+
 int nlc_batch(nlc_ctx *c, uint32_t steps, double norm, double *linenergies, double *entropies) {
 	if (!c) return NLC_EARG;
-	if (!steps) return c->fail(NLC_EARG, "--kernel_batching must be a positive number");
-	if (!c->graph_exec || c->graph_steps != steps || c->graph_norm != norm) {
-		drop_graph(c);
-		NLC_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
-		int rc = enqueue_move(c, steps);
-		if (!rc) rc = enqueue_energy(c);
-		if (!rc) rc = enqueue_results(c, norm);
-		hipGraph_t g = nullptr;
-		const hipError_t e = hipStreamEndCapture(c->stream, &g);
-		if (rc) {
-			if (g) hipGraphDestroy(g);
-			return rc;
-		}
-		if (e != hipSuccess) return c->hip_fail(e, "hipStreamEndCapture");
-		c->graph = g;
-		NLC_HIP(c, hipGraphInstantiate(&c->graph_exec, c->graph, nullptr, nullptr, 0));
-		c->graph_steps = steps;
-		c->graph_norm = norm;
-	}
-	NLC_HIP(c, hipGraphLaunch(c->graph_exec, c->stream));
+	NLC_TRY(ensure_batch_graph(c, NLC_BATCH_SLOTS, steps, norm, c->h_results));
+	NLC_HIP(c, hipGraphLaunch(c->graphs[NLC_BATCH_SLOTS].exec, c->stream));
 	NLC_HIP(c, hipStreamSynchronize(c->stream));
-	unpack_results(c, linenergies, entropies);
+	unpack_results(c, c->h_results, linenergies, entropies);
+	return NLC_OK;
+}
+
+int nlc_batch_submit(nlc_ctx *c, uint32_t steps, double norm) {
+	if (!c) return NLC_EARG;
+	if (c->slot_count == NLC_BATCH_SLOTS)
+		return c->fail(NLC_EARG, "%d batches are already queued; collect one first", NLC_BATCH_SLOTS);
+	if (!c->h_slots) {
+		hipError_t e = hipHostMalloc((void **) &c->h_slots, sizeof(double) * c->results_len() * NLC_BATCH_SLOTS,
+		                             hipHostMallocCoherent);
+		if (e != hipSuccess) {
+			c->h_slots = nullptr;
+			return c->fail(NLC_ENOMEM, "hipHostMalloc(batch result slots): %s", hipGetErrorString(e));
+		}
+		for (hipEvent_t &ev : c->slot_ev) NLC_HIP(c, hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+	}
+	const uint32_t slot = c->slot_head;
+	NLC_TRY(ensure_batch_graph(c, slot, steps, norm, c->h_slots + c->results_len() * slot));
+	NLC_HIP(c, hipGraphLaunch(c->graphs[slot].exec, c->stream));
+	NLC_HIP(c, hipEventRecord(c->slot_ev[slot], c->stream));
+	c->slot_head = (slot + 1) % NLC_BATCH_SLOTS;
+	++c->slot_count;
+	return NLC_OK;
+}
+
+int nlc_batch_collect(nlc_ctx *c, double *linenergies, double *entropies) {
+	if (!c) return NLC_EARG;
+	if (!c->slot_count) return c->fail(NLC_EARG, "no batch is queued");
+	const uint32_t slot = (c->slot_head + NLC_BATCH_SLOTS - c->slot_count) % NLC_BATCH_SLOTS;
+	NLC_HIP(c, hipEventSynchronize(c->slot_ev[slot]));
+	--c->slot_count;
+	unpack_results(c, c->h_slots + c->results_len() * slot, linenergies, entropies);
 	return NLC_OK;
 }
 

@@ -246,16 +246,29 @@ int run_loop(Config &c, nlc_params p, bool split_kernel, const std::string &shap
 	}
 	bool synched = false;
 	// energies at the start; afterwards every batch is one replayed hipGraph
-	// (move, energy pass, entropies, result copy)
+	// (move, energy pass, entropies), up to two queued ahead of the results
+	// being written (loop_control.hpp:46-57); tq is the time the state reaches
+	// after the queued batches.  A batch that starts at a dump time is queued
+	// only once that dump has been taken.
+	constexpr uint32_t kAhead = 2;
+	uint64_t tq = t;
+	uint32_t queued = 0;
 	ctx.check(nlc_linenergies(ctx.ctx, 0.0, e.data(), s), "linenergies");
 	while (true) {
 		res.write_entropy(t, s);
 		res.check_entropy(s);
 		if (t % c.dump_interval == 0) dump(t);
-		// loop_control::break_now: on a quit request the end step becomes the current one
-		if (quit_requested && !synched) c.steps = t, synched = true;
+		// loop_control::break_now: on a quit request the end step becomes the
+		// step the queued work reaches, as in the reference
+		if (quit_requested && !synched) c.steps = tq, synched = true;
 		if (c.steps == t && (synched || c.steps)) break;
-		ctx.check(nlc_batch(ctx.ctx, c.batch, 0.0, e.data(), s), "batch");
+		while (queued < kAhead && !(c.steps && tq >= c.steps) && (tq == t || tq % c.dump_interval != 0)) {
+			ctx.check(nlc_batch_submit(ctx.ctx, c.batch, 0.0), "batch");
+			tq += c.batch;
+			++queued;
+		}
+		ctx.check(nlc_batch_collect(ctx.ctx, e.data(), s), "batch");
+		--queued;
 		t += c.batch;
 	}
 	if (t % c.dump_interval != 0) dump(t);

@@ -108,6 +108,52 @@ def test_graph_batches_match_eager(gpu, ragged):
             e.chain_energies()  # unrelated eager work between replays
 
 
+@pytest.mark.parametrize("ragged", [False, True])
+def test_queued_batches_match_synchronous(gpu, ragged):
+    """nlc_batch_submit / nlc_batch_collect: results come back in submission
+    order, equal to nlc_batch's; the slot ring wraps; the state after the
+    queue is the state after all queued batches; the queue depth is enforced."""
+    N = gpu
+    if ragged:
+        lengths = N.ic_ragged_lengths(100, [32, 64, 256], 3)
+        s0 = N.ic_ragged_plane_waves(lengths, 1.0, 0.05, 0.1, 0.3, 3)
+        mk = lambda: N.Ensemble.ragged(N.FPUT, lengths, 0.05, beta=1.0)  # noqa: E731
+    else:
+        s0 = N.ic_plane_waves(64, 1024, 0.0, 1, 5, 0.1, 1)
+        mk = lambda: N.Ensemble(N.FPUT, 64, 1024, 0.1, beta=1.0)  # noqa: E731
+    with mk() as q, mk() as s:
+        q.upload(s0)
+        s.upload(s0)
+        want = [s.batch(20) for _ in range(10)]
+        got = []
+        for i in range(N.BATCH_SLOTS):
+            q.batch_submit(20)
+        with pytest.raises(N.NlcError):
+            q.batch_submit(20)
+        got.append(q.batch_collect())
+        for i in range(6):  # keep three queued while collecting, wrapping the ring
+            q.batch_submit(20)
+            got.append(q.batch_collect())
+        while len(got) < 10:
+            got.append(q.batch_collect())
+        with pytest.raises(N.NlcError):
+            q.batch_collect()
+        for i, ((eg, sg), (ew, sw)) in enumerate(zip(got, want)):
+            assert np.array_equal(eg, ew) and np.array_equal(sg, sw), i
+        assert np.array_equal(q.download(), s.download())
+        # queued work followed by eager calls: they see the state after the queue
+        q.batch_submit(20)
+        q.batch_submit(20)
+        first = s.batch(20)
+        second = s.batch(20)
+        assert np.array_equal(q.download(), s.download())
+        eq, sq = q.linenergies()
+        assert np.array_equal(eq, second[0]) and np.array_equal(sq, second[1])
+        for want_e, want_s in (first, second):
+            eg, sg = q.batch_collect()
+            assert np.array_equal(eg, want_e) and np.array_equal(sg, want_s)
+
+
 def test_autotune_keeps_state_and_parity(gpu):
     N = gpu
     s0 = N.ic_plane_waves(64, 500, 0.0, 1, 8, 0.3, 5)
