@@ -93,7 +93,9 @@ typedef struct nlc_params {
 	double alpha;             /* --alpha (FPUT cubic term; Toda steepness, != 0) */
 	double beta;              /* --beta (DNKG, FPUT, dDNKG, DNLS) */
 	const double *masses;     /* dDNKG: m_x, chain_length doubles (content of the -m file); copied */
-	uint32_t lanes_per_chain; /* wave shape override (0 = default); used by the autotuner and tests */
+	uint32_t lanes_per_chain; /* wave shape override (0 = default); used by the autotuner and tests.
+	                             1..64 (power of two): lanes of one wavefront; 128..512 (multiple
+	                             of 64): one workgroup of lanes_per_chain/64 wavefronts per chain */
 	uint32_t sites_per_lane;  /* wave shape override (0 = default); DNLS with chain_length 4096:
 	                             16 or 8 complex values per thread (radix-16 or radix-8 kernel) */
 	int32_t projection;       /* nlc_projection (dDNKG only) */
@@ -232,7 +234,8 @@ int nlc_yoshida(double *c, double *d);
 int nlc_dispersion(int model, uint32_t n, double m, double *omega);
 /* Eigensystem of diag(mp2) - periodic adjacency (dDNKG/eigensystem.cpp:13-36):
  * omegas ascending = sqrt(eigenvalues); eigenvectors column-major n x n.
- * Householder tridiagonalisation + implicit QL. */
+ * n <= 512, or no HIP device: Householder tridiagonalisation + implicit QL on
+ * the host; larger n with a device: rocSOLVER dsyevd (loaded on first use). */
 int nlc_eigensystem(uint32_t n, const double *mp2, double *omegas, double *eigenvectors);
 
 /* Seeded initial conditions (SplitMix64 streams, one per copy).  The

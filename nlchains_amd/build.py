@@ -61,7 +61,7 @@ def build_library(jobs: int | None = None) -> Path:
         for fut in [ex.submit(_run, c) for c in cmds]:
             fut.result()
     tmp = LIB.with_suffix(".so.tmp")
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-o", str(tmp), *map(str, objs), "-lpthread"])
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-o", str(tmp), *map(str, objs), "-lpthread", "-ldl"])
     os.replace(tmp, LIB)
     return LIB
 

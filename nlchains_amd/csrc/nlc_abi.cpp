@@ -137,7 +137,29 @@ int log2u(uint32_t x) {
 	return l;
 }
 
+// A workgroup shape (L = 64 W lanes, W = 2..8) that the group launcher accepts for n.
+bool group_shape_ok(uint32_t n, int L, int S) {
+	if (L < 128 || L > 512 || L % 64 || S < 8 || S > 32) return false;
+	const uint64_t G = 64u * ((uint64_t(n) + 64u * S - 1) / (64u * S));
+	return G == uint64_t(L) && G * (S - 1) < n && G * S >= n;
+}
+
+nlc::StepLauncher lookup_group(int model, int S, bool exact) {
+	nlc::StepLauncher (*f)(int, bool) = nullptr;
+	switch (model) {
+	case NLC_DNKG: f = nlc::lookup_group_dnkg; break;
+	case NLC_FPUT: f = nlc::lookup_group_fput; break;
+	case NLC_TODA: f = nlc::lookup_group_toda; break;
+	case NLC_DDNKG: f = nlc::lookup_group_ddnkg; break;
+	default: return nullptr;
+	}
+	// an exact fit without its own instance runs on the general one
+	nlc::StepLauncher l = f(S, exact);
+	return l ? l : (exact ? f(S, false) : nullptr);
+}
+
 nlc::StepLauncher lookup(int model, int L, int S, bool exact) {
+	if (L > 64) return lookup_group(model, S, exact);
 	switch (model) {
 	case NLC_DNKG: return nlc::lookup_step_dnkg(L, S, exact);
 	case NLC_FPUT: return nlc::lookup_step_fput(L, S, exact);
@@ -174,7 +196,23 @@ bool default_shape(uint32_t n, int &L, int &S, bool &exact) {
 	return true;
 }
 
-// Every instantiated wave shape that holds a chain of n sites.
+// Chains longer than a wavefront holds: the workgroup shape with the fewest
+// wavefronts (fewest lanes meeting at a barrier) with S <= 32 sites per lane;
+// Toda, whose exp keeps more registers live, prefers S <= 16 first.  Measured
+// on 2^24-site ensembles at 3000 to 16384 sites per chain.
+bool group_default(int model, uint32_t n, int &L, int &S, bool &exact) {
+	const int smaxes[2] = {model == NLC_TODA ? 16 : 32, 32};
+	for (int smax : smaxes)
+		for (int W = 2; W <= 8; ++W) {
+			const int s = int((n + 64u * W - 1) / (64u * W));
+			if (s > smax || !group_shape_ok(n, 64 * W, s)) continue;
+			L = 64 * W, S = s, exact = uint64_t(L) * S == n;
+			return true;
+		}
+	return false;
+}
+
+// Every instantiated wave and workgroup shape that holds a chain of n sites.
 struct Shape {
 	int L, S;
 	bool exact;
@@ -186,6 +224,12 @@ std::vector<Shape> wave_shapes(int model, uint32_t n) {
 		if (S < 1 || S > 32) continue;
 		const bool exact = uint32_t(L * S) == n;
 		if (!exact && (L != 64 || S < 2)) continue;
+		if (lookup(model, L, S, exact)) out.push_back({L, S, exact});
+	}
+	for (int L = 128; L <= 512; L += 64) {
+		const int S = int((n + L - 1) / L);
+		if (!group_shape_ok(n, L, S)) continue;
+		const bool exact = uint64_t(L) * S == n;
 		if (lookup(model, L, S, exact)) out.push_back({L, S, exact});
 	}
 	return out;
@@ -368,11 +412,12 @@ int select_uniform(nlc_ctx *c) {
 			L = int(p.lanes_per_chain);
 			S = int(p.sites_per_lane);
 			exact = uint64_t(L) * S == n;
-			have = L >= 1 && L <= 64 && is_pow2(uint32_t(L)) && S >= 1 && uint64_t(L) * S >= n &&
-			       uint64_t(L) * (S - 1) < n;
+			have = (L >= 1 && L <= 64 && is_pow2(uint32_t(L)) && S >= 1 && uint64_t(L) * S >= n &&
+			        uint64_t(L) * (S - 1) < n) ||
+			       group_shape_ok(n, L, S);
 			if (!have) return c->fail(NLC_EARG, "wave shape %d lanes x %d sites cannot hold chain_length %u", L, S, n);
 		} else {
-			have = default_shape(n, L, S, exact);
+			have = default_shape(n, L, S, exact) || group_default(p.model, n, L, S, exact);
 		}
 		c->launcher = have ? lookup(p.model, L, S, exact) : nullptr;
 		if (c->launcher) {
@@ -1084,8 +1129,8 @@ int nlc_describe(const nlc_ctx *c, char *buf, size_t len) {
 	else if (c->use_split)
 		snprintf(buf, len, "model=%s n=%u copies=%u kernel=split", names[c->p.model], c->n, c->copies);
 	else
-		snprintf(buf, len, "model=%s n=%u copies=%u kernel=wave lanes_per_chain=%d sites_per_lane=%d exact=%d",
-		         names[c->p.model], c->n, c->copies, c->L, c->S, int(c->exact));
+		snprintf(buf, len, "model=%s n=%u copies=%u kernel=%s lanes_per_chain=%d sites_per_lane=%d exact=%d",
+		         names[c->p.model], c->n, c->copies, c->L > 64 ? "group" : "wave", c->L, c->S, int(c->exact));
 	if (c->p.model == NLC_DDNKG) {
 		const size_t at = strlen(buf);
 		if (at + 1 < len)

@@ -130,6 +130,9 @@ void tridiagonalize(uint32_t n, std::vector<double> &a, std::vector<double> &q, 
 	}
 }
 
+// Largest chain length solved on the host when a device is present.
+constexpr uint32_t kHostEigenMax = 512;
+
 // Implicit symmetric QL iteration with Wilkinson-type shifts on the
 // tridiagonal (d, e) (Bowdler, Martin, Reinsch & Wilkinson, Numer. Math. 11,
 // 1968), applying every plane rotation to the columns of z.
@@ -185,8 +188,13 @@ bool tridiagonal_ql(uint32_t n, std::vector<double> &d, std::vector<double> &e, 
 
 } // namespace
 
+// rocSOLVER on the device (nlc_eigen.cpp); nonzero when unavailable.
+int eigensystem_device(uint32_t n, const double *mp2, double *omegas, double *V);
+
 int eigensystem(uint32_t n, const double *mp2, double *omegas, double *V) {
 	if (n < 2 || !mp2 || !omegas || !V) return NLC_EARG;
+	// long chains: the O(n^3) host solver takes seconds from n = 1024 on
+	if (n > kHostEigenMax && eigensystem_device(n, mp2, omegas, V) == 0) return NLC_OK;
 	// A = diag(m_x + 2) - adjacency, periodic corners (dDNKG/eigensystem.cpp:15-18)
 	std::vector<double> a(size_t(n) * n, 0.0), q, d, e;
 	for (uint32_t i = 0; i < n; ++i) {

@@ -35,6 +35,15 @@ StepLauncher lookup_step_fput(int L, int S, bool exact);
 StepLauncher lookup_step_toda(int L, int S, bool exact);
 StepLauncher lookup_step_ddnkg(int L, int S, bool exact);
 
+// Workgroup shapes for long chains: one chain per workgroup of W = 2..8
+// wavefronts (L = 64 W lanes), S sites per lane, W = ceil(n / (64 S)) and
+// 64 W (S - 1) < n <= 64 W S; exact instances for S = 8, 16, 32, others for
+// S = 9..32.
+StepLauncher lookup_group_dnkg(int S, bool exact);
+StepLauncher lookup_group_fput(int S, bool exact);
+StepLauncher lookup_group_toda(int S, bool exact);
+StepLauncher lookup_group_ddnkg(int S, bool exact);
+
 // Ragged ensembles: an item is one wavefront's worth (nchains <= 64/L) of
 // consecutive chains of n = L*S sites starting at site index site0.
 struct RaggedItem {

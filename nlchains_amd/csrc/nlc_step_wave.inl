@@ -31,82 +31,7 @@
 namespace nlc {
 namespace {
 
-__constant__ double kExp2Tab[kExpTabSize] = NLC_EXP2_TABLE;
-
-// Toda kernels: copy the exp table into LDS (every thread of the workgroup
-// reaches the barrier).
-template <int MODEL>
-__device__ __forceinline__ void load_exp_table(double *lds) {
-	if constexpr (MODEL == TODA) {
-		if (threadIdx.x < kExpTabSize) lds[threadIdx.x] = kExp2Tab[threadIdx.x];
-		__syncthreads();
-	}
-}
-
-template <int CTRL>
-__device__ __forceinline__ double dpp_f64(double v) {
-	const int lo = __double2loint(v), hi = __double2hiint(v);
-	// every control used here reads a valid lane, so bound_ctrl never applies
-	const int rlo = __builtin_amdgcn_mov_dpp(lo, CTRL, 0xF, 0xF, true);
-	const int rhi = __builtin_amdgcn_mov_dpp(hi, CTRL, 0xF, 0xF, true);
-	return __hiloint2double(rhi, rlo);
-}
-
-__device__ __forceinline__ double bperm_f64(double v, int src_lane) {
-	const int addr = src_lane << 2;
-	const int lo = __builtin_amdgcn_ds_bpermute(addr, __double2loint(v));
-	const int hi = __builtin_amdgcn_ds_bpermute(addr, __double2hiint(v));
-	return __hiloint2double(hi, lo);
-}
-
-// DPP controls (GFX9 encoding): wave_rol:1 = 0x134, wave_ror:1 = 0x13C,
-// row_ror:n = 0x120+n, quad_perm = 8-bit lane selector.
-constexpr int kWaveRol1 = 0x134;
-constexpr int kWaveRor1 = 0x13C;
-constexpr int kRowRor1 = 0x121;
-constexpr int kRowRor15 = 0x12F;
-constexpr int kQuadLeft = (3 << 0) | (0 << 2) | (1 << 4) | (2 << 6);  // lane i <- lane (i-1)&3
-constexpr int kQuadRight = (1 << 0) | (2 << 2) | (3 << 4) | (0 << 6); // lane i <- lane (i+1)&3
-constexpr int kQuadSwap = (1 << 0) | (0 << 2) | (3 << 4) | (2 << 6);  // lane i <- lane i^1
-
-// Value held by the lane on the left (lane-1 within the chain's L-lane ring).
-template <int L>
-__device__ __forceinline__ double from_left(double v, int lane) {
-	if constexpr (L == 1) return v;
-	else if constexpr (L == 2) return dpp_f64<kQuadSwap>(v);
-	else if constexpr (L == 4) return dpp_f64<kQuadLeft>(v);
-	else if constexpr (L == 16) return dpp_f64<kRowRor1>(v);
-	else if constexpr (L == 64) return dpp_f64<kWaveRor1>(v);
-	else return bperm_f64(v, (lane & ~(L - 1)) | ((lane - 1) & (L - 1)));
-}
-
-// Value held by the lane on the right (lane+1 within the ring).
-template <int L>
-__device__ __forceinline__ double from_right(double v, int lane) {
-	if constexpr (L == 1) return v;
-	else if constexpr (L == 2) return dpp_f64<kQuadSwap>(v);
-	else if constexpr (L == 4) return dpp_f64<kQuadRight>(v);
-	else if constexpr (L == 16) return dpp_f64<kRowRor15>(v);
-	else if constexpr (L == 64) return dpp_f64<kWaveRol1>(v);
-	else return bperm_f64(v, (lane & ~(L - 1)) | ((lane + 1) & (L - 1)));
-}
-
-// Bond force term of FPUT / Toda for the bond x = q_{i} - q_{i-1}; the force on
-// site i is bond(x_{i+1}) - bond(x_i).
-//   FPUT: V'(x) = x + alpha x^2 + beta x^3  (Horner), the same force as
-//         (l+r-2c)(1+alpha(r-l)) + beta(d_r^3-d_l^3) of DNKG_FPUT_Toda.cu:41-58;
-//   Toda: e^{2 alpha x}, the force of DNKG_FPUT_Toda.cu:62-74 without its factor
-//         1/(2 alpha), which the kick folds into its weight (the constant
-//         -1/(2 alpha) of V' cancels in the difference).  FP64 table-driven exp
-//         (fewest instructions); for S >= 25 sites per lane, where the state
-//         takes 100+ VGPRs and hoisted table loads would leave one wave
-//         per SIMD, the table-free form.
-template <int MODEL, int S>
-__device__ __forceinline__ double bond_force(double x, const StepArgs &a, const double *etab) {
-	if constexpr (MODEL == FPUT) return x * fma(x, fma(a.beta, x, a.alpha), 1.0);
-	else if constexpr (S >= 25) return exp_poly(x); // Toda: x is already 2 alpha (q_i - q_{i-1})
-	else return exp_tab(x, etab);
-}
+#include "nlc_step_common.inl"
 
 template <int MODEL, int L, int S, bool EXACT>
 struct Chain {

@@ -7,6 +7,7 @@ Bars (the plain-form acceptance of the build):
 Inputs are seeded synthetic ensembles (the reference ships no initial states).
 """
 import sys
+import time
 
 import numpy as np
 import pytest
@@ -98,9 +99,59 @@ def test_move_ragged_lanes(gpu, model_name, n):
 @pytest.mark.parametrize("n", [3, 64, 2500])
 def test_move_split_kernel(gpu, model_name, n):
     """The HBM-streaming kernel (reference --split_kernel, and the fallback
-    for chain lengths without a wave shape)."""
-    shape = dict(kernel=gpu.KERNEL_SPLIT) if n == 64 else {}
+    for chain lengths without a wave or workgroup shape, such as 3)."""
+    shape = dict(kernel=gpu.KERNEL_SPLIT) if n != 3 else {}
     check_move(gpu, model_name, n, 33, steps=40, **shape)
+
+
+GROUP_CASES = [(m, n, c) for n, c in [(2500, 5), (3001, 3), (4096, 4), (8192, 3), (16384, 2)]
+               for m in ("DNKG", "FPUT", "Toda", "dDNKG") if m != "dDNKG" or n <= 4096]
+
+
+@pytest.mark.parametrize("model_name,n,copies", GROUP_CASES)
+def test_move_group_default(gpu, model_name, n, copies):
+    """Chains longer than a wavefront holds (n > 2048) run on the
+    workgroup-resident kernel by default."""
+    banner(f"group move {model_name} n={n} copies={copies}")
+    model, kw, masses, s0 = make_case(gpu, model_name, n, copies, 4)
+    with gpu.Ensemble(model, n, copies, 0.05, masses=masses, **kw) as ens:
+        desc = ens.describe()
+        ens.upload(s0)
+        ens.move(60)
+        got = ens.download()
+    want = oracle_move(model, kw, masses, s0, 60, 0.05)
+    err = rel_err(got, want)
+    banner(f"  {desc}: rel err {err:.3e}")
+    assert "kernel=group" in desc
+    assert np.isfinite(got).all() and err <= STATE_TOL, (desc, err)
+
+
+@pytest.mark.parametrize("n,L,S", [(4096, 128, 32), (4096, 512, 8), (3000, 192, 16), (2048, 128, 16),
+                                   (1024, 128, 8), (16384, 512, 32), (5000, 320, 16), (2100, 192, 11)])
+def test_move_group_shapes(gpu, n, L, S):
+    for model_name in ("DNKG", "Toda"):
+        check_move(gpu, model_name, n, 3, steps=40, lanes_per_chain=L, sites_per_lane=S)
+
+
+def test_move_group_rejects_invalid_shapes(gpu):
+    for n, L, S in [(4096, 192, 16), (4096, 1024, 4), (4096, 256, 8), (100000, 512, 32)]:
+        with pytest.raises(gpu.NlcError):
+            gpu.Ensemble(gpu.DNKG, n, 2, 0.05, m=1.0, beta=1.0, lanes_per_chain=L, sites_per_lane=S)
+
+
+def test_autotune_includes_group_shapes(gpu):
+    N = gpu
+    model, kw, masses, s0 = make_case(N, "FPUT", 4096, 64, 3)
+    with N.Ensemble(model, 4096, 64, 0.05, **kw) as ens:
+        ens.upload(s0)
+        report = ens.autotune(5)
+        banner(f"autotune n=4096: {report} -> {ens.describe()}")
+        assert "256x16" in report and "128x32" in report and "split" in report
+        assert np.array_equal(ens.download(), s0)
+        ens.move(30)
+        got = ens.download()
+    want = oracle_move(model, kw, masses, s0[:4], 30, 0.05)
+    assert rel_err(got[:4], want) <= STATE_TOL
 
 
 def test_move_single_copy_and_single_step(gpu):
@@ -132,6 +183,22 @@ def test_move_is_deterministic(gpu):
             e.move(30)
             outs.append(e.download())
     assert np.array_equal(outs[0], outs[1])
+
+
+@pytest.mark.parametrize("n", [1024, 4096])
+def test_eigensystem_device_path(gpu, n):
+    """Above 512 sites the eigensystem runs on the device (rocSOLVER dsyevd):
+    eigenvalues against LAPACK, the eigen-equation and orthonormality."""
+    masses = gpu.ic_masses(n, 0.5, 1.5, 7)
+    t0 = time.perf_counter()
+    om, V = gpu.eigensystem(masses + 2)
+    banner(f"eigensystem n={n}: {time.perf_counter() - t0:.2f} s")
+    A = np.diag(masses + 2) - np.eye(n, k=1) - np.eye(n, k=-1)
+    A[0, -1] = A[-1, 0] = -1
+    lam = np.linalg.eigvalsh(A)
+    np.testing.assert_allclose(om, np.sqrt(lam), rtol=1e-12)
+    assert np.abs(A @ V - V * om ** 2).max() <= 1e-11 * np.abs(lam).max()
+    assert np.abs(V.T @ V - np.eye(n)).max() <= 1e-12
 
 
 # ------------------------------------------------------------ energy pass
