@@ -189,7 +189,16 @@ bool default_shape(uint32_t n, int &L, int &S, bool &exact) {
 		exact = (uint32_t(S) * 64 == n);
 		return S <= 32;
 	}
-	if (!is_pow2(n)) return false;
+	if (!is_pow2(n)) {
+		// short chains of other lengths: a thread per chain up to 32 sites,
+		// else 4 lanes with 9..16 sites (the last lanes one fewer)
+		if (n <= 32) {
+			L = 1, S = int(n), exact = true;
+		} else {
+			L = 4, S = int((n + 3) / 4), exact = false;
+		}
+		return true;
+	}
 	S = int(std::min<uint32_t>(n, 8));
 	L = int(n) / S;
 	exact = true;
@@ -223,7 +232,7 @@ std::vector<Shape> wave_shapes(int model, uint32_t n) {
 		const int S = int((n + L - 1) / L);
 		if (S < 1 || S > 32) continue;
 		const bool exact = uint32_t(L * S) == n;
-		if (!exact && (L != 64 || S < 2)) continue;
+		if (!exact && S < 2) continue;
 		if (lookup(model, L, S, exact)) out.push_back({L, S, exact});
 	}
 	for (int L = 128; L <= 512; L += 64) {

@@ -232,12 +232,23 @@ RaggedLauncher NLC_STEP_RAGGED(int S) {
 #endif
 
 // Shapes instantiated: every power-of-two (L, S) with 2 <= L*S <= 2048, S <= 32
-// (exact fit), plus L = 64 with S = 2..32 for any n in (64(S-1), 64S].
+// (exact fit), plus L = 64 with S = 2..32 for any n in (64(S-1), 64S]; for
+// short chains whose length is not a power of two, L = 1 with S = n <= 32
+// (thread per chain) and L = 4 with S = 9..16 for n in 33..63.
 StepLauncher NLC_STEP_LOOKUP(int L, int S, bool exact) {
 #define NLC_EXACT(LL, SS)                                                                                          \
 	if (exact && L == LL && S == SS) return &launch_wave<NLC_STEP_MODEL, LL, SS, true>;
 #define NLC_RAGGED64(SS)                                                                                           \
 	if (!exact && L == 64 && S == SS) return &launch_wave<NLC_STEP_MODEL, 64, SS, false>;
+#define NLC_RAGGED4(SS)                                                                                            \
+	if (!exact && L == 4 && S == SS) return &launch_wave<NLC_STEP_MODEL, 4, SS, false>;
+	NLC_EXACT(1, 3) NLC_EXACT(1, 5) NLC_EXACT(1, 6) NLC_EXACT(1, 7) NLC_EXACT(1, 9) NLC_EXACT(1, 10)
+	NLC_EXACT(1, 11) NLC_EXACT(1, 12) NLC_EXACT(1, 13) NLC_EXACT(1, 14) NLC_EXACT(1, 15) NLC_EXACT(1, 17)
+	NLC_EXACT(1, 18) NLC_EXACT(1, 19) NLC_EXACT(1, 20) NLC_EXACT(1, 21) NLC_EXACT(1, 22) NLC_EXACT(1, 23)
+	NLC_EXACT(1, 24) NLC_EXACT(1, 25) NLC_EXACT(1, 26) NLC_EXACT(1, 27) NLC_EXACT(1, 28) NLC_EXACT(1, 29)
+	NLC_EXACT(1, 30) NLC_EXACT(1, 31)
+	NLC_RAGGED4(9) NLC_RAGGED4(10) NLC_RAGGED4(11) NLC_RAGGED4(12) NLC_RAGGED4(13) NLC_RAGGED4(14)
+	NLC_RAGGED4(15) NLC_RAGGED4(16)
 	NLC_EXACT(1, 2) NLC_EXACT(1, 4) NLC_EXACT(1, 8) NLC_EXACT(1, 16) NLC_EXACT(1, 32)
 	NLC_EXACT(2, 1) NLC_EXACT(2, 2) NLC_EXACT(2, 4) NLC_EXACT(2, 8) NLC_EXACT(2, 16) NLC_EXACT(2, 32)
 	NLC_EXACT(4, 1) NLC_EXACT(4, 2) NLC_EXACT(4, 4) NLC_EXACT(4, 8) NLC_EXACT(4, 16) NLC_EXACT(4, 32)
@@ -253,6 +264,7 @@ StepLauncher NLC_STEP_LOOKUP(int L, int S, bool exact) {
 	NLC_RAGGED64(32)
 #undef NLC_EXACT
 #undef NLC_RAGGED64
+#undef NLC_RAGGED4
 	return nullptr;
 }
 

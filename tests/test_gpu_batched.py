@@ -43,6 +43,7 @@ def oracle_ragged(model, lengths, s0, steps, dt, kw):
     (1, dict(alpha=0.25, beta=1.0), [64, 256]),
     (0, dict(m=1.0, beta=1.0), [8, 16, 32, 64]),
     (2, dict(alpha=1.0), [70, 100, 300]),  # no persistent shape: one uniform kernel per class
+    (1, dict(beta=1.0), [5, 24, 45, 128]),  # short classes: thread per chain, 4 lanes
 ])
 def test_ragged_parity(gpu, model, kw, choices):
     N = gpu

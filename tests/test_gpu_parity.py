@@ -99,9 +99,27 @@ def test_move_ragged_lanes(gpu, model_name, n):
 @pytest.mark.parametrize("n", [3, 64, 2500])
 def test_move_split_kernel(gpu, model_name, n):
     """The HBM-streaming kernel (reference --split_kernel, and the fallback
-    for chain lengths without a wave or workgroup shape, such as 3)."""
-    shape = dict(kernel=gpu.KERNEL_SPLIT) if n != 3 else {}
-    check_move(gpu, model_name, n, 33, steps=40, **shape)
+    for chains longer than 16384 sites)."""
+    check_move(gpu, model_name, n, 33, steps=40, kernel=gpu.KERNEL_SPLIT)
+
+
+@pytest.mark.parametrize("model_name", ["DNKG", "FPUT", "Toda", "dDNKG"])
+@pytest.mark.parametrize("n", [3, 5, 7, 12, 17, 31, 33, 45, 63])
+def test_move_short_chains(gpu, model_name, n):
+    """Short chains whose length is not a power of two: a thread per chain up
+    to 32 sites, 4 lanes with one site fewer on the last lanes above."""
+    banner(f"short chains {model_name} n={n}")
+    model, kw, masses, s0 = make_case(gpu, model_name, n, 70, 6)
+    with gpu.Ensemble(model, n, 70, 0.05, masses=masses, **kw) as ens:
+        desc = ens.describe()
+        ens.upload(s0)
+        ens.move(100)
+        got = ens.download()
+    want = oracle_move(model, kw, masses, s0, 100, 0.05)
+    err = rel_err(got, want)
+    banner(f"  {desc}: rel err {err:.3e}")
+    assert "kernel=wave" in desc and f"lanes_per_chain={1 if n <= 32 else 4}" in desc
+    assert np.isfinite(got).all() and err <= STATE_TOL, (desc, err)
 
 
 GROUP_CASES = [(m, n, c) for n, c in [(2500, 5), (3001, 3), (4096, 4), (8192, 3), (16384, 2)]
