@@ -70,7 +70,7 @@ struct nlc_ctx {
 	double *eigvec = nullptr;  // dDNKG V, column-major
 	int proj_variant = nlc::kProjMfma; // dDNKG projection: FP64 matrix cores, vector ALU or split BF16
 	uint16_t *vplanes = nullptr, *splanes = nullptr; // split BF16 form: V and state splits
-	double *split_q = nullptr, *split_p = nullptr;
+	double *split_q[2] = {nullptr, nullptr}, *split_p[2] = {nullptr, nullptr}; // ping-pong pair
 	double *energy_d = nullptr, *norm_d = nullptr;
 	double2 *dnls_table = nullptr; // DNLS Fourier multipliers
 	int dnls_variant = nlc::kDnlsRadix16; // n = 4096: register kernel, radix 16 or radix 8
@@ -268,7 +268,7 @@ void free_all(nlc_ctx *c) {
 	for (void *ptr :
 	     {(void *) c->state, (void *) c->omega, (void *) c->tw, (void *) c->partial, (void *) c->reduce_scratch,
 	      (void *) c->e_raw, (void *) c->modes, (void *) c->mp2, (void *) c->masses_d,
-	      (void *) c->eigvec, (void *) c->split_q, (void *) c->split_p, (void *) c->energy_d, (void *) c->norm_d,
+	      (void *) c->eigvec, (void *) c->split_q[0], (void *) c->split_p[0], (void *) c->split_q[1], (void *) c->split_p[1], (void *) c->energy_d, (void *) c->norm_d,
 	      (void *) c->dnls_table, (void *) c->items, (void *) c->vplanes, (void *) c->splanes})
 		if (ptr) hipFree(ptr);
 	if (c->h_results) hipHostFree(c->h_results);
@@ -439,8 +439,10 @@ int select_uniform(nlc_ctx *c) {
 	}
 	if (want_split) {
 		c->use_split = true;
-		if (!c->split_q) NLC_TRY(dev_alloc(c, &c->split_q, size_t(n) * c->copies, "split q"));
-		if (!c->split_p) NLC_TRY(dev_alloc(c, &c->split_p, size_t(n) * c->copies, "split p"));
+		for (int b = 0; b < 2; ++b) {
+			if (!c->split_q[b]) NLC_TRY(dev_alloc(c, &c->split_q[b], size_t(n) * c->copies, "split q"));
+			if (!c->split_p[b]) NLC_TRY(dev_alloc(c, &c->split_p[b], size_t(n) * c->copies, "split p"));
+		}
 	}
 	return NLC_OK;
 }
@@ -490,10 +492,11 @@ int enqueue_move(nlc_ctx *c, uint32_t steps) {
 				NLC_HIP(c, k.launcher(c->stream, c->state + k.site0, k.count, k.n, steps, c->args, nullptr));
 		}
 	} else if (c->use_split) {
-		NLC_HIP(c, nlc::launch_planar_to_split(c->stream, c->state, c->split_q, c->split_p, c->copies, c->n));
-		NLC_HIP(c, nlc::launch_step_split(c->stream, c->p.model, c->split_q, c->split_p, c->copies, c->n, steps,
+		int cur = 0;
+		NLC_HIP(c, nlc::launch_planar_to_split(c->stream, c->state, c->split_q[0], c->split_p[0], c->copies, c->n));
+		NLC_HIP(c, nlc::launch_step_split(c->stream, c->p.model, c->split_q, c->split_p, cur, c->copies, c->n, steps,
 		                                  c->args, c->mp2));
-		NLC_HIP(c, nlc::launch_split_to_planar(c->stream, c->split_q, c->split_p, c->state, c->copies, c->n));
+		NLC_HIP(c, nlc::launch_split_to_planar(c->stream, c->split_q[cur], c->split_p[cur], c->state, c->copies, c->n));
 	} else {
 		NLC_HIP(c, c->launcher(c->stream, c->state, c->copies, c->n, steps, c->args, c->mp2));
 	}

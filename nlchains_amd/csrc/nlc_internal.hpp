@@ -59,9 +59,11 @@ RaggedLauncher lookup_ragged_dnkg(int S);
 RaggedLauncher lookup_ragged_fput(int S);
 RaggedLauncher lookup_ragged_toda(int S);
 
-// HBM-streaming step over the split layout q[n][copies], p[n][copies].
-hipError_t launch_step_split(hipStream_t, int model, double *q, double *p, uint32_t copies, uint32_t n,
-                             uint32_t steps, const StepArgs &args, const double *mp2);
+// HBM-streaming step over the split layout q[n][copies], p[n][copies]: one
+// launch per substep from buffer pair cur to pair cur ^ 1 (cur is updated; the
+// result is in q[cur], p[cur]).
+hipError_t launch_step_split(hipStream_t, int model, double *q[2], double *p[2], int &cur, uint32_t copies,
+                             uint32_t n, uint32_t steps, const StepArgs &args, const double *mp2);
 // planar double2[copies][n] <-> split q[n][copies], p[n][copies]
 hipError_t launch_planar_to_split(hipStream_t, const double2 *planar, double *q, double *p, uint32_t copies,
                                   uint32_t n);

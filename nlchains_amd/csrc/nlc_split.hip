@@ -1,16 +1,21 @@
-// HBM-streaming step over the ensemble-major ("split") layout, for chain
-// lengths without a wave shape and for the reference's --split_kernel.
+// HBM-streaming step over the ensemble-major ("split") layout, for chains
+// longer than the register- and LDS-resident kernels hold and for the
+// reference's --split_kernel.
 //
 // Reference behaviour followed: move_split, DNKG_FPUT_Toda.cu:210-259 and
-// dDNKG.cu:154-196 (one thread per copy, layout q[site][copy], p[site][copy],
-// a single pass per substep that drifts q_i and kicks p_{i-1}); the
-// planar<->split conversion of plane2split, common/utilities_cuda.cu:27-62.
+// dDNKG.cu:154-196 (layout q[site][copy], p[site][copy], one pass per substep
+// that drifts q and kicks p); the planar<->split conversion of plane2split,
+// common/utilities_cuda.cu:27-62.
 //
-// Design: consecutive threads own consecutive copies, so every access of the
-// pass is a fully coalesced 512 B wavefront load/store; sites 0 and 1 and p_0,
-// which the wrap-around needs at the end of each pass, stay in registers for
-// the whole batch.  32 B of HBM traffic per site and substep.  The layout
-// conversion is an LDS-tiled transpose (coalesced on both sides).
+// Design: the reference gives each copy one thread that walks the chain, which
+// leaves the GPU nearly idle unless there are hundreds of thousands of copies.
+// Here one launch per substep covers every (copy, 16-site segment) pair:
+// consecutive threads take consecutive copies, so each access is a coalesced
+// wavefront load or store; a thread drifts its segment and the two sites
+// around it, kicks its segment and writes it to the other buffer of a
+// ping-pong pair (neighbouring segments read the old values in the same
+// launch).  34 B of HBM traffic per site and substep (16 B written, 16 B plus
+// the 2/16 halo read).  The layout conversion is an LDS-tiled transpose.
 #include "nlc_internal.hpp"
 
 namespace nlc {
@@ -33,43 +38,62 @@ __device__ __forceinline__ double force(double l, double c, double r, double mas
 	}
 }
 
-template <int MODEL>
+constexpr uint32_t kSeg = 16; // sites per thread and substep
+
+// One substep: q += cc p over sites [i0 - 1, i0 + kSeg] (periodic), then, with
+// KICK, p += d F(q) on [i0, i0 + kSeg); results to (qo, po).
+template <int MODEL, bool KICK>
 __global__ void __launch_bounds__(256)
-k_step_split(double *__restrict__ q, double *__restrict__ p, uint32_t copies, uint32_t n, uint32_t steps, StepArgs a,
+k_step_split(const double *__restrict__ qi, const double *__restrict__ pi, double *__restrict__ qo,
+             double *__restrict__ po, uint32_t copies, uint32_t n, double cc, double d, StepArgs a,
              const double *__restrict__ mp2) {
-	const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-	if (c >= copies) return;
+	const uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+	const uint32_t nseg = (n + kSeg - 1) / kSeg;
+	if (t >= uint64_t(copies) * nseg) return;
 	const size_t C = copies;
-	auto mass = [&](uint32_t i) { return MODEL == DDNKG ? mp2[i] : a.m2; };
-	double q0 = q[c], q1 = q[C + c], p0 = p[c];
-	for (uint32_t s = 0; s < steps; ++s) {
-		for (int k = 0; k < 7; ++k) {
-			const double cc = (k == 0 && s) ? 2.0 * a.dc[0] : a.dc[k];
-			const double d = a.dd[k];
-			const double p1 = p[C + c];
-			q0 = fma(cc, p0, q0);
-			q1 = fma(cc, p1, q1);
-			double qm2 = q0, qm1 = q1, pm1 = p1; // q_{i-2}, q_{i-1} (drifted), p_{i-1} (not yet kicked)
-			for (uint32_t i = 2; i < n; ++i) {
-				const double pi = p[i * C + c];
-				const double qi = fma(cc, pi, q[i * C + c]);
-				q[i * C + c] = qi;
-				p[(i - 1) * C + c] = fma(d, force<MODEL>(qm2, qm1, qi, mass(i - 1), a), pm1);
-				pm1 = pi;
-				qm2 = qm1;
-				qm1 = qi;
+	const uint32_t c = uint32_t(t % copies), i0 = uint32_t(t / copies) * kSeg;
+	const uint32_t cnt = min(kSeg, n - i0);
+	const uint32_t il = i0 ? i0 - 1 : n - 1;
+	double ql = fma(cc, pi[il * C + c], qi[il * C + c]);
+	double pc = pi[i0 * C + c];
+	double qc = fma(cc, pc, qi[i0 * C + c]);
+#pragma unroll
+	for (uint32_t j = 0; j < kSeg; ++j) {
+		if (j < cnt) {
+			const uint32_t i = i0 + j, inext = (i + 1 == n) ? 0 : i + 1;
+			const double pn = pi[inext * C + c];
+			const double qn = fma(cc, pn, qi[inext * C + c]);
+			qo[i * C + c] = qc;
+			if constexpr (KICK) {
+				const double mass = (MODEL == DDNKG) ? mp2[i] : a.m2;
+				po[i * C + c] = fma(d, force<MODEL>(ql, qc, qn, mass, a), pc);
+			} else {
+				po[i * C + c] = pc;
 			}
-			p[(n - 1) * C + c] = fma(d, force<MODEL>(qm2, qm1, q0, mass(n - 1), a), pm1);
-			p0 = fma(d, force<MODEL>(qm1, q0, q1, mass(0), a), p0);
+			ql = qc;
+			qc = qn;
+			pc = pn;
 		}
 	}
-	const double c7 = a.dc[7];
-	q0 = fma(c7, p0, q0);
-	q1 = fma(c7, p[C + c], q1);
-	for (uint32_t i = 2; i < n; ++i) q[i * C + c] = fma(c7, p[i * C + c], q[i * C + c]);
-	q[c] = q0;
-	q[C + c] = q1;
-	p[c] = p0;
+}
+
+template <int MODEL>
+hipError_t launch_split_model(hipStream_t st, double *q[2], double *p[2], int &cur, uint32_t copies, uint32_t n,
+                              uint32_t steps, const StepArgs &a, const double *mp2) {
+	const uint64_t threads = uint64_t(copies) * ((n + kSeg - 1) / kSeg);
+	const dim3 grid(uint32_t((threads + 255) / 256)), block(256);
+	for (uint32_t s = 0; s < steps; ++s)
+		for (int k = 0; k < 7; ++k) {
+			// the c7 drift of a step merged with the c0 drift of the next
+			const double cc = (k == 0 && s) ? 2.0 * a.dc[0] : a.dc[k];
+			hipLaunchKernelGGL((k_step_split<MODEL, true>), grid, block, 0, st, q[cur], p[cur], q[cur ^ 1],
+			                   p[cur ^ 1], copies, n, cc, a.dd[k], a, mp2);
+			cur ^= 1;
+		}
+	hipLaunchKernelGGL((k_step_split<MODEL, false>), grid, block, 0, st, q[cur], p[cur], q[cur ^ 1], p[cur ^ 1],
+	                   copies, n, a.dc[7], 0.0, a, mp2);
+	cur ^= 1;
+	return hipGetLastError();
 }
 
 constexpr int kTile = 32;
@@ -112,17 +136,15 @@ k_split_to_planar(const double *__restrict__ q, const double *__restrict__ p, do
 
 } // namespace
 
-hipError_t launch_step_split(hipStream_t st, int model, double *q, double *p, uint32_t copies, uint32_t n,
-                             uint32_t steps, const StepArgs &args, const double *mp2) {
-	const dim3 grid((copies + 255) / 256), block(256);
+hipError_t launch_step_split(hipStream_t st, int model, double *q[2], double *p[2], int &cur, uint32_t copies,
+                             uint32_t n, uint32_t steps, const StepArgs &args, const double *mp2) {
 	switch (model) {
-	case DNKG: hipLaunchKernelGGL(k_step_split<DNKG>, grid, block, 0, st, q, p, copies, n, steps, args, mp2); break;
-	case FPUT: hipLaunchKernelGGL(k_step_split<FPUT>, grid, block, 0, st, q, p, copies, n, steps, args, mp2); break;
-	case TODA: hipLaunchKernelGGL(k_step_split<TODA>, grid, block, 0, st, q, p, copies, n, steps, args, mp2); break;
-	case DDNKG: hipLaunchKernelGGL(k_step_split<DDNKG>, grid, block, 0, st, q, p, copies, n, steps, args, mp2); break;
+	case DNKG: return launch_split_model<DNKG>(st, q, p, cur, copies, n, steps, args, mp2);
+	case FPUT: return launch_split_model<FPUT>(st, q, p, cur, copies, n, steps, args, mp2);
+	case TODA: return launch_split_model<TODA>(st, q, p, cur, copies, n, steps, args, mp2);
+	case DDNKG: return launch_split_model<DDNKG>(st, q, p, cur, copies, n, steps, args, mp2);
 	default: return hipErrorInvalidValue;
 	}
-	return hipGetLastError();
 }
 
 hipError_t launch_planar_to_split(hipStream_t st, const double2 *planar, double *q, double *p, uint32_t copies,
