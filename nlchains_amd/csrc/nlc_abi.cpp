@@ -332,9 +332,7 @@ int setup_common(nlc_ctx *c) {
 			tw[k.tab0 + j] = make_double2(std::cos(ang), -std::sin(ang));
 		}
 		if (p.model != NLC_DDNKG) nlc_dispersion(p.model, k.n, p.m, omega.data() + k.tab0);
-		// one partial row per workgroup; long chains get fewer rows (at most 256 MiB of partials)
-		k.rows = p.model == NLC_DDNKG ? nlc::ddnkg_partial_rows(k.count)
-		                              : std::min<uint32_t>({k.count, 1024u, std::max<uint32_t>(1u, (1u << 25) / k.n)});
+		k.rows = p.model == NLC_DDNKG ? nlc::ddnkg_partial_rows(k.count) : nlc::energy_rows(k.n, k.count);
 		partial_len = std::max(partial_len, size_t(k.rows) * k.n);
 		scratch_len = std::max(scratch_len, size_t(nlc::reduce_scratch_rows(k.rows)) * k.n);
 	}
@@ -503,35 +501,6 @@ int enqueue_move(nlc_ctx *c, uint32_t steps) {
 	return NLC_OK;
 }
 
-// Energy pass of every class into e_raw (raw sums over copies).
-int enqueue_energy(nlc_ctx *c) {
-	for (const auto &k : c->cls) {
-		const double2 *st = c->state + k.site0;
-		double *er = c->e_raw + k.tab0;
-		switch (c->p.model) {
-		case NLC_DNKG:
-		case NLC_FPUT:
-		case NLC_TODA:
-			NLC_HIP(c, nlc::launch_energy_real(c->stream, st, k.n, k.count, c->tw + k.tab0, c->omega + k.tab0,
-			                                   c->partial, k.rows));
-			NLC_HIP(c, nlc::launch_reduce_partial(c->stream, c->partial, k.rows, k.n, nullptr, er, c->reduce_scratch));
-			break;
-		case NLC_DDNKG:
-			NLC_HIP(c, nlc::launch_energy_ddnkg(c->stream, st, k.n, k.count, c->eigvec, c->omega, c->partial,
-			                                    c->proj_variant, c->vplanes, c->splanes));
-			NLC_HIP(c, nlc::launch_reduce_partial(c->stream, c->partial, k.rows, k.n, nullptr, er, c->reduce_scratch));
-			break;
-		case NLC_DNLS:
-			NLC_HIP(c, nlc::launch_energy_dnls(c->stream, st, k.n, k.count, c->tw + k.tab0, c->partial, k.rows));
-			NLC_HIP(c, nlc::launch_reduce_partial(c->stream, c->partial, k.rows, k.n, c->omega + k.tab0, er,
-			                                      c->reduce_scratch));
-			break;
-		default: return c->fail(NLC_EUNSUPPORTED, "no energy pass for this model");
-		}
-	}
-	return NLC_OK;
-}
-
 // results::calc_linenergies normalisation of the drivers for a class
 double default_norm(int model, const Klass &k) {
 	switch (model) {
@@ -541,13 +510,41 @@ double default_norm(int model, const Klass &k) {
 	}
 }
 
-// Normalisation and entropies, written to dst (pinned host memory, results_len doubles).
-int enqueue_results(nlc_ctx *c, double norm, double *dst) {
+// Energy pass of every class: raw sums over copies into e_raw and, with dst,
+// the normalisation and entropies written to dst (pinned host memory,
+// results_len doubles).  The classes share the partial rows, so each class is
+// reduced before the next one's pass.
+int enqueue_energy(nlc_ctx *c, double norm = 0.0, double *dst = nullptr) {
 	for (size_t i = 0; i < c->cls.size(); ++i) {
 		const Klass &k = c->cls[i];
-		const double nm = norm > 0 ? norm : default_norm(c->p.model, k);
-		NLC_HIP(c, nlc::launch_results(c->stream, c->e_raw + k.tab0, k.n, nm, c->a0is0, c->modes,
-		                               c->ragged ? 0 : c->nmodes, dst + k.tab0 + 2 * i));
+		const double2 *st = c->state + k.site0;
+		const double *scale = nullptr;
+		switch (c->p.model) {
+		case NLC_DNKG:
+		case NLC_FPUT:
+		case NLC_TODA:
+			NLC_HIP(c, nlc::launch_energy_real(c->stream, st, k.n, k.count, c->tw + k.tab0, c->omega + k.tab0,
+			                                   c->partial, k.rows));
+			break;
+		case NLC_DDNKG:
+			NLC_HIP(c, nlc::launch_energy_ddnkg(c->stream, st, k.n, k.count, c->eigvec, c->omega, c->partial,
+			                                    c->proj_variant, c->vplanes, c->splanes));
+			break;
+		case NLC_DNLS:
+			NLC_HIP(c, nlc::launch_energy_dnls(c->stream, st, k.n, k.count, c->tw + k.tab0, c->partial, k.rows));
+			scale = c->omega + k.tab0;
+			break;
+		default: return c->fail(NLC_EUNSUPPORTED, "no energy pass for this model");
+		}
+		double *er = c->e_raw + k.tab0;
+		if (dst) {
+			const double nm = norm > 0 ? norm : default_norm(c->p.model, k);
+			NLC_HIP(c, nlc::launch_reduce_results(c->stream, c->partial, k.rows, k.n, scale, er, c->reduce_scratch, nm,
+			                                      c->a0is0, c->modes, c->ragged ? 0 : c->nmodes,
+			                                      dst + k.tab0 + 2 * i));
+		} else {
+			NLC_HIP(c, nlc::launch_reduce_partial(c->stream, c->partial, k.rows, k.n, scale, er, c->reduce_scratch));
+		}
 	}
 	return NLC_OK;
 }
@@ -614,8 +611,7 @@ int ensure_batch_graph(nlc_ctx *c, uint32_t which, uint32_t steps, double norm, 
 		}
 		NLC_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
 		int rc = enqueue_move(c, steps);
-		if (!rc) rc = enqueue_energy(c);
-		if (!rc) rc = enqueue_results(c, norm, dst);
+		if (!rc) rc = enqueue_energy(c, norm, dst);
 		hipGraph_t g = nullptr;
 		const hipError_t e = hipStreamEndCapture(c->stream, &g);
 		if (rc) {
@@ -917,8 +913,7 @@ int nlc_linenergies_raw(nlc_ctx *c, double *e_raw) {
 
 int nlc_linenergies(nlc_ctx *c, double norm, double *linenergies, double *entropies) {
 	if (!c) return NLC_EARG;
-	NLC_TRY(enqueue_energy(c));
-	NLC_TRY(enqueue_results(c, norm, c->h_results));
+	NLC_TRY(enqueue_energy(c, norm, c->h_results));
 	NLC_HIP(c, hipStreamSynchronize(c->stream));
 	unpack_results(c, c->h_results, linenergies, entropies);
 	return NLC_OK;

@@ -22,6 +22,8 @@
 // a fixed order, so results are bit-reproducible run to run.
 #include "nlc_internal.hpp"
 
+#include <algorithm>
+
 namespace nlc {
 namespace {
 
@@ -32,6 +34,37 @@ constexpr uint32_t kEnergyFftLarge = 8192;
 
 __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
 	return make_double2(fma(a.x, b.x, -a.y * b.y), fma(a.x, b.y, a.y * b.x));
+}
+
+// Radix-4 Stockham butterfly j (< n/4) of the pass with stride ns: inputs
+// v[r] = buf[j + r n/4] (read before the pass's barrier), outputs at
+// (j/ns) 4 ns + j%ns + r ns.  span = n / (4 ns) is the twiddle stride.
+__device__ __forceinline__ void radix4_write(double2 *buf, const double2 v[4], uint32_t j, uint32_t ns, uint32_t span,
+                                             const double2 *__restrict__ tw) {
+	const uint32_t k = j % ns;
+	const double2 a0 = v[0];
+	const double2 a1 = cmul(v[1], tw[k * span]);
+	const double2 a2 = cmul(v[2], tw[2 * k * span]);
+	const double2 a3 = cmul(v[3], tw[3 * k * span]);
+	const double2 t0 = make_double2(a0.x + a2.x, a0.y + a2.y);
+	const double2 t1 = make_double2(a0.x - a2.x, a0.y - a2.y);
+	const double2 t2 = make_double2(a1.x + a3.x, a1.y + a3.y);
+	const double2 t3 = make_double2(a1.x - a3.x, a1.y - a3.y);
+	const uint32_t o = (j / ns) * ns * 4 + k;
+	buf[o] = make_double2(t0.x + t2.x, t0.y + t2.y);
+	buf[o + ns] = make_double2(t1.x + t3.y, t1.y - t3.x);     // t1 - i t3
+	buf[o + 2 * ns] = make_double2(t0.x - t2.x, t0.y - t2.y);
+	buf[o + 3 * ns] = make_double2(t1.x - t3.y, t1.y + t3.x); // t1 + i t3
+}
+
+// Final radix-2 butterfly j (< n/2): inputs buf[j], buf[j + n/2].
+__device__ __forceinline__ void radix2_write(double2 *buf, double2 v0, double2 v1, uint32_t j, uint32_t ns, uint32_t n,
+                                             const double2 *__restrict__ tw) {
+	const uint32_t k = j % ns;
+	const double2 b = cmul(v1, tw[k * (n / (2 * ns))]);
+	const uint32_t o = (j / ns) * ns * 2 + k;
+	buf[o] = make_double2(v0.x + b.x, v0.y + b.y);
+	buf[o + ns] = make_double2(v0.x - b.x, v0.y - b.y);
 }
 
 // In-place radix-4 (plus one final radix-2 pass when log2 n is odd) Stockham
@@ -58,22 +91,7 @@ __device__ void lds_fft(double2 *buf, uint32_t n, const double2 *__restrict__ tw
 #pragma unroll
 		for (int t = 0; t < kMaxPerThread; ++t) {
 			const uint32_t j = tid + t * kThreads;
-			if (j < quarter) {
-				const uint32_t k = j % ns;
-				const double2 a0 = v[t][0];
-				const double2 a1 = cmul(v[t][1], tw[k * span]);
-				const double2 a2 = cmul(v[t][2], tw[2 * k * span]);
-				const double2 a3 = cmul(v[t][3], tw[3 * k * span]);
-				const double2 t0 = make_double2(a0.x + a2.x, a0.y + a2.y);
-				const double2 t1 = make_double2(a0.x - a2.x, a0.y - a2.y);
-				const double2 t2 = make_double2(a1.x + a3.x, a1.y + a3.y);
-				const double2 t3 = make_double2(a1.x - a3.x, a1.y - a3.y);
-				const uint32_t o = (j / ns) * ns * 4 + k;
-				buf[o] = make_double2(t0.x + t2.x, t0.y + t2.y);
-				buf[o + ns] = make_double2(t1.x + t3.y, t1.y - t3.x);     // t1 - i t3
-				buf[o + 2 * ns] = make_double2(t0.x - t2.x, t0.y - t2.y);
-				buf[o + 3 * ns] = make_double2(t1.x - t3.y, t1.y + t3.x); // t1 + i t3
-			}
+			if (j < quarter) radix4_write(buf, v[t], j, ns, span, tw);
 		}
 		__syncthreads();
 	}
@@ -89,13 +107,7 @@ __device__ void lds_fft(double2 *buf, uint32_t n, const double2 *__restrict__ tw
 #pragma unroll
 		for (int t = 0; t < 2 * kMaxPerThread; ++t) {
 			const uint32_t j = tid + t * kThreads;
-			if (j < half) {
-				const uint32_t k = j % ns;
-				const double2 b = cmul(v[t][1], tw[k * (n / (2 * ns))]);
-				const uint32_t o = (j / ns) * ns * 2 + k;
-				buf[o] = make_double2(v[t][0].x + b.x, v[t][0].y + b.y);
-				buf[o + ns] = make_double2(v[t][0].x - b.x, v[t][0].y - b.y);
-			}
+			if (j < half) radix2_write(buf, v[t][0], v[t][1], j, ns, n, tw);
 		}
 		__syncthreads();
 	}
@@ -221,6 +233,75 @@ k_energy_gdft(const double2 *__restrict__ state, uint32_t n, uint32_t copies, co
 	}
 }
 
+// Short chains (N = 8..256 sites, a power of two): N/4 threads per chain, so a
+// workgroup transforms 1024/N chains at once, one radix-4 butterfly per thread
+// and pass (the butterflies of lds_fft).  The chains of a workgroup are summed
+// in LDS in a fixed order into one partial row per workgroup.
+template <bool DNLS_MODE, uint32_t N>
+__global__ void __launch_bounds__(kThreads)
+k_energy_small(const double2 *__restrict__ state, uint32_t copies, const double2 *__restrict__ tw,
+               const double *__restrict__ omega, double *__restrict__ partial) {
+	constexpr uint32_t T = N / 4, G = kThreads / T;
+	__shared__ double2 lds[G * N];
+	const uint32_t tid = threadIdx.x, g = tid / T, j = tid % T;
+	double2 *buf = lds + g * N;
+	double acc[4] = {0.0, 0.0, 0.0, 0.0}; // modes j + t T
+	const uint32_t per_round = G * gridDim.x;
+	const uint32_t rounds = (copies + per_round - 1) / per_round; // the same for every thread
+	for (uint32_t r = 0; r < rounds; ++r) {
+		const uint32_t c = r * per_round + blockIdx.x * G + g;
+		const bool live = c < copies;
+#pragma unroll
+		for (uint32_t t = 0; t < 4; ++t) buf[j + t * T] = live ? state[size_t(c) * N + j + t * T] : make_double2(0, 0);
+		__syncthreads();
+		uint32_t ns = 1;
+		for (; ns * 4 <= N; ns *= 4) {
+			double2 v[4];
+#pragma unroll
+			for (uint32_t q = 0; q < 4; ++q) v[q] = buf[j + q * T];
+			__syncthreads();
+			radix4_write(buf, v, j, ns, N / (ns * 4), tw);
+			__syncthreads();
+		}
+		if (ns < N) {
+			double2 v[2][2];
+#pragma unroll
+			for (uint32_t q = 0; q < 2; ++q) v[q][0] = buf[j + q * T], v[q][1] = buf[j + q * T + N / 2];
+			__syncthreads();
+#pragma unroll
+			for (uint32_t q = 0; q < 2; ++q) radix2_write(buf, v[q][0], v[q][1], j + q * T, ns, N, tw);
+			__syncthreads();
+		}
+		if (live) {
+#pragma unroll
+			for (uint32_t t = 0; t < 4; ++t) {
+				const uint32_t k = j + t * T;
+				const double2 z = buf[k];
+				if constexpr (DNLS_MODE) {
+					acc[t] += z.x * z.x + z.y * z.y;
+				} else {
+					const double2 zm = buf[(N - k) & (N - 1)];
+					const double w = omega[k];
+					const double a = 1.0 + w, b = 1.0 - w;
+					const double er = fma(a, z.x, -b * zm.x);
+					const double ei = fma(a, z.y, b * zm.y);
+					acc[t] += 0.25 * fma(er, er, ei * ei);
+				}
+			}
+		}
+		__syncthreads();
+	}
+	double *red = reinterpret_cast<double *>(lds); // G x N doubles
+#pragma unroll
+	for (uint32_t t = 0; t < 4; ++t) red[g * N + j + t * T] = acc[t];
+	__syncthreads();
+	for (uint32_t k = tid; k < N; k += kThreads) {
+		double sum = 0.0;
+		for (uint32_t q = 0; q < G; ++q) sum += red[q * N + k];
+		partial[size_t(blockIdx.x) * N + k] = sum;
+	}
+}
+
 __global__ void k_zero(double *p, size_t count) {
 	for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < count; i += size_t(gridDim.x) * blockDim.x)
 		p[i] = 0.0;
@@ -251,15 +332,15 @@ __global__ void k_reduce_partial(const double *__restrict__ partial, uint32_t gr
 	const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
 	if (k >= n) return;
 	double s = 0.0;
+#pragma unroll 8
 	for (uint32_t g = 0; g < grid; ++g) s += partial[size_t(g) * n + k];
 	e_raw[k] = scale ? s * scale[k] : s;
 }
 
 // Single workgroup: normalisation (results_entropy.cpp:11-19) and entropies
 // (results_entropy.cpp:21-46) with a fixed-order tree reduction.
-__global__ void __launch_bounds__(kThreads)
-k_results(const double *__restrict__ e_raw, uint32_t n, double norm, int a0is0, const uint32_t *__restrict__ idx,
-          uint32_t nidx, double *__restrict__ out) {
+__device__ void results_body(const double *e_raw, uint32_t n, double norm, int a0is0, const uint32_t *__restrict__ idx,
+                             uint32_t nidx, double *__restrict__ out) {
 	__shared__ double red[3][kThreads];
 	const uint32_t tid = threadIdx.x;
 	for (uint32_t k = tid; k < n; k += kThreads) out[k] = (a0is0 && k == 0) ? 0.0 : e_raw[k] * norm;
@@ -292,6 +373,56 @@ k_results(const double *__restrict__ e_raw, uint32_t n, double norm, int a0is0, 
 		out[n] = -(totlog + double(cnt) * lognorm);
 		out[n + 1] = normalization * (totelog + lognorm * tot);
 	}
+}
+
+__global__ void __launch_bounds__(kThreads)
+k_results(const double *__restrict__ e_raw, uint32_t n, double norm, int a0is0, const uint32_t *__restrict__ idx,
+          uint32_t nidx, double *__restrict__ out) {
+	results_body(e_raw, n, norm, a0is0, idx, nidx, out);
+}
+
+constexpr uint32_t kFusedMaxN = 2048;
+
+// k_reduce_partial and k_results in one workgroup (n <= kFusedMaxN): the
+// per-mode sums over the rows (same order as k_reduce_partial) go to e_raw and
+// to LDS, where the entropy reduction reads them.
+__global__ void __launch_bounds__(kThreads)
+k_reduce_results(const double *__restrict__ partial, uint32_t rows, uint32_t n, const double *__restrict__ scale,
+                 double *__restrict__ e_raw, double norm, int a0is0, const uint32_t *__restrict__ idx, uint32_t nidx,
+                 double *__restrict__ out) {
+	__shared__ double e[kFusedMaxN];
+	__shared__ double part[kThreads];
+	const uint32_t tid = threadIdx.x;
+	if (2 * n <= kThreads) {
+		// several threads per mode, each summing every slices-th row; the row
+		// loads are independent, so they are unrolled to keep several in flight
+		const uint32_t slices = kThreads / n, k = tid % n, sl = tid / n;
+		double sum = 0.0;
+		if (sl < slices) {
+#pragma unroll 8
+			for (uint32_t g = sl; g < rows; g += slices) sum += partial[size_t(g) * n + k];
+		}
+		part[tid] = sum;
+		__syncthreads();
+		if (tid < n) {
+			double t = 0.0;
+			for (uint32_t q = 0; q < slices; ++q) t += part[q * n + tid];
+			t = scale ? t * scale[tid] : t;
+			e_raw[tid] = t;
+			e[tid] = t;
+		}
+	} else {
+		for (uint32_t k = tid; k < n; k += kThreads) {
+			double sum = 0.0;
+#pragma unroll 8
+			for (uint32_t g = 0; g < rows; ++g) sum += partial[size_t(g) * n + k];
+			sum = scale ? sum * scale[k] : sum;
+			e_raw[k] = sum;
+			e[k] = sum;
+		}
+	}
+	__syncthreads();
+	results_body(e, n, norm, a0is0, idx, nidx, out);
 }
 
 // One workgroup per chain: total energy (and DNLS norm), fixed-order reduction.
@@ -353,9 +484,28 @@ hipError_t launch_lds_energy(hipStream_t st, const double2 *state, uint32_t n, u
 	return hipGetLastError();
 }
 
+bool small_path(uint32_t n) { return is_pow2(n) && n >= 8 && n <= 256; }
+
+template <bool DNLS_MODE, uint32_t N>
+hipError_t launch_small(hipStream_t st, const double2 *state, uint32_t copies, const double2 *tw, const double *omega,
+                        double *partial, uint32_t grid) {
+	hipLaunchKernelGGL((k_energy_small<DNLS_MODE, N>), dim3(grid), dim3(kThreads), 0, st, state, copies, tw, omega,
+	                   partial);
+	return hipGetLastError();
+}
+
 template <bool DNLS_MODE>
 hipError_t launch_energy_mode(hipStream_t st, const double2 *state, uint32_t n, uint32_t copies, const double2 *tw,
                               const double *omega, double *partial, uint32_t grid) {
+	switch (small_path(n) ? n : 0) {
+	case 8: return launch_small<DNLS_MODE, 8>(st, state, copies, tw, omega, partial, grid);
+	case 16: return launch_small<DNLS_MODE, 16>(st, state, copies, tw, omega, partial, grid);
+	case 32: return launch_small<DNLS_MODE, 32>(st, state, copies, tw, omega, partial, grid);
+	case 64: return launch_small<DNLS_MODE, 64>(st, state, copies, tw, omega, partial, grid);
+	case 128: return launch_small<DNLS_MODE, 128>(st, state, copies, tw, omega, partial, grid);
+	case 256: return launch_small<DNLS_MODE, 256>(st, state, copies, tw, omega, partial, grid);
+	default: break;
+	}
 	if (is_pow2(n) && n <= kEnergyFftMax)
 		return launch_lds_energy<DNLS_MODE, kEnergyFftMax>(st, state, n, copies, tw, omega, partial, grid, 1);
 	if (is_pow2(n) && n <= kEnergyFftLarge)
@@ -370,6 +520,13 @@ hipError_t launch_energy_mode(hipStream_t st, const double2 *state, uint32_t n, 
 }
 
 } // namespace
+
+uint32_t energy_rows(uint32_t n, uint32_t copies) {
+	// one partial row per workgroup: short chains 1024/n per workgroup, others
+	// one at a time; long chains get fewer rows (at most 256 MiB of partials)
+	if (small_path(n)) return std::min<uint32_t>((copies + 1024 / n - 1) / (1024 / n), 1024u);
+	return std::min<uint32_t>({copies, 1024u, std::max<uint32_t>(1u, (1u << 25) / n)});
+}
 
 static hipError_t launch_energy_common(hipStream_t st, bool dnls, const double2 *state, uint32_t n, uint32_t copies,
                                        const double2 *tw, const double *omega, double *partial, uint32_t grid) {
@@ -398,6 +555,24 @@ hipError_t launch_reduce_partial(hipStream_t st, const double *partial, uint32_t
 		grid = chunks;
 	}
 	hipLaunchKernelGGL(k_reduce_partial, dim3((n + 255) / 256), dim3(256), 0, st, partial, grid, n, scale, e_raw);
+	return hipGetLastError();
+}
+
+hipError_t launch_reduce_results(hipStream_t st, const double *partial, uint32_t grid, uint32_t n, const double *scale,
+                                 double *e_raw, double *scratch, double norm, int a0is0, const uint32_t *idx,
+                                 uint32_t nidx, double *out) {
+	if (n > kFusedMaxN) {
+		hipError_t e = launch_reduce_partial(st, partial, grid, n, scale, e_raw, scratch);
+		return e != hipSuccess ? e : launch_results(st, e_raw, n, norm, a0is0, idx, nidx, out);
+	}
+	if (grid > 64) {
+		const uint32_t chunks = reduce_scratch_rows(grid);
+		hipLaunchKernelGGL(k_reduce_rows, dim3((n + 63) / 64, chunks), dim3(256), 0, st, partial, grid, n, scratch);
+		partial = scratch;
+		grid = chunks;
+	}
+	hipLaunchKernelGGL(k_reduce_results, dim3(1), dim3(kThreads), 0, st, partial, grid, n, scale, e_raw, norm, a0is0,
+	                   idx, nidx, out);
 	return hipGetLastError();
 }
 

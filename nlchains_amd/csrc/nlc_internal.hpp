@@ -71,7 +71,9 @@ hipError_t launch_split_to_planar(hipStream_t, const double *q, const double *p,
                                   uint32_t n);
 
 // Energy pass of the plane-wave models (DNKG/FPUT/Toda): per-block partial sums
-// of e_k over copies into partial[grid][n].  tw[j] = exp(-2 pi i j/n).
+// of e_k over copies into partial[grid][n], grid = energy_rows(n, copies).
+// tw[j] = exp(-2 pi i j/n).
+uint32_t energy_rows(uint32_t n, uint32_t copies);
 hipError_t launch_energy_real(hipStream_t, const double2 *state, uint32_t n, uint32_t copies, const double2 *tw,
                               const double *omega, double *partial, uint32_t grid);
 // DNLS energies: partial[g][k] = sum |psi_hat_k|^2 (omega applied in the reduction).
@@ -85,6 +87,11 @@ hipError_t launch_reduce_partial(hipStream_t, const double *partial, uint32_t gr
 // out[0..n) = normalised linenergies, out[n] = S_WT, out[n+1] = S_INF.
 hipError_t launch_results(hipStream_t, const double *e_raw, uint32_t n, double norm, int a0is0,
                           const uint32_t *idx, uint32_t nidx, double *out);
+// launch_reduce_partial followed by launch_results, fused into one workgroup
+// for n <= 2048.
+hipError_t launch_reduce_results(hipStream_t, const double *partial, uint32_t grid, uint32_t n, const double *scale,
+                                 double *e_raw, double *scratch, double norm, int a0is0, const uint32_t *idx,
+                                 uint32_t nidx, double *out);
 // Total energy per copy (and DNLS norm).
 hipError_t launch_chain_energy(hipStream_t, int model, const double2 *state, uint32_t n, uint32_t copies,
                                const StepArgs &args, const double *masses, double *energy, double *norm);

@@ -225,7 +225,11 @@ def test_eigensystem_device_path(gpu, n):
                                                  ("Toda", 128, 20), ("dDNKG", 512, 70), ("dDNKG", 100, 9),
                                                  ("DNKG", 3000, 3), ("FPUT", 96, 5),
                                                  # 8192-point LDS FFT; DFT in 160 KB of LDS; DFT from L1/L2
-                                                 ("DNKG", 8192, 5), ("Toda", 5120, 3), ("FPUT", 6000, 3)])
+                                                 ("DNKG", 8192, 5), ("Toda", 5120, 3), ("FPUT", 6000, 3),
+                                                 # short chains, 1024/n per workgroup: partial groups,
+                                                 # several rounds per workgroup
+                                                 ("DNKG", 8, 129), ("FPUT", 16, 3), ("Toda", 32, 1000),
+                                                 ("FPUT", 64, 70000)])
 def test_linenergies_and_entropy(gpu, model_name, n, copies):
     N = gpu
     banner(f"linenergies {model_name} n={n} copies={copies}")
