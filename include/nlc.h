@@ -87,7 +87,8 @@ typedef struct nlc_params {
 	int32_t kernel;           /* nlc_kernel */
 	uint32_t chain_length;    /* -n, >= 2; DNLS: at most 4096, or a power of two up to 8192
 	                             (NLC_EUNSUPPORTED otherwise) */
-	uint32_t copies;          /* -c, >= 1 */
+	uint32_t copies;          /* -c, >= 1; chain_length * copies <= 2^31 (32 GiB of state);
+	                             DNLS: copies < 2^24 */
 	double dt;                /* --dt, > 0 */
 	double m;                 /* -m (DNKG linear parameter) */
 	double alpha;             /* --alpha (FPUT cubic term; Toda steepness, != 0) */

@@ -369,6 +369,8 @@ int setup_common(nlc_ctx *c) {
 int setup_dnls(nlc_ctx *c) {
 	const uint32_t n = c->n;
 	const nlc_params &p = c->p;
+	if (c->copies >= (1u << 24))
+		return c->fail(NLC_EUNSUPPORTED, "DNLS runs one workgroup per chain: at most 2^24 - 1 copies");
 	if (!nlc::dnls_supported(n))
 		return c->fail(NLC_EUNSUPPORTED, "DNLS needs a chain_length of 2 to %u, or a power of two up to %u", 4096u,
 		               nlc::kDnlsLdsMax);
@@ -569,8 +571,10 @@ int validate_params(const nlc_params *p, bool ragged) {
 	if (!ragged) {
 		if (p->chain_length < 2) return reject(NLC_EARG, "--chain_length must be >= 2");
 		if (!p->copies) return reject(NLC_EARG, "--copies must be a positive number");
-		if (uint64_t(p->chain_length) * p->copies >= 0x40000000ULL)
-			return reject(NLC_EARG, "(--chain_length * --copies) must be < 2^30");
+		// every launch has at most about one work-item per site (per-chain kernels
+		// with more are grid-strided or capped), so launch sizes stay below 2^32
+		if (uint64_t(p->chain_length) * p->copies > (1ULL << 31))
+			return reject(NLC_EARG, "(--chain_length * --copies) must be <= 2^31");
 	}
 	if (!(p->dt > 0)) return reject(NLC_EARG, "--dt must be a positive number");
 	if (p->model == NLC_TODA && p->alpha == 0) return reject(NLC_EARG, "Toda needs --alpha != 0");
@@ -696,7 +700,7 @@ int nlc_create_ragged(nlc_ctx **out, const nlc_params *p, const uint32_t *chain_
 		if (chain_lengths[i] < 2) return reject(NLC_EARG, "every chain length must be >= 2");
 		total += chain_lengths[i];
 	}
-	if (total >= 0x40000000ULL) return reject(NLC_EARG, "total number of sites must be < 2^30");
+	if (total > (1ULL << 31)) return reject(NLC_EARG, "total number of sites must be <= 2^31");
 	nlc_ctx *c = new (std::nothrow) nlc_ctx;
 	if (!c) return reject(NLC_ENOMEM, "context allocation failed");
 	c->p = *p;

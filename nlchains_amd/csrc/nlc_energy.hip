@@ -427,44 +427,49 @@ k_reduce_results(const double *__restrict__ partial, uint32_t rows, uint32_t n, 
 
 // One workgroup per chain: total energy (and DNLS norm), fixed-order reduction.
 __global__ void __launch_bounds__(kThreads)
-k_chain_energy(int model, const double2 *__restrict__ state, uint32_t n, StepArgs a,
+k_chain_energy(int model, const double2 *__restrict__ state, uint32_t n, uint32_t copies, StepArgs a,
                const double *__restrict__ masses, double *__restrict__ energy, double *__restrict__ norm) {
 	__shared__ double red[2][kThreads];
-	const uint32_t c = blockIdx.x, tid = threadIdx.x;
-	const double2 *s = state + size_t(c) * n;
-	double e = 0, nr = 0;
-	for (uint32_t x = tid; x < n; x += kThreads) {
-		const double2 v = s[x], w = s[x + 1 == n ? 0 : x + 1];
-		if (model == DNLS) {
-			const double dr = w.x - v.x, di = w.y - v.y, a2 = v.x * v.x + v.y * v.y;
-			e += dr * dr + di * di + a.beta * 0.5 * a2 * a2;
-			nr += a2;
-		} else {
-			const double q = v.x, p = v.y, d = w.x - q;
-			double h = 0.5 * p * p;
-			switch (model) {
-			case DNKG: h += 0.5 * d * d + 0.5 * a.m * q * q + 0.25 * a.beta * q * q * q * q; break;
-			case DDNKG: h += 0.5 * d * d + 0.5 * masses[x] * q * q + 0.25 * a.beta * q * q * q * q; break;
-			case FPUT: h += 0.5 * d * d + a.alpha * d * d * d / 3.0 + 0.25 * a.beta * d * d * d * d; break;
-			case TODA: {
-				const double al = 0.5 * a.a2, inv4a2 = 1.0 / (4.0 * al * al);
-				h += exp(a.a2 * d) * inv4a2 - d * a.a2i - inv4a2;
-				break;
+	const uint32_t tid = threadIdx.x;
+	// grid-stride over the chains (the grid is capped so that the launch stays
+	// within 2^32 work-items for any number of copies)
+	for (uint32_t c = blockIdx.x; c < copies; c += gridDim.x) {
+		const double2 *s = state + size_t(c) * n;
+		double e = 0, nr = 0;
+		for (uint32_t x = tid; x < n; x += kThreads) {
+			const double2 v = s[x], w = s[x + 1 == n ? 0 : x + 1];
+			if (model == DNLS) {
+				const double dr = w.x - v.x, di = w.y - v.y, a2 = v.x * v.x + v.y * v.y;
+				e += dr * dr + di * di + a.beta * 0.5 * a2 * a2;
+				nr += a2;
+			} else {
+				const double q = v.x, p = v.y, d = w.x - q;
+				double h = 0.5 * p * p;
+				switch (model) {
+				case DNKG: h += 0.5 * d * d + 0.5 * a.m * q * q + 0.25 * a.beta * q * q * q * q; break;
+				case DDNKG: h += 0.5 * d * d + 0.5 * masses[x] * q * q + 0.25 * a.beta * q * q * q * q; break;
+				case FPUT: h += 0.5 * d * d + a.alpha * d * d * d / 3.0 + 0.25 * a.beta * d * d * d * d; break;
+				case TODA: {
+					const double al = 0.5 * a.a2, inv4a2 = 1.0 / (4.0 * al * al);
+					h += exp(a.a2 * d) * inv4a2 - d * a.a2i - inv4a2;
+					break;
+				}
+				}
+				e += h;
 			}
-			}
-			e += h;
 		}
-	}
-	red[0][tid] = e;
-	red[1][tid] = nr;
-	__syncthreads();
-	for (uint32_t w = kThreads / 2; w > 0; w >>= 1) {
-		if (tid < w) red[0][tid] += red[0][tid + w], red[1][tid] += red[1][tid + w];
+		red[0][tid] = e;
+		red[1][tid] = nr;
 		__syncthreads();
-	}
-	if (tid == 0) {
-		energy[c] = red[0][0];
-		if (norm) norm[c] = red[1][0];
+		for (uint32_t w = kThreads / 2; w > 0; w >>= 1) {
+			if (tid < w) red[0][tid] += red[0][tid + w], red[1][tid] += red[1][tid + w];
+			__syncthreads();
+		}
+		if (tid == 0) {
+			energy[c] = red[0][0];
+			if (norm) norm[c] = red[1][0];
+		}
+		__syncthreads(); // red is rewritten by the next chain
 	}
 }
 
@@ -584,8 +589,8 @@ hipError_t launch_results(hipStream_t st, const double *e_raw, uint32_t n, doubl
 
 hipError_t launch_chain_energy(hipStream_t st, int model, const double2 *state, uint32_t n, uint32_t copies,
                                const StepArgs &args, const double *masses, double *energy, double *norm) {
-	hipLaunchKernelGGL(k_chain_energy, dim3(copies), dim3(kThreads), 0, st, model, state, n, args, masses, energy,
-	                   norm);
+	hipLaunchKernelGGL(k_chain_energy, dim3(std::min<uint32_t>(copies, 1u << 20)), dim3(kThreads), 0, st, model, state,
+	                   n, copies, args, masses, energy, norm);
 	return hipGetLastError();
 }
 

@@ -117,8 +117,8 @@ void run_common(Options &o, int argc, char **argv, Config &c) {
 		if (c.steps % c.batch) throw std::invalid_argument("--steps must be a multiple of --kernel_batching");
 		if (c.time_offset && c.steps && c.time_offset >= c.steps)
 			throw std::invalid_argument("--time_offset must be less than --steps");
-		if (uint64_t(c.n) * c.copies >= 0x40000000ULL)
-			throw std::invalid_argument("(--chain_length * --copies) must be < 2^30");
+		if (uint64_t(c.n) * c.copies > (1ULL << 31))
+			throw std::invalid_argument("(--chain_length * --copies) must be <= 2^31");
 	} catch (const std::invalid_argument &e) {
 		throw std::invalid_argument(std::string(e.what()) + "\n" + o.usage());
 	}

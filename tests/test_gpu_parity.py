@@ -219,6 +219,40 @@ def test_eigensystem_device_path(gpu, n):
     assert np.abs(V.T @ V - np.eye(n)).max() <= 1e-12
 
 
+def test_ensemble_beyond_2_30_sites(gpu):
+    """FPUT 64 sites x 4096*4097 copies = 1.07e9 sites (17 GB of state, past
+    2^30), built as 4097 tiles of one seeded 4096-copy ensemble: every tile
+    ends bit-identical, sampled chains match the oracle, the chain energies of
+    all tiles agree, and the mode energies are the tile's times 4097."""
+    N = gpu
+    n, base_copies, tiles, steps = 64, 4096, 4097, 20
+    model, kw, masses, base = make_case(N, "FPUT", n, base_copies, 8)
+    copies = base_copies * tiles
+    assert n * copies > 1 << 30
+    banner(f"large ensemble: {n} x {copies} = {n * copies:.3e} sites")
+    s0 = np.tile(base, (tiles, 1, 1))
+    with N.Ensemble(model, n, copies, 0.1, **kw) as ens:
+        ens.upload(s0)
+        del s0
+        ens.move(steps)
+        got = ens.download().reshape(tiles, base_copies, n, 2)
+        E, _ = ens.chain_energies()
+        raw = ens.linenergies_raw()
+    first = got[0]
+    for t in range(1, tiles):
+        assert np.array_equal(got[t], first), t
+    del got
+    want = oracle_move(model, kw, masses, base[:8], steps, 0.1)
+    assert rel_err(first[:8], want) <= STATE_TOL
+    E = E.reshape(tiles, base_copies)
+    assert np.array_equal(E, np.broadcast_to(E[0], E.shape))
+    np.testing.assert_allclose(E[0, :8], O.chain_energy(O.FPUT, want, **kw), rtol=1e-12)
+    e_tile = O.linenergies_real(first, O.dispersion_real(n, 0.0))
+    assert rel_err(raw, tiles * e_tile) <= 1e-11
+    with pytest.raises(N.NlcError):
+        N.Ensemble(N.DNLS, 2, 1 << 24, 0.01, beta=1.0)  # one workgroup per DNLS chain: < 2^24 copies
+
+
 # ------------------------------------------------------------ energy pass
 
 @pytest.mark.parametrize("model_name,n,copies", [("DNKG", 64, 50), ("DNKG", 1024, 64), ("FPUT", 256, 33),

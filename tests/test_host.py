@@ -39,7 +39,7 @@ def test_strerror_and_params_init():
 @pytest.mark.parametrize("kw,code", [
     (dict(n=1), N.NLC_EARG), (dict(copies=0), N.NLC_EARG), (dict(dt=0.0), N.NLC_EARG),
     (dict(dt=-1.0), N.NLC_EARG), (dict(model=N.TODA, alpha=0.0), N.NLC_EARG),
-    (dict(model=N.DDNKG), N.NLC_EARG), (dict(n=1 << 15, copies=1 << 15), N.NLC_EARG),
+    (dict(model=N.DDNKG), N.NLC_EARG), (dict(n=1 << 16, copies=(1 << 15) + 1), N.NLC_EARG),
 ])
 def test_create_rejects_invalid_arguments(kw, code):
     """The argument rules of common/main.cpp:83-103 (checked before any device call)."""
