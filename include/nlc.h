@@ -58,9 +58,10 @@ typedef enum { NLC_FP64 = 0, NLC_FP32C = 1 } nlc_precision;
  * (DNKG_FPUT_Toda.cpp:43,91-94; dDNKG.cpp:34): HBM-streaming over the
  * ensemble-major (split) layout.  NLC_KERNEL_WAVE keeps each chain in the
  * registers of one or more wavefront lanes (the reference's move_chain_in_warp /
- * move_chain_in_thread).  NLC_KERNEL_AUTO picks WAVE when a shape exists for
- * the chain length, else SPLIT (the reference's move<> dispatch,
- * DNKG_FPUT_Toda.cu:262-297). */
+ * move_chain_in_thread), or of a workgroup of 2-8 wavefronts for chains of
+ * 2049-16384 sites.  NLC_KERNEL_AUTO picks WAVE when a shape exists for the
+ * chain length (every length up to 16384), else SPLIT (the reference's move<>
+ * dispatch, DNKG_FPUT_Toda.cu:262-297). */
 typedef enum { NLC_KERNEL_AUTO = 0, NLC_KERNEL_WAVE = 1, NLC_KERNEL_SPLIT = 2 } nlc_kernel;
 
 /* dDNKG energy projection (the two cublasDgemm of dDNKG.cpp:118-135 and the
@@ -211,8 +212,8 @@ int nlc_batch(nlc_ctx *ctx, uint32_t steps, double norm, double *linenergies, do
 int nlc_batch_submit(nlc_ctx *ctx, uint32_t steps, double norm);
 int nlc_batch_collect(nlc_ctx *ctx, double *linenergies, double *entropies);
 
-/* Times every applicable step-kernel shape (wave shapes lanes x sites and the
- * split kernel for uniform ensembles; sites per lane of the persistent kernel
+/* Times every applicable step-kernel shape (wave and workgroup shapes lanes x
+ * sites and the split kernel for uniform ensembles; sites per lane of the persistent kernel
  * for ragged ones) on the context's own state for trial_steps steps, keeps the
  * fastest, restores the state, and writes a one-line report.  Replaces the
  * reference's compile-time optimized_chain_length choice (CMakeLists.txt:74-78)
