@@ -62,6 +62,8 @@ struct nlc_ctx {
 	double *partial = nullptr; // energy partial sums
 	double *reduce_scratch = nullptr;
 	double *e_raw = nullptr;   // table_len
+	double2 *fft_scratch = nullptr; // two-level FFT of chains above 8192 sites
+	size_t fft_scratch_sites = 0;
 	uint32_t *modes = nullptr; // entropy mode indices
 	uint32_t nmodes = 0;
 	int a0is0 = 0;
@@ -269,7 +271,7 @@ void free_all(nlc_ctx *c) {
 	     {(void *) c->state, (void *) c->omega, (void *) c->tw, (void *) c->partial, (void *) c->reduce_scratch,
 	      (void *) c->e_raw, (void *) c->modes, (void *) c->mp2, (void *) c->masses_d,
 	      (void *) c->eigvec, (void *) c->split_q[0], (void *) c->split_p[0], (void *) c->split_q[1], (void *) c->split_p[1], (void *) c->energy_d, (void *) c->norm_d,
-	      (void *) c->dnls_table, (void *) c->items, (void *) c->vplanes, (void *) c->splanes})
+	      (void *) c->dnls_table, (void *) c->fft_scratch, (void *) c->items, (void *) c->vplanes, (void *) c->splanes})
 		if (ptr) hipFree(ptr);
 	if (c->h_results) hipHostFree(c->h_results);
 	if (c->h_slots) hipHostFree(c->h_slots);
@@ -333,6 +335,8 @@ int setup_common(nlc_ctx *c) {
 		}
 		if (p.model != NLC_DDNKG) nlc_dispersion(p.model, k.n, p.m, omega.data() + k.tab0);
 		k.rows = p.model == NLC_DDNKG ? nlc::ddnkg_partial_rows(k.count) : nlc::energy_rows(k.n, k.count);
+		if (p.model != NLC_DDNKG)
+			c->fft_scratch_sites = std::max(c->fft_scratch_sites, nlc::energy_scratch_sites(k.n, k.count));
 		partial_len = std::max(partial_len, size_t(k.rows) * k.n);
 		scratch_len = std::max(scratch_len, size_t(nlc::reduce_scratch_rows(k.rows)) * k.n);
 	}
@@ -361,6 +365,7 @@ int setup_common(nlc_ctx *c) {
 	}
 	NLC_HIP(c, hipMemcpy(c->omega, omega.data(), sizeof(double) * c->table_len, hipMemcpyHostToDevice));
 	NLC_TRY(dev_alloc(c, &c->partial, partial_len, "energy partials"));
+	if (c->fft_scratch_sites) NLC_TRY(dev_alloc(c, &c->fft_scratch, c->fft_scratch_sites, "FFT scratch"));
 	NLC_TRY(dev_alloc(c, &c->reduce_scratch, scratch_len, "reduction scratch"));
 	return NLC_OK;
 }
@@ -373,7 +378,7 @@ int setup_dnls(nlc_ctx *c) {
 		return c->fail(NLC_EUNSUPPORTED, "DNLS runs one workgroup per chain: at most 2^24 - 1 copies");
 	if (!nlc::dnls_supported(n))
 		return c->fail(NLC_EUNSUPPORTED, "DNLS needs a chain_length of 2 to %u, or a power of two up to %u", 4096u,
-		               nlc::kDnlsLdsMax);
+		               nlc::kLargeFftMax);
 	const double bdt = p.beta * p.dt;
 	for (int k = 0; k < 8; ++k) c->theta[k] = (bdt * 1.) * nlc::kYoshidaC[k];
 	c->theta[8] = (bdt * 2.) * nlc::kYoshidaC[0];
@@ -392,6 +397,18 @@ int setup_dnls(nlc_ctx *c) {
 	if (const uint32_t M = nlc::dnls_bluestein_size(n)) {
 		tab.resize(size_t(7) * n + size_t(2) * M + n);
 		nlc::dnls_bluestein_tables(n, tab.data() + size_t(7) * n);
+	} else if (nlc::dnls_two_level(n)) {
+		// the two-level FFT keeps the spectrum in HBM in [r][k] order: the
+		// multipliers in that order, and a scratch of the state's size
+		nlc::dnls_two_level_tables(n, tab.data(), perm.data());
+		tab.swap(perm);
+		if (c->fft_scratch_sites < c->total_sites) {
+			if (c->fft_scratch) hipFree(c->fft_scratch);
+			c->fft_scratch = nullptr;
+			c->fft_scratch_sites = 0;
+			NLC_TRY(dev_alloc(c, &c->fft_scratch, c->total_sites, "DNLS spectrum"));
+			c->fft_scratch_sites = c->total_sites;
+		}
 	} else if (nlc::dnls_fast_path(n)) {
 		perm.resize(size_t(8) * n);
 		nlc::dnls_permute_tables(n, nlc::kDnlsRadix16, tab.data(), perm.data());
@@ -483,7 +500,7 @@ int enqueue_move(nlc_ctx *c, uint32_t steps) {
 	if (c->p.model == NLC_DNLS) {
 		const bool r8 = nlc::dnls_fast_path(c->n) && c->dnls_variant == nlc::kDnlsRadix8;
 		NLC_HIP(c, nlc::launch_dnls(c->stream, c->state, c->n, c->copies, steps, c->theta, c->tw,
-		                            c->dnls_table + (r8 ? size_t(4) * c->n : 0), c->dnls_variant));
+		                            c->dnls_table + (r8 ? size_t(4) * c->n : 0), c->dnls_variant, c->fft_scratch));
 	} else if (c->ragged) {
 		if (c->ragged_launcher) {
 			NLC_HIP(c, c->ragged_launcher(c->stream, c->state, c->items, c->nitems, steps, c->args, 0));
@@ -526,14 +543,15 @@ int enqueue_energy(nlc_ctx *c, double norm = 0.0, double *dst = nullptr) {
 		case NLC_FPUT:
 		case NLC_TODA:
 			NLC_HIP(c, nlc::launch_energy_real(c->stream, st, k.n, k.count, c->tw + k.tab0, c->omega + k.tab0,
-			                                   c->partial, k.rows));
+			                                   c->partial, k.rows, c->fft_scratch, c->fft_scratch_sites));
 			break;
 		case NLC_DDNKG:
 			NLC_HIP(c, nlc::launch_energy_ddnkg(c->stream, st, k.n, k.count, c->eigvec, c->omega, c->partial,
 			                                    c->proj_variant, c->vplanes, c->splanes));
 			break;
 		case NLC_DNLS:
-			NLC_HIP(c, nlc::launch_energy_dnls(c->stream, st, k.n, k.count, c->tw + k.tab0, c->partial, k.rows));
+			NLC_HIP(c, nlc::launch_energy_dnls(c->stream, st, k.n, k.count, c->tw + k.tab0, c->partial, k.rows,
+			                                   c->fft_scratch, c->fft_scratch_sites));
 			scale = c->omega + k.tab0;
 			break;
 		default: return c->fail(NLC_EUNSUPPORTED, "no energy pass for this model");
@@ -1129,7 +1147,8 @@ int nlc_describe(const nlc_ctx *c, char *buf, size_t len) {
 	static const char *names[] = {"DNKG", "FPUT", "Toda", "dDNKG", "DNLS"};
 	if (c->p.model == NLC_DNLS)
 		snprintf(buf, len, "model=DNLS n=%u copies=%u kernel=%s", c->n, c->copies,
-		         nlc::dnls_bluestein_size(c->n)            ? "dnls_lds_bluestein"
+		         nlc::dnls_two_level(c->n)                 ? "dnls_two_level_fft"
+		         : nlc::dnls_bluestein_size(c->n)          ? "dnls_lds_bluestein"
 		         : !nlc::dnls_fast_path(c->n)              ? "dnls_lds_stockham"
 		         : c->dnls_variant == nlc::kDnlsRadix8 ? "dnls_radix8_registers"
 		                                                : "dnls_radix16_registers");

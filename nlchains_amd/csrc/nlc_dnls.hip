@@ -33,6 +33,7 @@
 //    |r| <= pi/16 (degree 11 and 12), after a Cody & Waite reduction by pi/8
 //    for larger angles.
 #include "nlc_internal.hpp"
+#include "nlc_fft_large.hpp"
 
 #include <cmath>
 #include <utility>
@@ -609,15 +610,15 @@ k_dnls_4096_r8(double2 *__restrict__ state, uint32_t copies, uint32_t steps, Dnl
 // ------------------------------------------------------------ generic n
 
 // In-place radix-4 (+ final radix-2) Stockham FFT of n <= MAXN points, forward
-// sign, natural order; tw[j] = e^{-2 pi i j/n}.
+// sign, natural order; tw[j tws] = e^{-2 pi i j/n} (tws > 1: a longer table).
 template <uint32_t MAXN>
-__device__ void lds_fft_fwd(double2 *buf, uint32_t n, const double2 *__restrict__ tw) {
+__device__ void lds_fft_fwd(double2 *buf, uint32_t n, const double2 *__restrict__ tw, uint32_t tws = 1) {
 	constexpr int kMaxPer = MAXN / 4 / kT; // radix-4 butterflies per thread per pass
 	const uint32_t tid = threadIdx.x, quarter = n >> 2;
 	uint32_t ns = 1;
 	for (; ns * 4 <= n; ns *= 4) {
 		double2 v[kMaxPer][4];
-		const uint32_t span = n / (ns * 4);
+		const uint32_t span = n / (ns * 4) * tws;
 #pragma unroll
 		for (int q = 0; q < kMaxPer; ++q) {
 			const uint32_t j = tid + q * kT;
@@ -654,7 +655,7 @@ __device__ void lds_fft_fwd(double2 *buf, uint32_t n, const double2 *__restrict_
 			const uint32_t j = tid + q * kT;
 			if (j < half) {
 				const uint32_t k = j % ns;
-				const double2 b = cmul(v[q][1], tw[k * (n / (2 * ns))]);
+				const double2 b = cmul(v[q][1], tw[k * (n / (2 * ns)) * tws]);
 				const uint32_t o = (j / ns) * ns * 2 + k;
 				buf[o] = cadd(v[q][0], b);
 				buf[o + ns] = csub(v[q][0], b);
@@ -744,6 +745,87 @@ k_dnls_bluestein(double2 *__restrict__ state, uint32_t n, uint32_t M, uint32_t c
 	for (uint32_t x = tid; x < n; x += kT) psi[x] = nonlinear(cmul(buf[x], cconj(chirp[x])), a.theta[7]);
 }
 
+// Chains above 8192 sites (powers of two up to 131072, n = R M): the two-level
+// FFT of nlc_fft_large.hpp with the chain and its spectrum in HBM (scratch:
+// copies x n values, the spectrum in the order [r][k]).  A substep is two
+// launches: the row kernel (M-point FFT in LDS, Fourier multiplier, inverse
+// FFT, conjugate twiddle) and the column kernel, which finishes the inverse
+// transform of one substep (R-point DFT), applies the nonlinear phase of the
+// next and starts its forward transform (R-point DFT and twiddle).
+// MODE 0: the first substep, reads the state; 1: between substeps; 2: after
+// the last one, applies theta_7 and writes the state.
+template <uint32_t R, int MODE>
+__global__ void __launch_bounds__(kT)
+k_dnls_col(double2 *__restrict__ state, double2 *__restrict__ scratch, uint32_t n, uint32_t M, uint32_t copies,
+           double th, const double2 *__restrict__ tw) {
+	const uint64_t t = uint64_t(blockIdx.x) * kT + threadIdx.x;
+	if (t >= uint64_t(copies) * M) return;
+	const uint32_t c = uint32_t(t / M), m = uint32_t(t % M);
+	double2 root[R], x[R], y[R];
+	small_roots<R>(root, tw, n);
+	double2 *psi = state + size_t(c) * n + m, *sc = scratch + size_t(c) * n + m;
+	if constexpr (MODE == 0) {
+#pragma unroll
+		for (uint32_t j = 0; j < R; ++j) x[j] = psi[size_t(j) * M];
+	} else {
+#pragma unroll
+		for (uint32_t r = 0; r < R; ++r) y[r] = sc[size_t(r) * M];
+		small_dft<R, true>(y, x, root);
+	}
+#pragma unroll
+	for (uint32_t j = 0; j < R; ++j) x[j] = nonlinear(x[j], th);
+	if constexpr (MODE == 2) {
+#pragma unroll
+		for (uint32_t j = 0; j < R; ++j) psi[size_t(j) * M] = x[j];
+	} else {
+		small_dft<R, false>(x, y, root);
+#pragma unroll
+		for (uint32_t r = 0; r < R; ++r) sc[size_t(r) * M] = cmul(y[r], tw[m * r]);
+	}
+}
+
+// One row (chain c, block r) per workgroup: Z = FFT_M(y_r), U = Z T_k (the
+// multipliers in the same [r][k] order), v_r[m] = IFFT_M(U)[m] W_N^{-m r}
+// (unnormalised; T carries the 1/n), the inverse as conj(FFT(conj(.))).
+template <uint32_t M>
+__global__ void __launch_bounds__(kT)
+k_dnls_row(double2 *__restrict__ scratch, uint32_t R, const double2 *__restrict__ tw, const double2 *__restrict__ tk) {
+	extern __shared__ double2 buf[];
+	const uint32_t r = blockIdx.x % R;
+	double2 *row = scratch + size_t(blockIdx.x) * M;
+	for (uint32_t x = threadIdx.x; x < M; x += kT) buf[x] = row[x];
+	__syncthreads();
+	lds_fft_fwd<M>(buf, M, tw, R);
+	const double2 *t = tk + size_t(r) * M;
+	for (uint32_t k = threadIdx.x; k < M; k += kT) buf[k] = cconj(cmul(buf[k], t[k]));
+	__syncthreads();
+	lds_fft_fwd<M>(buf, M, tw, R);
+	for (uint32_t m = threadIdx.x; m < M; m += kT) row[m] = cconj(cmul(buf[m], tw[m * r]));
+}
+
+template <uint32_t R, uint32_t M>
+hipError_t launch_dnls_two_level(hipStream_t st, double2 *state, double2 *scratch, uint32_t n, uint32_t copies,
+                                 uint32_t steps, const DnlsArgs &a, const double2 *tw, const double2 *table) {
+	if (M * sizeof(double2) > 64 * 1024) {
+		hipError_t e = hipFuncSetAttribute((const void *) k_dnls_row<M>, hipFuncAttributeMaxDynamicSharedMemorySize,
+		                                   int(M * sizeof(double2)));
+		if (e != hipSuccess) return e;
+	}
+	const dim3 cols(uint32_t((uint64_t(copies) * M + kT - 1) / kT)), rows(copies * R), block(kT);
+	const size_t lds = M * sizeof(double2);
+	for (uint32_t s = 0; s < steps; ++s)
+		for (int k = 0; k < 7; ++k) {
+			const double th = (k == 0 && s) ? a.theta[8] : a.theta[k];
+			if (s == 0 && k == 0)
+				hipLaunchKernelGGL((k_dnls_col<R, 0>), cols, block, 0, st, state, scratch, n, M, copies, th, tw);
+			else
+				hipLaunchKernelGGL((k_dnls_col<R, 1>), cols, block, 0, st, state, scratch, n, M, copies, th, tw);
+			hipLaunchKernelGGL(k_dnls_row<M>, rows, block, lds, st, scratch, R, tw, table + size_t(k) * n);
+		}
+	hipLaunchKernelGGL((k_dnls_col<R, 2>), cols, block, 0, st, state, scratch, n, M, copies, a.theta[7], tw);
+	return hipGetLastError();
+}
+
 template <typename K>
 hipError_t launch_lds_chain_kernel(K kernel, size_t lds) {
 	if (lds > 64 * 1024)
@@ -784,7 +866,26 @@ uint32_t dnls_bluestein_size(uint32_t n) {
 	return M <= kDnlsBluesteinMax ? M : 0;
 }
 
-bool dnls_supported(uint32_t n) { return (pow2(n) && n <= kDnlsLdsMax) || dnls_bluestein_size(n) != 0; }
+bool dnls_supported(uint32_t n) {
+	uint32_t R, M;
+	return (pow2(n) && n <= kDnlsLdsMax) || dnls_bluestein_size(n) != 0 || large_fft_split(n, R, M);
+}
+
+bool dnls_two_level(uint32_t n) {
+	uint32_t R, M;
+	return large_fft_split(n, R, M);
+}
+
+// Multipliers of the two-level path in the spectrum's [r][k] order:
+// out[k][r M + q] = table[k][r + R q].
+void dnls_two_level_tables(uint32_t n, const double2 *table, double2 *out) {
+	uint32_t R, M;
+	if (!large_fft_split(n, R, M)) return;
+	for (int k = 0; k < 7; ++k)
+		for (uint32_t r = 0; r < R; ++r)
+			for (uint32_t q = 0; q < M; ++q)
+				out[size_t(k) * n + size_t(r) * M + q] = table[size_t(k) * n + r + size_t(R) * q];
+}
 
 void dnls_bluestein_tables(uint32_t n, double2 *out) {
 	const uint32_t M = dnls_bluestein_size(n);
@@ -828,9 +929,20 @@ void dnls_permute_tables(uint32_t n, int variant, const double2 *table, double2 
 }
 
 hipError_t launch_dnls(hipStream_t st, double2 *state, uint32_t n, uint32_t copies, uint32_t steps,
-                       const double *theta9, const double2 *tw, const double2 *table, int variant) {
+                       const double *theta9, const double2 *tw, const double2 *table, int variant, double2 *scratch) {
 	DnlsArgs a;
 	for (int i = 0; i < 9; ++i) a.theta[i] = theta9[i];
+	uint32_t R = 0, M = 0;
+	if (large_fft_split(n, R, M)) {
+		if (!scratch) return hipErrorInvalidValue;
+		if (M == 8192) return launch_dnls_two_level<16, 8192>(st, state, scratch, n, copies, steps, a, tw, table);
+		switch (R) {
+		case 4: return launch_dnls_two_level<4, 4096>(st, state, scratch, n, copies, steps, a, tw, table);
+		case 8: return launch_dnls_two_level<8, 4096>(st, state, scratch, n, copies, steps, a, tw, table);
+		case 16: return launch_dnls_two_level<16, 4096>(st, state, scratch, n, copies, steps, a, tw, table);
+		}
+		return hipErrorInvalidValue;
+	}
 	if (dnls_fast_path(n) && variant == kDnlsRadix8) {
 		hipError_t e = hipFuncSetAttribute((const void *) k_dnls_4096_r8, hipFuncAttributeMaxDynamicSharedMemorySize,
 		                                   int(kR8Lds));

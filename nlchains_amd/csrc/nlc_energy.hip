@@ -21,6 +21,7 @@
 // of transforming q and p separately).  Per-block partial sums are reduced in
 // a fixed order, so results are bit-reproducible run to run.
 #include "nlc_internal.hpp"
+#include "nlc_fft_large.hpp"
 
 #include <algorithm>
 
@@ -57,7 +58,8 @@ __device__ __forceinline__ void radix4_write(double2 *buf, const double2 v[4], u
 	buf[o + 3 * ns] = make_double2(t1.x - t3.y, t1.y + t3.x); // t1 + i t3
 }
 
-// Final radix-2 butterfly j (< n/2): inputs buf[j], buf[j + n/2].
+// Final radix-2 butterfly j (< n/2): inputs buf[j], buf[j + n/2]; n here is
+// the length of the twiddle table (the FFT length times its stride).
 __device__ __forceinline__ void radix2_write(double2 *buf, double2 v0, double2 v1, uint32_t j, uint32_t ns, uint32_t n,
                                              const double2 *__restrict__ tw) {
 	const uint32_t k = j % ns;
@@ -70,8 +72,9 @@ __device__ __forceinline__ void radix2_write(double2 *buf, double2 v0, double2 v
 // In-place radix-4 (plus one final radix-2 pass when log2 n is odd) Stockham
 // FFT of buf[0..n), forward sign, natural-order output.  tw[j] = exp(-2 pi i j/n).
 // Each pass reads all inputs into registers, synchronises, then writes.
+// tws: stride of tw (an M-point FFT can use the N-point table with tws = N/M).
 template <uint32_t MAXN>
-__device__ void lds_fft(double2 *buf, uint32_t n, const double2 *__restrict__ tw) {
+__device__ void lds_fft(double2 *buf, uint32_t n, const double2 *__restrict__ tw, uint32_t tws = 1) {
 	constexpr int kMaxPerThread = MAXN / 4 / kThreads; // radix-4 butterflies per thread per pass
 	const uint32_t tid = threadIdx.x;
 	uint32_t ns = 1;
@@ -91,7 +94,7 @@ __device__ void lds_fft(double2 *buf, uint32_t n, const double2 *__restrict__ tw
 #pragma unroll
 		for (int t = 0; t < kMaxPerThread; ++t) {
 			const uint32_t j = tid + t * kThreads;
-			if (j < quarter) radix4_write(buf, v[t], j, ns, span, tw);
+			if (j < quarter) radix4_write(buf, v[t], j, ns, span * tws, tw);
 		}
 		__syncthreads();
 	}
@@ -107,7 +110,7 @@ __device__ void lds_fft(double2 *buf, uint32_t n, const double2 *__restrict__ tw
 #pragma unroll
 		for (int t = 0; t < 2 * kMaxPerThread; ++t) {
 			const uint32_t j = tid + t * kThreads;
-			if (j < half) radix2_write(buf, v[t][0], v[t][1], j, ns, n, tw);
+			if (j < half) radix2_write(buf, v[t][0], v[t][1], j, ns, n * tws, tw);
 		}
 		__syncthreads();
 	}
@@ -300,6 +303,67 @@ k_energy_small(const double2 *__restrict__ state, uint32_t copies, const double2
 		for (uint32_t q = 0; q < G; ++q) sum += red[q * N + k];
 		partial[size_t(blockIdx.x) * N + k] = sum;
 	}
+}
+
+// Two-level FFT of chains of N = R M sites (powers of two above 8192, see
+// nlc_fft_large.hpp) for a chunk of copies: column pass over z = q + i p (or
+// psi) into scratch[c][r][m], forward M-point FFT of every row in LDS, then the
+// mode energies from Z_k and Z_{N-k} (stored at (k mod R) M + k / R).
+template <uint32_t R>
+__global__ void __launch_bounds__(kThreads)
+k_large_col(const double2 *__restrict__ state, uint32_t n, uint32_t M, uint32_t copies, const double2 *__restrict__ tw,
+            double2 *__restrict__ scratch) {
+	const uint64_t t = uint64_t(blockIdx.x) * kThreads + threadIdx.x;
+	if (t >= uint64_t(copies) * M) return;
+	const uint32_t c = uint32_t(t / M), m = uint32_t(t % M);
+	const double2 *x = state + size_t(c) * n;
+	double2 root[R], in[R], out[R];
+	small_roots<R>(root, tw, n);
+#pragma unroll
+	for (uint32_t j = 0; j < R; ++j) in[j] = x[size_t(j) * M + m];
+	small_dft<R, false>(in, out, root);
+	double2 *y = scratch + size_t(c) * n + m;
+#pragma unroll
+	for (uint32_t r = 0; r < R; ++r) y[size_t(r) * M] = cmul(out[r], tw[m * r]);
+}
+
+template <uint32_t M>
+__global__ void __launch_bounds__(kThreads)
+k_large_row(double2 *__restrict__ scratch, const double2 *__restrict__ tw, uint32_t R) {
+	extern __shared__ double2 buf[];
+	double2 *row = scratch + size_t(blockIdx.x) * M;
+	for (uint32_t x = threadIdx.x; x < M; x += kThreads) buf[x] = row[x];
+	__syncthreads();
+	lds_fft<M>(buf, M, tw, R);
+	for (uint32_t x = threadIdx.x; x < M; x += kThreads) row[x] = buf[x];
+}
+
+// partial[g][k] += e_k summed over the chunk's chains c = g, g + rows, ...
+// (one thread per stored position p = r M + k2, k = r + R k2).
+template <bool DNLS_MODE>
+__global__ void __launch_bounds__(kThreads)
+k_large_modes(const double2 *__restrict__ scratch, uint32_t n, uint32_t R, uint32_t M, uint32_t copies,
+              const double *__restrict__ omega, double *__restrict__ partial, uint32_t rows) {
+	const uint32_t p = blockIdx.x * kThreads + threadIdx.x, g = blockIdx.y;
+	if (p >= n) return;
+	const uint32_t k = p / M + R * (p % M);
+	const uint32_t km = (n - k) & (n - 1);
+	const uint32_t pm = (km % R) * M + km / R;
+	double sum = 0.0;
+	for (uint32_t c = g; c < copies; c += rows) {
+		const double2 z = scratch[size_t(c) * n + p];
+		if constexpr (DNLS_MODE) {
+			sum += z.x * z.x + z.y * z.y;
+		} else {
+			const double2 zm = scratch[size_t(c) * n + pm];
+			const double w = omega[k];
+			const double a = 1.0 + w, b = 1.0 - w;
+			const double er = fma(a, z.x, -b * zm.x);
+			const double ei = fma(a, z.y, b * zm.y);
+			sum += 0.25 * fma(er, er, ei * ei);
+		}
+	}
+	partial[size_t(g) * n + k] += sum;
 }
 
 __global__ void k_zero(double *p, size_t count) {
@@ -499,9 +563,42 @@ hipError_t launch_small(hipStream_t st, const double2 *state, uint32_t copies, c
 	return hipGetLastError();
 }
 
+// Chunks of copies through the two-level FFT; scratch holds chunk * n values.
+template <bool DNLS_MODE, uint32_t R, uint32_t M>
+hipError_t launch_large(hipStream_t st, const double2 *state, uint32_t n, uint32_t copies, const double2 *tw,
+                        const double *omega, double *partial, uint32_t grid, double2 *scratch, uint32_t chunk) {
+	if (M * sizeof(double2) > 64 * 1024) {
+		hipError_t e = hipFuncSetAttribute((const void *) k_large_row<M>, hipFuncAttributeMaxDynamicSharedMemorySize,
+		                                   int(M * sizeof(double2)));
+		if (e != hipSuccess) return e;
+	}
+	hipLaunchKernelGGL(k_zero, dim3(1024), dim3(256), 0, st, partial, size_t(grid) * n);
+	for (uint32_t c0 = 0; c0 < copies; c0 += chunk) {
+		const uint32_t cnt = std::min(chunk, copies - c0);
+		const uint64_t cols = uint64_t(cnt) * M;
+		hipLaunchKernelGGL(k_large_col<R>, dim3(uint32_t((cols + kThreads - 1) / kThreads)), dim3(kThreads), 0, st,
+		                   state + size_t(c0) * n, n, M, cnt, tw, scratch);
+		hipLaunchKernelGGL(k_large_row<M>, dim3(cnt * R), dim3(kThreads), M * sizeof(double2), st, scratch, tw, R);
+		hipLaunchKernelGGL(k_large_modes<DNLS_MODE>, dim3((n + kThreads - 1) / kThreads, grid), dim3(kThreads), 0, st,
+		                   scratch, n, R, M, cnt, omega, partial, grid);
+	}
+	return hipGetLastError();
+}
+
 template <bool DNLS_MODE>
 hipError_t launch_energy_mode(hipStream_t st, const double2 *state, uint32_t n, uint32_t copies, const double2 *tw,
-                              const double *omega, double *partial, uint32_t grid) {
+                              const double *omega, double *partial, uint32_t grid, double2 *scratch,
+                              size_t scratch_sites) {
+	uint32_t R = 0, M = 0;
+	if (scratch && large_fft_split(n, R, M) && scratch_sites >= n) {
+		const uint32_t chunk = uint32_t(std::min<size_t>(scratch_sites / n, copies));
+		if (M == 8192) return launch_large<DNLS_MODE, 16, 8192>(st, state, n, copies, tw, omega, partial, grid, scratch, chunk);
+		switch (R) {
+		case 4: return launch_large<DNLS_MODE, 4, 4096>(st, state, n, copies, tw, omega, partial, grid, scratch, chunk);
+		case 8: return launch_large<DNLS_MODE, 8, 4096>(st, state, n, copies, tw, omega, partial, grid, scratch, chunk);
+		case 16: return launch_large<DNLS_MODE, 16, 4096>(st, state, n, copies, tw, omega, partial, grid, scratch, chunk);
+		}
+	}
 	switch (small_path(n) ? n : 0) {
 	case 8: return launch_small<DNLS_MODE, 8>(st, state, copies, tw, omega, partial, grid);
 	case 16: return launch_small<DNLS_MODE, 16>(st, state, copies, tw, omega, partial, grid);
@@ -533,20 +630,22 @@ uint32_t energy_rows(uint32_t n, uint32_t copies) {
 	return std::min<uint32_t>({copies, 1024u, std::max<uint32_t>(1u, (1u << 25) / n)});
 }
 
-static hipError_t launch_energy_common(hipStream_t st, bool dnls, const double2 *state, uint32_t n, uint32_t copies,
-                                       const double2 *tw, const double *omega, double *partial, uint32_t grid) {
-	return dnls ? launch_energy_mode<true>(st, state, n, copies, tw, omega, partial, grid)
-	            : launch_energy_mode<false>(st, state, n, copies, tw, omega, partial, grid);
+size_t energy_scratch_sites(uint32_t n, uint32_t copies) {
+	uint32_t R, M;
+	if (!large_fft_split(n, R, M)) return 0;
+	// chunks of copies of at most 2^25 sites (512 MiB), whole chains
+	return size_t(std::min<uint64_t>(copies, std::max<uint64_t>(1, (uint64_t(1) << 25) / n))) * n;
 }
 
 hipError_t launch_energy_real(hipStream_t st, const double2 *state, uint32_t n, uint32_t copies, const double2 *tw,
-                              const double *omega, double *partial, uint32_t grid) {
-	return launch_energy_common(st, false, state, n, copies, tw, omega, partial, grid);
+                              const double *omega, double *partial, uint32_t grid, double2 *scratch,
+                              size_t scratch_sites) {
+	return launch_energy_mode<false>(st, state, n, copies, tw, omega, partial, grid, scratch, scratch_sites);
 }
 
 hipError_t launch_energy_dnls(hipStream_t st, const double2 *state, uint32_t n, uint32_t copies, const double2 *tw,
-                              double *partial, uint32_t grid) {
-	return launch_energy_common(st, true, state, n, copies, tw, nullptr, partial, grid);
+                              double *partial, uint32_t grid, double2 *scratch, size_t scratch_sites) {
+	return launch_energy_mode<true>(st, state, n, copies, tw, nullptr, partial, grid, scratch, scratch_sites);
 }
 
 uint32_t reduce_scratch_rows(uint32_t grid) { return (grid + 63) / 64; }

@@ -5,6 +5,8 @@
 #include <cstddef>
 #include <cstdint>
 
+#include "nlc_fft_large.hpp"
+
 namespace nlc {
 
 enum Model { DNKG = 0, FPUT = 1, TODA = 2, DDNKG = 3, DNLS = 4 };
@@ -73,12 +75,17 @@ hipError_t launch_split_to_planar(hipStream_t, const double *q, const double *p,
 // Energy pass of the plane-wave models (DNKG/FPUT/Toda): per-block partial sums
 // of e_k over copies into partial[grid][n], grid = energy_rows(n, copies).
 // tw[j] = exp(-2 pi i j/n).
+// Chains of 8192 < n <= 131072 sites (powers of two) go through a two-level
+// FFT in chunks of copies; scratch holds energy_scratch_sites(n, copies)
+// values (0: not needed; without it such chains use the O(n^2) direct DFT).
 uint32_t energy_rows(uint32_t n, uint32_t copies);
+size_t energy_scratch_sites(uint32_t n, uint32_t copies);
 hipError_t launch_energy_real(hipStream_t, const double2 *state, uint32_t n, uint32_t copies, const double2 *tw,
-                              const double *omega, double *partial, uint32_t grid);
+                              const double *omega, double *partial, uint32_t grid, double2 *scratch,
+                              size_t scratch_sites);
 // DNLS energies: partial[g][k] = sum |psi_hat_k|^2 (omega applied in the reduction).
 hipError_t launch_energy_dnls(hipStream_t, const double2 *state, uint32_t n, uint32_t copies, const double2 *tw,
-                              double *partial, uint32_t grid);
+                              double *partial, uint32_t grid, double2 *scratch, size_t scratch_sites);
 // e_raw[k] = scale_k * sum_g partial[g][k] (scale = omega for DNLS, else NULL = 1),
 // fixed-order; scratch holds reduce_scratch_rows(grid) * n doubles.
 uint32_t reduce_scratch_rows(uint32_t grid);
@@ -130,11 +137,17 @@ namespace nlc {
 // multipliers.
 constexpr int kDnlsRadix16 = 0, kDnlsRadix8 = 1;
 constexpr uint32_t kDnlsLdsMax = 8192, kDnlsBluesteinMax = 8192;
+// Powers of two above 8192 up to 131072 (dnls_two_level) use the two-level
+// FFT in HBM with the multipliers permuted by dnls_two_level_tables and a
+// scratch of copies x n values.
 bool dnls_supported(uint32_t n);
 uint32_t dnls_bluestein_size(uint32_t n);
 void dnls_bluestein_tables(uint32_t n, double2 *out);
+bool dnls_two_level(uint32_t n);
+void dnls_two_level_tables(uint32_t n, const double2 *table, double2 *out);
 bool dnls_fast_path(uint32_t n);
 void dnls_permute_tables(uint32_t n, int variant, const double2 *table, double2 *out);
 hipError_t launch_dnls(hipStream_t, double2 *state, uint32_t n, uint32_t copies, uint32_t steps,
-                       const double *theta9, const double2 *tw, const double2 *table, int variant);
+                       const double *theta9, const double2 *tw, const double2 *table, int variant,
+                       double2 *scratch);
 } // namespace nlc

@@ -36,11 +36,13 @@ def run(N, n, copies, steps, dt, beta, s0, spl=0):
                                             (256, 5, 100), (1024, 3, 100), (2048, 3, 60), (8192, 3, 20),
                                             # Bluestein: convolution sizes 8, 16, 256, 2048, 8192, 8192
                                             (3, 5, 100), (5, 4, 100), (100, 5, 100), (1000, 3, 30),
-                                            (3000, 2, 5), (4095, 2, 3)])
+                                            (3000, 2, 5), (4095, 2, 3),
+                                            # two-level FFT in HBM: 4 x 4096, 8 x 4096, 16 x 4096, 16 x 8192
+                                            (16384, 3, 10), (32768, 2, 6), (65536, 2, 3), (131072, 1, 3)])
 def test_dnls_move_parity(gpu, n, copies, steps):
     N = gpu
     dt, beta = 0.01, 1.0
-    s0 = N.ic_dnls(n, copies, max(1, min(n // 16, n // 2)), 1.0, 3)
+    s0 = N.ic_dnls(n, copies, max(1, min(n // 16, 256)), 1.0, 3)
     banner(f"move n={n} copies={copies} steps={steps}")
     got, desc = run(N, n, copies, steps, dt, beta, s0)
     want = O.move_dnls(s0, steps, dt, beta)
@@ -94,10 +96,10 @@ def test_dnls_batches_compose(gpu, spl):
     assert rel_err(a, b) < 1e-13
 
 
-@pytest.mark.parametrize("n,copies", [(4096, 5), (256, 9), (64, 3), (100, 7), (8192, 2)])
+@pytest.mark.parametrize("n,copies", [(4096, 5), (256, 9), (64, 3), (100, 7), (8192, 2), (16384, 3), (131072, 2)])
 def test_dnls_linenergies(gpu, n, copies):
     N = gpu
-    s0 = N.ic_dnls(n, copies, max(1, n // 16), 1.0, 4)
+    s0 = N.ic_dnls(n, copies, max(1, min(n // 16, 256)), 1.0, 4)
     s0 = s0 + 0.01 * np.random.default_rng(3).normal(size=s0.shape)
     with N.Ensemble(N.DNLS, n, copies, 0.01, beta=1.0) as ens:
         ens.upload(s0)
@@ -147,17 +149,17 @@ def test_dnls_autotune_and_unsupported_shapes(gpu):
         N.Ensemble(N.DNLS, 4096, 4, 0.01, beta=1.0, sites_per_lane=4)
     with pytest.raises(N.NlcError):
         N.Ensemble(N.DNLS, 1024, 4, 0.01, beta=1.0, sites_per_lane=8)
-    for n in (1, 4097, 16384):  # beyond the LDS kernels
+    for n in (1, 4097, 12288, 262144):  # no kernel: not a power of two above 4096, or above 131072
         with pytest.raises(N.NlcError):
             N.Ensemble(N.DNLS, n, 4, 0.01, beta=1.0)
 
 
-@pytest.mark.parametrize("n", [100, 8192])
+@pytest.mark.parametrize("n", [100, 8192, 32768])
 def test_dnls_generic_kernels_describe_and_compose(gpu, n):
-    """The Bluestein and 8192-point LDS kernels: kernel names, and two batches
-    of 10 steps equal one of 20 to rounding."""
+    """The Bluestein, 8192-point LDS and two-level kernels: kernel names, and
+    two batches of 10 steps equal one of 20 to rounding."""
     N = gpu
-    s0 = N.ic_dnls(n, 3, max(1, n // 16), 1.0, 5)
+    s0 = N.ic_dnls(n, 3, max(1, min(n // 16, 256)), 1.0, 5)
     with N.Ensemble(N.DNLS, n, 3, 0.01, beta=1.0) as ens:
         desc = ens.describe()
         ens.upload(s0)
@@ -169,4 +171,5 @@ def test_dnls_generic_kernels_describe_and_compose(gpu, n):
         b = ens.download()
     banner(f"  {desc}")
     assert ("bluestein" in desc) == (n == 100) and ("stockham" in desc) == (n == 8192)
+    assert ("two_level" in desc) == (n == 32768)
     assert rel_err(a, b) < 1e-12

@@ -263,12 +263,20 @@ def test_ensemble_beyond_2_30_sites(gpu):
                                                  # short chains, 1024/n per workgroup: partial groups,
                                                  # several rounds per workgroup
                                                  ("DNKG", 8, 129), ("FPUT", 16, 3), ("Toda", 32, 1000),
-                                                 ("FPUT", 64, 70000)])
+                                                 ("FPUT", 64, 70000),
+                                                 # two-level FFT above 8192 sites (600 copies of 65536
+                                                 # sites: two chunks of the 2^25-site scratch)
+                                                 ("DNKG", 16384, 3), ("Toda", 32768, 2), ("FPUT", 131072, 2),
+                                                 ("DNKG", 65536, 600)])
 def test_linenergies_and_entropy(gpu, model_name, n, copies):
     N = gpu
     banner(f"linenergies {model_name} n={n} copies={copies}")
-    model, kw, masses, s0 = make_case(N, model_name, n, copies, 9)
-    s0 = s0 + 0.01 * np.random.default_rng(1).normal(size=s0.shape)  # populate every mode
+    if n > 8192:  # plane-wave starts cost O(sites x modes) to generate: a seeded random state instead
+        model, kw, masses = N.MODELS[model_name], dict(MODEL_KW[model_name]), None
+        s0 = 0.1 * np.random.default_rng(9).normal(size=(copies, n, 2))
+    else:
+        model, kw, masses, s0 = make_case(N, model_name, n, copies, 9)
+        s0 = s0 + 0.01 * np.random.default_rng(1).normal(size=s0.shape)  # populate every mode
     with N.Ensemble(model, n, copies, 0.05, masses=masses, **kw) as ens:
         ens.upload(s0)
         e_gpu, s_gpu = ens.linenergies()
