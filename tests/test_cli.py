@@ -209,3 +209,43 @@ def test_cli_dnls_run(gpu, tmp_path):
     assert np.abs(got - want).max() <= 1e-10 * np.abs(want).max()
     e = O.calc_linenergies(O.linenergies_dnls(want), 1.0 / (n * c), True)
     np.testing.assert_allclose(np.fromfile(f"{pre}-linenergies-100"), e, rtol=0, atol=1e-9 * e.max())
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prog,n,c,extra", [
+    ("DNKG", 4096, 3, ["-m", 1, "--beta", 1]),                    # workgroup-resident kernel
+    ("Toda", 3000, 2, ["--alpha", 1]),                            # workgroup kernel, ragged lanes
+    ("FPUT", 100, 5, ["--alpha", 0.25, "--beta", 1, "--split_kernel"]),  # reference's --split_kernel
+])
+def test_cli_long_chains_and_split_kernel(gpu, tmp_path, prog, n, c, extra):
+    kw = {"DNKG": dict(m=1.0, beta=1.0), "Toda": dict(alpha=1.0), "FPUT": dict(alpha=0.25, beta=1.0)}[prog]
+    s0 = 0.2 * np.random.default_rng(7).normal(size=(c, n, 2))
+    s0.tofile(tmp_path / "init")
+    pre = tmp_path / "r"
+    p = run(prog, "-n", n, "-c", c, "--dt", 0.05, "-b", 20, "-s", 40, "-p", pre, "-i", tmp_path / "init", "-v", *extra,
+            check=0)
+    assert ("kernel=split" in p.stderr) == ("--split_kernel" in extra), p.stderr
+    want = O.move_real(getattr(O, prog.upper()), s0, 40, 0.05, **kw)
+    got = np.fromfile(f"{pre}-40").reshape(c, n, 2)
+    assert np.abs(got - want).max() <= 1e-10 * np.abs(want).max()
+    m = kw.get("m", 0.0)
+    e = O.calc_linenergies(O.linenergies_real(want, O.dispersion_real(n, m)), 0.5 / c / n, m == 0.0)
+    np.testing.assert_allclose(np.fromfile(f"{pre}-linenergies-40"), e, rtol=0, atol=1e-9 * e.max())
+
+
+@pytest.mark.gpu
+def test_cli_dnls_two_level(gpu, tmp_path):
+    n, c = 16384, 2
+    # every mode populated: with a band-limited start some modes come out exactly zero
+    # (which ones depends on the FFT's operation order), log 0 makes the entropy
+    # non-finite and the run stops at t = 0, as the reference's check_entropy does
+    s0 = gpu.ic_dnls(n, c, 64, 1.0, 3) + 1e-3 * np.random.default_rng(2).normal(size=(c, n, 2))
+    s0.tofile(tmp_path / "init")
+    pre = tmp_path / "s"
+    p = run("DNLS", "-n", n, "-c", c, "--dt", 0.01, "-b", 5, "-s", 10, "-p", pre, "-i", tmp_path / "init", "--beta", 1,
+            "-v", check=0)
+    assert "dnls_two_level_fft" in p.stderr
+    assert read_entropy(pre).shape[0] == 3  # t = 0, 5, 10
+    want = O.move_dnls(s0, 10, 0.01, 1.0)
+    got = np.fromfile(f"{pre}-10").reshape(c, n, 2)
+    assert np.abs(got - want).max() <= 1e-10 * np.abs(want).max()
