@@ -86,7 +86,7 @@ typedef struct nlc_params {
 	int32_t model;            /* nlc_model */
 	int32_t precision;        /* nlc_precision */
 	int32_t kernel;           /* nlc_kernel */
-	uint32_t chain_length;    /* -n, >= 2; DNLS: at most 4096, or a power of two up to 131072
+	uint32_t chain_length;    /* -n, >= 2; DNLS: at most 65536, or a power of two up to 131072
 	                             (NLC_EUNSUPPORTED otherwise) */
 	uint32_t copies;          /* -c, >= 1; chain_length * copies <= 2^31 (32 GiB of state);
 	                             DNLS: copies < 2^24 */

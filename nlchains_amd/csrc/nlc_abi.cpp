@@ -62,8 +62,9 @@ struct nlc_ctx {
 	double *partial = nullptr; // energy partial sums
 	double *reduce_scratch = nullptr;
 	double *e_raw = nullptr;   // table_len
-	double2 *fft_scratch = nullptr; // two-level FFT of chains above 8192 sites
+	double2 *fft_scratch = nullptr; // two-level FFT of chains above 8192 sites, Bluestein above 4096
 	size_t fft_scratch_sites = 0;
+	double2 *blue_table = nullptr;  // Bluestein tables of a uniform length that is not a power of two
 	uint32_t *modes = nullptr; // entropy mode indices
 	uint32_t nmodes = 0;
 	int a0is0 = 0;
@@ -271,7 +272,7 @@ void free_all(nlc_ctx *c) {
 	     {(void *) c->state, (void *) c->omega, (void *) c->tw, (void *) c->partial, (void *) c->reduce_scratch,
 	      (void *) c->e_raw, (void *) c->modes, (void *) c->mp2, (void *) c->masses_d,
 	      (void *) c->eigvec, (void *) c->split_q[0], (void *) c->split_p[0], (void *) c->split_q[1], (void *) c->split_p[1], (void *) c->energy_d, (void *) c->norm_d,
-	      (void *) c->dnls_table, (void *) c->fft_scratch, (void *) c->items, (void *) c->vplanes, (void *) c->splanes})
+	      (void *) c->dnls_table, (void *) c->fft_scratch, (void *) c->blue_table, (void *) c->items, (void *) c->vplanes, (void *) c->splanes})
 		if (ptr) hipFree(ptr);
 	if (c->h_results) hipHostFree(c->h_results);
 	if (c->h_slots) hipHostFree(c->h_slots);
@@ -366,6 +367,12 @@ int setup_common(nlc_ctx *c) {
 	NLC_HIP(c, hipMemcpy(c->omega, omega.data(), sizeof(double) * c->table_len, hipMemcpyHostToDevice));
 	NLC_TRY(dev_alloc(c, &c->partial, partial_len, "energy partials"));
 	if (c->fft_scratch_sites) NLC_TRY(dev_alloc(c, &c->fft_scratch, c->fft_scratch_sites, "FFT scratch"));
+	if (const uint32_t L = nlc::bluestein_two_level_size(c->n); L && !c->ragged && p.model != NLC_DDNKG) {
+		std::vector<double2> blue(size_t(2) * L + c->n);
+		nlc::bluestein_two_level_tables(c->n, blue.data());
+		NLC_TRY(dev_alloc(c, &c->blue_table, blue.size(), "Bluestein tables"));
+		NLC_HIP(c, hipMemcpy(c->blue_table, blue.data(), sizeof(double2) * blue.size(), hipMemcpyHostToDevice));
+	}
 	NLC_TRY(dev_alloc(c, &c->reduce_scratch, scratch_len, "reduction scratch"));
 	return NLC_OK;
 }
@@ -377,8 +384,8 @@ int setup_dnls(nlc_ctx *c) {
 	if (c->copies >= (1u << 24))
 		return c->fail(NLC_EUNSUPPORTED, "DNLS runs one workgroup per chain: at most 2^24 - 1 copies");
 	if (!nlc::dnls_supported(n))
-		return c->fail(NLC_EUNSUPPORTED, "DNLS needs a chain_length of 2 to %u, or a power of two up to %u", 4096u,
-		               nlc::kLargeFftMax);
+		return c->fail(NLC_EUNSUPPORTED, "DNLS needs a chain_length of 2 to %u, or a power of two up to %u",
+		               nlc::kLargeFftMax / 2, nlc::kLargeFftMax);
 	const double bdt = p.beta * p.dt;
 	for (int k = 0; k < 8; ++k) c->theta[k] = (bdt * 1.) * nlc::kYoshidaC[k];
 	c->theta[8] = (bdt * 2.) * nlc::kYoshidaC[0];
@@ -397,6 +404,18 @@ int setup_dnls(nlc_ctx *c) {
 	if (const uint32_t M = nlc::dnls_bluestein_size(n)) {
 		tab.resize(size_t(7) * n + size_t(2) * M + n);
 		nlc::dnls_bluestein_tables(n, tab.data() + size_t(7) * n);
+	} else if (const uint32_t L = nlc::bluestein_two_level_size(n)) {
+		// Bluestein through the two-level FFT: the 7 natural tables (the
+		// Bluestein tables are shared with the energy pass) and L values of
+		// scratch per chain
+		const size_t need = size_t(L) * c->copies;
+		if (c->fft_scratch_sites < need) {
+			if (c->fft_scratch) hipFree(c->fft_scratch);
+			c->fft_scratch = nullptr;
+			c->fft_scratch_sites = 0;
+			NLC_TRY(dev_alloc(c, &c->fft_scratch, need, "DNLS Bluestein buffer"));
+			c->fft_scratch_sites = need;
+		}
 	} else if (nlc::dnls_two_level(n)) {
 		// the two-level FFT keeps the spectrum in HBM in [r][k] order: the
 		// multipliers in that order, and a scratch of the state's size
@@ -500,7 +519,8 @@ int enqueue_move(nlc_ctx *c, uint32_t steps) {
 	if (c->p.model == NLC_DNLS) {
 		const bool r8 = nlc::dnls_fast_path(c->n) && c->dnls_variant == nlc::kDnlsRadix8;
 		NLC_HIP(c, nlc::launch_dnls(c->stream, c->state, c->n, c->copies, steps, c->theta, c->tw,
-		                            c->dnls_table + (r8 ? size_t(4) * c->n : 0), c->dnls_variant, c->fft_scratch));
+		                            c->dnls_table + (r8 ? size_t(4) * c->n : 0), c->dnls_variant, c->fft_scratch,
+		                            c->blue_table));
 	} else if (c->ragged) {
 		if (c->ragged_launcher) {
 			NLC_HIP(c, c->ragged_launcher(c->stream, c->state, c->items, c->nitems, steps, c->args, 0));
@@ -518,6 +538,14 @@ int enqueue_move(nlc_ctx *c, uint32_t steps) {
 		NLC_HIP(c, c->launcher(c->stream, c->state, c->copies, c->n, steps, c->args, c->mp2));
 	}
 	return NLC_OK;
+}
+
+nlc::LongChainFft long_chain_fft(const nlc_ctx *c) {
+	nlc::LongChainFft lf;
+	lf.scratch = c->fft_scratch;
+	lf.scratch_sites = c->fft_scratch_sites;
+	lf.blue = c->blue_table;
+	return lf;
 }
 
 // results::calc_linenergies normalisation of the drivers for a class
@@ -543,7 +571,7 @@ int enqueue_energy(nlc_ctx *c, double norm = 0.0, double *dst = nullptr) {
 		case NLC_FPUT:
 		case NLC_TODA:
 			NLC_HIP(c, nlc::launch_energy_real(c->stream, st, k.n, k.count, c->tw + k.tab0, c->omega + k.tab0,
-			                                   c->partial, k.rows, c->fft_scratch, c->fft_scratch_sites));
+			                                   c->partial, k.rows, long_chain_fft(c)));
 			break;
 		case NLC_DDNKG:
 			NLC_HIP(c, nlc::launch_energy_ddnkg(c->stream, st, k.n, k.count, c->eigvec, c->omega, c->partial,
@@ -551,7 +579,7 @@ int enqueue_energy(nlc_ctx *c, double norm = 0.0, double *dst = nullptr) {
 			break;
 		case NLC_DNLS:
 			NLC_HIP(c, nlc::launch_energy_dnls(c->stream, st, k.n, k.count, c->tw + k.tab0, c->partial, k.rows,
-			                                   c->fft_scratch, c->fft_scratch_sites));
+			                                   long_chain_fft(c)));
 			scale = c->omega + k.tab0;
 			break;
 		default: return c->fail(NLC_EUNSUPPORTED, "no energy pass for this model");
@@ -1147,7 +1175,8 @@ int nlc_describe(const nlc_ctx *c, char *buf, size_t len) {
 	static const char *names[] = {"DNKG", "FPUT", "Toda", "dDNKG", "DNLS"};
 	if (c->p.model == NLC_DNLS)
 		snprintf(buf, len, "model=DNLS n=%u copies=%u kernel=%s", c->n, c->copies,
-		         nlc::dnls_two_level(c->n)                 ? "dnls_two_level_fft"
+		         nlc::bluestein_two_level_size(c->n)       ? "dnls_bluestein_two_level"
+		         : nlc::dnls_two_level(c->n)               ? "dnls_two_level_fft"
 		         : nlc::dnls_bluestein_size(c->n)          ? "dnls_lds_bluestein"
 		         : !nlc::dnls_fast_path(c->n)              ? "dnls_lds_stockham"
 		         : c->dnls_variant == nlc::kDnlsRadix8 ? "dnls_radix8_registers"

@@ -587,8 +587,12 @@ hipError_t launch_large(hipStream_t st, const double2 *state, uint32_t n, uint32
 
 template <bool DNLS_MODE>
 hipError_t launch_energy_mode(hipStream_t st, const double2 *state, uint32_t n, uint32_t copies, const double2 *tw,
-                              const double *omega, double *partial, uint32_t grid, double2 *scratch,
-                              size_t scratch_sites) {
+                              const double *omega, double *partial, uint32_t grid, const LongChainFft &lf) {
+	double2 *scratch = lf.scratch;
+	const size_t scratch_sites = lf.scratch_sites;
+	if (lf.blue && scratch && bluestein_two_level_size(n))
+		return launch_energy_bluestein(st, DNLS_MODE, state, n, copies, lf.blue, omega, partial, grid, scratch,
+		                               scratch_sites);
 	uint32_t R = 0, M = 0;
 	if (scratch && large_fft_split(n, R, M) && scratch_sites >= n) {
 		const uint32_t chunk = uint32_t(std::min<size_t>(scratch_sites / n, copies));
@@ -631,21 +635,22 @@ uint32_t energy_rows(uint32_t n, uint32_t copies) {
 }
 
 size_t energy_scratch_sites(uint32_t n, uint32_t copies) {
-	uint32_t R, M;
-	if (!large_fft_split(n, R, M)) return 0;
-	// chunks of copies of at most 2^25 sites (512 MiB), whole chains
-	return size_t(std::min<uint64_t>(copies, std::max<uint64_t>(1, (uint64_t(1) << 25) / n))) * n;
+	uint32_t R, M, len = 0;
+	if (large_fft_split(n, R, M)) len = n;
+	else len = bluestein_two_level_size(n);
+	if (!len) return 0;
+	// chunks of copies of at most 2^25 values (512 MiB), whole chains
+	return size_t(std::min<uint64_t>(copies, std::max<uint64_t>(1, (uint64_t(1) << 25) / len))) * len;
 }
 
 hipError_t launch_energy_real(hipStream_t st, const double2 *state, uint32_t n, uint32_t copies, const double2 *tw,
-                              const double *omega, double *partial, uint32_t grid, double2 *scratch,
-                              size_t scratch_sites) {
-	return launch_energy_mode<false>(st, state, n, copies, tw, omega, partial, grid, scratch, scratch_sites);
+                              const double *omega, double *partial, uint32_t grid, const LongChainFft &lf) {
+	return launch_energy_mode<false>(st, state, n, copies, tw, omega, partial, grid, lf);
 }
 
 hipError_t launch_energy_dnls(hipStream_t st, const double2 *state, uint32_t n, uint32_t copies, const double2 *tw,
-                              double *partial, uint32_t grid, double2 *scratch, size_t scratch_sites) {
-	return launch_energy_mode<true>(st, state, n, copies, tw, nullptr, partial, grid, scratch, scratch_sites);
+                              double *partial, uint32_t grid, const LongChainFft &lf) {
+	return launch_energy_mode<true>(st, state, n, copies, tw, nullptr, partial, grid, lf);
 }
 
 uint32_t reduce_scratch_rows(uint32_t grid) { return (grid + 63) / 64; }

@@ -78,14 +78,20 @@ hipError_t launch_split_to_planar(hipStream_t, const double *q, const double *p,
 // Chains of 8192 < n <= 131072 sites (powers of two) go through a two-level
 // FFT in chunks of copies; scratch holds energy_scratch_sites(n, copies)
 // values (0: not needed; without it such chains use the O(n^2) direct DFT).
+// Lengths that are not powers of two, 4096 < n <= 65536, use Bluestein's
+// convolution through the same FFT when the Bluestein tables are given.
+struct LongChainFft {
+	double2 *scratch = nullptr;    // energy_scratch_sites(n, copies) values
+	size_t scratch_sites = 0;
+	const double2 *blue = nullptr; // bluestein_two_level_tables(n), or null
+};
 uint32_t energy_rows(uint32_t n, uint32_t copies);
 size_t energy_scratch_sites(uint32_t n, uint32_t copies);
 hipError_t launch_energy_real(hipStream_t, const double2 *state, uint32_t n, uint32_t copies, const double2 *tw,
-                              const double *omega, double *partial, uint32_t grid, double2 *scratch,
-                              size_t scratch_sites);
+                              const double *omega, double *partial, uint32_t grid, const LongChainFft &lf);
 // DNLS energies: partial[g][k] = sum |psi_hat_k|^2 (omega applied in the reduction).
 hipError_t launch_energy_dnls(hipStream_t, const double2 *state, uint32_t n, uint32_t copies, const double2 *tw,
-                              double *partial, uint32_t grid, double2 *scratch, size_t scratch_sites);
+                              double *partial, uint32_t grid, const LongChainFft &lf);
 // e_raw[k] = scale_k * sum_g partial[g][k] (scale = omega for DNLS, else NULL = 1),
 // fixed-order; scratch holds reduce_scratch_rows(grid) * n doubles.
 uint32_t reduce_scratch_rows(uint32_t grid);
@@ -149,5 +155,15 @@ bool dnls_fast_path(uint32_t n);
 void dnls_permute_tables(uint32_t n, int variant, const double2 *table, double2 *out);
 hipError_t launch_dnls(hipStream_t, double2 *state, uint32_t n, uint32_t copies, uint32_t steps,
                        const double *theta9, const double2 *tw, const double2 *table, int variant,
-                       double2 *scratch);
+                       double2 *scratch, const double2 *blue);
+// Lengths that are not powers of two, 4096 < n <= 65536: Bluestein's
+// convolution of length L = bluestein_two_level_size(n) (a power of two
+// >= 2n - 1) through the two-level FFT, with the tables of
+// bluestein_two_level_tables (2L + n values) and a scratch of L values per
+// chain (DNLS step: every copy; energy pass: chunks of copies).
+uint32_t bluestein_two_level_size(uint32_t n);
+void bluestein_two_level_tables(uint32_t n, double2 *out);
+hipError_t launch_energy_bluestein(hipStream_t, bool dnls, const double2 *state, uint32_t n, uint32_t copies,
+                                   const double2 *blue, const double *omega, double *partial, uint32_t grid,
+                                   double2 *scratch, size_t scratch_sites);
 } // namespace nlc

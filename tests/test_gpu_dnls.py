@@ -38,7 +38,9 @@ def run(N, n, copies, steps, dt, beta, s0, spl=0):
                                             (3, 5, 100), (5, 4, 100), (100, 5, 100), (1000, 3, 30),
                                             (3000, 2, 5), (4095, 2, 3),
                                             # two-level FFT in HBM: 4 x 4096, 8 x 4096, 16 x 4096, 16 x 8192
-                                            (16384, 3, 10), (32768, 2, 6), (65536, 2, 3), (131072, 1, 3)])
+                                            (16384, 3, 10), (32768, 2, 6), (65536, 2, 3), (131072, 1, 3),
+                                            # Bluestein through the two-level FFT (L = 16384, 16384, 65536)
+                                            (5000, 2, 5), (6007, 2, 3), (20001, 1, 1)])
 def test_dnls_move_parity(gpu, n, copies, steps):
     N = gpu
     dt, beta = 0.01, 1.0
@@ -96,7 +98,8 @@ def test_dnls_batches_compose(gpu, spl):
     assert rel_err(a, b) < 1e-13
 
 
-@pytest.mark.parametrize("n,copies", [(4096, 5), (256, 9), (64, 3), (100, 7), (8192, 2), (16384, 3), (131072, 2)])
+@pytest.mark.parametrize("n,copies", [(4096, 5), (256, 9), (64, 3), (100, 7), (8192, 2), (16384, 3), (131072, 2),
+                                      (6007, 3), (33001, 1)])
 def test_dnls_linenergies(gpu, n, copies):
     N = gpu
     s0 = N.ic_dnls(n, copies, max(1, min(n // 16, 256)), 1.0, 4)
@@ -149,12 +152,39 @@ def test_dnls_autotune_and_unsupported_shapes(gpu):
         N.Ensemble(N.DNLS, 4096, 4, 0.01, beta=1.0, sites_per_lane=4)
     with pytest.raises(N.NlcError):
         N.Ensemble(N.DNLS, 1024, 4, 0.01, beta=1.0, sites_per_lane=8)
-    for n in (1, 4097, 12288, 262144):  # no kernel: not a power of two above 4096, or above 131072
+    for n in (1, 65537, 262144):  # no kernel: not a power of two above 65536, or above 131072
         with pytest.raises(N.NlcError):
             N.Ensemble(N.DNLS, n, 4, 0.01, beta=1.0)
 
 
-@pytest.mark.parametrize("n", [100, 8192, 32768])
+@pytest.mark.parametrize("n", [33001, 50000])
+def test_dnls_bluestein_two_level_linear_closed_form(gpu, n):
+    """beta = 0 (L = 131072 = 16 x 8192): every Fourier mode only turns its
+    phase, psi_hat_k(t) = psi_hat_k(0) e^{-i omega_k t} (numpy's FFT as the
+    reference); beta = 1: the norm of every chain is conserved."""
+    N = gpu
+    s0 = N.ic_dnls(n, 2, 128, 1.0, 6) + 1e-3 * np.random.default_rng(5).normal(size=(2, n, 2))
+    steps, dt = 4, 0.01
+    with N.Ensemble(N.DNLS, n, 2, dt, beta=0.0) as ens:
+        banner(f"  {ens.describe()}")
+        assert "bluestein_two_level" in ens.describe()
+        ens.upload(s0)
+        ens.move(steps)
+        got = ens.download()
+    psi0 = s0[..., 0] + 1j * s0[..., 1]
+    om = O.dispersion_dnls(n)
+    want = np.fft.ifft(np.fft.fft(psi0, axis=1) * np.exp(-1j * om * dt * steps), axis=1)
+    g = got[..., 0] + 1j * got[..., 1]
+    assert np.abs(g - want).max() <= 1e-11 * np.abs(want).max()
+    with N.Ensemble(N.DNLS, n, 2, dt, beta=1.0) as ens:
+        ens.upload(s0)
+        _, n0 = ens.chain_energies()
+        ens.move(steps)
+        _, n1 = ens.chain_energies()
+    np.testing.assert_allclose(n1, n0, rtol=1e-12)
+
+
+@pytest.mark.parametrize("n", [100, 8192, 32768, 5000])
 def test_dnls_generic_kernels_describe_and_compose(gpu, n):
     """The Bluestein, 8192-point LDS and two-level kernels: kernel names, and
     two batches of 10 steps equal one of 20 to rounding."""
@@ -170,6 +200,6 @@ def test_dnls_generic_kernels_describe_and_compose(gpu, n):
         ens.move(20)
         b = ens.download()
     banner(f"  {desc}")
-    assert ("bluestein" in desc) == (n == 100) and ("stockham" in desc) == (n == 8192)
-    assert ("two_level" in desc) == (n == 32768)
+    assert ("lds_bluestein" in desc) == (n == 100) and ("stockham" in desc) == (n == 8192)
+    assert ("two_level_fft" in desc) == (n == 32768) and ("bluestein_two_level" in desc) == (n == 5000)
     assert rel_err(a, b) < 1e-12

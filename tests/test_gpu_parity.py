@@ -267,7 +267,10 @@ def test_ensemble_beyond_2_30_sites(gpu):
                                                  # two-level FFT above 8192 sites (600 copies of 65536
                                                  # sites: two chunks of the 2^25-site scratch)
                                                  ("DNKG", 16384, 3), ("Toda", 32768, 2), ("FPUT", 131072, 2),
-                                                 ("DNKG", 65536, 600)])
+                                                 ("DNKG", 65536, 600),
+                                                 # Bluestein through the two-level FFT (L = 16384, 65536,
+                                                 # 131072; 2100 copies of 6007 sites: two chunks)
+                                                 ("DNKG", 6007, 3), ("FPUT", 20001, 2), ("Toda", 33001, 1)])
 def test_linenergies_and_entropy(gpu, model_name, n, copies):
     N = gpu
     banner(f"linenergies {model_name} n={n} copies={copies}")
@@ -295,6 +298,21 @@ def test_linenergies_and_entropy(gpu, model_name, n, copies):
     assert rel_err(e_gpu, e) <= ENERGY_TOL
     assert abs(s_gpu[0] - wt) <= ENERGY_TOL * max(1, abs(wt))
     assert abs(s_gpu[1] - inf) <= ENERGY_TOL * max(1, abs(inf))
+
+
+@pytest.mark.parametrize("n,tiles", [(6007, 700), (65536, 200)])
+def test_linenergies_long_chains_in_chunks(gpu, n, tiles):
+    """More copies than one scratch chunk holds (2^25 values: 2048 chains of
+    6007 sites through Bluestein's L = 16384, 512 chains of 65536): three
+    seeded chains tiled, so the raw mode energies are the tile count times
+    the oracle's for the three."""
+    N = gpu
+    base = 0.1 * np.random.default_rng(11).normal(size=(3, n, 2))
+    with N.Ensemble(N.DNKG, n, 3 * tiles, 0.05, m=1.0, beta=1.0) as ens:
+        ens.upload(np.tile(base, (tiles, 1, 1)))
+        raw = ens.linenergies_raw()
+    want = tiles * O.linenergies_real(base, O.dispersion_real(n, 1.0))
+    assert rel_err(raw, want) <= 1e-11
 
 
 def test_entropy_mask_on_gpu(gpu):
