@@ -1,0 +1,43 @@
+"""bench.py's output contract: one JSON line with the fields the harness reads
+(metric, value, unit, steps, roofline, cpu_baseline, ...), and the refusal of
+runs on more than one GPU."""
+import json
+import subprocess
+import sys
+from pathlib import Path
+
+import pytest
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+def run_bench(*args, timeout=600):
+    return subprocess.run([sys.executable, str(ROOT / "bench.py"), *args], capture_output=True, text=True,
+                          timeout=timeout, cwd=ROOT)
+
+
+def test_bench_refuses_more_than_one_gpu():
+    p = run_bench("--gpus", "2")
+    assert p.returncode != 0
+    msg = json.loads(p.stdout.strip().splitlines()[-1])
+    assert "one GPU" in msg["error"]
+
+
+@pytest.mark.gpu
+def test_bench_json_contract(gpu):
+    p = run_bench("--steps", "3", "--warmup", "1", "--cpu-seconds", "1")
+    assert p.returncode == 0, p.stderr[-2000:]
+    line = p.stdout.strip().splitlines()[-1]
+    d = json.loads(line)
+    for key in ("metric", "value", "unit", "n_gpus", "steps", "warmup", "ms_per_step", "higher_is_better", "scaling",
+                "vs_baseline", "dtype", "data", "config", "roofline", "cpu_baseline"):
+        assert key in d, key
+    assert d["n_gpus"] == 1 and d["steps"] == 3 and d["warmup"] == 1 and d["higher_is_better"] is True
+    assert d["unit"] == "site-steps/s" and d["value"] > 1e11 and d["dtype"] == "f64"
+    assert "workload" in d["config"] and "model" not in d["config"]
+    r = d["roofline"]
+    assert r["bound"] in ("hbm", "mfma") and r["unit"] in ("GB/s", "TFLOP/s")
+    assert 0 < r["frac"] <= 1 and abs(r["frac"] - r["achieved"] / r["peak"]) < 1e-9
+    assert r["traffic"] is None or r["traffic"] > 0
+    c = d["cpu_baseline"]
+    assert c["kind"] in ("reference", "port") and c["value"] > 0 and 1 <= c["cores"] <= 16 and c["sample"]
