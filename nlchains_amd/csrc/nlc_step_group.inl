@@ -14,12 +14,15 @@
 //    whole batch, or S - 1 and a dummy last slot when n < 64 W S (the lanes
 //    with S sites come first);
 //  * halos: DPP wave_ror / wave_rol inside a wavefront; at the wavefront ends
-//    lane 63 and lane 0 exchange their boundary q through a double-buffered
-//    LDS ring (one workgroup barrier per kick; the buffer alternates, so a
-//    buffer is rewritten only after every wave has passed the barrier that
-//    follows its last read);
+//    lanes 63 and 0 exchange through a double-buffered LDS ring on every
+//    other kick (one workgroup barrier per two kicks; the buffer alternates,
+//    so a buffer is rewritten only after every wave has passed the barrier
+//    that follows its last read).  An exchange carries the edge q, the q next
+//    to it and the edge p; the receiving lane kicks and drifts the
+//    neighbour's edge site itself with the same expressions, which gives the
+//    halo of the following kick bit for bit;
 //  * FPUT / Toda evaluate the lane's right boundary bond themselves (one bond
-//    term more per lane and kick) instead of a second exchange and barrier;
+//    term more per lane and kick) instead of exchanging bond terms;
 //  * HBM is touched once per batch: 16 B/site in, 16 B/site out.
 #include <algorithm>
 
@@ -34,30 +37,62 @@ namespace {
 
 constexpr int kGroupMaxWaves = 8;
 
+// Force on a site of DNKG / dDNKG from its neighbours l, r (the expression of
+// the kick loop, shared so that a predicted halo is bit-identical).
+__device__ __forceinline__ double site_force(double l, double q, double r, double mass, const StepArgs &a) {
+	const double t = fma(a.beta, q * q, mass);
+	return fma(-q, t, l + r);
+}
+
 template <int MODEL, int S, bool EXACT>
 struct GroupChain {
 	double q[S], p[S];
 	double mp[MODEL == DDNKG ? S : 1];
+	double m_l, m_r; // dDNKG: m_x + 2 of the neighbouring sites across the lane edges
 	bool full;
 	uint32_t lane, wave, waves;
 	const double *etab;
-	double (*ring)[2][kGroupMaxWaves]; // [buffer][0: left halos, 1: right halos][wave]
+	// [buffer][0: sent to the next wave, 1: sent to the previous wave][wave][3]
+	double (*ring)[2][kGroupMaxWaves][3];
 	uint32_t buf;
+	bool exchange;        // this kick exchanges through LDS; the next one uses the predictions
+	double pred_l, pred_r; // lane 0 / lane 63: the neighbour's edge q after this kick and drift
+
+	__device__ __forceinline__ double last_q() const { return (EXACT || full) ? q[S - 1] : q[S - 2]; }
+	__device__ __forceinline__ double before_last_q() const { return (EXACT || full) ? q[S - 2] : q[S - 3]; }
+	__device__ __forceinline__ double last_p() const { return (EXACT || full) ? p[S - 1] : p[S - 2]; }
 
 	// Left and right halo of the lane: the last real q of lane g-1 and the first
-	// q of lane g+1 (ring over the 64 W lanes of the chain).
-	__device__ __forceinline__ void halos(double &hl, double &hr) {
-		const double send_l = (EXACT || full) ? q[S - 1] : q[S > 1 ? S - 2 : 0];
-		hl = dpp_f64<kWaveRor1>(send_l);
+	// q of lane g+1 (ring over the 64 W lanes of the chain).  Inside a
+	// wavefront by DPP; across wavefronts through LDS on exchange kicks (edge
+	// q, the q next to it and the edge p, from which the receiving lane
+	// predicts the neighbour's edge q for the following kick), else from the
+	// prediction.
+	__device__ __forceinline__ void halos(double &hl, double &hr, double (&xl)[3], double (&xr)[3]) {
+		hl = dpp_f64<kWaveRor1>(last_q());
 		hr = dpp_f64<kWaveRol1>(q[0]);
-		if (lane == 63) ring[buf][0][wave] = send_l;
-		if (lane == 0) ring[buf][1][wave] = q[0];
-		__syncthreads();
-		const double xl = ring[buf][0][wave == 0 ? waves - 1 : wave - 1];
-		const double xr = ring[buf][1][wave + 1 == waves ? 0 : wave + 1];
-		if (lane == 0) hl = xl;
-		if (lane == 63) hr = xr;
-		buf ^= 1;
+		if (exchange) {
+			if (lane == 63) {
+				ring[buf][0][wave][0] = before_last_q();
+				ring[buf][0][wave][1] = last_q();
+				ring[buf][0][wave][2] = last_p();
+			}
+			if (lane == 0) {
+				ring[buf][1][wave][0] = q[0];
+				ring[buf][1][wave][1] = q[1];
+				ring[buf][1][wave][2] = p[0];
+			}
+			__syncthreads();
+			const uint32_t wl = wave == 0 ? waves - 1 : wave - 1, wr = wave + 1 == waves ? 0 : wave + 1;
+#pragma unroll
+			for (int i = 0; i < 3; ++i) xl[i] = ring[buf][0][wl][i], xr[i] = ring[buf][1][wr][i];
+			if (lane == 0) hl = xl[1];
+			if (lane == 63) hr = xr[0];
+			buf ^= 1;
+		} else {
+			if (lane == 0) hl = pred_l;
+			if (lane == 63) hr = pred_r;
+		}
 	}
 
 	__device__ __forceinline__ void drift(double c) {
@@ -65,26 +100,38 @@ struct GroupChain {
 		for (int j = 0; j < S; ++j) q[j] = fma(c, p[j], q[j]);
 	}
 
-	__device__ __forceinline__ void kick(double d, const StepArgs &a) {
-		double hl, hr;
-		halos(hl, hr);
+	// Kick with weight d; c_next is the drift that follows (used by the
+	// predictions of an exchange kick).
+	__device__ __forceinline__ void kick(double d, double c_next, const StepArgs &a) {
+		double hl, hr, xl[3], xr[3];
+		halos(hl, hr, xl, xr);
 		if constexpr (MODEL == DNKG || MODEL == DDNKG) {
+			if (exchange) {
+				// the neighbours' edge sites, kicked and drifted as their own lanes do
+				const double ml = (MODEL == DDNKG) ? m_l : a.m2, mr = (MODEL == DDNKG) ? m_r : a.m2;
+				pred_l = fma(c_next, fma(d, site_force(xl[0], xl[1], q[0], ml, a), xl[2]), xl[1]);
+				pred_r = fma(c_next, fma(d, site_force(last_q(), xr[0], xr[1], mr, a), xr[2]), xr[0]);
+			}
 			if (!EXACT && !full) q[S - 1] = hr;
 #pragma unroll
 			for (int j = 0; j < S; ++j) {
 				const double lft = (j == 0) ? hl : q[j - 1];
 				const double rgt = (j == S - 1) ? hr : q[j + 1];
 				const double mass = (MODEL == DDNKG) ? mp[j] : a.m2;
-				const double t = fma(a.beta, q[j] * q[j], mass);
-				const double f = fma(-q[j], t, lft + rgt);
-				p[j] = fma(d, f, p[j]);
+				p[j] = fma(d, site_force(lft, q[j], rgt, mass, a), p[j]);
 			}
 			if (!EXACT && !full) p[S - 1] = 0.0;
 		} else {
 			const double w = (MODEL == TODA) ? d * a.a2i : d;
-			const double last = (EXACT || full) ? q[S - 1] : q[S > 1 ? S - 2 : 0];
+			const double last = last_q();
 			const double bs = bond_force<MODEL, S>(hr - last, a, etab); // right bond of the last real site
 			double bl = bond_force<MODEL, S>(q[0] - hl, a, etab);
+			if (exchange) {
+				const double bll = bond_force<MODEL, S>(xl[1] - xl[0], a, etab); // left bond of the left neighbour
+				const double brr = bond_force<MODEL, S>(xr[1] - xr[0], a, etab); // right bond of the right neighbour
+				pred_l = fma(c_next, fma(w, bl - bll, xl[2]), xl[1]);
+				pred_r = fma(c_next, fma(w, brr - bs, xr[2]), xr[0]);
+			}
 #pragma unroll
 			for (int j = 0; j < S; ++j) {
 				double br;
@@ -96,6 +143,7 @@ struct GroupChain {
 			}
 			if (!EXACT && !full) p[S - 1] = 0.0;
 		}
+		exchange = !exchange;
 	}
 };
 
@@ -106,7 +154,7 @@ template <int MODEL, int S, bool EXACT>
 __global__ void __launch_bounds__(64 * kGroupMaxWaves)
 k_step_group(double2 *__restrict__ state, uint32_t n, uint32_t steps, StepArgs a, const double *__restrict__ mp2g) {
 	__shared__ double etab[MODEL == TODA ? kExpTabSize : 1];
-	__shared__ double ring[2][2][kGroupMaxWaves];
+	__shared__ double ring[2][2][kGroupMaxWaves][3];
 	load_exp_table<MODEL>(etab);
 	const uint32_t g = threadIdx.x, G = blockDim.x;
 
@@ -117,6 +165,8 @@ k_step_group(double2 *__restrict__ state, uint32_t n, uint32_t steps, StepArgs a
 	ch.etab = etab;
 	ch.ring = ring;
 	ch.buf = 0;
+	ch.exchange = true;
+	ch.pred_l = ch.pred_r = 0.0;
 	const uint32_t full_lanes = EXACT ? G : n - uint32_t(S - 1) * G;
 	ch.full = EXACT || g < full_lanes;
 	const uint32_t off = ch.full ? g * S : full_lanes * S + (g - full_lanes) * (S - 1);
@@ -131,6 +181,13 @@ k_step_group(double2 *__restrict__ state, uint32_t n, uint32_t steps, StepArgs a
 		ch.p[j] = v.y;
 		if constexpr (MODEL == DDNKG) ch.mp[j] = real_site ? mp2g[off + j] : 0.0;
 	}
+	if constexpr (MODEL == DDNKG) {
+		ch.m_l = mp2g[off == 0 ? n - 1 : off - 1];
+		const uint32_t end = off + (ch.full ? S : S - 1);
+		ch.m_r = mp2g[end == n ? 0 : end];
+	} else {
+		ch.m_l = ch.m_r = 0.0;
+	}
 
 	const double cs = (MODEL == TODA) ? a.a2 : 1.0;
 	const double dc0 = cs * a.dc[0];
@@ -140,14 +197,16 @@ k_step_group(double2 *__restrict__ state, uint32_t n, uint32_t steps, StepArgs a
 		if constexpr (MODEL == TODA) {
 #pragma unroll 1
 			for (int k = 0; k < 7; ++k) {
-				ch.kick(a.dd[k], a);
-				ch.drift(k < 6 ? cs * a.dc[k + 1] : c_last);
+				const double c = k < 6 ? cs * a.dc[k + 1] : c_last;
+				ch.kick(a.dd[k], c, a);
+				ch.drift(c);
 			}
 		} else {
 #pragma unroll
 			for (int k = 0; k < 7; ++k) {
-				ch.kick(a.dd[k], a);
-				ch.drift(k < 6 ? a.dc[k + 1] : c_last);
+				const double c = k < 6 ? a.dc[k + 1] : c_last;
+				ch.kick(a.dd[k], c, a);
+				ch.drift(c);
 			}
 		}
 	}
