@@ -23,18 +23,12 @@
 //    offset, and neither exchange has bank conflicts.
 //  * k_dnls_4096_r8: the same transform as four radix-8 passes on 512 threads
 //    x 8 values (more wavefronts per SIMD, one more LDS exchange).
-//  * k_dnls_lds: any power-of-two n <= 8192, chain in LDS, radix-4 Stockham
-//    passes (the inverse transform as conj(FFT(conj(.)))).
-//  * k_dnls_bluestein: any other n <= 4096.  Each n-point DFT is Bluestein's
-//    chirp-z convolution (L. Bluestein, 1970): with w_j = e^{-i pi j^2/n},
-//    X_k = w_k sum_x (w_x psi_x) conj(w_{k-x}), a cyclic convolution of length
-//    M >= 2n - 1 (a power of two) done as two M-point FFTs in LDS.
-//  * the nonlinear phase e^{-i theta}: Taylor polynomials of sin and cos on
-//    |r| <= pi/16 (degree 11 and 12), after a Cody & Waite reduction by pi/8
-//    for larger angles.
+//  * other lengths: nlc_dnls_generic.hip (LDS kernels up to 8192 sites) and
+//    nlc_dnls_long.hip (two-level FFT in HBM beyond); launch_dnls dispatches.
 #include "nlc_internal.hpp"
 #include "nlc_fft_large.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <utility>
 #include <vector>
@@ -42,121 +36,9 @@
 namespace nlc {
 namespace {
 
-constexpr int kT = 256; // threads per chain
-
-__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
-	return make_double2(fma(a.x, b.x, -a.y * b.y), fma(a.x, b.y, a.y * b.x));
-}
-__device__ __forceinline__ double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
-__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
-__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
-
-// (cos r, sin r) for |r| <= pi/16: Taylor series about 0 in x = r^2 (degree
-// 12 and 11); the first omitted terms are below 1e-18 relative.
-__device__ __forceinline__ double2 expi_small(double r) {
-	const double x = r * r;
-	double s = -2.505210838544172e-08;         // -1/11!
-	s = fma(s, x, 2.7557319223985893e-06);     // 1/9!
-	s = fma(s, x, -0.0001984126984126984);     // -1/7!
-	s = fma(s, x, 0.008333333333333333);       // 1/5!
-	s = fma(s, x, -0.16666666666666666);       // -1/3!
-	double c = 2.08767569878681e-09;           // 1/12!
-	c = fma(c, x, -2.755731922398589e-07);     // -1/10!
-	c = fma(c, x, 2.48015873015873e-05);       // 1/8!
-	c = fma(c, x, -0.001388888888888889);      // -1/6!
-	c = fma(c, x, 0.041666666666666664);       // 1/4!
-	c = fma(c, x, -0.5);                       // -1/2!
-	return make_double2(fma(c, x, 1.0), fma(r * x, s, r));
-}
-
-// e^{i k pi/8}, k = 0..15
-__constant__ double2 kOctant[16] = {
-    {1.0, 0.0},
-    {0.9238795325112867, 0.3826834323650898},
-    {0.7071067811865476, 0.7071067811865476},
-    {0.3826834323650898, 0.9238795325112867},
-    {0.0, 1.0},
-    {-0.3826834323650898, 0.9238795325112867},
-    {-0.7071067811865476, 0.7071067811865476},
-    {-0.9238795325112867, 0.3826834323650898},
-    {-1.0, 0.0},
-    {-0.9238795325112867, -0.3826834323650898},
-    {-0.7071067811865476, -0.7071067811865476},
-    {-0.3826834323650898, -0.9238795325112867},
-    {0.0, -1.0},
-    {0.3826834323650898, -0.9238795325112867},
-    {0.7071067811865476, -0.7071067811865476},
-    {0.9238795325112867, -0.3826834323650898},
-};
-
-// (cos a, sin a).  |a| <= pi/16 directly (the nonlinear angles of a DNLS run
-// are far smaller, so the other branch is normally skipped by the whole
-// wavefront); otherwise Cody & Waite reduction a = k pi/8 + r, |r| <= pi/16,
-// with pi/8 split in three parts (33, 33 and 53 significant bits, so k*Q1 and
-// k*Q2 are exact for |k| < 2^20), and e^{ia} = e^{i k pi/8} e^{ir}.
-constexpr double kExpiSmall = 0.19634954084936207; // pi/16
-
-// The reduced form, valid for every angle; for |a| <= pi/16 it gives exactly
-// expi_small(a) (k = 0, r = a, and the octant factor is 1 + 0i).
-__device__ __forceinline__ double2 expi_reduced(double a) {
-	constexpr double kEightOverPi = 2.5464790894703255, kQ1 = 0x1.921fb54400000p-2, kQ2 = 0x1.0b4611a600000p-36,
-	                 kQ3 = 0x1.3198a2e037073p-71;
-	const double k = rint(a * kEightOverPi);
-	double r = fma(-k, kQ1, a);
-	r = fma(-k, kQ2, r);
-	r = fma(-k, kQ3, r);
-	return cmul(expi_small(r), kOctant[int(k) & 15]);
-}
-
-__device__ __forceinline__ double2 expi(double a) {
-	return fabs(a) <= kExpiSmall ? expi_small(a) : expi_reduced(a);
-}
-
-__device__ __forceinline__ double2 rotate(double2 p, double2 e) {
-	return make_double2(fma(p.x, e.x, -p.y * e.y), fma(p.x, e.y, p.y * e.x));
-}
-
-// psi * e^{-i theta |psi|^2} (DNLS.cu:11-17)
-__device__ __forceinline__ double2 nonlinear(double2 p, double theta) {
-	return rotate(p, expi(-theta * fma(p.x, p.x, p.y * p.y)));
-}
-
-// The nonlinear step on all N values of a thread.  The branch between the
-// short and the reduced form is taken once for the whole wavefront (the
-// reduced form only when some angle of some lane exceeds pi/16), so each path
-// is straight-line code over the N values: their polynomial chains interleave
-// instead of running one after another behind a per-value branch.
-template <int N>
-__device__ __forceinline__ void nonlinear_all(double2 (&v)[N], double theta) {
-	double a[N];
-	bool small = true;
-#pragma unroll
-	for (int j = 0; j < N; ++j) {
-		a[j] = -theta * fma(v[j].x, v[j].x, v[j].y * v[j].y);
-		small = small && fabs(a[j]) <= kExpiSmall;
-	}
-	if (__all(small)) {
-#pragma unroll
-		for (int j = 0; j < N; ++j) v[j] = rotate(v[j], expi_small(a[j]));
-	} else {
-#pragma unroll
-		for (int j = 0; j < N; ++j) v[j] = rotate(v[j], expi_reduced(a[j]));
-	}
-}
+#include "nlc_dnls_common.inl"
 
 // ---------------------------------------------------------------- radix 16
-
-// 4-point DFT in place, SIGN = -1 forward, +1 inverse.
-template <int SIGN>
-__device__ __forceinline__ void dft4(double2 &a0, double2 &a1, double2 &a2, double2 &a3) {
-	const double2 t0 = cadd(a0, a2), t1 = csub(a0, a2), t2 = cadd(a1, a3), t3 = csub(a1, a3);
-	a0 = cadd(t0, t2);
-	a2 = csub(t0, t2);
-	// (SIGN i) t3
-	const double2 it3 = SIGN < 0 ? make_double2(t3.y, -t3.x) : make_double2(-t3.y, t3.x);
-	a1 = cadd(t1, it3);
-	a3 = csub(t1, it3);
-}
 
 // exp(SIGN 2 pi i e / 16) for e in {1, 2, 3, 6, 9}
 template <int SIGN, int E>
@@ -354,10 +236,6 @@ __device__ __forceinline__ void fft4096_inv(double2 v[16], Lds4352 &L, const dou
 	dft16_n2t<+1>(v);
 }
 
-// theta[k] = beta dt c_k (k < 7), theta[7] = beta dt c_7, theta[8] = beta dt 2 c_0.
-struct DnlsArgs {
-	double theta[9];
-};
 
 __global__ void __launch_bounds__(kT, 2)
 k_dnls_4096(double2 *__restrict__ state, uint32_t copies, uint32_t steps, DnlsArgs a, const double2 *__restrict__ tw,
@@ -607,482 +485,13 @@ k_dnls_4096_r8(double2 *__restrict__ state, uint32_t copies, uint32_t steps, Dnl
 	for (int j = 0; j < 8; ++j) psi[t + 512 * j] = nonlinear(v[pos8(j)], a.theta[7]);
 }
 
-// ------------------------------------------------------------ generic n
-
-// In-place radix-4 (+ final radix-2) Stockham FFT of n <= MAXN points, forward
-// sign, natural order; tw[j tws] = e^{-2 pi i j/n} (tws > 1: a longer table).
-template <uint32_t MAXN>
-__device__ void lds_fft_fwd(double2 *buf, uint32_t n, const double2 *__restrict__ tw, uint32_t tws = 1) {
-	constexpr int kMaxPer = MAXN / 4 / kT; // radix-4 butterflies per thread per pass
-	const uint32_t tid = threadIdx.x, quarter = n >> 2;
-	uint32_t ns = 1;
-	for (; ns * 4 <= n; ns *= 4) {
-		double2 v[kMaxPer][4];
-		const uint32_t span = n / (ns * 4) * tws;
-#pragma unroll
-		for (int q = 0; q < kMaxPer; ++q) {
-			const uint32_t j = tid + q * kT;
-			if (j < quarter)
-#pragma unroll
-				for (int r = 0; r < 4; ++r) v[q][r] = buf[j + r * quarter];
-		}
-		__syncthreads();
-#pragma unroll
-		for (int q = 0; q < kMaxPer; ++q) {
-			const uint32_t j = tid + q * kT;
-			if (j < quarter) {
-				const uint32_t k = j % ns;
-				double2 a0 = v[q][0], a1 = cmul(v[q][1], tw[k * span]), a2 = cmul(v[q][2], tw[2 * k * span]),
-				        a3 = cmul(v[q][3], tw[3 * k * span]);
-				dft4<-1>(a0, a1, a2, a3);
-				const uint32_t o = (j / ns) * ns * 4 + k;
-				buf[o] = a0, buf[o + ns] = a1, buf[o + 2 * ns] = a2, buf[o + 3 * ns] = a3;
-			}
-		}
-		__syncthreads();
-	}
-	if (ns < n) {
-		const uint32_t half = n >> 1;
-		double2 v[2 * kMaxPer][2];
-#pragma unroll
-		for (int q = 0; q < 2 * kMaxPer; ++q) {
-			const uint32_t j = tid + q * kT;
-			if (j < half) v[q][0] = buf[j], v[q][1] = buf[j + half];
-		}
-		__syncthreads();
-#pragma unroll
-		for (int q = 0; q < 2 * kMaxPer; ++q) {
-			const uint32_t j = tid + q * kT;
-			if (j < half) {
-				const uint32_t k = j % ns;
-				const double2 b = cmul(v[q][1], tw[k * (n / (2 * ns)) * tws]);
-				const uint32_t o = (j / ns) * ns * 2 + k;
-				buf[o] = cadd(v[q][0], b);
-				buf[o + ns] = csub(v[q][0], b);
-			}
-		}
-		__syncthreads();
-	}
-}
-
-// Chain in LDS.  Between substeps the buffer holds conj(psi): the inverse
-// transform is conj(FFT(conj(X T))).
-template <uint32_t MAXN>
-__global__ void __launch_bounds__(kT)
-k_dnls_lds(double2 *__restrict__ state, uint32_t n, uint32_t copies, uint32_t steps, DnlsArgs a,
-           const double2 *__restrict__ tw, const double2 *__restrict__ table) {
-	extern __shared__ double2 buf[];
-	const uint32_t chain = blockIdx.x, tid = threadIdx.x;
-	double2 *psi = state + size_t(chain) * n;
-	for (uint32_t x = tid; x < n; x += kT) buf[x] = psi[x];
-	bool conj_state = false;
-	__syncthreads();
-	for (uint32_t s = 0; s < steps; ++s) {
-		for (int k = 0; k < 7; ++k) {
-			const double th = (k == 0 && s) ? a.theta[8] : a.theta[k];
-			for (uint32_t x = tid; x < n; x += kT) {
-				const double2 p = conj_state ? cconj(buf[x]) : buf[x];
-				buf[x] = nonlinear(p, th);
-			}
-			__syncthreads();
-			lds_fft_fwd<MAXN>(buf, n, tw);
-			const double2 *tk = table + size_t(k) * n;
-			for (uint32_t j = tid; j < n; j += kT) buf[j] = cconj(cmul(buf[j], tk[j]));
-			__syncthreads();
-			lds_fft_fwd<MAXN>(buf, n, tw);
-			conj_state = true;
-		}
-	}
-	for (uint32_t x = tid; x < n; x += kT) {
-		const double2 p = conj_state ? cconj(buf[x]) : buf[x];
-		psi[x] = nonlinear(p, a.theta[7]);
-	}
-}
-
-// Cyclic convolution of buf[0..M) with b_j = conj(w_j) (|j| < n, zero
-// elsewhere), returned conjugated: buf <- conj(IFFT(FFT(buf) bhat)) with
-// bhat = FFT(b)/M, as FFT(conj(FFT(buf) bhat)).
-template <uint32_t MAXN>
-__device__ void bluestein_conv(double2 *buf, uint32_t M, const double2 *__restrict__ twm,
-                               const double2 *__restrict__ bhat) {
-	lds_fft_fwd<MAXN>(buf, M, twm);
-	for (uint32_t j = threadIdx.x; j < M; j += kT) buf[j] = cconj(cmul(buf[j], bhat[j]));
-	__syncthreads();
-	lds_fft_fwd<MAXN>(buf, M, twm);
-}
-
-// Chain in LDS as u_x = w_x psi_x (x < n), M entries.  One substep:
-//   nonlinear: |u_x| = |psi_x|, so u_x <- u_x e^{-i theta |u_x|^2};
-//   forward DFT: conv(u) holds conj(C_k), X_k = w_k C_k;
-//   backward DFT of Y = X T as conj(DFT(conj Y)): its Bluestein input
-//   conj(Y_k) w_k = conj(C_k) conj(T_k), and the second conv(.) leaves
-//   conj(C'_x) = w_x psi'_x, the same representation: the chirps cancel
-//   between substeps and are applied only at load and store.
-// table: 7 multipliers of n entries, then bhat (M), the M-point twiddles (M),
-// the chirp w (n).
-template <uint32_t MAXN>
-__global__ void __launch_bounds__(kT)
-k_dnls_bluestein(double2 *__restrict__ state, uint32_t n, uint32_t M, uint32_t copies, uint32_t steps, DnlsArgs a,
-                 const double2 *__restrict__ table) {
-	extern __shared__ double2 buf[];
-	const double2 *bhat = table + size_t(7) * n, *twm = bhat + M, *chirp = twm + M;
-	const uint32_t chain = blockIdx.x, tid = threadIdx.x;
-	const double2 zero = make_double2(0.0, 0.0);
-	double2 *psi = state + size_t(chain) * n;
-	for (uint32_t x = tid; x < n; x += kT) buf[x] = cmul(psi[x], chirp[x]);
-	for (uint32_t s = 0; s < steps; ++s) {
-		for (int k = 0; k < 7; ++k) {
-			const double th = (k == 0 && s) ? a.theta[8] : a.theta[k];
-			for (uint32_t x = tid; x < M; x += kT) buf[x] = x < n ? nonlinear(buf[x], th) : zero;
-			__syncthreads();
-			bluestein_conv<MAXN>(buf, M, twm, bhat);
-			const double2 *tk = table + size_t(k) * n;
-			for (uint32_t j = tid; j < M; j += kT) buf[j] = j < n ? cmul(buf[j], cconj(tk[j])) : zero;
-			__syncthreads();
-			bluestein_conv<MAXN>(buf, M, twm, bhat);
-		}
-	}
-	for (uint32_t x = tid; x < n; x += kT) psi[x] = nonlinear(cmul(buf[x], cconj(chirp[x])), a.theta[7]);
-}
-
-// Chains above 8192 sites (powers of two up to 131072, n = R M): the two-level
-// FFT of nlc_fft_large.hpp with the chain and its spectrum in HBM (scratch:
-// copies x n values, the spectrum in the order [r][k]).  A substep is two
-// launches: the row kernel (M-point FFT in LDS, Fourier multiplier, inverse
-// FFT, conjugate twiddle) and the column kernel, which finishes the inverse
-// transform of one substep (R-point DFT), applies the nonlinear phase of the
-// next and starts its forward transform (R-point DFT and twiddle).
-// MODE 0: the first substep, reads the state; 1: between substeps; 2: after
-// the last one, applies theta_7 and writes the state.
-template <uint32_t R, int MODE>
-__global__ void __launch_bounds__(kT)
-k_dnls_col(double2 *__restrict__ state, double2 *__restrict__ scratch, uint32_t n, uint32_t M, uint32_t copies,
-           double th, const double2 *__restrict__ tw) {
-	const uint64_t t = uint64_t(blockIdx.x) * kT + threadIdx.x;
-	if (t >= uint64_t(copies) * M) return;
-	const uint32_t c = uint32_t(t / M), m = uint32_t(t % M);
-	double2 root[R], x[R], y[R];
-	small_roots<R>(root, tw, n);
-	double2 *psi = state + size_t(c) * n + m, *sc = scratch + size_t(c) * n + m;
-	if constexpr (MODE == 0) {
-#pragma unroll
-		for (uint32_t j = 0; j < R; ++j) x[j] = psi[size_t(j) * M];
-	} else {
-#pragma unroll
-		for (uint32_t r = 0; r < R; ++r) y[r] = sc[size_t(r) * M];
-		small_dft<R, true>(y, x, root);
-	}
-#pragma unroll
-	for (uint32_t j = 0; j < R; ++j) x[j] = nonlinear(x[j], th);
-	if constexpr (MODE == 2) {
-#pragma unroll
-		for (uint32_t j = 0; j < R; ++j) psi[size_t(j) * M] = x[j];
-	} else {
-		small_dft<R, false>(x, y, root);
-#pragma unroll
-		for (uint32_t r = 0; r < R; ++r) sc[size_t(r) * M] = cmul(y[r], tw[m * r]);
-	}
-}
-
-// One row (chain c, block r) per workgroup: Z = FFT_M(y_r), U = Z T_k (the
-// multipliers in the same [r][k] order), v_r[m] = IFFT_M(U)[m] W_N^{-m r}
-// (unnormalised; T carries the 1/n), the inverse as conj(FFT(conj(.))).
-template <uint32_t M>
-__global__ void __launch_bounds__(kT)
-k_dnls_row(double2 *__restrict__ scratch, uint32_t R, const double2 *__restrict__ tw, const double2 *__restrict__ tk) {
-	extern __shared__ double2 buf[];
-	const uint32_t r = blockIdx.x % R;
-	double2 *row = scratch + size_t(blockIdx.x) * M;
-	for (uint32_t x = threadIdx.x; x < M; x += kT) buf[x] = row[x];
-	__syncthreads();
-	lds_fft_fwd<M>(buf, M, tw, R);
-	const double2 *t = tk + size_t(r) * M;
-	for (uint32_t k = threadIdx.x; k < M; k += kT) buf[k] = cconj(cmul(buf[k], t[k]));
-	__syncthreads();
-	lds_fft_fwd<M>(buf, M, tw, R);
-	for (uint32_t m = threadIdx.x; m < M; m += kT) row[m] = cconj(cmul(buf[m], tw[m * r]));
-}
-
-template <uint32_t R, uint32_t M>
-hipError_t launch_dnls_two_level(hipStream_t st, double2 *state, double2 *scratch, uint32_t n, uint32_t copies,
-                                 uint32_t steps, const DnlsArgs &a, const double2 *tw, const double2 *table) {
-	if (M * sizeof(double2) > 64 * 1024) {
-		hipError_t e = hipFuncSetAttribute((const void *) k_dnls_row<M>, hipFuncAttributeMaxDynamicSharedMemorySize,
-		                                   int(M * sizeof(double2)));
-		if (e != hipSuccess) return e;
-	}
-	const dim3 cols(uint32_t((uint64_t(copies) * M + kT - 1) / kT)), rows(copies * R), block(kT);
-	const size_t lds = M * sizeof(double2);
-	for (uint32_t s = 0; s < steps; ++s)
-		for (int k = 0; k < 7; ++k) {
-			const double th = (k == 0 && s) ? a.theta[8] : a.theta[k];
-			if (s == 0 && k == 0)
-				hipLaunchKernelGGL((k_dnls_col<R, 0>), cols, block, 0, st, state, scratch, n, M, copies, th, tw);
-			else
-				hipLaunchKernelGGL((k_dnls_col<R, 1>), cols, block, 0, st, state, scratch, n, M, copies, th, tw);
-			hipLaunchKernelGGL(k_dnls_row<M>, rows, block, lds, st, scratch, R, tw, table + size_t(k) * n);
-		}
-	hipLaunchKernelGGL((k_dnls_col<R, 2>), cols, block, 0, st, state, scratch, n, M, copies, a.theta[7], tw);
-	return hipGetLastError();
-}
-
-// Lengths n that are not powers of two, 4096 < n <= 65536: Bluestein's
-// convolution (as in k_dnls_bluestein) of length L = R M >= 2n - 1 through the
-// two-level FFT, one L-entry buffer per chain (natural order between the
-// passes, [r][k] order inside a transform; the column passes work in place on
-// the R entries j M + m of their offset m).  conv(a) = IFFT_L(FFT_L(a) bhat)
-// is k_blue_fwd, k_dnls_row with bhat as the multiplier, k_blue_inv.
-// SRC of k_blue_fwd: 0 w psi from the state with the nonlinear phase, 1 the
-// buffer with the nonlinear phase, 2 the buffer, 3 w z from the state (energy
-// pass).  Entries at or beyond n are zero.
-template <uint32_t R, int SRC>
-__global__ void __launch_bounds__(kT)
-k_blue_fwd(const double2 *__restrict__ state, double2 *__restrict__ buf, uint32_t n, uint32_t L, uint32_t M,
-           uint32_t copies, double th, const double2 *__restrict__ twL, const double2 *__restrict__ chirp) {
-	const uint64_t t = uint64_t(blockIdx.x) * kT + threadIdx.x;
-	if (t >= uint64_t(copies) * M) return;
-	const uint32_t c = uint32_t(t / M), m = uint32_t(t % M);
-	double2 root[R], a[R], y[R];
-	small_roots<R>(root, twL, L);
-	double2 *b = buf + size_t(c) * L + m;
-#pragma unroll
-	for (uint32_t j = 0; j < R; ++j) {
-		const uint32_t x = j * M + m;
-		double2 v = make_double2(0.0, 0.0);
-		if (x < n) {
-			if constexpr (SRC == 0 || SRC == 3) v = cmul(state[size_t(c) * n + x], chirp[x]);
-			else v = b[size_t(j) * M];
-			if constexpr (SRC == 0 || SRC == 1) v = nonlinear(v, th);
-		}
-		a[j] = v;
-	}
-	small_dft<R, false>(a, y, root);
-#pragma unroll
-	for (uint32_t r = 0; r < R; ++r) b[size_t(r) * M] = cmul(y[r], twL[m * r]);
-}
-
-// DST of k_blue_inv, from C = conv(.) at x = j M + m: 0 C (energy pass); 1
-// conj(C T_k) for x < n, else 0; 2 conj(C) for x < n, else 0; 3 the state:
-// psi = conj(w C) with the nonlinear phase (after the last substep).
-template <uint32_t R, int DST>
-__global__ void __launch_bounds__(kT)
-k_blue_inv(double2 *__restrict__ buf, double2 *__restrict__ state, uint32_t n, uint32_t L, uint32_t M,
-           uint32_t copies, double th, const double2 *__restrict__ twL, const double2 *__restrict__ chirp,
-           const double2 *__restrict__ tk) {
-	const uint64_t t = uint64_t(blockIdx.x) * kT + threadIdx.x;
-	if (t >= uint64_t(copies) * M) return;
-	const uint32_t c = uint32_t(t / M), m = uint32_t(t % M);
-	double2 root[R], v[R], x[R];
-	small_roots<R>(root, twL, L);
-	double2 *b = buf + size_t(c) * L + m;
-#pragma unroll
-	for (uint32_t r = 0; r < R; ++r) v[r] = b[size_t(r) * M];
-	small_dft<R, true>(v, x, root);
-#pragma unroll
-	for (uint32_t j = 0; j < R; ++j) {
-		const uint32_t at = j * M + m;
-		const double2 zero = make_double2(0.0, 0.0);
-		if constexpr (DST == 0) b[size_t(j) * M] = x[j];
-		else if constexpr (DST == 1) b[size_t(j) * M] = at < n ? cconj(cmul(x[j], tk[at])) : zero;
-		else if constexpr (DST == 2) b[size_t(j) * M] = at < n ? cconj(x[j]) : zero;
-		else if (at < n) state[size_t(c) * n + at] = nonlinear(cconj(cmul(x[j], chirp[at])), th);
-	}
-}
-
-__global__ void k_zero_rows(double *p, size_t count) {
-	for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < count; i += size_t(gridDim.x) * blockDim.x)
-		p[i] = 0.0;
-}
-
-// Energy pass from C = conv(w z): X_k = w_k C_k; partial[g][k] += e_k over the
-// chunk's chains c = g, g + rows, ...
-template <bool DNLS_MODE>
-__global__ void __launch_bounds__(kT)
-k_blue_modes(const double2 *__restrict__ buf, uint32_t n, uint32_t L, uint32_t copies, const double2 *__restrict__ chirp,
-             const double *__restrict__ omega, double *__restrict__ partial, uint32_t rows) {
-	const uint32_t k = blockIdx.x * kT + threadIdx.x, g = blockIdx.y;
-	if (k >= n) return;
-	const uint32_t km = k ? n - k : 0;
-	const double2 wk = chirp[k], wm = chirp[km];
-	double sum = 0.0;
-	for (uint32_t c = g; c < copies; c += rows) {
-		const double2 z = cmul(buf[size_t(c) * L + k], wk);
-		if constexpr (DNLS_MODE) {
-			sum += z.x * z.x + z.y * z.y;
-		} else {
-			const double2 zm = cmul(buf[size_t(c) * L + km], wm);
-			const double w = omega[k];
-			const double a = 1.0 + w, b = 1.0 - w;
-			const double er = fma(a, z.x, -b * zm.x);
-			const double ei = fma(a, z.y, b * zm.y);
-			sum += 0.25 * fma(er, er, ei * ei);
-		}
-	}
-	partial[size_t(g) * n + k] += sum;
-}
-
-template <uint32_t M>
-hipError_t row_attribute() {
-	if (M * sizeof(double2) > 64 * 1024)
-		return hipFuncSetAttribute((const void *) k_dnls_row<M>, hipFuncAttributeMaxDynamicSharedMemorySize,
-		                           int(M * sizeof(double2)));
-	return hipSuccess;
-}
-
-// blue: bhat in [r][k] order (L), the L-point roots (L), the chirp (n).
-template <uint32_t R, uint32_t M>
-hipError_t launch_dnls_blue(hipStream_t st, double2 *state, double2 *buf, uint32_t n, uint32_t copies, uint32_t steps,
-                            const DnlsArgs &a, const double2 *table, const double2 *blue) {
-	if (hipError_t e = row_attribute<M>()) return e;
-	constexpr uint32_t L = R * M;
-	const double2 *bhat = blue, *twL = blue + L, *chirp = blue + 2 * L;
-	const dim3 cols(uint32_t((uint64_t(copies) * M + kT - 1) / kT)), rows(copies * R), block(kT);
-	const size_t lds = M * sizeof(double2);
-	for (uint32_t s = 0; s < steps; ++s)
-		for (int k = 0; k < 7; ++k) {
-			const double th = (k == 0 && s) ? a.theta[8] : a.theta[k];
-			if (s == 0 && k == 0)
-				hipLaunchKernelGGL((k_blue_fwd<R, 0>), cols, block, 0, st, state, buf, n, L, M, copies, th, twL, chirp);
-			else
-				hipLaunchKernelGGL((k_blue_fwd<R, 1>), cols, block, 0, st, state, buf, n, L, M, copies, th, twL, chirp);
-			hipLaunchKernelGGL(k_dnls_row<M>, rows, block, lds, st, buf, R, twL, bhat);
-			hipLaunchKernelGGL((k_blue_inv<R, 1>), cols, block, 0, st, buf, state, n, L, M, copies, 0.0, twL, chirp,
-			                   table + size_t(k) * n);
-			hipLaunchKernelGGL((k_blue_fwd<R, 2>), cols, block, 0, st, state, buf, n, L, M, copies, 0.0, twL, chirp);
-			hipLaunchKernelGGL(k_dnls_row<M>, rows, block, lds, st, buf, R, twL, bhat);
-			if (s + 1 == steps && k == 6)
-				hipLaunchKernelGGL((k_blue_inv<R, 3>), cols, block, 0, st, buf, state, n, L, M, copies, a.theta[7], twL,
-				                   chirp, nullptr);
-			else
-				hipLaunchKernelGGL((k_blue_inv<R, 2>), cols, block, 0, st, buf, state, n, L, M, copies, 0.0, twL, chirp,
-				                   nullptr);
-		}
-	return hipGetLastError();
-}
-
-template <bool DNLS_MODE, uint32_t R, uint32_t M>
-hipError_t launch_energy_blue_rm(hipStream_t st, const double2 *state, uint32_t n, uint32_t copies, const double2 *blue,
-                                 const double *omega, double *partial, uint32_t grid, double2 *scratch,
-                                 uint32_t chunk) {
-	if (hipError_t e = row_attribute<M>()) return e;
-	constexpr uint32_t L = R * M;
-	const double2 *bhat = blue, *twL = blue + L, *chirp = blue + 2 * L;
-	for (uint32_t c0 = 0; c0 < copies; c0 += chunk) {
-		const uint32_t cnt = std::min(chunk, copies - c0);
-		const dim3 cols(uint32_t((uint64_t(cnt) * M + kT - 1) / kT)), block(kT);
-		hipLaunchKernelGGL((k_blue_fwd<R, 3>), cols, block, 0, st, state + size_t(c0) * n, scratch, n, L, M, cnt, 0.0,
-		                   twL, chirp);
-		hipLaunchKernelGGL(k_dnls_row<M>, dim3(cnt * R), block, M * sizeof(double2), st, scratch, R, twL, bhat);
-		hipLaunchKernelGGL((k_blue_inv<R, 0>), cols, block, 0, st, scratch, nullptr, n, L, M, cnt, 0.0, twL, chirp,
-		                   nullptr);
-		hipLaunchKernelGGL(k_blue_modes<DNLS_MODE>, dim3((n + kT - 1) / kT, grid), block, 0, st, scratch, n, L, cnt,
-		                   chirp, omega, partial, grid);
-	}
-	return hipGetLastError();
-}
-
-template <typename K>
-hipError_t launch_lds_chain_kernel(K kernel, size_t lds) {
-	if (lds > 64 * 1024)
-		return hipFuncSetAttribute((const void *) kernel, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
-	return hipSuccess;
-}
-
-bool pow2(uint32_t n) { return n >= 2 && (n & (n - 1)) == 0; }
-
-// In-place radix-2 FFT on the host (forward sign, exact twiddles), for bhat.
-void host_fft(std::vector<double2> &v) {
-	const size_t n = v.size();
-	for (size_t i = 1, j = 0; i < n; ++i) {
-		size_t bit = n >> 1;
-		for (; j & bit; bit >>= 1) j ^= bit;
-		j ^= bit;
-		if (i < j) std::swap(v[i], v[j]);
-	}
-	for (size_t len = 2; len <= n; len <<= 1)
-		for (size_t j = 0; j < len / 2; ++j) {
-			const double ang = -2.0 * M_PI * double(j) / double(len);
-			const double wr = std::cos(ang), wi = std::sin(ang);
-			for (size_t b = 0; b < n; b += len) {
-				double2 &u = v[b + j], &t = v[b + j + len / 2];
-				const double tr = t.x * wr - t.y * wi, ti = t.x * wi + t.y * wr;
-				t = make_double2(u.x - tr, u.y - ti);
-				u = make_double2(u.x + tr, u.y + ti);
-			}
-		}
-}
-
 } // namespace
-
-uint32_t dnls_bluestein_size(uint32_t n) {
-	if (n < 2 || pow2(n)) return 0;
-	uint32_t M = 1;
-	while (M < 2 * n - 1) M <<= 1;
-	return M <= kDnlsBluesteinMax ? M : 0;
-}
 
 bool dnls_supported(uint32_t n) {
 	uint32_t R, M;
-	return (pow2(n) && n <= kDnlsLdsMax) || dnls_bluestein_size(n) != 0 || large_fft_split(n, R, M) ||
+	const bool pow2 = n >= 2 && (n & (n - 1)) == 0;
+	return (pow2 && n <= kDnlsLdsMax) || dnls_bluestein_size(n) != 0 || large_fft_split(n, R, M) ||
 	       bluestein_two_level_size(n) != 0;
-}
-
-bool dnls_two_level(uint32_t n) {
-	uint32_t R, M;
-	return large_fft_split(n, R, M);
-}
-
-// Multipliers of the two-level path in the spectrum's [r][k] order:
-// out[k][r M + q] = table[k][r + R q].
-void dnls_two_level_tables(uint32_t n, const double2 *table, double2 *out) {
-	uint32_t R, M;
-	if (!large_fft_split(n, R, M)) return;
-	for (int k = 0; k < 7; ++k)
-		for (uint32_t r = 0; r < R; ++r)
-			for (uint32_t q = 0; q < M; ++q)
-				out[size_t(k) * n + size_t(r) * M + q] = table[size_t(k) * n + r + size_t(R) * q];
-}
-
-namespace {
-// bhat = FFT_M(b)/M of b_j = conj(w_j) (|j| < n, placed cyclically), the M-point
-// roots e^{-2 pi i j/M} and the chirp w_j = e^{-i pi j^2/n}: out[0, M), out[M, 2M),
-// out[2M, 2M + n).  With R > 1 bhat is stored in the two-level order [r][k]
-// (bhat[r + R k] at r M/R + k).
-void bluestein_tables(uint32_t n, uint32_t M, uint32_t R, double2 *out) {
-	std::vector<double2> b(M, make_double2(0.0, 0.0));
-	// w_j = e^{-i pi j^2/n}, with j^2 reduced mod 2n so the angle stays in [0, 2 pi)
-	auto chirp_angle = [n](uint32_t j) { return M_PI * double((uint64_t(j) * j) % (2 * uint64_t(n))) / double(n); };
-	for (uint32_t j = 0; j < n; ++j) {
-		const double ang = chirp_angle(j);
-		b[j] = make_double2(std::cos(ang), std::sin(ang)); // conj(w_j)
-		if (j) b[M - j] = b[j];
-		out[size_t(2) * M + j] = make_double2(std::cos(ang), -std::sin(ang)); // w_j
-	}
-	host_fft(b);
-	const uint32_t rows = M / R;
-	for (uint32_t j = 0; j < M; ++j) {
-		const uint32_t at = R > 1 ? (j % R) * rows + j / R : j;
-		out[at] = make_double2(b[j].x / M, b[j].y / M);
-		const double ang = 2.0 * M_PI * double(j) / double(M);
-		out[M + j] = make_double2(std::cos(ang), -std::sin(ang));
-	}
-}
-} // namespace
-
-void dnls_bluestein_tables(uint32_t n, double2 *out) { bluestein_tables(n, dnls_bluestein_size(n), 1, out); }
-
-uint32_t bluestein_two_level_size(uint32_t n) {
-	if (n <= 4096 || pow2(n) || n > kLargeFftMax / 2) return 0;
-	uint32_t L = 1;
-	while (L < 2 * n - 1) L <<= 1;
-	return L;
-}
-
-void bluestein_two_level_tables(uint32_t n, double2 *out) {
-	const uint32_t L = bluestein_two_level_size(n);
-	uint32_t R, M;
-	if (!L || !large_fft_split(L, R, M)) return;
-	bluestein_tables(n, L, R, out);
 }
 
 // Fourier multiplier tables (host side).  table[k][j] = T_k[j] for the generic
@@ -1107,85 +516,24 @@ void dnls_permute_tables(uint32_t n, int variant, const double2 *table, double2 
 		}
 }
 
-hipError_t launch_energy_bluestein(hipStream_t st, bool dnls, const double2 *state, uint32_t n, uint32_t copies,
-                                   const double2 *blue, const double *omega, double *partial, uint32_t grid,
-                                   double2 *scratch, size_t scratch_sites) {
-	const uint32_t L = bluestein_two_level_size(n);
-	uint32_t R = 0, M = 0;
-	if (!L || !blue || !scratch || scratch_sites < L || !large_fft_split(L, R, M)) return hipErrorInvalidValue;
-	const uint32_t chunk = uint32_t(std::min<size_t>(scratch_sites / L, copies));
-	hipLaunchKernelGGL(k_zero_rows, dim3(1024), dim3(256), 0, st, partial, size_t(grid) * n);
-#define NLC_BLUE(DM, RR, MM)                                                                                        \
-	return launch_energy_blue_rm<DM, RR, MM>(st, state, n, copies, blue, omega, partial, grid, scratch, chunk)
-	if (dnls) {
-		if (M == 8192) NLC_BLUE(true, 16, 8192);
-		if (R == 4) NLC_BLUE(true, 4, 4096);
-		if (R == 8) NLC_BLUE(true, 8, 4096);
-		NLC_BLUE(true, 16, 4096);
-	}
-	if (M == 8192) NLC_BLUE(false, 16, 8192);
-	if (R == 4) NLC_BLUE(false, 4, 4096);
-	if (R == 8) NLC_BLUE(false, 8, 4096);
-	NLC_BLUE(false, 16, 4096);
-#undef NLC_BLUE
-}
-
 hipError_t launch_dnls(hipStream_t st, double2 *state, uint32_t n, uint32_t copies, uint32_t steps,
                        const double *theta9, const double2 *tw, const double2 *table, int variant, double2 *scratch,
                        const double2 *blue) {
+	if (!dnls_fast_path(n)) {
+		uint32_t R, M;
+		if (bluestein_two_level_size(n) || large_fft_split(n, R, M))
+			return launch_dnls_long(st, state, n, copies, steps, theta9, tw, table, scratch, blue);
+		return launch_dnls_generic(st, state, n, copies, steps, theta9, tw, table);
+	}
 	DnlsArgs a;
 	for (int i = 0; i < 9; ++i) a.theta[i] = theta9[i];
-	uint32_t R = 0, M = 0;
-	if (const uint32_t L = bluestein_two_level_size(n)) {
-		if (!scratch || !blue || !large_fft_split(L, R, M)) return hipErrorInvalidValue;
-		if (M == 8192) return launch_dnls_blue<16, 8192>(st, state, scratch, n, copies, steps, a, table, blue);
-		switch (R) {
-		case 4: return launch_dnls_blue<4, 4096>(st, state, scratch, n, copies, steps, a, table, blue);
-		case 8: return launch_dnls_blue<8, 4096>(st, state, scratch, n, copies, steps, a, table, blue);
-		case 16: return launch_dnls_blue<16, 4096>(st, state, scratch, n, copies, steps, a, table, blue);
-		}
-		return hipErrorInvalidValue;
-	}
-	if (large_fft_split(n, R, M)) {
-		if (!scratch) return hipErrorInvalidValue;
-		if (M == 8192) return launch_dnls_two_level<16, 8192>(st, state, scratch, n, copies, steps, a, tw, table);
-		switch (R) {
-		case 4: return launch_dnls_two_level<4, 4096>(st, state, scratch, n, copies, steps, a, tw, table);
-		case 8: return launch_dnls_two_level<8, 4096>(st, state, scratch, n, copies, steps, a, tw, table);
-		case 16: return launch_dnls_two_level<16, 4096>(st, state, scratch, n, copies, steps, a, tw, table);
-		}
-		return hipErrorInvalidValue;
-	}
-	if (dnls_fast_path(n) && variant == kDnlsRadix8) {
+	if (variant == kDnlsRadix8) {
 		hipError_t e = hipFuncSetAttribute((const void *) k_dnls_4096_r8, hipFuncAttributeMaxDynamicSharedMemorySize,
 		                                   int(kR8Lds));
 		if (e != hipSuccess) return e;
 		hipLaunchKernelGGL(k_dnls_4096_r8, dim3(copies), dim3(kT8), kR8Lds, st, state, copies, steps, a, tw, table);
-	} else if (dnls_fast_path(n)) {
-		hipLaunchKernelGGL(k_dnls_4096, dim3(copies), dim3(kT), 0, st, state, copies, steps, a, tw, table);
-	} else if (pow2(n) && n <= kDnlsLdsMax) {
-		const size_t lds = sizeof(double2) * n;
-		if (n <= 4096) {
-			hipLaunchKernelGGL(k_dnls_lds<4096>, dim3(copies), dim3(kT), lds, st, state, n, copies, steps, a, tw, table);
-		} else {
-			hipError_t e = launch_lds_chain_kernel(k_dnls_lds<kDnlsLdsMax>, lds);
-			if (e != hipSuccess) return e;
-			hipLaunchKernelGGL(k_dnls_lds<kDnlsLdsMax>, dim3(copies), dim3(kT), lds, st, state, n, copies, steps, a, tw,
-			                   table);
-		}
-	} else if (const uint32_t M = dnls_bluestein_size(n)) {
-		const size_t lds = sizeof(double2) * M;
-		if (M <= 4096) {
-			hipLaunchKernelGGL(k_dnls_bluestein<4096>, dim3(copies), dim3(kT), lds, st, state, n, M, copies, steps, a,
-			                   table);
-		} else {
-			hipError_t e = launch_lds_chain_kernel(k_dnls_bluestein<kDnlsBluesteinMax>, lds);
-			if (e != hipSuccess) return e;
-			hipLaunchKernelGGL(k_dnls_bluestein<kDnlsBluesteinMax>, dim3(copies), dim3(kT), lds, st, state, n, M,
-			                   copies, steps, a, table);
-		}
 	} else {
-		return hipErrorInvalidValue;
+		hipLaunchKernelGGL(k_dnls_4096, dim3(copies), dim3(kT), 0, st, state, copies, steps, a, tw, table);
 	}
 	return hipGetLastError();
 }

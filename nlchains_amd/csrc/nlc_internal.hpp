@@ -163,6 +163,18 @@ hipError_t launch_dnls(hipStream_t, double2 *state, uint32_t n, uint32_t copies,
 // chain (DNLS step: every copy; energy pass: chunks of copies).
 uint32_t bluestein_two_level_size(uint32_t n);
 void bluestein_two_level_tables(uint32_t n, double2 *out);
+// bhat = FFT_M(b)/M of the chirp b_j = e^{i pi j^2/n} (|j| < n, cyclic), the
+// M-point roots and the chirp itself (M, M and n values; bhat in the
+// two-level [r][k] order when R > 1).
+void bluestein_tables(uint32_t n, uint32_t M, uint32_t R, double2 *out);
+// The DNLS launchers of the other chain lengths, called by launch_dnls:
+// LDS-resident kernels (nlc_dnls_generic.hip) and the HBM paths
+// (nlc_dnls_long.hip).
+hipError_t launch_dnls_generic(hipStream_t, double2 *state, uint32_t n, uint32_t copies, uint32_t steps,
+                               const double *theta9, const double2 *tw, const double2 *table);
+hipError_t launch_dnls_long(hipStream_t, double2 *state, uint32_t n, uint32_t copies, uint32_t steps,
+                            const double *theta9, const double2 *tw, const double2 *table, double2 *scratch,
+                            const double2 *blue);
 hipError_t launch_energy_bluestein(hipStream_t, bool dnls, const double2 *state, uint32_t n, uint32_t copies,
                                    const double2 *blue, const double *omega, double *partial, uint32_t grid,
                                    double2 *scratch, size_t scratch_sites);
