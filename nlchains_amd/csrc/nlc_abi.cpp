@@ -310,6 +310,7 @@ int setup_common(nlc_ctx *c) {
 	c->args.alpha = p.alpha;
 	c->args.a2 = 2 * p.alpha;
 	c->args.a2i = 1 / (2 * p.alpha);
+	c->args.form = nlc::force_form(p.model, p.alpha, p.beta, &c->args.scale);
 	c->a0is0 = (p.model == NLC_DNLS) ? 1 : (p.model == NLC_DDNKG ? 0 : (p.m == 0.0));
 
 	NLC_HIP(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
@@ -1190,6 +1191,15 @@ int nlc_describe(const nlc_ctx *c, char *buf, size_t len) {
 	else
 		snprintf(buf, len, "model=%s n=%u copies=%u kernel=%s lanes_per_chain=%d sites_per_lane=%d exact=%d",
 		         names[c->p.model], c->n, c->copies, c->L > 64 ? "group" : "wave", c->L, c->S, int(c->exact));
+	if (c->p.model != NLC_DNLS && !c->use_split) {
+		// force form of the register-resident kernels
+		const size_t at = strlen(buf);
+		if (at + 1 < len)
+			snprintf(buf + at, len - at, " force=%s",
+			         c->args.form == nlc::kFormCubic  ? "cubic_unit"
+			         : c->args.form == nlc::kFormQuad ? "quadratic_unit"
+			                                          : "plain");
+	}
 	if (c->p.model == NLC_DDNKG) {
 		const size_t at = strlen(buf);
 		if (at + 1 < len)

@@ -1,21 +1,24 @@
-// Double-precision e^y for the Toda bond force, table driven.
+// Double-precision exponential for the Toda bond force, table driven, on an
+// argument already expressed in 1/32 octaves: exp2_32(x) = 2^(x/32), so the
+// Toda kernels keep v = (64 alpha/ln2) q in registers and the bond term
+// e^{2 alpha (q_i - q_{i-1})} is exp2_32(v_i - v_{i-1}).
 //
 // Method: P. T. P. Tang, "Table-driven implementation of the exponential
 // function in IEEE floating-point arithmetic", ACM TOMS 15(2), 1989, with a
-// 32-entry table and a Taylor polynomial:
-//   y = N ln2/32 + r,  N = nearest integer to 32 y / ln2,  |r| <= ln2/64,
-//   e^y = 2^(N >> 5) * 2^((N & 31)/32) * e^r,
-//   e^r - 1 = r (1 + r/2! + r^2/3! + ... + r^5/6!)   (truncation < 4e-18).
-// N comes from the low word of fma(y, 32/ln2, 1.5 * 2^52) (no conversion
-// instruction); r = y - N ln2/32 is formed with two FMAs (ln2/32 split into a
-// rounded head and the rounded remainder); 2^(N >> 5) is applied with ldexp,
-// which saturates to +inf / 0 beyond the double range.  Error about 1 ulp:
-// the table entries are 2^(j/32) rounded to nearest, one rounding in the last
-// FMA, |e^r - 1| < 0.011 carries the polynomial's few-ulp error down to 1/100 ulp.
-// Valid for |y| < 4e7 (larger magnitudes are reached only by a diverged chain,
-// whose state is non-finite long before); NaN propagates.
+// 32-entry table and a Taylor polynomial, on the pre-scaled argument:
+//   x = N + r,  N = nearest integer to x,  |r| <= 1/2,
+//   2^(x/32) = 2^(N >> 5) * 2^((N & 31)/32) * 2^(r/32),
+//   2^(r/32) - 1 = sum_{k=1..6} (ln2/32)^k r^k / k!   (truncation < 4e-18).
+// N comes from the low word of x + 1.5 * 2^52 (no conversion instruction) and
+// r = x - N is exact, so the reduction of Tang's method (y - N ln2/32 in two
+// FMAs) is one subtraction here; 2^(N >> 5) is applied with ldexp, which
+// saturates to +inf / 0 beyond the double range.  Error about 1 ulp: the table
+// entries are 2^(j/32) rounded to nearest, one rounding in the last FMA, and
+// |2^(r/32) - 1| < 0.011 carries the polynomial's few-ulp error down to 1/100
+// ulp.  Valid for |x| < 2^31 (larger magnitudes are reached only by a
+// diverged chain, whose state is non-finite long before); NaN propagates.
 //
-// Cost on gfx950: 14 VALU + 1 LDS read per call, against about 25 VALU for the
+// Cost on gfx950: 13 VALU + 1 LDS read per call, against about 25 VALU for the
 // device library's exp (a degree-12 polynomial on |r| <= ln2/2 plus range
 // checks).  The table lives in LDS (256 bytes: entry j in banks 2j, 2j+1, so a
 // wavefront's reads never conflict).
@@ -43,13 +46,12 @@ constexpr int kExpTabSize = 32;
 	 0x1.ae89f995ad3adp+0, 0x1.b7f76f2fb5e47p+0, 0x1.c199bdd85529cp+0, 0x1.cb720dcef9069p+0,                        \
 	 0x1.d5818dcfba487p+0, 0x1.dfc97337b9b5fp+0, 0x1.ea4afa2a490dap+0, 0x1.f50765b6e4540p+0}
 
-constexpr double kExpInvStep = 0x1.71547652b82fep+5; // 32/ln2
-constexpr double kExpStepHi = 0x1.62e42fefa39efp-6;  // ln2/32 rounded
-constexpr double kExpStepLo = 0x1.abc9e3b39803fp-61; // ln2/32 - kExpStepHi, rounded
-constexpr double kExpShifter = 0x1.8p52;             // 1.5 * 2^52
+constexpr double kExpShifter = 0x1.8p52;   // 1.5 * 2^52: x + kExpShifter rounds x to an integer
+constexpr double kExpShifter32 = 0x1.8p57; // 1.5 * 2^57: rounds x to a multiple of 32
 
-NLC_EXP_QUAL double exp_tab(double y, const double *tab) {
-	const double s = __builtin_fma(y, kExpInvStep, kExpShifter);
+// 2^(x/32) = e^(x ln2/32).
+NLC_EXP_QUAL double exp2_32_tab(double x, const double *tab) {
+	const double s = x + kExpShifter;
 	const double nd = s - kExpShifter; // N as a double, exact
 	// low 32 bits of s's significand hold N in two's complement (|N| < 2^31)
 	union {
@@ -57,49 +59,45 @@ NLC_EXP_QUAL double exp_tab(double y, const double *tab) {
 		unsigned long long u;
 	} bits = {s};
 	const int n = int(unsigned(bits.u));
-	double r = __builtin_fma(-nd, kExpStepHi, y);
-	r = __builtin_fma(-nd, kExpStepLo, r);
-	double t = __builtin_fma(r, 0x1.6c16c16c16c17p-10, 0x1.1111111111111p-7); // 1/720, 1/120
-	t = __builtin_fma(t, r, 0x1.5555555555555p-5);                             // 1/24
-	t = __builtin_fma(t, r, 0x1.5555555555555p-3);                             // 1/6
-	t = __builtin_fma(t, r, 0.5);
-	t = __builtin_fma(t, r, 1.0);
-	const double q = t * r; // e^r - 1
+	const double r = x - nd; // exact, |r| <= 1/2
+	// 2^(r/32) - 1 = sum_k c_k r^k, c_k = (ln2/32)^k / k!
+	double t = __builtin_fma(r, 0x1.430912f86c787p-43, 0x1.5d87fe78a6731p-35); // c6, c5
+	t = __builtin_fma(t, r, 0x1.3b2ab6fba4e77p-27);                            // c4
+	t = __builtin_fma(t, r, 0x1.c6b08d704a0c0p-20);                            // c3
+	t = __builtin_fma(t, r, 0x1.ebfbdff82c58fp-13);                            // c2
+	t = __builtin_fma(t, r, 0x1.62e42fefa39efp-6);                             // c1
+	const double q = t * r;
 	const double T = tab[n & (kExpTabSize - 1)];
 	return __builtin_ldexp(__builtin_fma(T, q, T), n >> 5);
 }
 
-// Table-free variant: y = N ln2 + r, |r| <= ln2/2, e^r by its degree-13 Taylor
-// polynomial (truncation below 5e-18 relative), 2^N by ldexp.  18 VALU and no
-// memory access: with the table, the scheduler hoists the table loads of a
-// whole unrolled run of bonds to hide their latency and the kernel loses
-// occupancy; this form keeps a chain of bonds in few registers.
-constexpr double kExpInvLn2 = 0x1.71547652b82fep+0; // 1/ln2
-constexpr double kExpLn2Hi = 0x1.62e42fefa39efp-1;  // ln2 rounded
-constexpr double kExpLn2Lo = 0x1.abc9e3b39803fp-56; // ln2 - kExpLn2Hi, rounded
-
-NLC_EXP_QUAL double exp_poly(double y) {
-	const double s = __builtin_fma(y, kExpInvLn2, kExpShifter);
-	const double nd = s - kExpShifter;
+// Table-free variant: x = 32 N + r, |r| <= 16, 2^(r/32) by its degree-13
+// Taylor polynomial in r (truncation below 5e-18 relative), 2^N by ldexp.
+// 17 VALU and no memory access: with the table, the scheduler hoists the
+// table loads of a whole unrolled run of bonds to hide their latency and the
+// kernel loses occupancy; this form keeps a chain of bonds in few registers.
+NLC_EXP_QUAL double exp2_32_poly(double x) {
+	const double s = x + kExpShifter32; // the unit in the last place of s is 32
+	const double nd = s - kExpShifter32; // 32 N, exact
+	// low 32 bits of s's significand (in units of 32) hold N in two's complement
 	union {
 		double d;
 		unsigned long long u;
 	} bits = {s};
 	const int n = int(unsigned(bits.u));
-	double r = __builtin_fma(-nd, kExpLn2Hi, y);
-	r = __builtin_fma(-nd, kExpLn2Lo, r);
-	double t = __builtin_fma(r, 0x1.6124613a86d09p-33, 0x1.1eed8eff8d898p-29); // 1/13!, 1/12!
-	t = __builtin_fma(t, r, 0x1.ae64567f544e4p-26);                            // 1/11!
-	t = __builtin_fma(t, r, 0x1.27e4fb7789f5cp-22);                            // 1/10!
-	t = __builtin_fma(t, r, 0x1.71de3a556c734p-19);                            // 1/9!
-	t = __builtin_fma(t, r, 0x1.a01a01a01a01ap-16);                            // 1/8!
-	t = __builtin_fma(t, r, 0x1.a01a01a01a01ap-13);                            // 1/7!
-	t = __builtin_fma(t, r, 0x1.6c16c16c16c17p-10);                            // 1/6!
-	t = __builtin_fma(t, r, 0x1.1111111111111p-7);                             // 1/5!
-	t = __builtin_fma(t, r, 0x1.5555555555555p-5);                             // 1/4!
-	t = __builtin_fma(t, r, 0x1.5555555555555p-3);                             // 1/3!
-	t = __builtin_fma(t, r, 0.5);
-	t = __builtin_fma(t, r, 1.0);
+	const double r = x - nd; // exact, |r| <= 16
+	double t = __builtin_fma(r, 0x1.816193166d0f9p-105, 0x1.c3bd650fc2986p-96); // c13, c12
+	t = __builtin_fma(t, r, 0x1.e8cac7351bb25p-87);                             // c11
+	t = __builtin_fma(t, r, 0x1.e4cf5158b8ecap-78);                             // c10
+	t = __builtin_fma(t, r, 0x1.b5253d395e7c4p-69);                             // c9
+	t = __builtin_fma(t, r, 0x1.62c0223a5c824p-60);                             // c8
+	t = __builtin_fma(t, r, 0x1.ffcbfc588b0c7p-52);                             // c7
+	t = __builtin_fma(t, r, 0x1.430912f86c787p-43);                             // c6
+	t = __builtin_fma(t, r, 0x1.5d87fe78a6731p-35);                             // c5
+	t = __builtin_fma(t, r, 0x1.3b2ab6fba4e77p-27);                             // c4
+	t = __builtin_fma(t, r, 0x1.c6b08d704a0c0p-20);                             // c3
+	t = __builtin_fma(t, r, 0x1.ebfbdff82c58fp-13);                             // c2
+	t = __builtin_fma(t, r, 0x1.62e42fefa39efp-6);                              // c1
 	return __builtin_ldexp(__builtin_fma(t, r, 1.0), n);
 }
 

@@ -23,7 +23,32 @@ struct StepArgs {
 	double alpha; // FPUT cubic coefficient
 	double a2;    // Toda: 2*alpha
 	double a2i;   // Toda: 1/(2*alpha)
+	double scale; // sigma of the scaled force forms (see force_form)
+	int form;     // kFormPlain, kFormCubic or kFormQuad
 };
+
+// Force forms of the register-resident step kernels.  The scaled forms hold
+// sigma q and sigma p, which turns the leading nonlinear coefficient into 1:
+//   kFormCubic  DNKG / dDNKG with beta > 0, FPUT with alpha = 0 < beta:
+//               sigma = sqrt(beta);
+//   kFormQuad   FPUT with beta = 0 != alpha: sigma = alpha;
+//   kFormPlain  every other case; Toda keeps v = sigma q with unscaled p,
+//               sigma = 64 alpha/ln2 (its bond exponent in 1/32 octaves).
+enum ForceForm { kFormPlain = 0, kFormCubic = 1, kFormQuad = 2 };
+
+// The form and sigma for a model's parameters.
+inline ForceForm force_form(int model, double alpha, double beta, double *sigma) {
+	*sigma = model == TODA ? alpha * 0x1.71547652b82fep+6 : 1.0; // 64/ln2
+	if ((model == DNKG || model == DDNKG || (model == FPUT && alpha == 0.0)) && beta > 0.0) {
+		*sigma = __builtin_sqrt(beta);
+		return kFormCubic;
+	}
+	if (model == FPUT && beta == 0.0 && alpha != 0.0) {
+		*sigma = alpha;
+		return kFormQuad;
+	}
+	return kFormPlain;
+}
 
 // Enqueues `steps` Yoshida steps of a wave-resident step kernel.
 using StepLauncher = hipError_t (*)(hipStream_t, double2 *state, uint32_t copies, uint32_t n, uint32_t steps,

@@ -62,20 +62,54 @@ __device__ __forceinline__ double from_right(double v, int lane) {
 	else return bperm_f64(v, (lane & ~(L - 1)) | ((lane + 1) & (L - 1)));
 }
 
+// Force of DNKG / dDNKG on a site with neighbours l, r and mass = m_x + 2:
+// l + r - q (m_x + 2 + beta q^2) (DNKG_FPUT_Toda.cu:24-31, dDNKG.cu:17-19).
+// kFormCubic works on sigma q with sigma = sqrt(beta), where the cubic
+// coefficient is 1: one FMA for m + 2 + q^2 instead of a product and an FMA.
+template <int FORM>
+__device__ __forceinline__ double site_force(double l, double q, double r, double mass, const StepArgs &a) {
+	const double t = (FORM == kFormCubic) ? fma(q, q, mass) : fma(a.beta, q * q, mass);
+	return fma(-q, t, l + r);
+}
+
 // Bond force term of FPUT / Toda for the bond x = q_{i} - q_{i-1}; the force on
 // site i is bond(x_{i+1}) - bond(x_i).
 //   FPUT: V'(x) = x + alpha x^2 + beta x^3  (Horner), the same force as
 //         (l+r-2c)(1+alpha(r-l)) + beta(d_r^3-d_l^3) of DNKG_FPUT_Toda.cu:41-58;
+//         on sigma q, kFormCubic (alpha = 0, sigma = sqrt(beta)) is x + x^3 and
+//         kFormQuad (beta = 0, sigma = alpha) is x + x^2;
 //   Toda: e^{2 alpha x}, the force of DNKG_FPUT_Toda.cu:62-74 without its factor
 //         1/(2 alpha), which the kick folds into its weight (the constant
-//         -1/(2 alpha) of V' cancels in the difference).  FP64 table-driven exp
-//         (fewest instructions); for S >= 25 sites per lane, where the state
-//         takes 100+ VGPRs and hoisted table loads would leave one wave
-//         per SIMD, the table-free form.
-template <int MODEL, int S>
+//         -1/(2 alpha) of V' cancels in the difference), evaluated as
+//         2^(x'/32) on the bond x' = v_i - v_{i-1} of v = (64 alpha/ln2) q:
+//         table driven (fewest instructions), or for S >= 25 sites per lane,
+//         where the state takes 100+ VGPRs and hoisted table loads would
+//         leave one wave per SIMD, the table-free form.
+template <int MODEL, int S, int FORM>
 __device__ __forceinline__ double bond_force(double x, const StepArgs &a, const double *etab) {
-	if constexpr (MODEL == FPUT) return x * fma(x, fma(a.beta, x, a.alpha), 1.0);
-	else if constexpr (S >= 25) return exp_poly(x); // Toda: x is already 2 alpha (q_i - q_{i-1})
-	else return exp_tab(x, etab);
+	if constexpr (MODEL == FPUT) {
+		if constexpr (FORM == kFormCubic) return fma(x * x, x, x);
+		else if constexpr (FORM == kFormQuad) return fma(x, x, x);
+		else return x * fma(x, fma(a.beta, x, a.alpha), 1.0);
+	} else if constexpr (S >= 25) return exp2_32_poly(x); // Toda: x = v_i - v_{i-1}
+	else return exp2_32_tab(x, etab);
+}
+
+// The state as the kernels hold it: sigma q and sigma p for the scaled forms
+// (then sigma F(q) = F~(sigma q), so drift and kick weights stay as they
+// are), v = (64 alpha/ln2) q with unscaled p for Toda (sigma = 64 alpha/ln2),
+// else q and p.
+template <int MODEL, int FORM>
+__device__ __forceinline__ double2 to_registers(double2 v, const StepArgs &a) {
+	if constexpr (MODEL == TODA) return make_double2(a.scale * v.x, v.y);
+	else if constexpr (FORM != kFormPlain) return make_double2(a.scale * v.x, a.scale * v.y);
+	else return v;
+}
+
+template <int MODEL, int FORM>
+__device__ __forceinline__ double2 from_registers(double q, double p, const StepArgs &a) {
+	if constexpr (MODEL == TODA) return make_double2(q / a.scale, p);
+	else if constexpr (FORM != kFormPlain) return make_double2(q / a.scale, p / a.scale);
+	else return make_double2(q, p);
 }
 

@@ -37,14 +37,9 @@ namespace {
 
 constexpr int kGroupMaxWaves = 8;
 
-// Force on a site of DNKG / dDNKG from its neighbours l, r (the expression of
-// the kick loop, shared so that a predicted halo is bit-identical).
-__device__ __forceinline__ double site_force(double l, double q, double r, double mass, const StepArgs &a) {
-	const double t = fma(a.beta, q * q, mass);
-	return fma(-q, t, l + r);
-}
-
-template <int MODEL, int S, bool EXACT>
+// The kick loop and the predicted halos evaluate the force with the same
+// site_force / bond_force expressions, so a prediction is bit-identical.
+template <int MODEL, int S, bool EXACT, int FORM>
 struct GroupChain {
 	double q[S], p[S];
 	double mp[MODEL == DDNKG ? S : 1];
@@ -109,8 +104,8 @@ struct GroupChain {
 			if (exchange) {
 				// the neighbours' edge sites, kicked and drifted as their own lanes do
 				const double ml = (MODEL == DDNKG) ? m_l : a.m2, mr = (MODEL == DDNKG) ? m_r : a.m2;
-				pred_l = fma(c_next, fma(d, site_force(xl[0], xl[1], q[0], ml, a), xl[2]), xl[1]);
-				pred_r = fma(c_next, fma(d, site_force(last_q(), xr[0], xr[1], mr, a), xr[2]), xr[0]);
+				pred_l = fma(c_next, fma(d, site_force<FORM>(xl[0], xl[1], q[0], ml, a), xl[2]), xl[1]);
+				pred_r = fma(c_next, fma(d, site_force<FORM>(last_q(), xr[0], xr[1], mr, a), xr[2]), xr[0]);
 			}
 			if (!EXACT && !full) q[S - 1] = hr;
 #pragma unroll
@@ -118,17 +113,17 @@ struct GroupChain {
 				const double lft = (j == 0) ? hl : q[j - 1];
 				const double rgt = (j == S - 1) ? hr : q[j + 1];
 				const double mass = (MODEL == DDNKG) ? mp[j] : a.m2;
-				p[j] = fma(d, site_force(lft, q[j], rgt, mass, a), p[j]);
+				p[j] = fma(d, site_force<FORM>(lft, q[j], rgt, mass, a), p[j]);
 			}
 			if (!EXACT && !full) p[S - 1] = 0.0;
 		} else {
 			const double w = (MODEL == TODA) ? d * a.a2i : d;
 			const double last = last_q();
-			const double bs = bond_force<MODEL, S>(hr - last, a, etab); // right bond of the last real site
-			double bl = bond_force<MODEL, S>(q[0] - hl, a, etab);
+			const double bs = bond_force<MODEL, S, FORM>(hr - last, a, etab); // right bond of the last real site
+			double bl = bond_force<MODEL, S, FORM>(q[0] - hl, a, etab);
 			if (exchange) {
-				const double bll = bond_force<MODEL, S>(xl[1] - xl[0], a, etab); // left bond of the left neighbour
-				const double brr = bond_force<MODEL, S>(xr[1] - xr[0], a, etab); // right bond of the right neighbour
+				const double bll = bond_force<MODEL, S, FORM>(xl[1] - xl[0], a, etab); // left bond of the left neighbour
+				const double brr = bond_force<MODEL, S, FORM>(xr[1] - xr[0], a, etab); // right bond of the right neighbour
 				pred_l = fma(c_next, fma(w, bl - bll, xl[2]), xl[1]);
 				pred_r = fma(c_next, fma(w, brr - bs, xr[2]), xr[0]);
 			}
@@ -136,8 +131,8 @@ struct GroupChain {
 			for (int j = 0; j < S; ++j) {
 				double br;
 				if (j == S - 1) br = bs;
-				else if (!EXACT && j == S - 2) br = full ? bond_force<MODEL, S>(q[j + 1] - q[j], a, etab) : bs;
-				else br = bond_force<MODEL, S>(q[j + 1] - q[j], a, etab);
+				else if (!EXACT && j == S - 2) br = full ? bond_force<MODEL, S, FORM>(q[j + 1] - q[j], a, etab) : bs;
+				else br = bond_force<MODEL, S, FORM>(q[j + 1] - q[j], a, etab);
 				p[j] = fma(w, br - bl, p[j]);
 				bl = br;
 			}
@@ -150,7 +145,7 @@ struct GroupChain {
 // One chain per workgroup of 64 W threads, W = blockDim.x / 64.  Every thread
 // of the workgroup runs the same number of kicks, so every barrier is reached
 // by all of them.
-template <int MODEL, int S, bool EXACT>
+template <int MODEL, int S, bool EXACT, int FORM>
 __global__ void __launch_bounds__(64 * kGroupMaxWaves)
 k_step_group(double2 *__restrict__ state, uint32_t n, uint32_t steps, StepArgs a, const double *__restrict__ mp2g) {
 	__shared__ double etab[MODEL == TODA ? kExpTabSize : 1];
@@ -158,7 +153,7 @@ k_step_group(double2 *__restrict__ state, uint32_t n, uint32_t steps, StepArgs a
 	load_exp_table<MODEL>(etab);
 	const uint32_t g = threadIdx.x, G = blockDim.x;
 
-	GroupChain<MODEL, S, EXACT> ch;
+	GroupChain<MODEL, S, EXACT, FORM> ch;
 	ch.lane = g & 63;
 	ch.wave = g >> 6;
 	ch.waves = G >> 6;
@@ -176,8 +171,8 @@ k_step_group(double2 *__restrict__ state, uint32_t n, uint32_t steps, StepArgs a
 	for (int j = 0; j < S; ++j) {
 		const bool real_site = EXACT || j < S - 1 || ch.full;
 		double2 v = make_double2(0.0, 0.0);
-		if (real_site) v = base[j];
-		ch.q[j] = (MODEL == TODA) ? a.a2 * v.x : v.x;
+		if (real_site) v = to_registers<MODEL, FORM>(base[j], a);
+		ch.q[j] = v.x;
 		ch.p[j] = v.y;
 		if constexpr (MODEL == DDNKG) ch.mp[j] = real_site ? mp2g[off + j] : 0.0;
 	}
@@ -189,7 +184,7 @@ k_step_group(double2 *__restrict__ state, uint32_t n, uint32_t steps, StepArgs a
 		ch.m_l = ch.m_r = 0.0;
 	}
 
-	const double cs = (MODEL == TODA) ? a.a2 : 1.0;
+	const double cs = (MODEL == TODA) ? a.scale : 1.0;
 	const double dc0 = cs * a.dc[0];
 	ch.drift(dc0);
 	for (uint32_t s = 0; s < steps; ++s) {
@@ -213,7 +208,7 @@ k_step_group(double2 *__restrict__ state, uint32_t n, uint32_t steps, StepArgs a
 
 #pragma unroll
 	for (int j = 0; j < S; ++j)
-		if (EXACT || j < S - 1 || ch.full) base[j] = make_double2((MODEL == TODA) ? ch.q[j] / a.a2 : ch.q[j], ch.p[j]);
+		if (EXACT || j < S - 1 || ch.full) base[j] = from_registers<MODEL, FORM>(ch.q[j], ch.p[j], a);
 }
 
 // W = ceil(n / (64 S)) wavefronts; the sites must fill every lane with S or
@@ -225,7 +220,20 @@ hipError_t launch_group(hipStream_t st, double2 *state, uint32_t copies, uint32_
 	if (G < 128 || G > 64u * kGroupMaxWaves || uint64_t(G) * (S - 1) >= n || uint64_t(G) * S < n ||
 	    (EXACT && uint64_t(G) * S != n))
 		return hipErrorInvalidValue;
-	hipLaunchKernelGGL((k_step_group<MODEL, S, EXACT>), dim3(copies), dim3(G), 0, st, state, n, steps, a, mp2);
+	if constexpr (MODEL != TODA)
+		if (a.form == kFormCubic) {
+			hipLaunchKernelGGL((k_step_group<MODEL, S, EXACT, kFormCubic>), dim3(copies), dim3(G), 0, st, state, n,
+			                   steps, a, mp2);
+			return hipGetLastError();
+		}
+	if constexpr (MODEL == FPUT)
+		if (a.form == kFormQuad) {
+			hipLaunchKernelGGL((k_step_group<MODEL, S, EXACT, kFormQuad>), dim3(copies), dim3(G), 0, st, state, n,
+			                   steps, a, mp2);
+			return hipGetLastError();
+		}
+	hipLaunchKernelGGL((k_step_group<MODEL, S, EXACT, kFormPlain>), dim3(copies), dim3(G), 0, st, state, n, steps, a,
+	                   mp2);
 	return hipGetLastError();
 }
 

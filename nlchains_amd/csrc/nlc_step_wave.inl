@@ -14,10 +14,12 @@
 //  * nearest-neighbour halos move between lanes with DPP (wave_ror/wave_rol for
 //    L = 64, row_ror for L = 16, quad_perm for L = 4 and 2) or ds_bpermute
 //    (L = 32, 8), never through LDS memory;
-//  * the kick of DNKG/dDNKG is 5 VALU ops per site, the drift 1 FMA;
+//  * the kick of DNKG/dDNKG is 5 FP64 ops per site (4 when beta > 0: the
+//    state is held as sqrt(beta) q, sqrt(beta) p, which makes the cubic
+//    coefficient 1), the drift 1 FMA;
 //    FPUT and Toda exchange the bond force of the lane's first bond instead of
 //    recomputing it, so each bond term (a cubic or an FP64 exp) is evaluated
-//    once per substep; Toda works on u = 2 alpha q with the exp of
+//    once per substep; Toda works on v = (64 alpha/ln2) q with the 2^(x/32) of
 //    nlc_exp.hpp;
 //  * the 7 substeps are unrolled with their weights in SGPRs; the c7 drift of a
 //    step is fused with the c0 drift of the next (the reference's 2*c0 merge).
@@ -33,7 +35,7 @@ namespace {
 
 #include "nlc_step_common.inl"
 
-template <int MODEL, int L, int S, bool EXACT>
+template <int MODEL, int L, int S, bool EXACT, int FORM>
 struct Chain {
 	double q[S], p[S];
 	double mp[MODEL == DDNKG ? S : 1];
@@ -58,24 +60,21 @@ struct Chain {
 				const double lft = (j == 0) ? hl : q[j - 1];
 				const double rgt = (j == S - 1) ? hr : q[j + 1];
 				const double mass = (MODEL == DDNKG) ? mp[j] : a.m2;
-				// F = l + r - q (m + 2 + beta q^2)
-				const double t = fma(a.beta, q[j] * q[j], mass);
-				const double f = fma(-q[j], t, lft + rgt);
-				p[j] = fma(d, f, p[j]);
+				p[j] = fma(d, site_force<FORM>(lft, q[j], rgt, mass, a), p[j]);
 			}
 			if (!EXACT && !full) p[S - 1] = 0.0;
 		} else {
 			// bond terms rolled through the chain: only two are live at a time
 			const double w = (MODEL == TODA) ? d * a.a2i : d;
-			const double b0 = bond_force<MODEL, S>(q[0] - hl, a, etab);
+			const double b0 = bond_force<MODEL, S, FORM>(q[0] - hl, a, etab);
 			const double bs = from_right<L>(b0, lane); // right bond of the lane's last site
 			double bl = b0;
 #pragma unroll
 			for (int j = 0; j < S; ++j) {
 				double br;
 				if (j == S - 1) br = bs;
-				else if (!EXACT && j == S - 2) br = full ? bond_force<MODEL, S>(q[j + 1] - q[j], a, etab) : bs;
-				else br = bond_force<MODEL, S>(q[j + 1] - q[j], a, etab);
+				else if (!EXACT && j == S - 2) br = full ? bond_force<MODEL, S, FORM>(q[j + 1] - q[j], a, etab) : bs;
+				else br = bond_force<MODEL, S, FORM>(q[j + 1] - q[j], a, etab);
 				p[j] = fma(w, br - bl, p[j]);
 				bl = br;
 			}
@@ -87,9 +86,11 @@ struct Chain {
 // One wavefront integrates `live_chains` (<= 64/L) consecutive chains of
 // length n starting at chain0 for `steps` steps.  All 64 lanes take part in
 // the halo exchange even when the wave is only partly filled.  Toda keeps
-// u = 2 alpha q in registers (the exponent of its bond term is then a plain
-// difference) and drifts u with 2 alpha c; q = u / (2 alpha) is stored back.
-template <int MODEL, int L, int S, bool EXACT>
+// v = (64 alpha/ln2) q in registers (the exponent of its bond term is then a
+// plain difference in 1/32 octaves) and drifts v with (64 alpha/ln2) c;
+// q = v / (64 alpha/ln2) is stored back.
+// The scaled force forms hold sigma q and sigma p.
+template <int MODEL, int L, int S, bool EXACT, int FORM>
 __device__ __forceinline__ void run_wave(double2 *__restrict__ chain0, uint32_t n, uint32_t live_chains, uint32_t steps,
                                          const StepArgs &a, const double *__restrict__ mp2g, const double *etab) {
 	const int lane = threadIdx.x & 63;
@@ -97,7 +98,7 @@ __device__ __forceinline__ void run_wave(double2 *__restrict__ chain0, uint32_t 
 	const uint32_t sub = uint32_t(lane / L);
 	const bool live = sub < live_chains;
 
-	Chain<MODEL, L, S, EXACT> ch;
+	Chain<MODEL, L, S, EXACT, FORM> ch;
 	ch.lane = lane;
 	ch.etab = etab;
 	const int full_lanes = EXACT ? L : int(n) - (S - 1) * L;
@@ -109,14 +110,14 @@ __device__ __forceinline__ void run_wave(double2 *__restrict__ chain0, uint32_t 
 	for (int j = 0; j < S; ++j) {
 		const bool real_site = EXACT || j < S - 1 || ch.full;
 		double2 v = make_double2(0.0, 0.0);
-		if (live && real_site) v = base[j];
-		ch.q[j] = (MODEL == TODA) ? a.a2 * v.x : v.x;
+		if (live && real_site) v = to_registers<MODEL, FORM>(base[j], a);
+		ch.q[j] = v.x;
 		ch.p[j] = v.y;
 		if constexpr (MODEL == DDNKG) ch.mp[j] = real_site ? mp2g[off + j] : 0.0;
 	}
 
-	// Toda drifts u = 2 alpha q: its drift weights carry the factor 2 alpha
-	const double cs = (MODEL == TODA) ? a.a2 : 1.0;
+	// Toda drifts v = (64 alpha/ln2) q: its drift weights carry that factor
+	const double cs = (MODEL == TODA) ? a.scale : 1.0;
 	const double dc0 = cs * a.dc[0];
 	ch.drift(dc0);
 	for (uint32_t s = 0; s < steps; ++s) {
@@ -141,11 +142,11 @@ __device__ __forceinline__ void run_wave(double2 *__restrict__ chain0, uint32_t 
 #pragma unroll
 		for (int j = 0; j < S; ++j)
 			if (EXACT || j < S - 1 || ch.full)
-				base[j] = make_double2((MODEL == TODA) ? ch.q[j] / a.a2 : ch.q[j], ch.p[j]);
+				base[j] = from_registers<MODEL, FORM>(ch.q[j], ch.p[j], a);
 	}
 }
 
-template <int MODEL, int L, int S, bool EXACT>
+template <int MODEL, int L, int S, bool EXACT, int FORM>
 __global__ void __launch_bounds__(256)
 k_step_wave(double2 *__restrict__ state, uint32_t copies, uint32_t n, uint32_t steps, StepArgs a,
             const double *__restrict__ mp2g) {
@@ -155,19 +156,31 @@ k_step_wave(double2 *__restrict__ state, uint32_t copies, uint32_t n, uint32_t s
 	const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
 	const uint32_t first_chain = wave * CPW;
 	if (first_chain >= copies) return; // whole wavefront idle: uniform exit
-	run_wave<MODEL, L, S, EXACT>(state + size_t(first_chain) * n, n, min(CPW, copies - first_chain), steps, a, mp2g,
+	run_wave<MODEL, L, S, EXACT, FORM>(state + size_t(first_chain) * n, n, min(CPW, copies - first_chain), steps, a, mp2g,
 	                             etab);
 }
 
-template <int MODEL, int L, int S, bool EXACT>
-hipError_t launch_wave(hipStream_t st, double2 *state, uint32_t copies, uint32_t n, uint32_t steps,
-                       const StepArgs &a, const double *mp2) {
+template <int MODEL, int L, int S, bool EXACT, int FORM>
+hipError_t launch_wave_form(hipStream_t st, double2 *state, uint32_t copies, uint32_t n, uint32_t steps,
+                            const StepArgs &a, const double *mp2) {
 	constexpr uint32_t CPW = 64 / L;
 	const uint64_t waves = (uint64_t(copies) + CPW - 1) / CPW;
 	const uint64_t blocks = (waves + 3) / 4;
-	hipLaunchKernelGGL((k_step_wave<MODEL, L, S, EXACT>), dim3(uint32_t(blocks)), dim3(256), 0, st, state, copies, n,
-	                   steps, a, mp2);
+	hipLaunchKernelGGL((k_step_wave<MODEL, L, S, EXACT, FORM>), dim3(uint32_t(blocks)), dim3(256), 0, st, state,
+	                   copies, n, steps, a, mp2);
 	return hipGetLastError();
+}
+
+// The instance for the force form of `a` (forms a model does not have are not
+// instantiated; the host never selects them).
+template <int MODEL, int L, int S, bool EXACT>
+hipError_t launch_wave(hipStream_t st, double2 *state, uint32_t copies, uint32_t n, uint32_t steps,
+                       const StepArgs &a, const double *mp2) {
+	if constexpr (MODEL != TODA)
+		if (a.form == kFormCubic) return launch_wave_form<MODEL, L, S, EXACT, kFormCubic>(st, state, copies, n, steps, a, mp2);
+	if constexpr (MODEL == FPUT)
+		if (a.form == kFormQuad) return launch_wave_form<MODEL, L, S, EXACT, kFormQuad>(st, state, copies, n, steps, a, mp2);
+	return launch_wave_form<MODEL, L, S, EXACT, kFormPlain>(st, state, copies, n, steps, a, mp2);
 }
 
 #ifdef NLC_STEP_RAGGED
@@ -175,7 +188,7 @@ hipError_t launch_wave(hipStream_t st, double2 *state, uint32_t copies, uint32_t
 // of chains of one length class (64 S / n chains of n = L S sites), so all
 // items cost the same; waves take items round-robin until none is left (no
 // atomics, deterministic, every wave reaches the end of the list).
-template <int MODEL, int S>
+template <int MODEL, int S, int FORM>
 __global__ void __launch_bounds__(256)
 k_step_ragged(double2 *__restrict__ state, const RaggedItem *__restrict__ items, uint32_t nitems, uint32_t steps,
               StepArgs a) {
@@ -187,24 +200,24 @@ k_step_ragged(double2 *__restrict__ state, const RaggedItem *__restrict__ items,
 		const RaggedItem it = items[i];
 		double2 *c0 = state + it.site0;
 		switch (it.log2_lanes) {
-		case 0: run_wave<MODEL, 1, S, true>(c0, it.n, it.nchains, steps, a, nullptr, etab); break;
-		case 1: run_wave<MODEL, 2, S, true>(c0, it.n, it.nchains, steps, a, nullptr, etab); break;
-		case 2: run_wave<MODEL, 4, S, true>(c0, it.n, it.nchains, steps, a, nullptr, etab); break;
-		case 3: run_wave<MODEL, 8, S, true>(c0, it.n, it.nchains, steps, a, nullptr, etab); break;
-		case 4: run_wave<MODEL, 16, S, true>(c0, it.n, it.nchains, steps, a, nullptr, etab); break;
-		case 5: run_wave<MODEL, 32, S, true>(c0, it.n, it.nchains, steps, a, nullptr, etab); break;
-		default: run_wave<MODEL, 64, S, true>(c0, it.n, it.nchains, steps, a, nullptr, etab); break;
+		case 0: run_wave<MODEL, 1, S, true, FORM>(c0, it.n, it.nchains, steps, a, nullptr, etab); break;
+		case 1: run_wave<MODEL, 2, S, true, FORM>(c0, it.n, it.nchains, steps, a, nullptr, etab); break;
+		case 2: run_wave<MODEL, 4, S, true, FORM>(c0, it.n, it.nchains, steps, a, nullptr, etab); break;
+		case 3: run_wave<MODEL, 8, S, true, FORM>(c0, it.n, it.nchains, steps, a, nullptr, etab); break;
+		case 4: run_wave<MODEL, 16, S, true, FORM>(c0, it.n, it.nchains, steps, a, nullptr, etab); break;
+		case 5: run_wave<MODEL, 32, S, true, FORM>(c0, it.n, it.nchains, steps, a, nullptr, etab); break;
+		default: run_wave<MODEL, 64, S, true, FORM>(c0, it.n, it.nchains, steps, a, nullptr, etab); break;
 		}
 	}
 }
 
-template <int MODEL, int S>
-hipError_t launch_ragged(hipStream_t st, double2 *state, const RaggedItem *items, uint32_t nitems, uint32_t steps,
-                         const StepArgs &a, uint32_t max_blocks) {
+template <int MODEL, int S, int FORM>
+hipError_t launch_ragged_form(hipStream_t st, double2 *state, const RaggedItem *items, uint32_t nitems, uint32_t steps,
+                              const StepArgs &a, uint32_t max_blocks) {
 	int per_cu = 0, cus = 0, dev = 0;
 	hipError_t e = hipGetDevice(&dev);
 	if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-	if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_step_ragged<MODEL, S>, 256, 0);
+	if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_step_ragged<MODEL, S, FORM>, 256, 0);
 	if (e != hipSuccess) return e;
 	// as many waves as fit at once, but no more rounds than the item count needs
 	const uint64_t slots = uint64_t(std::max(per_cu, 1)) * std::max(cus, 1) * 4;
@@ -212,9 +225,19 @@ hipError_t launch_ragged(hipStream_t st, double2 *state, const RaggedItem *items
 	uint64_t waves = rounds ? (nitems + rounds - 1) / rounds : 1;
 	uint64_t blocks = (waves + 3) / 4;
 	if (max_blocks) blocks = std::min<uint64_t>(blocks, max_blocks);
-	hipLaunchKernelGGL((k_step_ragged<MODEL, S>), dim3(uint32_t(std::max<uint64_t>(blocks, 1))), dim3(256), 0, st,
+	hipLaunchKernelGGL((k_step_ragged<MODEL, S, FORM>), dim3(uint32_t(std::max<uint64_t>(blocks, 1))), dim3(256), 0, st,
 	                   state, items, nitems, steps, a);
 	return hipGetLastError();
+}
+
+template <int MODEL, int S>
+hipError_t launch_ragged(hipStream_t st, double2 *state, const RaggedItem *items, uint32_t nitems, uint32_t steps,
+                         const StepArgs &a, uint32_t max_blocks) {
+	if constexpr (MODEL != TODA)
+		if (a.form == kFormCubic) return launch_ragged_form<MODEL, S, kFormCubic>(st, state, items, nitems, steps, a, max_blocks);
+	if constexpr (MODEL == FPUT)
+		if (a.form == kFormQuad) return launch_ragged_form<MODEL, S, kFormQuad>(st, state, items, nitems, steps, a, max_blocks);
+	return launch_ragged_form<MODEL, S, kFormPlain>(st, state, items, nitems, steps, a, max_blocks);
 }
 #endif
 

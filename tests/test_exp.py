@@ -1,6 +1,7 @@
-"""The two FP64 exp forms of the Toda kernels (nlchains_amd/csrc/nlc_exp.hpp:
-table-driven and table-free), compiled on the host from the same header,
-against e^y computed to 40 significant digits with Python's decimal module."""
+"""The two FP64 exponential forms of the Toda kernels
+(nlchains_amd/csrc/nlc_exp.hpp: table-driven and table-free 2^(x/32)), compiled
+on the host from the same header, against 2^(x/32) computed to 40 significant
+digits with Python's decimal module."""
 import ctypes
 import re
 import subprocess
@@ -17,10 +18,10 @@ WRAPPER = r"""
 #include "nlc_exp.hpp"
 static const double tab[nlc::kExpTabSize] = NLC_EXP2_TABLE;
 extern "C" void exp_many(const double *y, double *out, long n) {
-	for (long i = 0; i < n; ++i) out[i] = nlc::exp_tab(y[i], tab);
+	for (long i = 0; i < n; ++i) out[i] = nlc::exp2_32_tab(y[i], tab);
 }
 extern "C" void exp_poly_many(const double *y, double *out, long n) {
-	for (long i = 0; i < n; ++i) out[i] = nlc::exp_poly(y[i]);
+	for (long i = 0; i < n; ++i) out[i] = nlc::exp2_32_poly(y[i]);
 }
 """
 
@@ -53,11 +54,12 @@ def exp_fn(request, exp_tab):
     return exp_tab[request.param]
 
 
-def ulp_errors(y, got):
+def ulp_errors(x, got):
+    """Error of got = 2^(x/32) in units of the last place of the exact value."""
     getcontext().prec = 40
     errs = []
-    for yi, gi in zip(y, got):
-        ref = Decimal(float(yi)).exp()
+    for xi, gi in zip(x, got):
+        ref = Decimal(2) ** (Decimal(float(xi)) / 32)
         spacing = np.spacing(float(ref))
         errs.append(float(abs(Decimal(float(gi)) - ref) / Decimal(spacing)))
     return np.array(errs)
@@ -74,60 +76,49 @@ def test_table_entries_are_correctly_rounded():
         assert v == float(Decimal(2) ** (Decimal(j) / 32)), j
 
 
-def test_reduction_constants():
+def test_polynomial_coefficients():
+    """Every literal annotated c_k in the header is (ln2/32)^k / k! rounded to
+    nearest, and the shifters are 1.5 * 2^52 and 1.5 * 2^57."""
     getcontext().prec = 60
-    ln2 = Decimal(2).ln()
-    text = HEADER.read_text()
-
-    def const(name):
-        return float.fromhex(re.search(name + r" = (0x[0-9a-f.]+p[+-]\d+)", text).group(1))
-    hi, lo = const("kExpStepHi"), const("kExpStepLo")
-    assert hi == float(ln2 / 32) and lo == float(ln2 / 32 - Decimal(hi))
-    assert const("kExpInvStep") == float(32 / ln2)
-    hi, lo = const("kExpLn2Hi"), const("kExpLn2Lo")
-    assert hi == float(ln2) and lo == float(ln2 - Decimal(hi))
-    assert const("kExpInvLn2") == float(1 / ln2)
-
-
-def test_polynomial_coefficients_are_reciprocal_factorials():
-    """Every literal annotated 1/k! in the header is 1/k! rounded to nearest."""
-    getcontext().prec = 60
+    b = Decimal(2).ln() / 32
     fact = [Decimal(1)]
     for k in range(1, 20):
         fact.append(fact[-1] * k)
+    text = HEADER.read_text()
     checked = 0
-    for line in HEADER.read_text().splitlines():
-        m = re.search(r"//\s*((?:1/\d+!?,?\s*)+)$", line)
+    for line in text.splitlines():
+        m = re.search(r"//\s*(c\d+(?:,\s*c\d+)*)\s*$", line)
         if not m:
             continue
-        ks = [int(x) for x in re.findall(r"1/(\d+)", m.group(1))]
+        ks = [int(x) for x in re.findall(r"c(\d+)", m.group(1))]
         lits = [float.fromhex(h) for h in re.findall(r"0x[0-9a-f.]+p[+-]\d+", line)]
-        # a "1/n" without "!" is the value itself (1/720 = 1/6!, 1/120 = 1/5!, ...)
+        assert len(lits) == len(ks), line
         for k, lit in zip(ks, lits):
-            want = 1 / fact[k] if f"1/{k}!" in m.group(1) else 1 / Decimal(k)
-            assert lit == float(want), (line, k)
+            assert lit == float(b ** k / fact[k]), (line, k)
             checked += 1
-    assert checked >= 15
+    assert checked == 6 + 13
+    assert float.fromhex(re.search(r"kExpShifter = (0x[0-9a-f.]+p[+-]?\d+)", text).group(1)) == 1.5 * 2.0 ** 52
+    assert float.fromhex(re.search(r"kExpShifter32 = (0x[0-9a-f.]+p[+-]?\d+)", text).group(1)) == 1.5 * 2.0 ** 57
 
 
-@pytest.mark.parametrize("lo,hi,count", [(-1.0, 1.0, 4000), (-30.0, 30.0, 4000), (-700.0, 700.0, 2000),
-                                         (-1e-6, 1e-6, 500)])
+# x = 32 y / ln2 for y in (-1, 1), (-30, 30), (-700, 700), (-1e-6, 1e-6)
+@pytest.mark.parametrize("lo,hi,count", [(-46.0, 46.0, 4000), (-1385.0, 1385.0, 4000), (-32300.0, 32300.0, 2000),
+                                         (-4.6e-5, 4.6e-5, 500)])
 def test_error_within_one_ulp(exp_fn, lo, hi, count):
-    y = np.random.default_rng(int(abs(hi) * 10)).uniform(lo, hi, count)
-    err = ulp_errors(y, exp_fn(y))
-    assert err.max() < 1.0, (err.max(), y[err.argmax()])
+    x = np.random.default_rng(int(abs(hi) * 10)).uniform(lo, hi, count)
+    err = ulp_errors(x, exp_fn(x))
+    assert err.max() < 1.0, (err.max(), x[err.argmax()])
 
 
 def test_reduction_boundaries_and_special_values(exp_fn):
-    ln2 = np.log(2.0)
-    k = np.arange(-2000, 2000)
-    # interval ends of both reductions (steps ln2/32 and ln2)
-    y = np.concatenate([k * ln2 / 32, k * ln2 / 32 + ln2 / 64, k * ln2 / 32 - ln2 / 64,
-                        k[::4] * ln2 + 0.4999999 * ln2, k[::4] * ln2 - 0.4999999 * ln2])
-    y = y[np.abs(y) < 700]
-    err = ulp_errors(y, exp_fn(y))
+    k = np.arange(-4000, 4000, dtype=np.float64)
+    # interval ends of both reductions (N = nearest integer to x, resp. to x/32)
+    x = np.concatenate([k, k + 0.5, k - 0.5, np.nextafter(k + 0.5, 0), np.nextafter(k - 0.5, 0),
+                        32 * k[::8] + 15.9999999, 32 * k[::8] - 15.9999999, 32 * k[::8] + 16.0])
+    x = x[np.abs(x) < 32000]
+    err = ulp_errors(x, exp_fn(x))
     assert err.max() < 1.0
-    out = exp_fn(np.array([0.0, -0.0, 709.7, 710.0, 1000.0, -746.0, -1000.0, np.nan]))
+    out = exp_fn(np.array([0.0, -0.0, 32767.0, 32768.0, 50000.0, -34500.0, -50000.0, np.nan]))
     assert out[0] == 1.0 and out[1] == 1.0
     assert np.isfinite(out[2]) and out[3] == np.inf and out[4] == np.inf
     assert out[5] == 0.0 and out[6] == 0.0 and np.isnan(out[7])

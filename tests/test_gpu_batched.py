@@ -44,6 +44,8 @@ def oracle_ragged(model, lengths, s0, steps, dt, kw):
     (0, dict(m=1.0, beta=1.0), [8, 16, 32, 64]),
     (2, dict(alpha=1.0), [70, 100, 300]),  # no persistent shape: one uniform kernel per class
     (1, dict(beta=1.0), [5, 24, 45, 128]),  # short classes: thread per chain, 4 lanes
+    (0, dict(m=1.0, beta=0.3), [32, 128, 1024]),  # scaled cubic form, sigma = sqrt(0.3)
+    (1, dict(alpha=0.2), [64, 256]),  # scaled quadratic form, sigma = 0.2
 ])
 def test_ragged_parity(gpu, model, kw, choices):
     N = gpu

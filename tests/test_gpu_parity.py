@@ -37,11 +37,16 @@ MODEL_KW = {
 }
 
 
-def make_case(N, model_name, n, copies, seed, dens=0.5):
+def make_case(N, model_name, n, copies, seed, dens=0.5, kw=None, plane=False):
+    """Seeded initial state; kw overrides the model parameters, plane=True
+    gives dDNKG plane waves instead of eigenmodes (no eigensystem)."""
     model = N.MODELS[model_name]
-    kw = dict(MODEL_KW[model_name])
+    kw = dict(MODEL_KW[model_name] if kw is None else kw)
     masses = None
-    if model == N.DDNKG:
+    if model == N.DDNKG and plane:
+        masses = N.ic_masses(n, 0.5, 1.5, seed)
+        s = N.ic_plane_waves(n, copies, 1.0, 0, max(1, n // 8), dens, seed)
+    elif model == N.DDNKG:
         masses = N.ic_masses(n, 0.5, 1.5, seed)
         om, V = N.eigensystem(masses + 2)
         s = N.ic_eigenmodes(n, copies, om, V, max(1, n // 8), dens, seed)
@@ -58,9 +63,10 @@ def oracle_move(model, kw, masses, s, steps, dt):
     return O.move_real(model, s, steps, dt, mp2=mp2, threads=1, **okw)
 
 
-def check_move(N, model_name, n, copies, steps=100, dt=0.05, seed=1, **shape):
-    banner(f"move {model_name} n={n} copies={copies} steps={steps} shape={shape}")
-    model, kw, masses, s0 = make_case(N, model_name, n, copies, seed)
+def check_move(N, model_name, n, copies, steps=100, dt=0.05, seed=1, kw=None, dens=0.5, plane=False, expect=None,
+               **shape):
+    banner(f"move {model_name} n={n} copies={copies} steps={steps} kw={kw} shape={shape}")
+    model, kw, masses, s0 = make_case(N, model_name, n, copies, seed, dens=dens, kw=kw, plane=plane)
     with N.Ensemble(model, n, copies, dt, masses=masses, **kw, **shape) as ens:
         ens.upload(s0)
         ens.move(steps)
@@ -71,6 +77,8 @@ def check_move(N, model_name, n, copies, steps=100, dt=0.05, seed=1, **shape):
     banner(f"  {desc}: rel err {err:.3e}")
     assert np.isfinite(got).all()
     assert err <= STATE_TOL, (desc, err)
+    if expect is not None:
+        assert expect in desc, desc
     return got
 
 
@@ -86,6 +94,29 @@ def test_move_default_shape(gpu, model_name, n, copies):
 def test_move_wave_shapes(gpu, n, L, S):
     check_move(gpu, "DNKG", n, 37, steps=60, lanes_per_chain=L, sites_per_lane=S)
     check_move(gpu, "FPUT", n, 37, steps=60, lanes_per_chain=L, sites_per_lane=S)
+
+
+# Parameters that select each force form of the register-resident kernels;
+# the scaled forms hold sigma q, sigma p with sigma = sqrt(beta) (cubic) or
+# alpha (quadratic), != 1 here.  n = 12: thread per chain; 1000 / 1024: wave
+# kernel with ragged / full lanes; 3000 / 4096: workgroup kernel.
+FORM_CASES = [
+    ("DNKG", dict(m=1.0, beta=0.3), "cubic_unit"),
+    ("DNKG", dict(m=0.7, beta=0.0), "plain"),
+    ("DNKG", dict(m=1.0, beta=-0.02), "plain"),
+    ("FPUT", dict(alpha=0.0, beta=2.5), "cubic_unit"),
+    ("FPUT", dict(alpha=0.3, beta=0.0), "quadratic_unit"),
+    ("FPUT", dict(alpha=-0.4, beta=0.0), "quadratic_unit"),
+    ("FPUT", dict(alpha=0.25, beta=0.7), "plain"),
+    ("dDNKG", dict(beta=0.4), "cubic_unit"),
+    ("dDNKG", dict(beta=0.0), "plain"),
+]
+
+
+@pytest.mark.parametrize("model_name,kw,form", FORM_CASES)
+@pytest.mark.parametrize("n", [12, 1000, 1024, 3000, 4096])
+def test_force_forms(gpu, model_name, kw, form, n):
+    check_move(gpu, model_name, n, 6, steps=60, kw=kw, dens=0.1, plane=True, expect=f"force={form}")
 
 
 @pytest.mark.parametrize("n", [65, 100, 1000, 1500, 2047])
