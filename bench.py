@@ -121,7 +121,7 @@ def cpu_baseline(w, s0, seconds_budget: float):
     if steps < 10:  # keep at least 10 steps: shrink the copy count instead
         copies = max(cores, min(copies, int(seconds_budget * rate / (10 * w["n"]))))
         steps = 10
-    steps = min(steps, w["batch"])
+    steps = min(steps, 20 * w["batch"])  # the budget, not one batch, bounds the sample
     t = time.perf_counter()
     move(s0[:copies], steps)
     el = time.perf_counter() - t
@@ -169,7 +169,7 @@ def cpu_baseline_ragged(w, lengths, s0, seconds_budget: float):
     lib = O.fast()
     cores = max(1, min(len(os.sched_getaffinity(0)), 16))
     offs = np.concatenate([[0], np.cumsum(lengths.astype(np.int64))])
-    per_class = max(cores, 64)
+    per_class = max(cores, 256)
     groups = []
     for n in sorted(set(int(x) for x in lengths)):
         idx = np.nonzero(lengths == n)[0][:per_class]
@@ -179,7 +179,7 @@ def cpu_baseline_ragged(w, lengths, s0, seconds_budget: float):
     for g in groups:
         O.move_real(O.TODA, g, 2, w["dt"], alpha=w["alpha"], threads=cores, lib=lib)
     rate = sites * 2 / (time.perf_counter() - t)
-    steps = int(max(4, min(w["batch"], seconds_budget * rate / sites)))
+    steps = int(max(4, min(20 * w["batch"], seconds_budget * rate / sites)))
     t = time.perf_counter()
     for g in groups:
         O.move_real(O.TODA, g, steps, w["dt"], alpha=w["alpha"], threads=cores, lib=lib)
