@@ -224,7 +224,13 @@ int nlc_autotune(nlc_ctx *ctx, uint32_t trial_steps, char *report, size_t len);
 int nlc_sync(nlc_ctx *ctx);
 /* The context's hipStream_t (for events and graph capture by the caller). */
 void *nlc_stream(nlc_ctx *ctx);
-/* Human-readable description of the selected kernels and launch shapes. */
+/* Human-readable description of the selected kernels and launch shapes, as
+ * space-separated key=value fields: model, n, copies, kernel, the shape
+ * (lanes_per_chain, sites_per_lane, exact), for the register-resident real
+ * kernels force=cubic_unit|quadratic_unit|plain (the state is held scaled so
+ * that the leading nonlinear coefficient is 1: sqrt(beta) for DNKG / dDNKG
+ * with beta > 0 and FPUT with alpha = 0, alpha for FPUT with beta = 0; the
+ * results are the reference's up to rounding), and projection= for dDNKG. */
 int nlc_describe(const nlc_ctx *ctx, char *buf, size_t len);
 
 /* ---- host helpers (no device needed) ------------------------------------ */
