@@ -212,7 +212,7 @@ def main():
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print(json.dumps({"error": "nlchains_amd runs on one GPU per process with no sharding; "
-                                   "multi-GPU runs are not part of this build (bench.py --gpus 1)"}))
+                                   "multi-GPU runs are not supported (bench.py --gpus 1)"}))
         sys.exit(2)
 
     import nlchains_amd as N

@@ -40,7 +40,7 @@ extern "C" {
 #define NLC_EARG (-1)        /* invalid argument: std::invalid_argument, common/main.cpp:83-103 */
 #define NLC_EHIP (-2)        /* HIP runtime failure: cuda_error from assertcu, common/utilities_cuda.cuh:19-37 */
 #define NLC_ENOMEM (-3)      /* device or pinned host allocation failed */
-#define NLC_EUNSUPPORTED (-4)/* valid request this build has no kernel for */
+#define NLC_EUNSUPPORTED (-4)/* valid request the library has no kernel for */
 #define NLC_ESTATE (-5)      /* call not valid in the context's current state */
 
 /* Subprograms of the reference registry (common/main.cpp:33-36, DNKG_FPUT_Toda.cpp:156-162,

@@ -311,6 +311,7 @@ int setup_common(nlc_ctx *c) {
 	c->args.a2 = 2 * p.alpha;
 	c->args.a2i = 1 / (2 * p.alpha);
 	c->args.form = nlc::force_form(p.model, p.alpha, p.beta, &c->args.scale);
+	c->args.inv_scale = 1 / c->args.scale;
 	c->a0is0 = (p.model == NLC_DNLS) ? 1 : (p.model == NLC_DDNKG ? 0 : (p.m == 0.0));
 
 	NLC_HIP(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
@@ -626,7 +627,7 @@ int validate_params(const nlc_params *p, bool ragged) {
 	if (!(p->dt > 0)) return reject(NLC_EARG, "--dt must be a positive number");
 	if (p->model == NLC_TODA && p->alpha == 0) return reject(NLC_EARG, "Toda needs --alpha != 0");
 	if (p->model == NLC_DDNKG && !p->masses) return reject(NLC_EARG, "dDNKG needs the masses (-m file content)");
-	if (p->precision != NLC_FP64) return reject(NLC_EUNSUPPORTED, "only FP64 precision is available in this build");
+	if (p->precision != NLC_FP64) return reject(NLC_EUNSUPPORTED, "only FP64 precision (NLC_FP64) is implemented");
 	if (p->kernel < NLC_KERNEL_AUTO || p->kernel > NLC_KERNEL_SPLIT) return reject(NLC_EARG, "unknown kernel family");
 	if (p->projection < NLC_PROJ_AUTO || p->projection > NLC_PROJ_SPLIT_BF16)
 		return reject(NLC_EARG, "unknown projection form");

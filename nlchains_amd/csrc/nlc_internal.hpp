@@ -24,6 +24,7 @@ struct StepArgs {
 	double a2;    // Toda: 2*alpha
 	double a2i;   // Toda: 1/(2*alpha)
 	double scale; // sigma of the scaled force forms (see force_form)
+	double inv_scale; // 1 / sigma
 	int form;     // kFormPlain, kFormCubic or kFormQuad
 };
 

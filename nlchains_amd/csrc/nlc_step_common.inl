@@ -98,7 +98,9 @@ __device__ __forceinline__ double bond_force(double x, const StepArgs &a, const 
 // The state as the kernels hold it: sigma q and sigma p for the scaled forms
 // (then sigma F(q) = F~(sigma q), so drift and kick weights stay as they
 // are), v = (64 alpha/ln2) q with unscaled p for Toda (sigma = 64 alpha/ln2),
-// else q and p.
+// else q and p.  The way back multiplies by 1/sigma: a division would put its
+// 10-instruction sequence between the stores of a lane, and the L2 then writes
+// partly filled lines back (a third more HBM write traffic measured).
 template <int MODEL, int FORM>
 __device__ __forceinline__ double2 to_registers(double2 v, const StepArgs &a) {
 	if constexpr (MODEL == TODA) return make_double2(a.scale * v.x, v.y);
@@ -108,8 +110,8 @@ __device__ __forceinline__ double2 to_registers(double2 v, const StepArgs &a) {
 
 template <int MODEL, int FORM>
 __device__ __forceinline__ double2 from_registers(double q, double p, const StepArgs &a) {
-	if constexpr (MODEL == TODA) return make_double2(q / a.scale, p);
-	else if constexpr (FORM != kFormPlain) return make_double2(q / a.scale, p / a.scale);
+	if constexpr (MODEL == TODA) return make_double2(q * a.inv_scale, p);
+	else if constexpr (FORM != kFormPlain) return make_double2(q * a.inv_scale, p * a.inv_scale);
 	else return make_double2(q, p);
 }
 

@@ -341,7 +341,7 @@ int main_ddnkg(int argc, char **argv) {
 }
 
 // DNLS: DNLS.cpp:30-41.  The cuFFT-callback switches are accepted for
-// compatibility; this build has no callbacks (the FFTs are fused on chip).
+// compatibility; the library has no callbacks (the FFTs are fused on chip).
 int main_dnls(int argc, char **argv) {
 	Options o(std::string("Options for ") + argv[0]);
 	add_common(o);
