@@ -214,13 +214,23 @@ bool default_shape(uint32_t n, int &L, int &S, bool &exact) {
 // shape at 4096 for every model, within 12 % of the best elsewhere).
 bool group_default(int model, uint32_t n, int &L, int &S, bool &exact) {
 	(void) model;
-	for (int smax : {16, 32})
-		for (int W = 2; W <= 8; ++W) {
+	// 2, 4 or 8 wavefronts spread evenly over the 4 SIMDs of a CU; with 3, 5, 6
+	// or 7 every barrier waits for a SIMD that runs two of the workgroup's
+	// wavefronts (measured 10-45 % slower).  The fewest such wavefronts with at
+	// most 24 sites per lane, else 32; other counts only as a last resort.
+	for (int smax : {24, 32})
+		for (int W : {2, 4, 8}) {
 			const int s = int((n + 64u * W - 1) / (64u * W));
 			if (s > smax || !group_shape_ok(n, 64 * W, s)) continue;
 			L = 64 * W, S = s, exact = uint64_t(L) * S == n;
 			return true;
 		}
+	for (int W = 3; W <= 7; ++W) {
+		const int s = int((n + 64u * W - 1) / (64u * W));
+		if (!group_shape_ok(n, 64 * W, s)) continue;
+		L = 64 * W, S = s, exact = uint64_t(L) * S == n;
+		return true;
+	}
 	return false;
 }
 

@@ -153,14 +153,14 @@ def test_move_short_chains(gpu, model_name, n):
     assert np.isfinite(got).all() and err <= STATE_TOL, (desc, err)
 
 
-GROUP_CASES = [(m, n, c) for n, c in [(2500, 5), (3001, 3), (4096, 4), (8192, 3), (16384, 2)]
+GROUP_CASES = [(m, n, c) for n, c in [(2500, 5), (3001, 3), (4096, 4), (6000, 3), (8192, 3), (16384, 2)]
                for m in ("DNKG", "FPUT", "Toda", "dDNKG") if m != "dDNKG" or n <= 4096]
 
 
 @pytest.mark.parametrize("model_name,n,copies", GROUP_CASES)
 def test_move_group_default(gpu, model_name, n, copies):
     """Chains longer than a wavefront holds (n > 2048) run on the
-    workgroup-resident kernel by default."""
+    workgroup-resident kernel by default, with 2, 4 or 8 wavefronts per chain."""
     banner(f"group move {model_name} n={n} copies={copies}")
     model, kw, masses, s0 = make_case(gpu, model_name, n, copies, 4)
     with gpu.Ensemble(model, n, copies, 0.05, masses=masses, **kw) as ens:
@@ -172,6 +172,7 @@ def test_move_group_default(gpu, model_name, n, copies):
     err = rel_err(got, want)
     banner(f"  {desc}: rel err {err:.3e}")
     assert "kernel=group" in desc
+    assert any(f"lanes_per_chain={L} " in desc for L in (128, 256, 512)), desc
     assert np.isfinite(got).all() and err <= STATE_TOL, (desc, err)
 
 
