@@ -163,13 +163,21 @@ nlc::StepLauncher lookup_group(int model, int S, bool exact) {
 
 nlc::StepLauncher lookup(int model, int L, int S, bool exact) {
 	if (L > 64) return lookup_group(model, S, exact);
+	nlc::StepLauncher (*f)(int, int, bool) = nullptr;
+	nlc::StepLauncher (*f16)(int) = nullptr;
 	switch (model) {
-	case NLC_DNKG: return nlc::lookup_step_dnkg(L, S, exact);
-	case NLC_FPUT: return nlc::lookup_step_fput(L, S, exact);
-	case NLC_TODA: return nlc::lookup_step_toda(L, S, exact);
-	case NLC_DDNKG: return nlc::lookup_step_ddnkg(L, S, exact);
+	case NLC_DNKG: f = nlc::lookup_step_dnkg, f16 = nlc::lookup_step16_dnkg; break;
+	case NLC_FPUT: f = nlc::lookup_step_fput, f16 = nlc::lookup_step16_fput; break;
+	case NLC_TODA: f = nlc::lookup_step_toda, f16 = nlc::lookup_step16_toda; break;
+	case NLC_DDNKG: f = nlc::lookup_step_ddnkg, f16 = nlc::lookup_step16_ddnkg; break;
+	default: return nullptr;
 	}
-	return nullptr;
+	// an exact fit without its own instance (n = 64 S, 16 S or 4 S with S not a
+	// power of two) runs on the ragged-lane one, whose lanes are then all full
+	nlc::StepLauncher l = f(L, S, exact);
+	if (!l && exact && S >= 2) l = f(L, S, false);
+	if (!l && L == 16) l = f16(S);
+	return l;
 }
 
 nlc::RaggedLauncher lookup_ragged(int model, int S) {
@@ -181,37 +189,49 @@ nlc::RaggedLauncher lookup_ragged(int model, int S) {
 	return nullptr;
 }
 
-// Default wave shape for a chain length: one chain per wavefront (L = 64) with
-// ceil(n/64) sites per lane when n >= 64; for shorter power-of-two chains
-// several chains share a wavefront with 8 sites per lane (thread-per-chain
-// below 8 sites).
-bool default_shape(uint32_t n, int &L, int &S, bool &exact) {
+// Default wave shape of a chain of n <= 2048 sites in an ensemble of `copies`
+// chains: 64 lanes from 64 sites up, except the powers of two 64..256, which
+// take S = 16 sites per lane (on 2^24-site ensembles 4 x 16 runs 38 % faster
+// than 64 x 1 at 64 sites, 16 x 16 6 % faster than 64 x 4 at 256), and the
+// other lengths 65..384, which take 16 lanes; powers of two below 64 take
+// S = min(n, 8).  A small ensemble trades sites per lane for lanes (L doubles,
+// S halves; 16 lanes become 64) until it has 4096 wavefronts or L = 64: then
+// the number of wavefronts, not their throughput, limits the step.
+bool default_shape(uint32_t n, uint32_t copies, int &L, int &S, bool &exact) {
+	if (is_pow2(n) && n <= 256) {
+		S = int(std::min<uint32_t>(n, n >= 64 ? 16 : 8));
+		L = int(n) / S;
+		while (L < 64 && S > 1 && uint64_t(copies) * L < 4096ull * 64) L *= 2, S /= 2;
+		exact = true;
+		return true;
+	}
+	if (n > 64 && n <= 384 && uint64_t(copies) * 16 >= 4096ull * 64) {
+		// other lengths up to 384: 16 lanes (a DPP row) with ragged lanes, four
+		// chains to a wavefront (1.5x the rate of 64 x 2 at 100 sites, 1.13x of
+		// 64 x 5 at 300; from 24 sites per lane on, 64 lanes are as fast or faster)
+		S = int((n + 15) / 16);
+		L = 16;
+		exact = (uint32_t(S) * 16 == n);
+		return true;
+	}
 	if (n >= 64) {
 		S = int((n + 63) / 64);
 		L = 64;
 		exact = (uint32_t(S) * 64 == n);
 		return S <= 32;
 	}
-	if (!is_pow2(n)) {
-		// short chains of other lengths: a thread per chain up to 32 sites,
-		// else 4 lanes with 9..16 sites (the last lanes one fewer)
-		if (n <= 32) {
-			L = 1, S = int(n), exact = true;
-		} else {
-			L = 4, S = int((n + 3) / 4), exact = false;
-		}
-		return true;
+	// short chains of other lengths: a thread per chain up to 32 sites, else 4
+	// lanes with 9..16 sites (the last lanes one fewer)
+	if (n <= 32) {
+		L = 1, S = int(n), exact = true;
+	} else {
+		L = 4, S = int((n + 3) / 4), exact = false;
 	}
-	S = int(std::min<uint32_t>(n, 8));
-	L = int(n) / S;
-	exact = true;
 	return true;
 }
 
-// Chains longer than a wavefront holds: the workgroup shape with the fewest
-// wavefronts that keeps S <= 16 sites per lane, else S <= 32 (measured on
-// 2^24-site ensembles at 3000 to 16384 sites per chain: S = 16 is the best
-// shape at 4096 for every model, within 12 % of the best elsewhere).
+// Chains longer than a wavefront holds: the workgroup shape (measured on
+// 2^24-site ensembles at 2500 to 16384 sites per chain).
 bool group_default(int model, uint32_t n, int &L, int &S, bool &exact) {
 	(void) model;
 	// 2, 4 or 8 wavefronts spread evenly over the 4 SIMDs of a CU; with 3, 5, 6
@@ -474,7 +494,7 @@ int select_uniform(nlc_ctx *c) {
 			       group_shape_ok(n, L, S);
 			if (!have) return c->fail(NLC_EARG, "wave shape %d lanes x %d sites cannot hold chain_length %u", L, S, n);
 		} else {
-			have = default_shape(n, L, S, exact) || group_default(p.model, n, L, S, exact);
+			have = default_shape(n, c->copies, L, S, exact) || group_default(p.model, n, L, S, exact);
 		}
 		c->launcher = have ? lookup(p.model, L, S, exact) : nullptr;
 		if (c->launcher) {
@@ -520,7 +540,7 @@ int select_ragged(nlc_ctx *c, int S) {
 	c->ragged_S = 0;
 	c->nitems = 0;
 	for (auto &k : c->cls) {
-		if (!default_shape(k.n, k.L, k.S, k.exact) || !(k.launcher = lookup(c->p.model, k.L, k.S, k.exact)))
+		if (!default_shape(k.n, k.count, k.L, k.S, k.exact) || !(k.launcher = lookup(c->p.model, k.L, k.S, k.exact)))
 			return c->fail(NLC_EUNSUPPORTED, "ragged ensembles need chain lengths with a wave shape (got %u)", k.n);
 	}
 	return NLC_OK;
@@ -1093,7 +1113,7 @@ int nlc_autotune(nlc_ctx *c, uint32_t trial_steps, char *report, size_t len) {
 		for (const auto &k : c->cls) {
 			int L, S;
 			bool exact;
-			per_class = per_class && default_shape(k.n, L, S, exact) && lookup(c->p.model, L, S, exact);
+			per_class = per_class && default_shape(k.n, k.count, L, S, exact) && lookup(c->p.model, L, S, exact);
 		}
 		if (per_class) cands.push_back({0, 0, false, false, 0, 0, 0});
 	} else {

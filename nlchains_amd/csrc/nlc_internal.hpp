@@ -62,6 +62,11 @@ StepLauncher lookup_step_dnkg(int L, int S, bool exact);
 StepLauncher lookup_step_fput(int L, int S, bool exact);
 StepLauncher lookup_step_toda(int L, int S, bool exact);
 StepLauncher lookup_step_ddnkg(int L, int S, bool exact);
+// 16 lanes with S = 5..32 sites per lane for any n in (16(S-1), 16S] (ragged lanes).
+StepLauncher lookup_step16_dnkg(int S);
+StepLauncher lookup_step16_fput(int S);
+StepLauncher lookup_step16_toda(int S);
+StepLauncher lookup_step16_ddnkg(int S);
 
 // Workgroup shapes for long chains: one chain per workgroup of W = 2..8
 // wavefronts (L = 64 W lanes), S sites per lane, W = ceil(n / (64 S)) and

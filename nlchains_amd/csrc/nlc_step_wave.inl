@@ -1,7 +1,8 @@
 // Wave-resident Yoshida-6 step kernels for the real models (gfx950, wave64).
 //
 // Included once per model by nlc_step_<model>.hip with NLC_STEP_MODEL and
-// NLC_STEP_LOOKUP defined, so the models compile in parallel.
+// NLC_STEP_LOOKUP defined, and by nlc_step16_<model>.hip with
+// NLC_STEP_LOOKUP16 (the 16-lane ragged shapes), so they compile in parallel.
 //
 // Reference behaviour followed: DNKG_FPUT_Toda.cu:82-147 (move_chain_in_warp),
 // :156-190 (move_chain_in_thread), dDNKG.cu:25-128; forces DNKG_FPUT_Toda.cu:21-74
@@ -254,10 +255,12 @@ RaggedLauncher NLC_STEP_RAGGED(int S) {
 }
 #endif
 
+#ifdef NLC_STEP_LOOKUP
 // Shapes instantiated: every power-of-two (L, S) with 2 <= L*S <= 2048, S <= 32
 // (exact fit), plus L = 64 with S = 2..32 for any n in (64(S-1), 64S]; for
 // short chains whose length is not a power of two, L = 1 with S = n <= 32
-// (thread per chain) and L = 4 with S = 9..16 for n in 33..63.
+// (thread per chain) and L = 4 with S = 9..16 for n in 33..63.  (L = 16 with
+// S = 5..32, for n in 65..512, is compiled in its own translation unit.)
 StepLauncher NLC_STEP_LOOKUP(int L, int S, bool exact) {
 #define NLC_EXACT(LL, SS)                                                                                          \
 	if (exact && L == LL && S == SS) return &launch_wave<NLC_STEP_MODEL, LL, SS, true>;
@@ -290,5 +293,24 @@ StepLauncher NLC_STEP_LOOKUP(int L, int S, bool exact) {
 #undef NLC_RAGGED4
 	return nullptr;
 }
+#endif
+
+#ifdef NLC_STEP_LOOKUP16
+// L = 16 lanes (one DPP row) with S = 5..32 sites per lane for any n in
+// (16(S-1), 16S]: chains of 65..512 sites whose length is not a power of two,
+// four to a wavefront.
+StepLauncher NLC_STEP_LOOKUP16(int S) {
+	switch (S) {
+#define NLC_R16(SS)                                                                                                \
+	case SS: return &launch_wave<NLC_STEP_MODEL, 16, SS, false>;
+		NLC_R16(5) NLC_R16(6) NLC_R16(7) NLC_R16(8) NLC_R16(9) NLC_R16(10) NLC_R16(11) NLC_R16(12) NLC_R16(13)
+		NLC_R16(14) NLC_R16(15) NLC_R16(16) NLC_R16(17) NLC_R16(18) NLC_R16(19) NLC_R16(20) NLC_R16(21)
+		NLC_R16(22) NLC_R16(23) NLC_R16(24) NLC_R16(25) NLC_R16(26) NLC_R16(27) NLC_R16(28) NLC_R16(29)
+		NLC_R16(30) NLC_R16(31) NLC_R16(32)
+#undef NLC_R16
+	}
+	return nullptr;
+}
+#endif
 
 } // namespace nlc

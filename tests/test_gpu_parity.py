@@ -204,6 +204,72 @@ def test_autotune_includes_group_shapes(gpu):
     assert rel_err(got[:4], want) <= STATE_TOL
 
 
+@pytest.mark.parametrize("n", [36, 40, 60, 192, 960, 1984])
+def test_exact_multiples_stay_on_the_wave_kernel(gpu, n):
+    """Lengths n = 4 S or 64 S with S not a power of two have no exact-fit
+    instance; they run on the ragged-lane one with every lane full, by default
+    and among the autotuner's candidates."""
+    N = gpu
+    model, kw, masses, s0 = make_case(N, "DNKG", n, 24, 5)
+    with N.Ensemble(model, n, 24, 0.05, **kw) as ens:
+        assert "kernel=wave" in ens.describe(), ens.describe()
+        ens.upload(s0)
+        report = ens.autotune(5)
+        desc = ens.describe()
+        banner(f"n={n}: {report} -> {desc}")
+        L = 4 if n < 64 else 64
+        assert f" {L}x{n // L} " in report, report  # a candidate of the autotuner
+        assert "kernel=split" not in desc, (report, desc)
+        ens.move(40)
+        got = ens.download()
+    want = oracle_move(model, kw, masses, s0, 40, 0.05)
+    assert rel_err(got, want) <= STATE_TOL
+
+
+@pytest.mark.parametrize("n,copies,shape", [(64, 65536, (4, 16)), (128, 32768, (8, 16)), (256, 16384, (16, 16)),
+                                            (64, 96, (64, 1)), (64, 4096, (64, 1)), (128, 2048, (64, 2)),
+                                            (256, 8192, (32, 8)), (32, 65536, (4, 8)), (32, 8192, (32, 1)),
+                                            (16, 96, (16, 1))])
+def test_default_shape_short_powers_of_two(gpu, n, copies, shape):
+    """Powers of two 64..256 default to 16 sites per lane, traded for lanes
+    while the ensemble has fewer than 4096 wavefronts."""
+    N = gpu
+    model, kw, masses, s0 = make_case(N, "FPUT", n, copies, 6)
+    with N.Ensemble(model, n, copies, 0.05, **kw) as ens:
+        desc = ens.describe()
+        assert f"lanes_per_chain={shape[0]} sites_per_lane={shape[1]} " in desc, desc
+        ens.upload(s0)
+        ens.move(20)
+        got = ens.download()
+    want = oracle_move(model, kw, masses, s0[:16], 20, 0.05)
+    assert rel_err(got[:16], want) <= STATE_TOL
+
+
+@pytest.mark.parametrize("model_name", ["DNKG", "FPUT", "Toda", "dDNKG"])
+@pytest.mark.parametrize("n", [65, 80, 100, 333, 511])
+def test_move_16_lane_ragged(gpu, model_name, n):
+    """16 lanes per chain (four chains to a wavefront, 9 copies: the last
+    wavefront partly filled) with S = ceil(n/16) sites per lane."""
+    check_move(gpu, model_name, n, 9, steps=40, lanes_per_chain=16, sites_per_lane=-(-n // 16))
+
+
+@pytest.mark.parametrize("n,copies,shape", [(300, 16384, (16, 19)), (100, 20000, (16, 7)), (300, 100, (64, 5)),
+                                            (500, 16384, (64, 8))])
+def test_default_shape_16_lanes(gpu, n, copies, shape):
+    """Lengths 65..384 that are not powers of two default to 16 lanes when the
+    ensemble fills 4096 wavefronts, else (and above 384 sites) to 64 lanes."""
+    N = gpu
+    model, kw, masses, s0 = make_case(N, "DNKG", n, copies, 7)
+    with N.Ensemble(model, n, copies, 0.05, **kw) as ens:
+        desc = ens.describe()
+        assert f"lanes_per_chain={shape[0]} sites_per_lane={shape[1]} exact=0" in desc, desc
+        ens.upload(s0)
+        ens.move(20)
+        got = ens.download()
+    want = oracle_move(model, kw, masses, s0[-8:], 20, 0.05)
+    assert rel_err(got[-8:], want) <= STATE_TOL
+
+
 def test_move_single_copy_and_single_step(gpu):
     check_move(gpu, "DNKG", 1024, 1, steps=1)
     check_move(gpu, "FPUT", 64, 1, steps=1)
