@@ -399,7 +399,8 @@ int setup_common(nlc_ctx *c) {
 	NLC_HIP(c, hipMemcpy(c->omega, omega.data(), sizeof(double) * c->table_len, hipMemcpyHostToDevice));
 	NLC_TRY(dev_alloc(c, &c->partial, partial_len, "energy partials"));
 	if (c->fft_scratch_sites) NLC_TRY(dev_alloc(c, &c->fft_scratch, c->fft_scratch_sites, "FFT scratch"));
-	if (const uint32_t L = nlc::bluestein_two_level_size(c->n); L && !c->ragged && p.model != NLC_DDNKG) {
+	if (const uint32_t L = nlc::bluestein_two_level_size(c->n);
+	    L && !c->ragged && p.model != NLC_DDNKG && !nlc::mixed_radix_plan(c->n, nullptr)) {
 		std::vector<double2> blue(size_t(2) * L + c->n);
 		nlc::bluestein_two_level_tables(c->n, blue.data());
 		NLC_TRY(dev_alloc(c, &c->blue_table, blue.size(), "Bluestein tables"));
@@ -436,7 +437,7 @@ int setup_dnls(nlc_ctx *c) {
 	if (const uint32_t M = nlc::dnls_bluestein_size(n)) {
 		tab.resize(size_t(7) * n + size_t(2) * M + n);
 		nlc::dnls_bluestein_tables(n, tab.data() + size_t(7) * n);
-	} else if (const uint32_t L = nlc::bluestein_two_level_size(n)) {
+	} else if (const uint32_t L = nlc::bluestein_two_level_size(n); L && !nlc::mixed_radix_plan(n, nullptr)) {
 		// Bluestein through the two-level FFT: the 7 natural tables (the
 		// Bluestein tables are shared with the energy pass) and L values of
 		// scratch per chain
@@ -1207,7 +1208,8 @@ int nlc_describe(const nlc_ctx *c, char *buf, size_t len) {
 	static const char *names[] = {"DNKG", "FPUT", "Toda", "dDNKG", "DNLS"};
 	if (c->p.model == NLC_DNLS)
 		snprintf(buf, len, "model=DNLS n=%u copies=%u kernel=%s", c->n, c->copies,
-		         nlc::bluestein_two_level_size(c->n)       ? "dnls_bluestein_two_level"
+		         nlc::mixed_radix_plan(c->n, nullptr)       ? "dnls_lds_mixed_radix"
+		         : nlc::bluestein_two_level_size(c->n)     ? "dnls_bluestein_two_level"
 		         : nlc::dnls_two_level(c->n)               ? "dnls_two_level_fft"
 		         : nlc::dnls_bluestein_size(c->n)          ? "dnls_lds_bluestein"
 		         : !nlc::dnls_fast_path(c->n)              ? "dnls_lds_stockham"

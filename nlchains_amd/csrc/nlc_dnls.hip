@@ -490,8 +490,8 @@ k_dnls_4096_r8(double2 *__restrict__ state, uint32_t copies, uint32_t steps, Dnl
 bool dnls_supported(uint32_t n) {
 	uint32_t R, M;
 	const bool pow2 = n >= 2 && (n & (n - 1)) == 0;
-	return (pow2 && n <= kDnlsLdsMax) || dnls_bluestein_size(n) != 0 || large_fft_split(n, R, M) ||
-	       bluestein_two_level_size(n) != 0;
+	return (pow2 && n <= kDnlsLdsMax) || mixed_radix_plan(n, nullptr) || dnls_bluestein_size(n) != 0 ||
+	       large_fft_split(n, R, M) || bluestein_two_level_size(n) != 0;
 }
 
 // Fourier multiplier tables (host side).  table[k][j] = T_k[j] for the generic
@@ -521,7 +521,7 @@ hipError_t launch_dnls(hipStream_t st, double2 *state, uint32_t n, uint32_t copi
                        const double2 *blue) {
 	if (!dnls_fast_path(n)) {
 		uint32_t R, M;
-		if (bluestein_two_level_size(n) || large_fft_split(n, R, M))
+		if (!mixed_radix_plan(n, nullptr) && (bluestein_two_level_size(n) || large_fft_split(n, R, M)))
 			return launch_dnls_long(st, state, n, copies, steps, theta9, tw, table, scratch, blue);
 		return launch_dnls_generic(st, state, n, copies, steps, theta9, tw, table);
 	}

@@ -5,12 +5,16 @@
 //
 //  * k_dnls_lds: any power-of-two n <= 8192, chain in LDS, radix-4 Stockham
 //    passes (the inverse transform as conj(FFT(conj(.)))).
+//  * k_dnls_mixed: the other n <= 8192 whose prime factors are 2, 3, 5 and 7
+//    (3000, 5000, 6000, ...), the same with the mixed-radix Stockham passes of
+//    nlc_fft_mixed.hpp.
 //  * k_dnls_bluestein: any other n <= 4096.  Each n-point DFT is Bluestein's
 //    chirp-z convolution (L. Bluestein, 1970): with w_j = e^{-i pi j^2/n},
 //    X_k = w_k sum_x (w_x psi_x) conj(w_{k-x}), a cyclic convolution of length
 //    M >= 2n - 1 (a power of two) done as two M-point FFTs in LDS.
 #include "nlc_internal.hpp"
 #include "nlc_fft_large.hpp"
+#include "nlc_fft_mixed.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -22,15 +26,13 @@ namespace {
 
 #include "nlc_dnls_common.inl"
 
-// Chain in LDS.  Between substeps the buffer holds conj(psi): the inverse
-// transform is conj(FFT(conj(X T))).
-template <uint32_t MAXN>
-__global__ void __launch_bounds__(kT)
-k_dnls_lds(double2 *__restrict__ state, uint32_t n, uint32_t copies, uint32_t steps, DnlsArgs a,
-           const double2 *__restrict__ tw, const double2 *__restrict__ table) {
-	extern __shared__ double2 buf[];
-	const uint32_t chain = blockIdx.x, tid = threadIdx.x;
-	double2 *psi = state + size_t(chain) * n;
+// One batch of steps on a chain held in LDS; fft(buf) is the forward n-point
+// FFT in place, natural order in and out.  Between substeps the buffer holds
+// conj(psi): the inverse transform is conj(FFT(conj(X T))).
+template <class Fft>
+__device__ __forceinline__ void dnls_lds_batch(double2 *buf, double2 *psi, uint32_t n, uint32_t steps,
+                                               const DnlsArgs &a, const double2 *__restrict__ table, Fft fft) {
+	const uint32_t tid = threadIdx.x;
 	for (uint32_t x = tid; x < n; x += kT) buf[x] = psi[x];
 	bool conj_state = false;
 	__syncthreads();
@@ -42,11 +44,11 @@ k_dnls_lds(double2 *__restrict__ state, uint32_t n, uint32_t copies, uint32_t st
 				buf[x] = nonlinear(p, th);
 			}
 			__syncthreads();
-			lds_fft_fwd<MAXN>(buf, n, tw);
+			fft(buf);
 			const double2 *tk = table + size_t(k) * n;
 			for (uint32_t j = tid; j < n; j += kT) buf[j] = cconj(cmul(buf[j], tk[j]));
 			__syncthreads();
-			lds_fft_fwd<MAXN>(buf, n, tw);
+			fft(buf);
 			conj_state = true;
 		}
 	}
@@ -54,6 +56,24 @@ k_dnls_lds(double2 *__restrict__ state, uint32_t n, uint32_t copies, uint32_t st
 		const double2 p = conj_state ? cconj(buf[x]) : buf[x];
 		psi[x] = nonlinear(p, a.theta[7]);
 	}
+}
+
+template <uint32_t MAXN>
+__global__ void __launch_bounds__(kT)
+k_dnls_lds(double2 *__restrict__ state, uint32_t n, uint32_t copies, uint32_t steps, DnlsArgs a,
+           const double2 *__restrict__ tw, const double2 *__restrict__ table) {
+	extern __shared__ double2 buf[];
+	dnls_lds_batch(buf, state + size_t(blockIdx.x) * n, n, steps, a, table,
+	               [&](double2 *b) { lds_fft_fwd<MAXN>(b, n, tw); });
+}
+
+template <uint32_t MAXN>
+__global__ void __launch_bounds__(kT)
+k_dnls_mixed(double2 *__restrict__ state, uint32_t n, uint32_t copies, uint32_t steps, DnlsArgs a, MixedPlan plan,
+             const double2 *__restrict__ tw, const double2 *__restrict__ table) {
+	extern __shared__ double2 buf[];
+	dnls_lds_batch(buf, state + size_t(blockIdx.x) * n, n, steps, a, table,
+	               [&](double2 *b) { mixed::lds_fft<MAXN, kT>(b, n, tw, plan); });
 }
 
 // Cyclic convolution of buf[0..M) with b_j = conj(w_j) (|j| < n, zero
@@ -136,7 +156,7 @@ void host_fft(std::vector<double2> &v) {
 } // namespace
 
 uint32_t dnls_bluestein_size(uint32_t n) {
-	if (n < 2 || pow2(n)) return 0;
+	if (n < 2 || pow2(n) || mixed_radix_plan(n, nullptr)) return 0;
 	uint32_t M = 1;
 	while (M < 2 * n - 1) M <<= 1;
 	return M <= kDnlsBluesteinMax ? M : 0;
@@ -181,6 +201,17 @@ hipError_t launch_dnls_generic(hipStream_t st, double2 *state, uint32_t n, uint3
 			if (e != hipSuccess) return e;
 			hipLaunchKernelGGL(k_dnls_lds<kDnlsLdsMax>, dim3(copies), dim3(kT), lds, st, state, n, copies, steps, a, tw,
 			                   table);
+		}
+	} else if (MixedPlan plan; mixed_radix_plan(n, &plan)) {
+		const size_t lds = sizeof(double2) * n;
+		if (n <= 4096) {
+			hipLaunchKernelGGL(k_dnls_mixed<4096>, dim3(copies), dim3(kT), lds, st, state, n, copies, steps, a, plan, tw,
+			                   table);
+		} else {
+			hipError_t e = launch_lds_chain_kernel(k_dnls_mixed<kDnlsLdsMax>, lds);
+			if (e != hipSuccess) return e;
+			hipLaunchKernelGGL(k_dnls_mixed<kDnlsLdsMax>, dim3(copies), dim3(kT), lds, st, state, n, copies, steps, a,
+			                   plan, tw, table);
 		}
 	} else if (const uint32_t M = dnls_bluestein_size(n)) {
 		const size_t lds = sizeof(double2) * M;

@@ -22,6 +22,7 @@
 // a fixed order, so results are bit-reproducible run to run.
 #include "nlc_internal.hpp"
 #include "nlc_fft_large.hpp"
+#include "nlc_fft_mixed.hpp"
 
 #include <algorithm>
 
@@ -131,10 +132,11 @@ __device__ void lds_dft(const double2 *buf, double2 *out_k, uint32_t k, uint32_t
 	*out_k = make_double2(sr, si);
 }
 
+// use_fft: 1 = power-of-two FFT, 2 = mixed-radix FFT through `plan`, 0 = direct DFT.
 template <bool DNLS_MODE, uint32_t MAXN>
 __global__ void __launch_bounds__(kThreads)
 k_energy(const double2 *__restrict__ state, uint32_t n, uint32_t copies, const double2 *__restrict__ tw,
-         const double *__restrict__ omega, double *__restrict__ partial, int use_fft) {
+         const double *__restrict__ omega, double *__restrict__ partial, int use_fft, MixedPlan plan) {
 	extern __shared__ double2 lds[]; // n (FFT) or 2n (DFT) entries
 	constexpr int kMaxModes = MAXN / kThreads;
 	double acc[kMaxModes];
@@ -148,7 +150,8 @@ k_energy(const double2 *__restrict__ state, uint32_t n, uint32_t copies, const d
 		for (uint32_t x = tid; x < n; x += kThreads) lds[x] = chain[x];
 		__syncthreads();
 		if (use_fft) {
-			lds_fft<MAXN>(lds, n, tw);
+			if (use_fft == 2) mixed::lds_fft<MAXN, kThreads>(lds, n, tw, plan);
+			else lds_fft<MAXN>(lds, n, tw);
 #pragma unroll
 			for (int t = 0; t < kMaxModes; ++t) {
 				const uint32_t k = tid + t * kThreads;
@@ -157,7 +160,7 @@ k_energy(const double2 *__restrict__ state, uint32_t n, uint32_t copies, const d
 					if constexpr (DNLS_MODE) {
 						acc[t] += z.x * z.x + z.y * z.y;
 					} else {
-						const double2 zm = lds[(n - k) & (n - 1)];
+						const double2 zm = lds[k ? n - k : 0];
 						const double w = omega[k];
 						const double a = 1.0 + w, b = 1.0 - w;
 						const double er = fma(a, z.x, -b * zm.x); // Re[(1+w)Z - (1-w)conj(Zm)]
@@ -541,7 +544,8 @@ bool is_pow2(uint32_t n) { return n >= 2 && (n & (n - 1)) == 0; }
 
 template <bool DNLS_MODE, uint32_t MAXN>
 hipError_t launch_lds_energy(hipStream_t st, const double2 *state, uint32_t n, uint32_t copies, const double2 *tw,
-                             const double *omega, double *partial, uint32_t grid, int use_fft) {
+                             const double *omega, double *partial, uint32_t grid, int use_fft,
+                             const MixedPlan &plan = MixedPlan{}) {
 	const size_t lds = sizeof(double2) * n * (use_fft ? 1 : 2);
 	if (lds > 64 * 1024) {
 		hipError_t e = hipFuncSetAttribute((const void *) k_energy<DNLS_MODE, MAXN>,
@@ -549,7 +553,7 @@ hipError_t launch_lds_energy(hipStream_t st, const double2 *state, uint32_t n, u
 		if (e != hipSuccess) return e;
 	}
 	hipLaunchKernelGGL((k_energy<DNLS_MODE, MAXN>), dim3(grid), dim3(kThreads), lds, st, state, n, copies, tw, omega,
-	                   partial, use_fft);
+	                   partial, use_fft, plan);
 	return hipGetLastError();
 }
 
@@ -590,6 +594,11 @@ hipError_t launch_energy_mode(hipStream_t st, const double2 *state, uint32_t n, 
                               const double *omega, double *partial, uint32_t grid, const LongChainFft &lf) {
 	double2 *scratch = lf.scratch;
 	const size_t scratch_sites = lf.scratch_sites;
+	if (MixedPlan plan; mixed_radix_plan(n, &plan)) // n <= 8192 with prime factors 2, 3, 5, 7
+		return n <= kEnergyFftMax
+		           ? launch_lds_energy<DNLS_MODE, kEnergyFftMax>(st, state, n, copies, tw, omega, partial, grid, 2, plan)
+		           : launch_lds_energy<DNLS_MODE, kEnergyFftLarge>(st, state, n, copies, tw, omega, partial, grid, 2,
+		                                                          plan);
 	if (lf.blue && scratch && bluestein_two_level_size(n))
 		return launch_energy_bluestein(st, DNLS_MODE, state, n, copies, lf.blue, omega, partial, grid, scratch,
 		                               scratch_sites);
@@ -637,7 +646,7 @@ uint32_t energy_rows(uint32_t n, uint32_t copies) {
 size_t energy_scratch_sites(uint32_t n, uint32_t copies) {
 	uint32_t R, M, len = 0;
 	if (large_fft_split(n, R, M)) len = n;
-	else len = bluestein_two_level_size(n);
+	else if (!mixed_radix_plan(n, nullptr)) len = bluestein_two_level_size(n);
 	if (!len) return 0;
 	// chunks of copies of at most 2^25 values (512 MiB), whole chains
 	return size_t(std::min<uint64_t>(copies, std::max<uint64_t>(1, (uint64_t(1) << 25) / len))) * len;

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <cstddef>
 #include <cstdint>
+#include <initializer_list>
 
 #include "nlc_fft_large.hpp"
 
@@ -179,6 +180,25 @@ constexpr uint32_t kDnlsLdsMax = 8192, kDnlsBluesteinMax = 8192;
 // scratch of copies x n values.
 bool dnls_supported(uint32_t n);
 uint32_t dnls_bluestein_size(uint32_t n);
+// Stockham passes (nlc_fft_mixed.hpp) of a length n <= 8192 that is not a
+// power of two and has no prime factor above 7: radix 4 first, then 2, 3, 5,
+// 7 (at most 8 passes); false for every other n.
+struct MixedPlan {
+	uint8_t radix[15];
+	uint8_t passes;
+};
+inline bool mixed_radix_plan(uint32_t n, MixedPlan *plan) {
+	if (n < 2 || (n & (n - 1)) == 0 || n > 8192) return false;
+	MixedPlan p{};
+	uint32_t r = n;
+	while (r % 4 == 0) p.radix[p.passes++] = 4, r /= 4;
+	if (r % 2 == 0) p.radix[p.passes++] = 2, r /= 2;
+	for (const uint32_t f : {3u, 5u, 7u})
+		while (r % f == 0) p.radix[p.passes++] = uint8_t(f), r /= f;
+	if (r != 1) return false;
+	if (plan) *plan = p;
+	return true;
+}
 void dnls_bluestein_tables(uint32_t n, double2 *out);
 bool dnls_two_level(uint32_t n);
 void dnls_two_level_tables(uint32_t n, const double2 *table, double2 *out);

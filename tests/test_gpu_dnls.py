@@ -34,13 +34,16 @@ def run(N, n, copies, steps, dt, beta, s0, spl=0):
 
 @pytest.mark.parametrize("n,copies,steps", [(4096, 6, 100), (2, 5, 100), (16, 5, 100), (64, 7, 100),
                                             (256, 5, 100), (1024, 3, 100), (2048, 3, 60), (8192, 3, 20),
-                                            # Bluestein: convolution sizes 8, 16, 256, 2048, 8192, 8192
-                                            (3, 5, 100), (5, 4, 100), (100, 5, 100), (1000, 3, 30),
-                                            (3000, 2, 5), (4095, 2, 3),
+                                            # mixed-radix Stockham (factors 2, 3, 5, 7 only)
+                                            (3, 5, 100), (5, 4, 100), (6, 5, 100), (7, 4, 100), (100, 5, 100),
+                                            (1000, 3, 30), (1225, 2, 8), (3000, 2, 5), (6561, 1, 1),
+                                            (7000, 1, 1), (8000, 1, 1),
+                                            # Bluestein: convolution sizes 32, 256, 4096, 8192
+                                            (11, 4, 100), (101, 5, 50), (1009, 2, 20), (4095, 2, 3),
                                             # two-level FFT in HBM: 4 x 4096, 8 x 4096, 16 x 4096, 16 x 8192
                                             (16384, 3, 10), (32768, 2, 6), (65536, 2, 3), (131072, 1, 3),
                                             # Bluestein through the two-level FFT (L = 16384, 16384, 65536)
-                                            (5000, 2, 5), (6007, 2, 3), (20001, 1, 1)])
+                                            (5003, 2, 5), (6007, 2, 3), (20001, 1, 1)])
 def test_dnls_move_parity(gpu, n, copies, steps):
     N = gpu
     dt, beta = 0.01, 1.0
@@ -184,10 +187,10 @@ def test_dnls_bluestein_two_level_linear_closed_form(gpu, n):
     np.testing.assert_allclose(n1, n0, rtol=1e-12)
 
 
-@pytest.mark.parametrize("n", [100, 8192, 32768, 5000])
+@pytest.mark.parametrize("n", [101, 8192, 32768, 5003, 100, 5000])
 def test_dnls_generic_kernels_describe_and_compose(gpu, n):
-    """The Bluestein, 8192-point LDS and two-level kernels: kernel names, and
-    two batches of 10 steps equal one of 20 to rounding."""
+    """The Bluestein, 8192-point LDS, two-level and mixed-radix kernels: kernel
+    names, and two batches of 10 steps equal one of 20 to rounding."""
     N = gpu
     s0 = N.ic_dnls(n, 3, max(1, min(n // 16, 256)), 1.0, 5)
     with N.Ensemble(N.DNLS, n, 3, 0.01, beta=1.0) as ens:
@@ -200,6 +203,7 @@ def test_dnls_generic_kernels_describe_and_compose(gpu, n):
         ens.move(20)
         b = ens.download()
     banner(f"  {desc}")
-    assert ("lds_bluestein" in desc) == (n == 100) and ("stockham" in desc) == (n == 8192)
-    assert ("two_level_fft" in desc) == (n == 32768) and ("bluestein_two_level" in desc) == (n == 5000)
+    assert ("lds_bluestein" in desc) == (n == 101) and ("stockham" in desc) == (n == 8192)
+    assert ("two_level_fft" in desc) == (n == 32768) and ("bluestein_two_level" in desc) == (n == 5003)
+    assert ("mixed_radix" in desc) == (n in (100, 5000))
     assert rel_err(a, b) < 1e-12

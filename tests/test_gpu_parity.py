@@ -355,9 +355,11 @@ def test_ensemble_beyond_2_30_sites(gpu):
 
 @pytest.mark.parametrize("model_name,n,copies", [("DNKG", 64, 50), ("DNKG", 1024, 64), ("FPUT", 256, 33),
                                                  ("Toda", 128, 20), ("dDNKG", 512, 70), ("dDNKG", 100, 9),
-                                                 ("DNKG", 3000, 3), ("FPUT", 96, 5),
-                                                 # 8192-point LDS FFT; DFT in 160 KB of LDS; DFT from L1/L2
+                                                 ("DNKG", 3000, 3), ("FPUT", 96, 5), ("DNKG", 7, 40),
+                                                 # 8192-point LDS FFT; mixed-radix FFT (5120 = 2^10 5,
+                                                 # 6000 = 2^4 3 5^3, 7000 = 2^3 5^3 7); DFT in LDS (3001)
                                                  ("DNKG", 8192, 5), ("Toda", 5120, 3), ("FPUT", 6000, 3),
+                                                 ("DNKG", 7000, 2), ("Toda", 3001, 3),
                                                  # short chains, 1024/n per workgroup: partial groups,
                                                  # several rounds per workgroup
                                                  ("DNKG", 8, 129), ("FPUT", 16, 3), ("Toda", 32, 1000),
