@@ -125,6 +125,11 @@ def cpu_baseline(w, s0, seconds_budget: float):
     t = time.perf_counter()
     move(s0[:copies], steps)
     el = time.perf_counter() - t
+    if el < 0.5 * seconds_budget:  # the small probe underestimates the rate: rescale once
+        steps = max(steps, min(int(steps * seconds_budget / el), 100 * w["batch"]))
+        t = time.perf_counter()
+        move(s0[:copies], steps)
+        el = time.perf_counter() - t
     return {"value": copies * w["n"] * steps / el, "unit": "site-steps/s", "cores": cores, "kind": "port",
             "sample": f"oracle/liboracle_fast.so (-O3 -ffast-math, OpenMP over copies), {w['model']} "
                       f"N={w['n']} x {copies} copies x {steps} steps, {el:.1f} s"}
@@ -180,10 +185,16 @@ def cpu_baseline_ragged(w, lengths, s0, seconds_budget: float):
         O.move_real(O.TODA, g, 2, w["dt"], alpha=w["alpha"], threads=cores, lib=lib)
     rate = sites * 2 / (time.perf_counter() - t)
     steps = int(max(4, min(20 * w["batch"], seconds_budget * rate / sites)))
-    t = time.perf_counter()
-    for g in groups:
-        O.move_real(O.TODA, g, steps, w["dt"], alpha=w["alpha"], threads=cores, lib=lib)
-    el = time.perf_counter() - t
+
+    def sample(steps):
+        t = time.perf_counter()
+        for g in groups:
+            O.move_real(O.TODA, g, steps, w["dt"], alpha=w["alpha"], threads=cores, lib=lib)
+        return time.perf_counter() - t
+    el = sample(steps)
+    if el < 0.5 * seconds_budget:  # the 2-step probe underestimates the rate: rescale once
+        steps = max(steps, min(int(steps * seconds_budget / el), 100 * w["batch"]))
+        el = sample(steps)
     return {"value": sites * steps / el, "unit": "site-steps/s", "cores": cores, "kind": "port",
             "sample": f"oracle/liboracle_fast.so (-O3 -ffast-math, OpenMP over copies), Toda, first {per_class} "
                       f"chains of each length class ({sites} sites) x {steps} steps, {el:.1f} s"}
