@@ -124,17 +124,23 @@ struct DnlsArgs {
 
 // In-place radix-4 (+ final radix-2) Stockham FFT of n <= MAXN points, forward
 // sign, natural order; tw[j tws] = e^{-2 pi i j/n} (tws > 1: a longer table).
-template <uint32_t MAXN>
-__device__ void lds_fft_fwd(double2 *buf, uint32_t n, const double2 *__restrict__ tw, uint32_t tws = 1) {
-	constexpr int kMaxPer = MAXN / 4 / kT; // radix-4 butterflies per thread per pass
-	const uint32_t tid = threadIdx.x, quarter = n >> 2;
+// Threads tid < nthr of the chain work on it (nthr = kT, or n/4 when several
+// chains share the workgroup: one radix-4 butterfly per thread and pass); every
+// thread of the workgroup reaches the same barriers.
+template <uint32_t MAXN, bool WHOLE>
+__device__ void lds_fft_fwd_impl(double2 *buf, uint32_t n, const double2 *__restrict__ tw, uint32_t tws,
+                                 uint32_t tid, uint32_t nthr_rt) {
+	constexpr int kMaxPer = MAXN / 4 / kT; // radix-4 butterflies per thread per pass (nthr = kT)
+	// a compile-time stride for the whole workgroup keeps the LDS offsets immediate
+	const uint32_t nthr = WHOLE ? uint32_t(kT) : nthr_rt;
+	const uint32_t quarter = n >> 2;
 	uint32_t ns = 1;
 	for (; ns * 4 <= n; ns *= 4) {
 		double2 v[kMaxPer][4];
 		const uint32_t span = n / (ns * 4) * tws;
 #pragma unroll
 		for (int q = 0; q < kMaxPer; ++q) {
-			const uint32_t j = tid + q * kT;
+			const uint32_t j = tid + q * nthr;
 			if (j < quarter)
 #pragma unroll
 				for (int r = 0; r < 4; ++r) v[q][r] = buf[j + r * quarter];
@@ -142,7 +148,7 @@ __device__ void lds_fft_fwd(double2 *buf, uint32_t n, const double2 *__restrict_
 		__syncthreads();
 #pragma unroll
 		for (int q = 0; q < kMaxPer; ++q) {
-			const uint32_t j = tid + q * kT;
+			const uint32_t j = tid + q * nthr;
 			if (j < quarter) {
 				const uint32_t k = j % ns;
 				double2 a0 = v[q][0], a1 = cmul(v[q][1], tw[k * span]), a2 = cmul(v[q][2], tw[2 * k * span]),
@@ -159,13 +165,13 @@ __device__ void lds_fft_fwd(double2 *buf, uint32_t n, const double2 *__restrict_
 		double2 v[2 * kMaxPer][2];
 #pragma unroll
 		for (int q = 0; q < 2 * kMaxPer; ++q) {
-			const uint32_t j = tid + q * kT;
+			const uint32_t j = tid + q * nthr;
 			if (j < half) v[q][0] = buf[j], v[q][1] = buf[j + half];
 		}
 		__syncthreads();
 #pragma unroll
 		for (int q = 0; q < 2 * kMaxPer; ++q) {
-			const uint32_t j = tid + q * kT;
+			const uint32_t j = tid + q * nthr;
 			if (j < half) {
 				const uint32_t k = j % ns;
 				const double2 b = cmul(v[q][1], tw[k * (n / (2 * ns)) * tws]);
@@ -176,4 +182,17 @@ __device__ void lds_fft_fwd(double2 *buf, uint32_t n, const double2 *__restrict_
 		}
 		__syncthreads();
 	}
+}
+
+// The whole workgroup on one chain.
+template <uint32_t MAXN>
+__device__ void lds_fft_fwd(double2 *buf, uint32_t n, const double2 *__restrict__ tw, uint32_t tws = 1) {
+	lds_fft_fwd_impl<MAXN, true>(buf, n, tw, tws, threadIdx.x, kT);
+}
+
+// Threads tid < nthr of the workgroup on one of its chains.
+template <uint32_t MAXN>
+__device__ void lds_fft_fwd(double2 *buf, uint32_t n, const double2 *__restrict__ tw, uint32_t tws, uint32_t tid,
+                            uint32_t nthr) {
+	lds_fft_fwd_impl<MAXN, false>(buf, n, tw, tws, tid, nthr);
 }

@@ -26,45 +26,66 @@ namespace {
 
 #include "nlc_dnls_common.inl"
 
-// One batch of steps on a chain held in LDS; fft(buf) is the forward n-point
-// FFT in place, natural order in and out.  Between substeps the buffer holds
-// conj(psi): the inverse transform is conj(FFT(conj(X T))).
-template <class Fft>
-__device__ __forceinline__ void dnls_lds_batch(double2 *buf, double2 *psi, uint32_t n, uint32_t steps,
-                                               const DnlsArgs &a, const double2 *__restrict__ table, Fft fft) {
-	const uint32_t tid = threadIdx.x;
-	for (uint32_t x = tid; x < n; x += kT) buf[x] = psi[x];
+// One batch of steps on a chain held in LDS by threads tid < nthr; fft(buf) is
+// the forward n-point FFT in place, natural order in and out.  Between substeps
+// the buffer holds conj(psi): the inverse transform is conj(FFT(conj(X T))).
+// A slot past the last chain (live = false) runs on zeros and writes nothing,
+// so that every thread reaches every barrier.
+// NT: the thread count as a compile-time constant (0: nthr at run time).
+template <uint32_t NT, class Fft>
+__device__ __forceinline__ void dnls_lds_batch(double2 *buf, double2 *psi, bool live, uint32_t n, uint32_t tid,
+                                               uint32_t nthr_rt, uint32_t steps, const DnlsArgs &a,
+                                               const double2 *__restrict__ table, Fft fft) {
+	const uint32_t nthr = NT ? NT : nthr_rt;
+	for (uint32_t x = tid; x < n; x += nthr) buf[x] = live ? psi[x] : make_double2(0.0, 0.0);
 	bool conj_state = false;
 	__syncthreads();
 	for (uint32_t s = 0; s < steps; ++s) {
 		for (int k = 0; k < 7; ++k) {
 			const double th = (k == 0 && s) ? a.theta[8] : a.theta[k];
-			for (uint32_t x = tid; x < n; x += kT) {
+			for (uint32_t x = tid; x < n; x += nthr) {
 				const double2 p = conj_state ? cconj(buf[x]) : buf[x];
 				buf[x] = nonlinear(p, th);
 			}
 			__syncthreads();
 			fft(buf);
 			const double2 *tk = table + size_t(k) * n;
-			for (uint32_t j = tid; j < n; j += kT) buf[j] = cconj(cmul(buf[j], tk[j]));
+			for (uint32_t j = tid; j < n; j += nthr) buf[j] = cconj(cmul(buf[j], tk[j]));
 			__syncthreads();
 			fft(buf);
 			conj_state = true;
 		}
 	}
-	for (uint32_t x = tid; x < n; x += kT) {
-		const double2 p = conj_state ? cconj(buf[x]) : buf[x];
-		psi[x] = nonlinear(p, a.theta[7]);
-	}
+	if (live)
+		for (uint32_t x = tid; x < n; x += nthr) {
+			const double2 p = conj_state ? cconj(buf[x]) : buf[x];
+			psi[x] = nonlinear(p, a.theta[7]);
+		}
 }
 
-template <uint32_t MAXN>
+// Threads per chain of k_dnls_lds: kT from 4 kT points up, else n/4 (one
+// radix-4 butterfly per thread and pass) with kT/(n/4) chains per workgroup.
+__host__ __device__ inline uint32_t lds_chain_threads(uint32_t n) {
+	return n >= 4 * uint32_t(kT) ? uint32_t(kT) : (n >= 4 ? n / 4 : 1u);
+}
+
+// SHARED: n < 4 kT, kT/(n/4) chains per workgroup; else one chain.
+template <uint32_t MAXN, bool SHARED>
 __global__ void __launch_bounds__(kT)
 k_dnls_lds(double2 *__restrict__ state, uint32_t n, uint32_t copies, uint32_t steps, DnlsArgs a,
            const double2 *__restrict__ tw, const double2 *__restrict__ table) {
 	extern __shared__ double2 buf[];
-	dnls_lds_batch(buf, state + size_t(blockIdx.x) * n, n, steps, a, table,
-	               [&](double2 *b) { lds_fft_fwd<MAXN>(b, n, tw); });
+	if constexpr (SHARED) {
+		const uint32_t nthr = lds_chain_threads(n), per_group = kT / nthr;
+		const uint32_t sub = threadIdx.x / nthr, tid = threadIdx.x % nthr;
+		const uint32_t chain = blockIdx.x * per_group + sub;
+		const bool live = chain < copies;
+		dnls_lds_batch<0>(buf + size_t(sub) * n, state + size_t(live ? chain : 0) * n, live, n, tid, nthr, steps, a,
+		                  table, [&](double2 *b) { lds_fft_fwd<MAXN>(b, n, tw, 1, tid, nthr); });
+	} else {
+		dnls_lds_batch<kT>(buf, state + size_t(blockIdx.x) * n, true, n, threadIdx.x, kT, steps, a, table,
+		                   [&](double2 *b) { lds_fft_fwd<MAXN>(b, n, tw); });
+	}
 }
 
 template <uint32_t MAXN>
@@ -72,8 +93,8 @@ __global__ void __launch_bounds__(kT)
 k_dnls_mixed(double2 *__restrict__ state, uint32_t n, uint32_t copies, uint32_t steps, DnlsArgs a, MixedPlan plan,
              const double2 *__restrict__ tw, const double2 *__restrict__ table) {
 	extern __shared__ double2 buf[];
-	dnls_lds_batch(buf, state + size_t(blockIdx.x) * n, n, steps, a, table,
-	               [&](double2 *b) { mixed::lds_fft<MAXN, kT>(b, n, tw, plan); });
+	dnls_lds_batch<kT>(buf, state + size_t(blockIdx.x) * n, true, n, threadIdx.x, kT, steps, a, table,
+	                   [&](double2 *b) { mixed::lds_fft<MAXN, kT>(b, n, tw, plan); });
 }
 
 // Cyclic convolution of buf[0..M) with b_j = conj(w_j) (|j| < n, zero
@@ -193,14 +214,19 @@ hipError_t launch_dnls_generic(hipStream_t st, double2 *state, uint32_t n, uint3
 	DnlsArgs a;
 	for (int i = 0; i < 9; ++i) a.theta[i] = theta9[i];
 	if (pow2(n) && n <= kDnlsLdsMax) {
-		const size_t lds = sizeof(double2) * n;
-		if (n <= 4096) {
-			hipLaunchKernelGGL(k_dnls_lds<4096>, dim3(copies), dim3(kT), lds, st, state, n, copies, steps, a, tw, table);
-		} else {
-			hipError_t e = launch_lds_chain_kernel(k_dnls_lds<kDnlsLdsMax>, lds);
-			if (e != hipSuccess) return e;
-			hipLaunchKernelGGL(k_dnls_lds<kDnlsLdsMax>, dim3(copies), dim3(kT), lds, st, state, n, copies, steps, a, tw,
+		const uint32_t per_group = kT / lds_chain_threads(n);
+		const size_t lds = sizeof(double2) * n * per_group;
+		if (per_group > 1) {
+			hipLaunchKernelGGL((k_dnls_lds<4 * kT, true>), dim3((copies + per_group - 1) / per_group), dim3(kT), lds, st,
+			                   state, n, copies, steps, a, tw, table);
+		} else if (n <= 4096) {
+			hipLaunchKernelGGL((k_dnls_lds<4096, false>), dim3(copies), dim3(kT), lds, st, state, n, copies, steps, a, tw,
 			                   table);
+		} else {
+			hipError_t e = launch_lds_chain_kernel(k_dnls_lds<kDnlsLdsMax, false>, lds);
+			if (e != hipSuccess) return e;
+			hipLaunchKernelGGL((k_dnls_lds<kDnlsLdsMax, false>), dim3(copies), dim3(kT), lds, st, state, n, copies, steps,
+			                   a, tw, table);
 		}
 	} else if (MixedPlan plan; mixed_radix_plan(n, &plan)) {
 		const size_t lds = sizeof(double2) * n;

@@ -34,6 +34,9 @@ def run(N, n, copies, steps, dt, beta, s0, spl=0):
 
 @pytest.mark.parametrize("n,copies,steps", [(4096, 6, 100), (2, 5, 100), (16, 5, 100), (64, 7, 100),
                                             (256, 5, 100), (1024, 3, 100), (2048, 3, 60), (8192, 3, 20),
+                                            # 1024/n chains per LDS workgroup: several groups, the
+                                            # last one partly filled
+                                            (2, 600, 50), (8, 300, 50), (64, 100, 20), (512, 5, 30),
                                             # mixed-radix Stockham (factors 2, 3, 5, 7 only)
                                             (3, 5, 100), (5, 4, 100), (6, 5, 100), (7, 4, 100), (100, 5, 100),
                                             (1000, 3, 30), (1225, 2, 8), (3000, 2, 5), (6561, 1, 1),
