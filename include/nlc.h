@@ -59,10 +59,14 @@ typedef enum { NLC_FP64 = 0, NLC_FP32C = 1 } nlc_precision;
  * ensemble-major (split) layout.  NLC_KERNEL_WAVE keeps each chain in the
  * registers of one or more wavefront lanes (the reference's move_chain_in_warp /
  * move_chain_in_thread), or of a workgroup of 2-8 wavefronts for chains of
- * 2049-16384 sites.  NLC_KERNEL_AUTO picks WAVE when a shape exists for the
- * chain length (every length up to 16384), else SPLIT (the reference's move<>
- * dispatch, DNKG_FPUT_Toda.cu:262-297). */
-typedef enum { NLC_KERNEL_AUTO = 0, NLC_KERNEL_WAVE = 1, NLC_KERNEL_SPLIT = 2 } nlc_kernel;
+ * 2049-16384 sites.  NLC_KERNEL_TILED keeps overlapping segments of 2048
+ * sites (1024 for Toda) in the registers of one wavefront for a few steps per
+ * launch (temporal blocking, any chain length from 256 sites; it needs a
+ * second buffer of the state's size).  NLC_KERNEL_AUTO picks WAVE when a shape
+ * exists for the chain length (every length up to 16384), else TILED (the
+ * reference's move<> dispatch, DNKG_FPUT_Toda.cu:262-297, which falls back to
+ * its split kernel there); WAVE without a shape override does the same. */
+typedef enum { NLC_KERNEL_AUTO = 0, NLC_KERNEL_WAVE = 1, NLC_KERNEL_SPLIT = 2, NLC_KERNEL_TILED = 3 } nlc_kernel;
 
 /* dDNKG energy projection (the two cublasDgemm of dDNKG.cpp:118-135 and the
  * reduction of dDNKG.cu:237-265, fused).  NLC_PROJ_AUTO = FP64 matrix cores

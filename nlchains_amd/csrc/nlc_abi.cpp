@@ -93,6 +93,9 @@ struct nlc_ctx {
 	nlc::StepLauncher launcher = nullptr; // uniform wave kernel
 	int L = 0, S = 0;
 	bool exact = false, use_split = false;
+	nlc::TiledLauncher tiled = nullptr; // temporally blocked kernel for long chains
+	double2 *tile_buf = nullptr;        // its second state buffer
+	bool use_tiled = false;
 
 	// cached graphs of one batch, one per result destination: [0, NLC_BATCH_SLOTS)
 	// for the slots of queued batches, NLC_BATCH_SLOTS for nlc_batch
@@ -178,6 +181,16 @@ nlc::StepLauncher lookup(int model, int L, int S, bool exact) {
 	if (!l && exact && S >= 2) l = f(L, S, false);
 	if (!l && L == 16) l = f16(S);
 	return l;
+}
+
+nlc::TiledLauncher lookup_tiled(int model) {
+	switch (model) {
+	case NLC_DNKG: return nlc::lookup_tiled_dnkg();
+	case NLC_FPUT: return nlc::lookup_tiled_fput();
+	case NLC_TODA: return nlc::lookup_tiled_toda();
+	case NLC_DDNKG: return nlc::lookup_tiled_ddnkg();
+	}
+	return nullptr;
 }
 
 nlc::RaggedLauncher lookup_ragged(int model, int S) {
@@ -302,7 +315,8 @@ void free_all(nlc_ctx *c) {
 	     {(void *) c->state, (void *) c->omega, (void *) c->tw, (void *) c->partial, (void *) c->reduce_scratch,
 	      (void *) c->e_raw, (void *) c->modes, (void *) c->mp2, (void *) c->masses_d,
 	      (void *) c->eigvec, (void *) c->split_q[0], (void *) c->split_p[0], (void *) c->split_q[1], (void *) c->split_p[1], (void *) c->energy_d, (void *) c->norm_d,
-	      (void *) c->dnls_table, (void *) c->fft_scratch, (void *) c->blue_table, (void *) c->items, (void *) c->vplanes, (void *) c->splanes})
+	      (void *) c->dnls_table, (void *) c->fft_scratch, (void *) c->blue_table, (void *) c->items, (void *) c->vplanes, (void *) c->splanes,
+	      (void *) c->tile_buf})
 		if (ptr) hipFree(ptr);
 	if (c->h_results) hipHostFree(c->h_results);
 	if (c->h_slots) hipHostFree(c->h_slots);
@@ -481,9 +495,9 @@ int setup_dnls(nlc_ctx *c) {
 int select_uniform(nlc_ctx *c) {
 	const nlc_params &p = c->p;
 	const uint32_t n = c->n;
-	bool want_split = p.kernel == NLC_KERNEL_SPLIT;
-	c->use_split = false;
-	if (!want_split) {
+	bool want_split = p.kernel == NLC_KERNEL_SPLIT, want_tiled = p.kernel == NLC_KERNEL_TILED;
+	c->use_split = c->use_tiled = false;
+	if (!want_split && !want_tiled) {
 		int L = 0, S = 0;
 		bool exact = false, have;
 		if (p.lanes_per_chain || p.sites_per_lane) {
@@ -500,11 +514,19 @@ int select_uniform(nlc_ctx *c) {
 		c->launcher = have ? lookup(p.model, L, S, exact) : nullptr;
 		if (c->launcher) {
 			c->L = L, c->S = S, c->exact = exact;
-		} else if (p.kernel == NLC_KERNEL_WAVE || p.lanes_per_chain) {
+		} else if (p.lanes_per_chain) {
 			return c->fail(NLC_EUNSUPPORTED, "no wave-resident kernel for chain_length %u (shape %d x %d)", n, L, S);
 		} else {
-			want_split = true;
+			// longer than a workgroup holds: segments in registers with temporal blocking
+			want_tiled = true;
 		}
+	}
+	if (want_tiled) {
+		if (!(c->tiled = lookup_tiled(p.model)) || n < 256)
+			return c->fail(NLC_EUNSUPPORTED, "no temporally blocked kernel for chain_length %u", n);
+		if (!c->tile_buf) NLC_TRY(dev_alloc(c, &c->tile_buf, size_t(n) * c->copies, "second state buffer"));
+		c->use_tiled = true;
+		return NLC_OK;
 	}
 	if (want_split) {
 		c->use_split = true;
@@ -561,6 +583,8 @@ int enqueue_move(nlc_ctx *c, uint32_t steps) {
 			for (const auto &k : c->cls)
 				NLC_HIP(c, k.launcher(c->stream, c->state + k.site0, k.count, k.n, steps, c->args, nullptr));
 		}
+	} else if (c->use_tiled) {
+		NLC_HIP(c, c->tiled(c->stream, c->state, c->tile_buf, c->copies, c->n, steps, c->args, c->mp2));
 	} else if (c->use_split) {
 		int cur = 0;
 		NLC_HIP(c, nlc::launch_planar_to_split(c->stream, c->state, c->split_q[0], c->split_p[0], c->copies, c->n));
@@ -659,7 +683,7 @@ int validate_params(const nlc_params *p, bool ragged) {
 	if (p->model == NLC_TODA && p->alpha == 0) return reject(NLC_EARG, "Toda needs --alpha != 0");
 	if (p->model == NLC_DDNKG && !p->masses) return reject(NLC_EARG, "dDNKG needs the masses (-m file content)");
 	if (p->precision != NLC_FP64) return reject(NLC_EUNSUPPORTED, "only FP64 precision (NLC_FP64) is implemented");
-	if (p->kernel < NLC_KERNEL_AUTO || p->kernel > NLC_KERNEL_SPLIT) return reject(NLC_EARG, "unknown kernel family");
+	if (p->kernel < NLC_KERNEL_AUTO || p->kernel > NLC_KERNEL_TILED) return reject(NLC_EARG, "unknown kernel family");
 	if (p->projection < NLC_PROJ_AUTO || p->projection > NLC_PROJ_SPLIT_BF16)
 		return reject(NLC_EARG, "unknown projection form");
 	int ndev = 0;
@@ -1100,6 +1124,7 @@ int nlc_autotune(nlc_ctx *c, uint32_t trial_steps, char *report, size_t len) {
 		int ragged_S;
 		float ms;
 		int dnls;
+		bool tiled;
 	};
 	std::vector<Cand> cands;
 	if (c->p.model == NLC_DNLS) {
@@ -1119,6 +1144,7 @@ int nlc_autotune(nlc_ctx *c, uint32_t trial_steps, char *report, size_t len) {
 		if (per_class) cands.push_back({0, 0, false, false, 0, 0, 0});
 	} else {
 		for (const Shape &s : wave_shapes(c->p.model, c->n)) cands.push_back({s.L, s.S, s.exact, false, 0, 0, 0});
+		if (c->n > 2048 && lookup_tiled(c->p.model)) cands.push_back({0, 0, false, false, 0, 0, 0, true});
 		cands.push_back({0, 0, false, true, 0, 0, 0});
 	}
 	if (!cands.empty()) {
@@ -1138,9 +1164,9 @@ int nlc_autotune(nlc_ctx *c, uint32_t trial_steps, char *report, size_t len) {
 			} else if (c->ragged) {
 				rc = select_ragged(c, cd.ragged_S);
 			} else {
-				c->p.kernel = cd.split ? NLC_KERNEL_SPLIT : NLC_KERNEL_WAVE;
-				c->p.lanes_per_chain = cd.split ? 0 : uint32_t(cd.L);
-				c->p.sites_per_lane = cd.split ? 0 : uint32_t(cd.S);
+				c->p.kernel = cd.split ? NLC_KERNEL_SPLIT : cd.tiled ? NLC_KERNEL_TILED : NLC_KERNEL_WAVE;
+				c->p.lanes_per_chain = uint32_t(cd.L);
+				c->p.sites_per_lane = uint32_t(cd.S);
 				rc = select_uniform(c);
 			}
 			for (int rep = 0; rep < 2 && !rc; ++rep) { // the first run warms up
@@ -1175,6 +1201,7 @@ int nlc_autotune(nlc_ctx *c, uint32_t trial_steps, char *report, size_t len) {
 			else if (c->ragged && cd.ragged_S) snprintf(buf, sizeof buf, " S=%d %.3f ms%s", cd.ragged_S, cd.ms, mark);
 			else if (c->ragged) snprintf(buf, sizeof buf, " per-class %.3f ms%s", cd.ms, mark);
 			else if (cd.split) snprintf(buf, sizeof buf, " split %.3f ms%s", cd.ms, mark);
+			else if (cd.tiled) snprintf(buf, sizeof buf, " tiled %.3f ms%s", cd.ms, mark);
 			else snprintf(buf, sizeof buf, " %dx%d%s %.3f ms%s", cd.L, cd.S, cd.exact ? "" : "r", cd.ms, mark);
 			rep += buf;
 		}
@@ -1184,9 +1211,9 @@ int nlc_autotune(nlc_ctx *c, uint32_t trial_steps, char *report, size_t len) {
 		} else if (c->ragged) {
 			NLC_TRY(select_ragged(c, best->ragged_S));
 		} else {
-			c->p.kernel = best->split ? NLC_KERNEL_SPLIT : NLC_KERNEL_WAVE;
-			c->p.lanes_per_chain = best->split ? 0 : uint32_t(best->L);
-			c->p.sites_per_lane = best->split ? 0 : uint32_t(best->S);
+			c->p.kernel = best->split ? NLC_KERNEL_SPLIT : best->tiled ? NLC_KERNEL_TILED : NLC_KERNEL_WAVE;
+			c->p.lanes_per_chain = uint32_t(best->L);
+			c->p.sites_per_lane = uint32_t(best->S);
 			NLC_TRY(select_uniform(c));
 		}
 	}
@@ -1221,6 +1248,8 @@ int nlc_describe(const nlc_ctx *c, char *buf, size_t len) {
 		         c->ragged_launcher ? "ragged_persistent" : "per_class_wave", c->ragged_S, c->nitems);
 	else if (c->use_split)
 		snprintf(buf, len, "model=%s n=%u copies=%u kernel=split", names[c->p.model], c->n, c->copies);
+	else if (c->use_tiled)
+		snprintf(buf, len, "model=%s n=%u copies=%u kernel=tiled", names[c->p.model], c->n, c->copies);
 	else
 		snprintf(buf, len, "model=%s n=%u copies=%u kernel=%s lanes_per_chain=%d sites_per_lane=%d exact=%d",
 		         names[c->p.model], c->n, c->copies, c->L > 64 ? "group" : "wave", c->L, c->S, int(c->exact));

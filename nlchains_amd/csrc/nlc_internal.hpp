@@ -63,6 +63,15 @@ StepLauncher lookup_step_dnkg(int L, int S, bool exact);
 StepLauncher lookup_step_fput(int L, int S, bool exact);
 StepLauncher lookup_step_toda(int L, int S, bool exact);
 StepLauncher lookup_step_ddnkg(int L, int S, bool exact);
+// Temporally blocked step for chains longer than a workgroup holds: `steps`
+// steps in launches of a few steps each, ping-ponging between state and
+// scratch (same size as state); the result ends in state.
+using TiledLauncher = hipError_t (*)(hipStream_t, double2 *state, double2 *scratch, uint32_t copies, uint32_t n,
+                                     uint32_t steps, const StepArgs &args, const double *mp2);
+TiledLauncher lookup_tiled_dnkg();
+TiledLauncher lookup_tiled_fput();
+TiledLauncher lookup_tiled_toda();
+TiledLauncher lookup_tiled_ddnkg();
 // 16 lanes with S = 5..32 sites per lane for any n in (16(S-1), 16S] (ragged lanes).
 StepLauncher lookup_step16_dnkg(int S);
 StepLauncher lookup_step16_fput(int S);

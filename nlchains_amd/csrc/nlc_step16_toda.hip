@@ -1,5 +1,7 @@
 // Wave-resident step kernels, model Toda: 16-lane shapes with ragged lanes for
-// chains of 65..512 sites (see nlc_step_wave.inl).
+// chains of 65..512 sites, and the temporally blocked kernel for long chains
+// (see nlc_step_wave.inl).
 #define NLC_STEP_MODEL ::nlc::TODA
 #define NLC_STEP_LOOKUP16 lookup_step16_toda
+#define NLC_STEP_TILED lookup_tiled_toda
 #include "nlc_step_wave.inl"

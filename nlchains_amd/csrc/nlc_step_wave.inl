@@ -26,6 +26,7 @@
 //    step is fused with the c0 drift of the next (the reference's 2*c0 merge).
 //  * HBM is touched once per batch: 16 B/site in, 16 B/site out.
 #include <algorithm>
+#include <utility>
 
 #include "nlc_internal.hpp"
 // after the HIP runtime header: nlc_exp.hpp uses its __device__ qualifiers
@@ -84,6 +85,33 @@ struct Chain {
 	}
 };
 
+// `steps` Yoshida steps of the chain part held by ch: the c7 drift of a step
+// fused with the c0 drift of the next (the reference's 2 c0 merge).  Toda
+// drifts v = (64 alpha/ln2) q: its drift weights carry that factor.
+template <int MODEL, class C>
+__device__ __forceinline__ void step_loop(C &ch, uint32_t steps, const StepArgs &a) {
+	const double cs = (MODEL == TODA) ? a.scale : 1.0;
+	const double dc0 = cs * a.dc[0];
+	ch.drift(dc0);
+	for (uint32_t s = 0; s < steps; ++s) {
+		const double c_last = (s + 1 < steps) ? 2.0 * dc0 : cs * a.dc[7];
+		if constexpr (MODEL == TODA) {
+			// exp is long: keep the substep loop rolled to bound code size
+#pragma unroll 1
+			for (int k = 0; k < 7; ++k) {
+				ch.kick(a.dd[k], a);
+				ch.drift(k < 6 ? cs * a.dc[k + 1] : c_last);
+			}
+		} else {
+#pragma unroll
+			for (int k = 0; k < 7; ++k) {
+				ch.kick(a.dd[k], a);
+				ch.drift(k < 6 ? a.dc[k + 1] : c_last);
+			}
+		}
+	}
+}
+
 // One wavefront integrates `live_chains` (<= 64/L) consecutive chains of
 // length n starting at chain0 for `steps` steps.  All 64 lanes take part in
 // the halo exchange even when the wave is only partly filled.  Toda keeps
@@ -117,27 +145,7 @@ __device__ __forceinline__ void run_wave(double2 *__restrict__ chain0, uint32_t 
 		if constexpr (MODEL == DDNKG) ch.mp[j] = real_site ? mp2g[off + j] : 0.0;
 	}
 
-	// Toda drifts v = (64 alpha/ln2) q: its drift weights carry that factor
-	const double cs = (MODEL == TODA) ? a.scale : 1.0;
-	const double dc0 = cs * a.dc[0];
-	ch.drift(dc0);
-	for (uint32_t s = 0; s < steps; ++s) {
-		const double c_last = (s + 1 < steps) ? 2.0 * dc0 : cs * a.dc[7];
-		if constexpr (MODEL == TODA) {
-			// exp is long: keep the substep loop rolled to bound code size
-#pragma unroll 1
-			for (int k = 0; k < 7; ++k) {
-				ch.kick(a.dd[k], a);
-				ch.drift(k < 6 ? cs * a.dc[k + 1] : c_last);
-			}
-		} else {
-#pragma unroll
-			for (int k = 0; k < 7; ++k) {
-				ch.kick(a.dd[k], a);
-				ch.drift(k < 6 ? a.dc[k + 1] : c_last);
-			}
-		}
-	}
+	step_loop<MODEL>(ch, steps, a);
 
 	if (live) {
 #pragma unroll
@@ -183,6 +191,92 @@ hipError_t launch_wave(hipStream_t st, double2 *state, uint32_t copies, uint32_t
 		if (a.form == kFormQuad) return launch_wave_form<MODEL, L, S, EXACT, kFormQuad>(st, state, copies, n, steps, a, mp2);
 	return launch_wave_form<MODEL, L, S, EXACT, kFormPlain>(st, state, copies, n, steps, a, mp2);
 }
+
+#ifdef NLC_STEP_TILED
+// Temporal blocking for chains longer than a workgroup holds.  A wavefront
+// takes an extended segment of E = 64 S consecutive sites (64 lanes x S, as a
+// 64 x S wave shape) around the C = E - 2 H sites it owns, H on each side, and
+// runs T <= H/7 steps on it as if it were a ring.  The two ends see wrong
+// neighbours, and a wrong value moves one site inward per kick (every kick
+// reads only nearest neighbours, drifts are local), so after 7 T kicks the C
+// owned sites are exact; only they are written, to the other buffer.  HBM
+// traffic: 16 E bytes in and 16 C out per segment and launch, about 2 B per
+// site and step at T = 18.
+template <int S>
+__host__ __device__ constexpr uint32_t tiled_halo() { return S >= 32 ? 126 : 63; } // 7 T: T = 18 or 9
+
+template <int MODEL, int S, int FORM>
+__global__ void __launch_bounds__(256)
+k_step_tiled(const double2 *__restrict__ src, double2 *__restrict__ dst, uint32_t n, uint32_t copies, uint32_t segs,
+             uint32_t steps, StepArgs a, const double *__restrict__ mp2g) {
+	constexpr uint32_t E = 64 * S, H = tiled_halo<S>(), C = E - 2 * H;
+	__shared__ double etab[MODEL == TODA ? kExpTabSize : 1];
+	load_exp_table<MODEL>(etab);
+	const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+	if (wave >= copies * segs) return; // whole wavefront idle: uniform exit, no barrier follows
+	const uint32_t copy = wave / segs, seg = wave % segs;
+	const int lane = threadIdx.x & 63;
+	Chain<MODEL, 64, S, true, FORM> ch;
+	ch.lane = lane;
+	ch.etab = etab;
+	ch.full = true;
+	const double2 *in = src + size_t(copy) * n;
+	// extended position e = lane S + j is global site (seg C - H + e) mod n
+	const uint64_t g0 = uint64_t(seg) * C + n - H; // n > H: no negative index
+#pragma unroll
+	for (int j = 0; j < S; ++j) {
+		const uint32_t g = uint32_t((g0 + uint32_t(lane * S + j)) % n);
+		const double2 v = to_registers<MODEL, FORM>(in[g], a);
+		ch.q[j] = v.x;
+		ch.p[j] = v.y;
+		if constexpr (MODEL == DDNKG) ch.mp[j] = mp2g[g];
+	}
+	step_loop<MODEL>(ch, steps, a);
+	double2 *out = dst + size_t(copy) * n;
+#pragma unroll
+	for (int j = 0; j < S; ++j) {
+		const uint32_t e = uint32_t(lane * S + j);
+		const uint64_t g = uint64_t(seg) * C + e - H;
+		if (e >= H && e < H + C && g < n) out[g] = from_registers<MODEL, FORM>(ch.q[j], ch.p[j], a);
+	}
+}
+
+// `steps` steps of every chain in launches of at most T steps, alternating
+// between state and scratch (the result ends in state: an odd launch count
+// copies it back).  Needs n > H.
+template <int MODEL, int S, int FORM>
+hipError_t launch_tiled_form(hipStream_t st, double2 *state, double2 *scratch, uint32_t copies, uint32_t n,
+                             uint32_t steps, const StepArgs &a, const double *mp2) {
+	constexpr uint32_t E = 64 * S, H = tiled_halo<S>(), C = E - 2 * H, T = H / 7;
+	if (n <= H) return hipErrorInvalidValue;
+	const uint32_t segs = (n + C - 1) / C;
+	const uint64_t blocks = (uint64_t(copies) * segs + 3) / 4;
+	if (blocks > 0x7fffffffu) return hipErrorInvalidValue;
+	double2 *from = state, *to = scratch;
+	for (uint32_t done = 0; done < steps;) {
+		const uint32_t k = std::min(T, steps - done);
+		hipLaunchKernelGGL((k_step_tiled<MODEL, S, FORM>), dim3(uint32_t(blocks)), dim3(256), 0, st, from, to, n, copies,
+		                   segs, k, a, mp2);
+		hipError_t e = hipGetLastError();
+		if (e != hipSuccess) return e;
+		std::swap(from, to);
+		done += k;
+	}
+	if (from != state)
+		return hipMemcpyAsync(state, from, sizeof(double2) * size_t(n) * copies, hipMemcpyDeviceToDevice, st);
+	return hipSuccess;
+}
+
+template <int MODEL, int S>
+hipError_t launch_tiled(hipStream_t st, double2 *state, double2 *scratch, uint32_t copies, uint32_t n, uint32_t steps,
+                        const StepArgs &a, const double *mp2) {
+	if constexpr (MODEL != TODA)
+		if (a.form == kFormCubic) return launch_tiled_form<MODEL, S, kFormCubic>(st, state, scratch, copies, n, steps, a, mp2);
+	if constexpr (MODEL == FPUT)
+		if (a.form == kFormQuad) return launch_tiled_form<MODEL, S, kFormQuad>(st, state, scratch, copies, n, steps, a, mp2);
+	return launch_tiled_form<MODEL, S, kFormPlain>(st, state, scratch, copies, n, steps, a, mp2);
+}
+#endif
 
 #ifdef NLC_STEP_RAGGED
 // Persistent step over a ragged ensemble: every item is one wavefront's worth
@@ -243,6 +337,14 @@ hipError_t launch_ragged(hipStream_t st, double2 *state, const RaggedItem *items
 #endif
 
 } // namespace
+
+#ifdef NLC_STEP_TILED
+// Toda: 64 x 16 sites with the table-driven exp (T = 9); the others 64 x 32 (T = 18).
+TiledLauncher NLC_STEP_TILED() {
+	constexpr int S = NLC_STEP_MODEL == TODA ? 16 : 32;
+	return &launch_tiled<NLC_STEP_MODEL, S>;
+}
+#endif
 
 #ifdef NLC_STEP_RAGGED
 RaggedLauncher NLC_STEP_RAGGED(int S) {

@@ -216,6 +216,7 @@ def test_cli_dnls_run(gpu, tmp_path):
     ("DNKG", 4096, 3, ["-m", 1, "--beta", 1]),                    # workgroup-resident kernel
     ("Toda", 3000, 2, ["--alpha", 1]),                            # workgroup kernel, ragged lanes
     ("FPUT", 100, 5, ["--alpha", 0.25, "--beta", 1, "--split_kernel"]),  # reference's --split_kernel
+    ("DNKG", 20000, 2, ["-m", 1, "--beta", 1]),                  # temporally blocked kernel
 ])
 def test_cli_long_chains_and_split_kernel(gpu, tmp_path, prog, n, c, extra):
     kw = {"DNKG": dict(m=1.0, beta=1.0), "Toda": dict(alpha=1.0), "FPUT": dict(alpha=0.25, beta=1.0)}[prog]
@@ -225,6 +226,7 @@ def test_cli_long_chains_and_split_kernel(gpu, tmp_path, prog, n, c, extra):
     p = run(prog, "-n", n, "-c", c, "--dt", 0.05, "-b", 20, "-s", 40, "-p", pre, "-i", tmp_path / "init", "-v", *extra,
             check=0)
     assert ("kernel=split" in p.stderr) == ("--split_kernel" in extra), p.stderr
+    assert ("kernel=tiled" in p.stderr) == (n > 16384), p.stderr
     want = O.move_real(getattr(O, prog.upper()), s0, 40, 0.05, **kw)
     got = np.fromfile(f"{pre}-40").reshape(c, n, 2)
     assert np.abs(got - want).max() <= 1e-10 * np.abs(want).max()

@@ -129,9 +129,45 @@ def test_move_ragged_lanes(gpu, model_name, n):
 @pytest.mark.parametrize("model_name", ["DNKG", "FPUT", "Toda", "dDNKG"])
 @pytest.mark.parametrize("n", [3, 64, 2500])
 def test_move_split_kernel(gpu, model_name, n):
-    """The HBM-streaming kernel (reference --split_kernel, and the fallback
-    for chains longer than 16384 sites)."""
+    """The HBM-streaming kernel (the reference's --split_kernel)."""
     check_move(gpu, model_name, n, 33, steps=40, kernel=gpu.KERNEL_SPLIT)
+
+
+@pytest.mark.parametrize("model_name", ["DNKG", "FPUT", "Toda", "dDNKG"])
+@pytest.mark.parametrize("n,steps", [(16385, 40), (20000, 36)])
+def test_move_tiled_default(gpu, model_name, n, steps):
+    """Chains longer than 16384 sites run on the temporally blocked kernel by
+    default (launches of 18 steps, 9 for Toda: odd and even launch counts)."""
+    check_move(gpu, model_name, n, 2, steps=steps, plane=True, expect="kernel=tiled")
+
+
+@pytest.mark.parametrize("model_name", ["DNKG", "FPUT", "Toda", "dDNKG"])
+@pytest.mark.parametrize("n", [300, 1000, 5000])
+def test_move_tiled_forced(gpu, model_name, n):
+    """Forced on shorter chains; at 300 and 1000 sites a segment (2048 sites,
+    1024 for Toda) wraps around the ring more than once."""
+    check_move(gpu, model_name, n, 5, steps=25, plane=True, kernel=gpu.KERNEL_TILED, expect="kernel=tiled")
+
+
+@pytest.mark.parametrize("model_name,kw", [("DNKG", dict(m=1.0, beta=0.3)), ("FPUT", dict(alpha=0.3, beta=0.0)),
+                                           ("FPUT", dict(alpha=0.25, beta=1.0))])
+def test_move_tiled_force_forms(gpu, model_name, kw):
+    check_move(gpu, model_name, 17000, 2, steps=20, kw=kw, dens=0.1, expect="kernel=tiled")
+
+
+def test_autotune_long_chain_includes_tiled(gpu):
+    N = gpu
+    model, kw, masses, s0 = make_case(N, "DNKG", 20000, 4, 3)
+    with N.Ensemble(model, 20000, 4, 0.05, **kw) as ens:
+        ens.upload(s0)
+        report = ens.autotune(20)
+        desc = ens.describe()
+        banner(f"autotune n=20000: {report} -> {desc}")
+        assert "tiled" in report and "split" in report
+        ens.move(30)
+        got = ens.download()
+    want = oracle_move(model, kw, masses, s0, 30, 0.05)
+    assert rel_err(got, want) <= STATE_TOL
 
 
 @pytest.mark.parametrize("model_name", ["DNKG", "FPUT", "Toda", "dDNKG"])
