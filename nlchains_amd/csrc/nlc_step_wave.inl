@@ -221,11 +221,11 @@ k_step_tiled(const double2 *__restrict__ src, double2 *__restrict__ dst, uint32_
 	ch.etab = etab;
 	ch.full = true;
 	const double2 *in = src + size_t(copy) * n;
-	// extended position e = lane S + j is global site (seg C - H + e) mod n
-	const uint64_t g0 = uint64_t(seg) * C + n - H; // n > H: no negative index
+	// extended position e = lane S + j is global site (seg C - H + e) mod n; one
+	// 64-bit reduction per lane (n > H >= S: the sites of a lane wrap at most once)
+	uint32_t g = uint32_t((uint64_t(seg) * C + n - H + uint32_t(lane * S)) % n);
 #pragma unroll
-	for (int j = 0; j < S; ++j) {
-		const uint32_t g = uint32_t((g0 + uint32_t(lane * S + j)) % n);
+	for (int j = 0; j < S; ++j, g = (g + 1 == n) ? 0 : g + 1) {
 		const double2 v = to_registers<MODEL, FORM>(in[g], a);
 		ch.q[j] = v.x;
 		ch.p[j] = v.y;
@@ -339,7 +339,9 @@ hipError_t launch_ragged(hipStream_t st, double2 *state, const RaggedItem *items
 } // namespace
 
 #ifdef NLC_STEP_TILED
-// Toda: 64 x 16 sites with the table-driven exp (T = 9); the others 64 x 32 (T = 18).
+// Toda: 64 x 16 sites with the table-driven exp (T = 9); the others 64 x 32
+// (T = 18: every launch reads and writes the whole state, about 0.14 ms at
+// 2^24 sites, so 64 x 16 with T = 9 measured 15 % slower).
 TiledLauncher NLC_STEP_TILED() {
 	constexpr int S = NLC_STEP_MODEL == TODA ? 16 : 32;
 	return &launch_tiled<NLC_STEP_MODEL, S>;
