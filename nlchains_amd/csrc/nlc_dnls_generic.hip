@@ -88,13 +88,28 @@ k_dnls_lds(double2 *__restrict__ state, uint32_t n, uint32_t copies, uint32_t st
 	}
 }
 
-template <uint32_t MAXN>
+// Chains per workgroup of k_dnls_mixed below 512 sites: ceil(n/4) threads per
+// chain (no pass then has more than 2 butterflies per thread), kT / that many
+// chains; the kT mod nthr threads left over run a dummy slot of their own.
+__host__ __device__ inline uint32_t mixed_chain_threads(uint32_t n) { return (n + 3) / 4; }
+
+// SHARED: several chains per workgroup (n <= 512), MAXN = 1024 bounds n per chunk.
+template <uint32_t MAXN, bool SHARED>
 __global__ void __launch_bounds__(kT)
 k_dnls_mixed(double2 *__restrict__ state, uint32_t n, uint32_t copies, uint32_t steps, DnlsArgs a, MixedPlan plan,
              const double2 *__restrict__ tw, const double2 *__restrict__ table) {
 	extern __shared__ double2 buf[];
-	dnls_lds_batch<kT>(buf, state + size_t(blockIdx.x) * n, true, n, threadIdx.x, kT, steps, a, table,
-	                   [&](double2 *b) { mixed::lds_fft<MAXN, kT>(b, n, tw, plan); });
+	if constexpr (SHARED) {
+		const uint32_t nthr = mixed_chain_threads(n), per_group = kT / nthr;
+		const uint32_t sub = threadIdx.x / nthr, tid = threadIdx.x % nthr; // sub == per_group: the dummy slot
+		const uint32_t chain = blockIdx.x * per_group + sub;
+		const bool live = sub < per_group && chain < copies;
+		dnls_lds_batch<0>(buf + size_t(sub) * n, state + size_t(live ? chain : 0) * n, live, n, tid, nthr, steps, a,
+		                  table, [&](double2 *b) { mixed::lds_fft<MAXN, kT, false>(b, n, tw, plan, tid, nthr); });
+	} else {
+		dnls_lds_batch<kT>(buf, state + size_t(blockIdx.x) * n, true, n, threadIdx.x, kT, steps, a, table,
+		                   [&](double2 *b) { mixed::lds_fft<MAXN, kT>(b, n, tw, plan); });
+	}
 }
 
 // Cyclic convolution of buf[0..M) with b_j = conj(w_j) (|j| < n, zero
@@ -229,15 +244,20 @@ hipError_t launch_dnls_generic(hipStream_t st, double2 *state, uint32_t n, uint3
 			                   a, tw, table);
 		}
 	} else if (MixedPlan plan; mixed_radix_plan(n, &plan)) {
-		const size_t lds = sizeof(double2) * n;
-		if (n <= 4096) {
-			hipLaunchKernelGGL(k_dnls_mixed<4096>, dim3(copies), dim3(kT), lds, st, state, n, copies, steps, a, plan, tw,
-			                   table);
+		const uint32_t nthr = mixed_chain_threads(n), per_group = kT / nthr;
+		if (per_group > 1) {
+			const uint32_t slots = (kT + nthr - 1) / nthr; // with the dummy slot of the left-over threads
+			hipLaunchKernelGGL((k_dnls_mixed<4 * kT, true>), dim3((copies + per_group - 1) / per_group), dim3(kT),
+			                   sizeof(double2) * n * slots, st, state, n, copies, steps, a, plan, tw, table);
+		} else if (n <= 4096) {
+			hipLaunchKernelGGL((k_dnls_mixed<4096, false>), dim3(copies), dim3(kT), sizeof(double2) * n, st, state, n,
+			                   copies, steps, a, plan, tw, table);
 		} else {
-			hipError_t e = launch_lds_chain_kernel(k_dnls_mixed<kDnlsLdsMax>, lds);
+			const size_t lds = sizeof(double2) * n;
+			hipError_t e = launch_lds_chain_kernel(k_dnls_mixed<kDnlsLdsMax, false>, lds);
 			if (e != hipSuccess) return e;
-			hipLaunchKernelGGL(k_dnls_mixed<kDnlsLdsMax>, dim3(copies), dim3(kT), lds, st, state, n, copies, steps, a,
-			                   plan, tw, table);
+			hipLaunchKernelGGL((k_dnls_mixed<kDnlsLdsMax, false>), dim3(copies), dim3(kT), lds, st, state, n, copies, steps,
+			                   a, plan, tw, table);
 		}
 	} else if (const uint32_t M = dnls_bluestein_size(n)) {
 		const size_t lds = sizeof(double2) * M;

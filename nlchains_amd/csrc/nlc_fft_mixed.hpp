@@ -21,15 +21,20 @@ __device__ __forceinline__ double2 mul(double2 a, double2 b) {
 	return make_double2(fma(a.x, b.x, -a.y * b.y), fma(a.x, b.y, a.y * b.x));
 }
 
-// One pass of radix P over n points in buf by NT threads (tw[t] = e^{-2 pi i t/n}).
-template <uint32_t MAXN, uint32_t NT, uint32_t P>
-__device__ void stockham_pass(double2 *buf, uint32_t n, uint32_t ns, const double2 *__restrict__ tw) {
+// One pass of radix P over n points in buf (tw[t] = e^{-2 pi i t/n}) by the
+// NT threads of the block (WHOLE), or by threads tid < nthr when several
+// chains share a block of NT threads (then n nthr-chunks stay within MAXN: at
+// most kPer butterflies per thread).
+template <uint32_t MAXN, uint32_t NT, uint32_t P, bool WHOLE>
+__device__ void stockham_pass(double2 *buf, uint32_t n, uint32_t ns, const double2 *__restrict__ tw, uint32_t tid,
+                              uint32_t nthr_rt) {
 	constexpr uint32_t kPer = (MAXN / P + NT - 1) / NT; // butterflies per thread
-	const uint32_t m = n / P, span = n / (ns * P), tid = threadIdx.x;
+	const uint32_t nthr = WHOLE ? NT : nthr_rt;
+	const uint32_t m = n / P, span = n / (ns * P);
 	double2 v[kPer][P];
 #pragma unroll
 	for (uint32_t q = 0; q < kPer; ++q) {
-		const uint32_t j = tid + q * NT;
+		const uint32_t j = tid + q * nthr;
 		if (j < m)
 #pragma unroll
 			for (uint32_t r = 0; r < P; ++r) v[q][r] = buf[j + r * m];
@@ -39,7 +44,7 @@ __device__ void stockham_pass(double2 *buf, uint32_t n, uint32_t ns, const doubl
 	if constexpr (P != 2 && P != 4) small_roots<P>(root, tw, n);
 #pragma unroll
 	for (uint32_t q = 0; q < kPer; ++q) {
-		const uint32_t j = tid + q * NT;
+		const uint32_t j = tid + q * nthr;
 		if (j < m) {
 			const uint32_t k = j % ns;
 			double2 a[P], y[P];
@@ -72,16 +77,18 @@ __device__ void stockham_pass(double2 *buf, uint32_t n, uint32_t ns, const doubl
 
 // Forward n-point FFT in LDS through the passes of `plan`; every thread of the
 // block calls it (it contains barriers).
-template <uint32_t MAXN, uint32_t NT>
-__device__ void lds_fft(double2 *buf, uint32_t n, const double2 *__restrict__ tw, const MixedPlan &plan) {
+template <uint32_t MAXN, uint32_t NT, bool WHOLE = true>
+__device__ void lds_fft(double2 *buf, uint32_t n, const double2 *__restrict__ tw, const MixedPlan &plan,
+                        uint32_t tid = 0, uint32_t nthr = NT) {
+	if (WHOLE) tid = threadIdx.x;
 	uint32_t ns = 1;
 	for (uint32_t i = 0; i < plan.passes; ++i) {
 		switch (plan.radix[i]) {
-		case 2: stockham_pass<MAXN, NT, 2>(buf, n, ns, tw); break;
-		case 3: stockham_pass<MAXN, NT, 3>(buf, n, ns, tw); break;
-		case 4: stockham_pass<MAXN, NT, 4>(buf, n, ns, tw); break;
-		case 5: stockham_pass<MAXN, NT, 5>(buf, n, ns, tw); break;
-		default: stockham_pass<MAXN, NT, 7>(buf, n, ns, tw); break;
+		case 2: stockham_pass<MAXN, NT, 2, WHOLE>(buf, n, ns, tw, tid, nthr); break;
+		case 3: stockham_pass<MAXN, NT, 3, WHOLE>(buf, n, ns, tw, tid, nthr); break;
+		case 4: stockham_pass<MAXN, NT, 4, WHOLE>(buf, n, ns, tw, tid, nthr); break;
+		case 5: stockham_pass<MAXN, NT, 5, WHOLE>(buf, n, ns, tw, tid, nthr); break;
+		default: stockham_pass<MAXN, NT, 7, WHOLE>(buf, n, ns, tw, tid, nthr); break;
 		}
 		ns *= plan.radix[i];
 	}

@@ -37,6 +37,9 @@ def run(N, n, copies, steps, dt, beta, s0, spl=0):
                                             # 1024/n chains per LDS workgroup: several groups, the
                                             # last one partly filled
                                             (2, 600, 50), (8, 300, 50), (64, 100, 20), (512, 5, 30),
+                                            # mixed radix, several chains per workgroup with left-over
+                                            # threads: 25 and 125 threads per chain
+                                            (100, 33, 20), (500, 5, 20), (6, 300, 30),
                                             # mixed-radix Stockham (factors 2, 3, 5, 7 only)
                                             (3, 5, 100), (5, 4, 100), (6, 5, 100), (7, 4, 100), (100, 5, 100),
                                             (1000, 3, 30), (1225, 2, 8), (3000, 2, 5), (6561, 1, 1),
