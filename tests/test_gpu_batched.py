@@ -87,24 +87,33 @@ def test_ragged_linenergies_per_class(gpu):
         at += n
 
 
-@pytest.mark.parametrize("ragged", [False, True])
-def test_graph_batches_match_eager(gpu, ragged):
+@pytest.mark.parametrize("kind", ["wave", "ragged", "tiled"])
+def test_graph_batches_match_eager(gpu, kind):
     """nlc_batch replays a captured graph; every replay (with eager work and
-    synchronisations in between) equals the eager move + energy pass."""
+    synchronisations in between) equals the eager move + energy pass.  The
+    tiled case (20000 sites, 40 steps = launches of 18, 18 and 4) ends its
+    batch with a device-to-device copy node."""
     N = gpu
-    if ragged:
+    steps = 25
+    if kind == "ragged":
         lengths = N.ic_ragged_lengths(120, [32, 64, 128], 2)
         s0 = N.ic_ragged_plane_waves(lengths, 1.0, 0.05, 0.1, 0.3, 2)
         mk = lambda: N.Ensemble.ragged(N.DNKG, lengths, 0.05, m=1.0, beta=1.0)  # noqa: E731
+    elif kind == "tiled":
+        s0 = N.ic_plane_waves(20000, 3, 1.0, 0, 64, 0.3, 2)
+        mk = lambda: N.Ensemble(N.DNKG, 20000, 3, 0.05, m=1.0, beta=1.0)  # noqa: E731
+        steps = 40
     else:
         s0 = N.ic_plane_waves(1024, 64, 1.0, 0, 128, 0.5, 2)
         mk = lambda: N.Ensemble(N.DNKG, 1024, 64, 0.05, m=1.0, beta=1.0)  # noqa: E731
     with mk() as g, mk() as e:
+        if kind == "tiled":
+            assert "kernel=tiled" in g.describe()
         g.upload(s0)
         e.upload(s0)
         for rep in range(4):
-            eg, sg = g.batch(25)
-            e.move(25)
+            eg, sg = g.batch(steps)
+            e.move(steps)
             ee, se = e.linenergies()
             assert np.array_equal(g.download(), e.download()), rep
             assert np.array_equal(eg, ee) and np.array_equal(sg, se), rep
