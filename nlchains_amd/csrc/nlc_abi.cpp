@@ -300,16 +300,16 @@ bool ragged_fits(const std::vector<Klass> &cls, int S) {
 
 void drop_graph(nlc_ctx *c) {
 	// a queued replay may still be running
-	if (c->slot_count) hipStreamSynchronize(c->stream);
+	if (c->slot_count) (void) hipStreamSynchronize(c->stream);
 	for (auto &g : c->graphs) {
-		if (g.exec) hipGraphExecDestroy(g.exec);
-		if (g.graph) hipGraphDestroy(g.graph);
+		if (g.exec) (void) hipGraphExecDestroy(g.exec);
+		if (g.graph) (void) hipGraphDestroy(g.graph);
 		g = nlc_ctx::BatchGraph();
 	}
 }
 
 void free_all(nlc_ctx *c) {
-	if (c->stream) hipStreamSynchronize(c->stream);
+	if (c->stream) (void) hipStreamSynchronize(c->stream);
 	drop_graph(c);
 	for (void *ptr :
 	     {(void *) c->state, (void *) c->omega, (void *) c->tw, (void *) c->partial, (void *) c->reduce_scratch,
@@ -317,16 +317,16 @@ void free_all(nlc_ctx *c) {
 	      (void *) c->eigvec, (void *) c->split_q[0], (void *) c->split_p[0], (void *) c->split_q[1], (void *) c->split_p[1], (void *) c->energy_d, (void *) c->norm_d,
 	      (void *) c->dnls_table, (void *) c->fft_scratch, (void *) c->blue_table, (void *) c->items, (void *) c->vplanes, (void *) c->splanes,
 	      (void *) c->tile_buf})
-		if (ptr) hipFree(ptr);
-	if (c->h_results) hipHostFree(c->h_results);
-	if (c->h_slots) hipHostFree(c->h_slots);
+		if (ptr) (void) hipFree(ptr);
+	if (c->h_results) (void) hipHostFree(c->h_results);
+	if (c->h_slots) (void) hipHostFree(c->h_slots);
 	for (hipEvent_t ev : c->slot_ev)
-		if (ev) hipEventDestroy(ev);
+		if (ev) (void) hipEventDestroy(ev);
 	for (int i = 0; i < 2; ++i) {
-		if (c->pinned[i]) hipHostFree(c->pinned[i]);
-		if (c->pinned_ev[i]) hipEventDestroy(c->pinned_ev[i]);
+		if (c->pinned[i]) (void) hipHostFree(c->pinned[i]);
+		if (c->pinned_ev[i]) (void) hipEventDestroy(c->pinned_ev[i]);
 	}
-	if (c->stream) hipStreamDestroy(c->stream);
+	if (c->stream) (void) hipStreamDestroy(c->stream);
 }
 
 template <class T>
@@ -457,7 +457,7 @@ int setup_dnls(nlc_ctx *c) {
 		// scratch per chain
 		const size_t need = size_t(L) * c->copies;
 		if (c->fft_scratch_sites < need) {
-			if (c->fft_scratch) hipFree(c->fft_scratch);
+			if (c->fft_scratch) (void) hipFree(c->fft_scratch);
 			c->fft_scratch = nullptr;
 			c->fft_scratch_sites = 0;
 			NLC_TRY(dev_alloc(c, &c->fft_scratch, need, "DNLS Bluestein buffer"));
@@ -469,7 +469,7 @@ int setup_dnls(nlc_ctx *c) {
 		nlc::dnls_two_level_tables(n, tab.data(), perm.data());
 		tab.swap(perm);
 		if (c->fft_scratch_sites < c->total_sites) {
-			if (c->fft_scratch) hipFree(c->fft_scratch);
+			if (c->fft_scratch) (void) hipFree(c->fft_scratch);
 			c->fft_scratch = nullptr;
 			c->fft_scratch_sites = 0;
 			NLC_TRY(dev_alloc(c, &c->fft_scratch, c->total_sites, "DNLS spectrum"));
@@ -542,7 +542,7 @@ int select_uniform(nlc_ctx *c) {
 // one uniform kernel per class when no S fits every class.
 int select_ragged(nlc_ctx *c, int S) {
 	if (c->items) {
-		hipFree(c->items);
+		(void) hipFree(c->items);
 		c->items = nullptr;
 	}
 	c->ragged_launcher = S ? lookup_ragged(c->p.model, S) : nullptr;
@@ -712,8 +712,8 @@ int ensure_batch_graph(nlc_ctx *c, uint32_t which, uint32_t steps, double norm, 
 	if (!bg.exec || bg.steps != steps || bg.norm != norm) {
 		if (bg.exec) {
 			if (c->slot_count) NLC_HIP(c, hipStreamSynchronize(c->stream));
-			hipGraphExecDestroy(bg.exec);
-			hipGraphDestroy(bg.graph);
+			(void) hipGraphExecDestroy(bg.exec);
+			(void) hipGraphDestroy(bg.graph);
 			bg = nlc_ctx::BatchGraph();
 		}
 		NLC_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
@@ -722,7 +722,7 @@ int ensure_batch_graph(nlc_ctx *c, uint32_t which, uint32_t steps, double norm, 
 		hipGraph_t g = nullptr;
 		const hipError_t e = hipStreamEndCapture(c->stream, &g);
 		if (rc) {
-			if (g) hipGraphDestroy(g);
+			if (g) (void) hipGraphDestroy(g);
 			return rc;
 		}
 		if (e != hipSuccess) return c->hip_fail(e, "hipStreamEndCapture");
@@ -1105,8 +1105,8 @@ static int tune_projection(nlc_ctx *c) {
 				rc = c->fail(NLC_EHIP, "autotune timing failed");
 		}
 	}
-	if (e0) hipEventDestroy(e0);
-	if (e1) hipEventDestroy(e1);
+	if (e0) (void) hipEventDestroy(e0);
+	if (e1) (void) hipEventDestroy(e1);
 	if (rc) return rc;
 	c->proj_variant = ms[nlc::kProjValu] < ms[nlc::kProjMfma] ? nlc::kProjValu : nlc::kProjMfma;
 	char buf[128];
@@ -1180,10 +1180,10 @@ int nlc_autotune(nlc_ctx *c, uint32_t trial_steps, char *report, size_t len) {
 			if (hipMemcpyAsync(c->state, backup, c->state_bytes, hipMemcpyDeviceToDevice, c->stream) && !rc)
 				rc = c->fail(NLC_EHIP, "autotune restore failed");
 		}
-		hipStreamSynchronize(c->stream);
-		if (e0) hipEventDestroy(e0);
-		if (e1) hipEventDestroy(e1);
-		hipFree(backup);
+		(void) hipStreamSynchronize(c->stream);
+		if (e0) (void) hipEventDestroy(e0);
+		if (e1) (void) hipEventDestroy(e1);
+		(void) hipFree(backup);
 		c->p = saved;
 		if (rc) return rc;
 		const Cand *best = &cands[0];

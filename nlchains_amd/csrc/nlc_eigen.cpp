@@ -51,7 +51,7 @@ struct DevBuf {
 		if (hipMalloc((void **) &p, sizeof(T) * count) != hipSuccess) p = nullptr;
 	}
 	~DevBuf() {
-		if (p) hipFree(p);
+		if (p) (void) hipFree(p);
 	}
 };
 
