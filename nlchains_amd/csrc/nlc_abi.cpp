@@ -466,6 +466,7 @@ int setup_dnls(nlc_ctx *c) {
 	} else if (nlc::dnls_two_level(n)) {
 		// the two-level FFT keeps the spectrum in HBM in [r][k] order: the
 		// multipliers in that order, and a scratch of the state's size
+		perm.resize(nlc::dnls_two_level_table_size(n));
 		nlc::dnls_two_level_tables(n, tab.data(), perm.data());
 		tab.swap(perm);
 		if (c->fft_scratch_sites < c->total_sites) {

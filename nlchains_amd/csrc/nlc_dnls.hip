@@ -267,6 +267,34 @@ k_dnls_4096(double2 *__restrict__ state, uint32_t copies, uint32_t steps, DnlsAr
 	for (int j = 0; j < 16; ++j) psi[t + 256 * j] = nonlinear(v[pos(j)], a.theta[7]);
 }
 
+// Row pass of the two-level FFT (nlc_dnls_long.hip) for rows of M = 4096
+// points: forward radix-16 FFT in registers, the multipliers T[r + R k] of
+// row r in this kernel's digit-reversed order (dnls_two_level_tables), the
+// inverse FFT, and the inverse column twiddle W_n^{-m r} (as k_dnls_row);
+// rows[row] in place, natural order in and out.  tw4096: the 4096-point
+// roots; twn: the n-point ones (n = 4096 R).
+__global__ void __launch_bounds__(kT, 2)
+k_dnls_rows4096(double2 *__restrict__ rows, uint32_t R, const double2 *__restrict__ tw4096,
+                const double2 *__restrict__ twn, const double2 *__restrict__ tk) {
+	__shared__ Lds4352 L;
+	const uint32_t row = blockIdx.x, t = threadIdx.x;
+	double2 *y = rows + size_t(row) * 4096;
+	double2 v[16];
+#pragma unroll
+	for (int j = 0; j < 16; ++j) v[pos(j)] = y[t + 256 * j];
+	fft4096_fwd(v, L, tw4096);
+	const uint32_t r = row % R;
+	const double2 *m = tk + size_t(r) * 4096 + t;
+#pragma unroll
+	for (int j = 0; j < 16; ++j) v[pos(j)] = cmul(v[pos(j)], m[256 * j]);
+	fft4096_inv(v, L, tw4096);
+#pragma unroll
+	for (int j = 0; j < 16; ++j) {
+		const uint32_t x = t + 256 * j; // x r < 4096 R = n
+		y[x] = cmul(v[pos(j)], cconj(twn[x * r]));
+	}
+}
+
 // ---------------------------------------------------------------- radix 8
 //
 // k_dnls_4096_r8: 512 threads x 8 complex values, 4096 = 8^4 in four radix-8
@@ -500,6 +528,12 @@ bool dnls_supported(uint32_t n) {
 //   radix 16: entry [k][256 j + t] = T_k[(t >> 4) + 16 (t & 15) + 256 j],
 //   radix 8:  entry [k][512 j + t] = T_k[(t >> 6) + 8 ((t >> 3) & 7) + 64 (t & 7) + 512 j].
 bool dnls_fast_path(uint32_t n) { return n == 4096; }
+
+hipError_t launch_dnls_rows4096(hipStream_t st, double2 *rows, uint32_t nrows, uint32_t R, const double2 *tw4096,
+                                const double2 *twn, const double2 *tk) {
+	hipLaunchKernelGGL(k_dnls_rows4096, dim3(nrows), dim3(kT), 0, st, rows, R, tw4096, twn, tk);
+	return hipGetLastError();
+}
 
 void dnls_permute_tables(uint32_t n, int variant, const double2 *table, double2 *out) {
 	for (int k = 0; k < 4; ++k)

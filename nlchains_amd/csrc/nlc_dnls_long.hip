@@ -93,7 +93,13 @@ hipError_t launch_dnls_two_level(hipStream_t st, double2 *state, double2 *scratc
 				hipLaunchKernelGGL((k_dnls_col<R, 0>), cols, block, 0, st, state, scratch, n, M, copies, th, tw);
 			else
 				hipLaunchKernelGGL((k_dnls_col<R, 1>), cols, block, 0, st, state, scratch, n, M, copies, th, tw);
-			hipLaunchKernelGGL(k_dnls_row<M>, rows, block, lds, st, scratch, R, tw, table + size_t(k) * n);
+			if constexpr (M == 4096) { // rows on the radix-16 register FFT
+				hipError_t e = launch_dnls_rows4096(st, scratch, copies * R, R, table + size_t(7) * n, tw,
+				                                    table + size_t(k) * n);
+				if (e != hipSuccess) return e;
+			} else {
+				hipLaunchKernelGGL(k_dnls_row<M>, rows, block, lds, st, scratch, R, tw, table + size_t(k) * n);
+			}
 		}
 	hipLaunchKernelGGL((k_dnls_col<R, 2>), cols, block, 0, st, state, scratch, n, M, copies, a.theta[7], tw);
 	return hipGetLastError();
@@ -262,14 +268,30 @@ bool dnls_two_level(uint32_t n) {
 }
 
 // Multipliers of the two-level path in the spectrum's [r][k] order:
-// out[k][r M + q] = table[k][r + R q].
+// out[k][r M + q] = table[k][r + R q].  With M = 4096 the rows run on the
+// radix-16 register FFT, whose spectrum is digit-reversed: slot e = t + 256 j
+// (thread t, register j) holds q = (t >> 4) + 16 (t & 15) + 256 j; and the
+// 4096-point roots e^{-2 pi i j/4096} follow at out[7 n, 7 n + 4096).
+size_t dnls_two_level_table_size(uint32_t n) {
+	uint32_t R, M;
+	if (!large_fft_split(n, R, M)) return 0;
+	return size_t(7) * n + (M == 4096 ? 4096 : 0);
+}
+
 void dnls_two_level_tables(uint32_t n, const double2 *table, double2 *out) {
 	uint32_t R, M;
 	if (!large_fft_split(n, R, M)) return;
 	for (int k = 0; k < 7; ++k)
 		for (uint32_t r = 0; r < R; ++r)
-			for (uint32_t q = 0; q < M; ++q)
-				out[size_t(k) * n + size_t(r) * M + q] = table[size_t(k) * n + r + size_t(R) * q];
+			for (uint32_t e = 0; e < M; ++e) {
+				const uint32_t q = M == 4096 ? (e & 255) / 16 + 16 * (e & 15) + 256 * (e >> 8) : e;
+				out[size_t(k) * n + size_t(r) * M + e] = table[size_t(k) * n + r + size_t(R) * q];
+			}
+	if (M == 4096)
+		for (uint32_t j = 0; j < 4096; ++j) {
+			const double ang = -2.0 * M_PI * double(j) / 4096.0;
+			out[size_t(7) * n + j] = make_double2(std::cos(ang), std::sin(ang));
+		}
 }
 
 uint32_t bluestein_two_level_size(uint32_t n) {

@@ -210,7 +210,13 @@ inline bool mixed_radix_plan(uint32_t n, MixedPlan *plan) {
 }
 void dnls_bluestein_tables(uint32_t n, double2 *out);
 bool dnls_two_level(uint32_t n);
+size_t dnls_two_level_table_size(uint32_t n);
 void dnls_two_level_tables(uint32_t n, const double2 *table, double2 *out);
+// Row pass of the two-level FFT for rows of 4096 points on the radix-16
+// register FFT (tk: the row-permuted multipliers of dnls_two_level_tables;
+// twn: the n-point roots for the inverse column twiddle).
+hipError_t launch_dnls_rows4096(hipStream_t, double2 *rows, uint32_t nrows, uint32_t R, const double2 *tw4096,
+                                const double2 *twn, const double2 *tk);
 bool dnls_fast_path(uint32_t n);
 void dnls_permute_tables(uint32_t n, int variant, const double2 *table, double2 *out);
 hipError_t launch_dnls(hipStream_t, double2 *state, uint32_t n, uint32_t copies, uint32_t steps,
