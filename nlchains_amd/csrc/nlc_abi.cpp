@@ -453,8 +453,12 @@ int setup_dnls(nlc_ctx *c) {
 		nlc::dnls_bluestein_tables(n, tab.data() + size_t(7) * n);
 	} else if (const uint32_t L = nlc::bluestein_two_level_size(n); L && !nlc::mixed_radix_plan(n, nullptr)) {
 		// Bluestein through the two-level FFT: the 7 natural tables (the
-		// Bluestein tables are shared with the energy pass) and L values of
-		// scratch per chain
+		// Bluestein tables are shared with the energy pass), the row-permuted
+		// copy of bhat for 4096-point rows, and L values of scratch per chain
+		if (const size_t extra = nlc::dnls_blue_row_table_size(n)) {
+			tab.resize(size_t(7) * n + extra);
+			nlc::dnls_blue_row_tables(n, tab.data() + size_t(7) * n);
+		}
 		const size_t need = size_t(L) * c->copies;
 		if (c->fft_scratch_sites < need) {
 			if (c->fft_scratch) (void) hipFree(c->fft_scratch);

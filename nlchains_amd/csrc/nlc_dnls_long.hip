@@ -212,6 +212,18 @@ hipError_t row_attribute() {
 template <uint32_t R, uint32_t M>
 hipError_t launch_dnls_blue(hipStream_t st, double2 *state, double2 *buf, uint32_t n, uint32_t copies, uint32_t steps,
                             const DnlsArgs &a, const double2 *table, const double2 *blue) {
+	// the row pass of a convolution: 4096-point rows on the register FFT with
+	// the permuted bhat of dnls_blue_row_tables (after the 7 multipliers), else LDS
+	auto blue_rows = [](hipStream_t st, double2 *buf, uint32_t copies, dim3 rows, size_t lds, uint32_t n,
+	                    const double2 *twL, const double2 *bhat, const double2 *table) -> hipError_t {
+		if constexpr (M == 4096) {
+			const double2 *perm = table + size_t(7) * n;
+			return launch_dnls_rows4096(st, buf, copies * R, R, perm + R * M, twL, perm);
+		} else {
+			hipLaunchKernelGGL(k_dnls_row<M>, rows, dim3(kT), lds, st, buf, R, twL, bhat);
+			return hipGetLastError();
+		}
+	};
 	if (hipError_t e = row_attribute<M>()) return e;
 	constexpr uint32_t L = R * M;
 	const double2 *bhat = blue, *twL = blue + L, *chirp = blue + 2 * L;
@@ -224,11 +236,11 @@ hipError_t launch_dnls_blue(hipStream_t st, double2 *state, double2 *buf, uint32
 				hipLaunchKernelGGL((k_blue_fwd<R, 0>), cols, block, 0, st, state, buf, n, L, M, copies, th, twL, chirp);
 			else
 				hipLaunchKernelGGL((k_blue_fwd<R, 1>), cols, block, 0, st, state, buf, n, L, M, copies, th, twL, chirp);
-			hipLaunchKernelGGL(k_dnls_row<M>, rows, block, lds, st, buf, R, twL, bhat);
+			if (hipError_t e = blue_rows(st, buf, copies, rows, lds, n, twL, bhat, table)) return e;
 			hipLaunchKernelGGL((k_blue_inv<R, 1>), cols, block, 0, st, buf, state, n, L, M, copies, 0.0, twL, chirp,
 			                   table + size_t(k) * n);
 			hipLaunchKernelGGL((k_blue_fwd<R, 2>), cols, block, 0, st, state, buf, n, L, M, copies, 0.0, twL, chirp);
-			hipLaunchKernelGGL(k_dnls_row<M>, rows, block, lds, st, buf, R, twL, bhat);
+			if (hipError_t e = blue_rows(st, buf, copies, rows, lds, n, twL, bhat, table)) return e;
 			if (s + 1 == steps && k == 6)
 				hipLaunchKernelGGL((k_blue_inv<R, 3>), cols, block, 0, st, buf, state, n, L, M, copies, a.theta[7], twL,
 				                   chirp, nullptr);
@@ -299,6 +311,33 @@ uint32_t bluestein_two_level_size(uint32_t n) {
 	uint32_t L = 1;
 	while (L < 2 * n - 1) L <<= 1;
 	return L;
+}
+
+// DNLS through Bluestein with rows of 4096 points: the chirp's transform bhat
+// in the row kernel's digit-reversed order (row r: slot e holds the [r][k]
+// entry k = (e & 255)/16 + 16 (e & 15) + 256 (e >> 8)), then the 4096-point
+// roots; appended after the 7 multipliers.  0 entries for other lengths.
+size_t dnls_blue_row_table_size(uint32_t n) {
+	const uint32_t L = bluestein_two_level_size(n);
+	uint32_t R, M;
+	return (L && large_fft_split(L, R, M) && M == 4096) ? size_t(L) + 4096 : 0;
+}
+
+void dnls_blue_row_tables(uint32_t n, double2 *out) {
+	const uint32_t L = bluestein_two_level_size(n);
+	uint32_t R, M;
+	if (!L || !large_fft_split(L, R, M) || M != 4096) return;
+	std::vector<double2> blue(size_t(2) * L + n);
+	bluestein_tables(n, L, R, blue.data()); // bhat in [r][k] order at [0, L)
+	for (uint32_t r = 0; r < R; ++r)
+		for (uint32_t e = 0; e < 4096; ++e) {
+			const uint32_t k = (e & 255) / 16 + 16 * (e & 15) + 256 * (e >> 8);
+			out[size_t(r) * 4096 + e] = blue[size_t(r) * 4096 + k];
+		}
+	for (uint32_t j = 0; j < 4096; ++j) {
+		const double ang = -2.0 * M_PI * double(j) / 4096.0;
+		out[size_t(L) + j] = make_double2(std::cos(ang), std::sin(ang));
+	}
 }
 
 void bluestein_two_level_tables(uint32_t n, double2 *out) {

@@ -211,6 +211,8 @@ inline bool mixed_radix_plan(uint32_t n, MixedPlan *plan) {
 void dnls_bluestein_tables(uint32_t n, double2 *out);
 bool dnls_two_level(uint32_t n);
 size_t dnls_two_level_table_size(uint32_t n);
+size_t dnls_blue_row_table_size(uint32_t n);
+void dnls_blue_row_tables(uint32_t n, double2 *out);
 void dnls_two_level_tables(uint32_t n, const double2 *table, double2 *out);
 // Row pass of the two-level FFT for rows of 4096 points on the radix-16
 // register FFT (tk: the row-permuted multipliers of dnls_two_level_tables;
