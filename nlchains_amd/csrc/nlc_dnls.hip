@@ -554,8 +554,7 @@ hipError_t launch_dnls(hipStream_t st, double2 *state, uint32_t n, uint32_t copi
                        const double *theta9, const double2 *tw, const double2 *table, int variant, double2 *scratch,
                        const double2 *blue) {
 	if (!dnls_fast_path(n)) {
-		uint32_t R, M;
-		if (!mixed_radix_plan(n, nullptr) && (bluestein_two_level_size(n) || large_fft_split(n, R, M)))
+		if (!mixed_radix_plan(n, nullptr) && (bluestein_two_level_size(n) || dnls_two_level(n)))
 			return launch_dnls_long(st, state, n, copies, steps, theta9, tw, table, scratch, blue);
 		return launch_dnls_generic(st, state, n, copies, steps, theta9, tw, table);
 	}

@@ -274,9 +274,20 @@ hipError_t launch_energy_blue_rm(hipStream_t st, const double2 *state, uint32_t 
 
 } // namespace
 
+// The DNLS step's two-level split: large_fft_split, and n = 8192 as 2 rows of
+// 4096 (on the register FFT those beat the 8192-point LDS kernel: one HBM
+// round trip per pass is cheaper than the LDS passes).
+bool dnls_split(uint32_t n, uint32_t &R, uint32_t &M) {
+	if (n == 8192) {
+		R = 2, M = 4096;
+		return true;
+	}
+	return large_fft_split(n, R, M);
+}
+
 bool dnls_two_level(uint32_t n) {
 	uint32_t R, M;
-	return large_fft_split(n, R, M);
+	return dnls_split(n, R, M);
 }
 
 // Multipliers of the two-level path in the spectrum's [r][k] order:
@@ -286,13 +297,13 @@ bool dnls_two_level(uint32_t n) {
 // 4096-point roots e^{-2 pi i j/4096} follow at out[7 n, 7 n + 4096).
 size_t dnls_two_level_table_size(uint32_t n) {
 	uint32_t R, M;
-	if (!large_fft_split(n, R, M)) return 0;
+	if (!dnls_split(n, R, M)) return 0;
 	return size_t(7) * n + (M == 4096 ? 4096 : 0);
 }
 
 void dnls_two_level_tables(uint32_t n, const double2 *table, double2 *out) {
 	uint32_t R, M;
-	if (!large_fft_split(n, R, M)) return;
+	if (!dnls_split(n, R, M)) return;
 	for (int k = 0; k < 7; ++k)
 		for (uint32_t r = 0; r < R; ++r)
 			for (uint32_t e = 0; e < M; ++e) {
@@ -386,10 +397,11 @@ hipError_t launch_dnls_long(hipStream_t st, double2 *state, uint32_t n, uint32_t
 		}
 		return hipErrorInvalidValue;
 	}
-	if (large_fft_split(n, R, M)) {
+	if (dnls_split(n, R, M)) {
 		if (!scratch) return hipErrorInvalidValue;
 		if (M == 8192) return launch_dnls_two_level<16, 8192>(st, state, scratch, n, copies, steps, a, tw, table);
 		switch (R) {
+		case 2: return launch_dnls_two_level<2, 4096>(st, state, scratch, n, copies, steps, a, tw, table);
 		case 4: return launch_dnls_two_level<4, 4096>(st, state, scratch, n, copies, steps, a, tw, table);
 		case 8: return launch_dnls_two_level<8, 4096>(st, state, scratch, n, copies, steps, a, tw, table);
 		case 16: return launch_dnls_two_level<16, 4096>(st, state, scratch, n, copies, steps, a, tw, table);

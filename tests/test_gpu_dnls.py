@@ -193,10 +193,11 @@ def test_dnls_bluestein_two_level_linear_closed_form(gpu, n):
     np.testing.assert_allclose(n1, n0, rtol=1e-12)
 
 
-@pytest.mark.parametrize("n", [101, 8192, 32768, 5003, 100, 5000])
+@pytest.mark.parametrize("n", [101, 2048, 8192, 32768, 5003, 100, 5000])
 def test_dnls_generic_kernels_describe_and_compose(gpu, n):
-    """The Bluestein, 8192-point LDS, two-level and mixed-radix kernels: kernel
-    names, and two batches of 10 steps equal one of 20 to rounding."""
+    """The Bluestein, LDS Stockham, two-level (8192 sites: two rows of 4096)
+    and mixed-radix kernels: kernel names, and two batches of 10 steps equal
+    one of 20 to rounding."""
     N = gpu
     s0 = N.ic_dnls(n, 3, max(1, min(n // 16, 256)), 1.0, 5)
     with N.Ensemble(N.DNLS, n, 3, 0.01, beta=1.0) as ens:
@@ -209,7 +210,7 @@ def test_dnls_generic_kernels_describe_and_compose(gpu, n):
         ens.move(20)
         b = ens.download()
     banner(f"  {desc}")
-    assert ("lds_bluestein" in desc) == (n == 101) and ("stockham" in desc) == (n == 8192)
-    assert ("two_level_fft" in desc) == (n == 32768) and ("bluestein_two_level" in desc) == (n == 5003)
+    assert ("lds_bluestein" in desc) == (n == 101) and ("stockham" in desc) == (n == 2048)
+    assert ("two_level_fft" in desc) == (n in (8192, 32768)) and ("bluestein_two_level" in desc) == (n == 5003)
     assert ("mixed_radix" in desc) == (n in (100, 5000))
     assert rel_err(a, b) < 1e-12
