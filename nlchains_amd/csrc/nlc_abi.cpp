@@ -30,6 +30,7 @@ struct Klass {
 	nlc::StepLauncher launcher = nullptr;
 	int L = 0, S = 0;
 	bool exact = false;
+	int side = 0; // per-class launches: 0 = context stream, 1 = the second stream
 };
 
 } // namespace
@@ -55,6 +56,10 @@ struct nlc_ctx {
 	uint32_t nitems = 0;
 	int ragged_S = 0;
 	nlc::RaggedLauncher ragged_launcher = nullptr;
+	// per-class launches run on two streams (fork and join by events), so that
+	// the waves of one class fill the tail of another class's launch
+	hipStream_t aux_stream = nullptr;
+	hipEvent_t fork_ev = nullptr, join_ev = nullptr;
 
 	double2 *state = nullptr;
 	double *omega = nullptr;   // per-class dispersion, table_len
@@ -326,6 +331,9 @@ void free_all(nlc_ctx *c) {
 		if (c->pinned[i]) (void) hipHostFree(c->pinned[i]);
 		if (c->pinned_ev[i]) (void) hipEventDestroy(c->pinned_ev[i]);
 	}
+	if (c->fork_ev) (void) hipEventDestroy(c->fork_ev);
+	if (c->join_ev) (void) hipEventDestroy(c->join_ev);
+	if (c->aux_stream) (void) hipStreamDestroy(c->aux_stream);
 	if (c->stream) (void) hipStreamDestroy(c->stream);
 }
 
@@ -585,8 +593,18 @@ int enqueue_move(nlc_ctx *c, uint32_t steps) {
 		if (c->ragged_launcher) {
 			NLC_HIP(c, c->ragged_launcher(c->stream, c->state, c->items, c->nitems, steps, c->args, 0));
 		} else {
+			const bool fork = c->aux_stream != nullptr;
+			if (fork) {
+				NLC_HIP(c, hipEventRecord(c->fork_ev, c->stream));
+				NLC_HIP(c, hipStreamWaitEvent(c->aux_stream, c->fork_ev, 0));
+			}
 			for (const auto &k : c->cls)
-				NLC_HIP(c, k.launcher(c->stream, c->state + k.site0, k.count, k.n, steps, c->args, nullptr));
+				NLC_HIP(c, k.launcher(fork && k.side ? c->aux_stream : c->stream, c->state + k.site0, k.count, k.n,
+				                      steps, c->args, nullptr));
+			if (fork) {
+				NLC_HIP(c, hipEventRecord(c->join_ev, c->aux_stream));
+				NLC_HIP(c, hipStreamWaitEvent(c->stream, c->join_ev, 0));
+			}
 		}
 	} else if (c->use_tiled) {
 		NLC_HIP(c, c->tiled(c->stream, c->state, c->tile_buf, c->copies, c->n, steps, c->args, c->mp2));
@@ -855,6 +873,24 @@ int nlc_create_ragged(nlc_ctx **out, const nlc_params *p, const uint32_t *chain_
 		caller += c->lengths[i];
 	}
 	int rc = setup_common(c);
+	if (!rc && c->cls.size() > 1) {
+		// split the classes over the two streams, longest processing time first
+		// (cost ~ sites of the class)
+		std::vector<size_t> order(c->cls.size());
+		for (size_t i = 0; i < order.size(); ++i) order[i] = i;
+		auto sites = [&](size_t i) { return uint64_t(c->cls[i].n) * c->cls[i].count; };
+		std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return sites(a) > sites(b); });
+		uint64_t load[2] = {0, 0};
+		for (size_t i : order) {
+			const int side = load[1] < load[0];
+			c->cls[i].side = side;
+			load[side] += sites(i);
+		}
+		if (hipStreamCreateWithFlags(&c->aux_stream, hipStreamNonBlocking) != hipSuccess ||
+		    hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming) != hipSuccess ||
+		    hipEventCreateWithFlags(&c->join_ev, hipEventDisableTiming) != hipSuccess)
+			rc = c->fail(NLC_EHIP, "stream or event creation failed");
+	}
 	if (!rc) {
 		int S = 0;
 		for (int s : {32, 16, 8})
