@@ -5,23 +5,27 @@
 //
 // Method: P. T. P. Tang, "Table-driven implementation of the exponential
 // function in IEEE floating-point arithmetic", ACM TOMS 15(2), 1989, with a
-// 32-entry table and a Taylor polynomial, on the pre-scaled argument:
-//   x = N + r,  N = nearest integer to x,  |r| <= 1/2,
-//   2^(x/32) = 2^(N >> 5) * 2^((N & 31)/32) * 2^(r/32),
-//   2^(r/32) - 1 = sum_{k=1..6} (ln2/32)^k r^k / k!   (truncation < 4e-18).
-// N comes from the low word of x + 1.5 * 2^52 (no conversion instruction) and
-// r = x - N is exact, so the reduction of Tang's method (y - N ln2/32 in two
-// FMAs) is one subtraction here; 2^(N >> 5) is applied with ldexp, which
-// saturates to +inf / 0 beyond the double range.  Error about 1 ulp: the table
-// entries are 2^(j/32) rounded to nearest, one rounding in the last FMA, and
-// |2^(r/32) - 1| < 0.011 carries the polynomial's few-ulp error down to 1/100
-// ulp.  Valid for |x| < 2^31 (larger magnitudes are reached only by a
-// diverged chain, whose state is non-finite long before); NaN propagates.
+// 64-entry table and a degree-5 polynomial, on the pre-scaled argument:
+//   2x = N + 2r,  N = nearest integer to 2x,  |r| <= 1/4,
+//   2^(x/32) = 2^(N >> 6) * 2^((N & 63)/64) * 2^(r/32),
+//   2^(r/32) - 1 = r (p0 + p1 r + ... + p4 r^4),
+// the p_k interpolating (2^(r/32) - 1)/r at the five Chebyshev nodes of
+// |r| <= 1/4 (computed in 60-digit decimal arithmetic; error of the
+// polynomial < 2.4e-18 on the interval, against 3.5e-17 for the degree-5
+// Taylor polynomial).  N comes from the low word of 2x + 1.5 * 2^52 (one FMA,
+// no conversion instruction) and r = x - N/2 is exact (one FMA), so the
+// reduction of Tang's method (y - N ln2/64 in two FMAs) costs nothing extra
+// here; 2^(N >> 6) is applied with ldexp, which saturates to +inf / 0 beyond
+// the double range.  Error about 1 ulp: the table entries are 2^(j/64)
+// rounded to nearest, one rounding in the last FMA, and |2^(r/32) - 1| <
+// 0.0055 carries the polynomial's rounding errors down to 1/100 ulp.  Valid
+// for |x| < 2^30 (larger magnitudes are reached only by a diverged chain,
+// whose state is non-finite long before); NaN propagates.
 //
-// Cost on gfx950: 13 VALU + 1 LDS read per call, against about 25 VALU for the
+// Cost on gfx950: 12 VALU + 1 LDS read per call, against about 25 VALU for the
 // device library's exp (a degree-12 polynomial on |r| <= ln2/2 plus range
-// checks).  The table lives in LDS (256 bytes: entry j in banks 2j, 2j+1, so a
-// wavefront's reads never conflict).
+// checks).  The table lives in LDS (512 bytes: entry j in banks 2j mod 64 and
+// 2j + 1 mod 64, so a wavefront's reads conflict at most two ways).
 //
 // The header is plain C++ apart from NLC_EXP_QUAL, so the same code can be
 // compiled on the host to measure its error.
@@ -33,42 +37,49 @@
 
 namespace nlc {
 
-constexpr int kExpTabSize = 32;
+constexpr int kExpTabSize = 64;
 
-// 2^(j/32), j = 0..31, rounded to nearest double.
+// 2^(j/64), j = 0..63, rounded to nearest double.
 #define NLC_EXP2_TABLE                                                                                              \
-	{0x1.0000000000000p+0, 0x1.059b0d3158574p+0, 0x1.0b5586cf9890fp+0, 0x1.11301d0125b51p+0,                        \
-	 0x1.172b83c7d517bp+0, 0x1.1d4873168b9aap+0, 0x1.2387a6e756238p+0, 0x1.29e9df51fdee1p+0,                        \
-	 0x1.306fe0a31b715p+0, 0x1.371a7373aa9cbp+0, 0x1.3dea64c123422p+0, 0x1.44e086061892dp+0,                        \
-	 0x1.4bfdad5362a27p+0, 0x1.5342b569d4f82p+0, 0x1.5ab07dd485429p+0, 0x1.6247eb03a5585p+0,                        \
-	 0x1.6a09e667f3bcdp+0, 0x1.71f75e8ec5f74p+0, 0x1.7a11473eb0187p+0, 0x1.82589994cce13p+0,                        \
-	 0x1.8ace5422aa0dbp+0, 0x1.93737b0cdc5e5p+0, 0x1.9c49182a3f090p+0, 0x1.a5503b23e255dp+0,                        \
-	 0x1.ae89f995ad3adp+0, 0x1.b7f76f2fb5e47p+0, 0x1.c199bdd85529cp+0, 0x1.cb720dcef9069p+0,                        \
-	 0x1.d5818dcfba487p+0, 0x1.dfc97337b9b5fp+0, 0x1.ea4afa2a490dap+0, 0x1.f50765b6e4540p+0}
+	{0x1.0000000000000p+0, 0x1.02c9a3e778061p+0, 0x1.059b0d3158574p+0, 0x1.0874518759bc8p+0,                           \
+	 0x1.0b5586cf9890fp+0, 0x1.0e3ec32d3d1a2p+0, 0x1.11301d0125b51p+0, 0x1.1429aaea92de0p+0,                           \
+	 0x1.172b83c7d517bp+0, 0x1.1a35beb6fcb75p+0, 0x1.1d4873168b9aap+0, 0x1.2063b88628cd6p+0,                           \
+	 0x1.2387a6e756238p+0, 0x1.26b4565e27cddp+0, 0x1.29e9df51fdee1p+0, 0x1.2d285a6e4030bp+0,                           \
+	 0x1.306fe0a31b715p+0, 0x1.33c08b26416ffp+0, 0x1.371a7373aa9cbp+0, 0x1.3a7db34e59ff7p+0,                           \
+	 0x1.3dea64c123422p+0, 0x1.4160a21f72e2ap+0, 0x1.44e086061892dp+0, 0x1.486a2b5c13cd0p+0,                           \
+	 0x1.4bfdad5362a27p+0, 0x1.4f9b2769d2ca7p+0, 0x1.5342b569d4f82p+0, 0x1.56f4736b527dap+0,                           \
+	 0x1.5ab07dd485429p+0, 0x1.5e76f15ad2148p+0, 0x1.6247eb03a5585p+0, 0x1.6623882552225p+0,                           \
+	 0x1.6a09e667f3bcdp+0, 0x1.6dfb23c651a2fp+0, 0x1.71f75e8ec5f74p+0, 0x1.75feb564267c9p+0,                           \
+	 0x1.7a11473eb0187p+0, 0x1.7e2f336cf4e62p+0, 0x1.82589994cce13p+0, 0x1.868d99b4492edp+0,                           \
+	 0x1.8ace5422aa0dbp+0, 0x1.8f1ae99157736p+0, 0x1.93737b0cdc5e5p+0, 0x1.97d829fde4e50p+0,                           \
+	 0x1.9c49182a3f090p+0, 0x1.a0c667b5de565p+0, 0x1.a5503b23e255dp+0, 0x1.a9e6b5579fdbfp+0,                           \
+	 0x1.ae89f995ad3adp+0, 0x1.b33a2b84f15fbp+0, 0x1.b7f76f2fb5e47p+0, 0x1.bcc1e904bc1d2p+0,                           \
+	 0x1.c199bdd85529cp+0, 0x1.c67f12e57d14bp+0, 0x1.cb720dcef9069p+0, 0x1.d072d4a07897cp+0,                           \
+	 0x1.d5818dcfba487p+0, 0x1.da9e603db3285p+0, 0x1.dfc97337b9b5fp+0, 0x1.e502ee78b3ff6p+0,                           \
+	 0x1.ea4afa2a490dap+0, 0x1.efa1bee615a27p+0, 0x1.f50765b6e4540p+0, 0x1.fa7c1819e90d8p+0}
 
-constexpr double kExpShifter = 0x1.8p52;   // 1.5 * 2^52: x + kExpShifter rounds x to an integer
+constexpr double kExpShifter = 0x1.8p52;   // 1.5 * 2^52: y + kExpShifter rounds y to an integer
 constexpr double kExpShifter32 = 0x1.8p57; // 1.5 * 2^57: rounds x to a multiple of 32
 
 // 2^(x/32) = e^(x ln2/32).
 NLC_EXP_QUAL double exp2_32_tab(double x, const double *tab) {
-	const double s = x + kExpShifter;
-	const double nd = s - kExpShifter; // N as a double, exact
+	const double s = __builtin_fma(x, 2.0, kExpShifter); // rounds 2x to the nearest integer N
+	const double nd = s - kExpShifter;                    // N as a double, exact
 	// low 32 bits of s's significand hold N in two's complement (|N| < 2^31)
 	union {
 		double d;
 		unsigned long long u;
 	} bits = {s};
 	const int n = int(unsigned(bits.u));
-	const double r = x - nd; // exact, |r| <= 1/2
-	// 2^(r/32) - 1 = sum_k c_k r^k, c_k = (ln2/32)^k / k!
-	double t = __builtin_fma(r, 0x1.430912f86c787p-43, 0x1.5d87fe78a6731p-35); // c6, c5
-	t = __builtin_fma(t, r, 0x1.3b2ab6fba4e77p-27);                            // c4
-	t = __builtin_fma(t, r, 0x1.c6b08d704a0c0p-20);                            // c3
-	t = __builtin_fma(t, r, 0x1.ebfbdff82c58fp-13);                            // c2
-	t = __builtin_fma(t, r, 0x1.62e42fefa39efp-6);                             // c1
+	const double r = __builtin_fma(nd, -0.5, x); // x - N/2, exact, |r| <= 1/4
+	// (2^(r/32) - 1) / r = sum_k p_k r^k
+	double t = __builtin_fma(r, 0x1.5d8812769ed72p-35, 0x1.3b2ad0385b409p-27); // p4, p3
+	t = __builtin_fma(t, r, 0x1.c6b08d70496c1p-20);                            // p2
+	t = __builtin_fma(t, r, 0x1.ebfbdff82ac52p-13);                            // p1
+	t = __builtin_fma(t, r, 0x1.62e42fefa39efp-6);                             // p0
 	const double q = t * r;
 	const double T = tab[n & (kExpTabSize - 1)];
-	return __builtin_ldexp(__builtin_fma(T, q, T), n >> 5);
+	return __builtin_ldexp(__builtin_fma(T, q, T), n >> 6);
 }
 
 // Table-free variant: x = 32 N + r, |r| <= 16, 2^(r/32) by its degree-13

@@ -3,6 +3,7 @@
 on the host from the same header, against 2^(x/32) computed to 40 significant
 digits with Python's decimal module."""
 import ctypes
+import math
 import re
 import subprocess
 from decimal import Decimal, getcontext
@@ -71,32 +72,69 @@ def test_table_entries_are_correctly_rounded():
     block = text[text.index("#define NLC_EXP2_TABLE"):]
     block = block[:block.index("}") + 1]
     vals = [float.fromhex(h) for h in re.findall(r"0x[0-9a-f.]+p[+-]\d+", block)]
-    assert len(vals) == 32
+    assert len(vals) == 64
     for j, v in enumerate(vals):
-        assert v == float(Decimal(2) ** (Decimal(j) / 32)), j
+        assert v == float(Decimal(2) ** (Decimal(j) / 64)), j
+
+
+def annotated_literals(text, tag):
+    """{k: literal} for the literals annotated `tag`k in comments of the header."""
+    out = {}
+    for line in text.splitlines():
+        m = re.search(r"//\s*(%s\d+(?:,\s*%s\d+)*)\s*$" % (tag, tag), line)
+        if not m:
+            continue
+        ks = [int(x) for x in re.findall(r"%s(\d+)" % tag, m.group(1))]
+        lits = [float.fromhex(h) for h in re.findall(r"0x[0-9a-f.]+p[+-]\d+", line)]
+        assert len(lits) == len(ks), line
+        out.update(zip(ks, lits))
+    return out
+
+
+def test_interpolant_of_the_table_form():
+    """The table form's p_k: r (p0 + ... + p4 r^4) is 2^(r/32) - 1 within
+    2.5e-18 on |r| <= 1/4 (60-digit check on a grid with the interval ends);
+    the Taylor polynomial of the same degree misses by 3.5e-17, and p_k off by
+    10^(3k - 15) relative (the size of its term at r = 1/4) fails."""
+    getcontext().prec = 60
+    p = annotated_literals(HEADER.read_text(), "p")
+    assert sorted(p) == [0, 1, 2, 3, 4]
+    ln2 = Decimal(2).ln()
+
+    def worst(coef):
+        w = Decimal(0)
+        for i in range(-400, 401):
+            r = Decimal(i) / 1600
+            acc = Decimal(0)
+            for k in sorted(coef, reverse=True):
+                acc = acc * r + Decimal(coef[k])
+            approx = r * acc
+            w = max(w, abs(approx - ((r * ln2 / 32).exp() - 1)))
+        return w
+
+    assert worst(p) < Decimal("2.5e-18")
+    taylor = {k: float((ln2 / 32) ** (k + 1) / math.factorial(k + 1)) for k in range(5)}
+    assert worst(taylor) > Decimal("3e-17")
+    for k, rel in ((0, 1e-15), (1, 1e-12), (2, 1e-9), (3, 1e-6), (4, 1e-3)):
+        bad = dict(p)
+        bad[k] = p[k] * (1 + rel)
+        assert worst(bad) > Decimal("2.5e-18"), k
 
 
 def test_polynomial_coefficients():
-    """Every literal annotated c_k in the header is (ln2/32)^k / k! rounded to
-    nearest, and the shifters are 1.5 * 2^52 and 1.5 * 2^57."""
+    """Every literal annotated c_k in the header (the table-free form) is
+    (ln2/32)^k / k! rounded to nearest, and the shifters are 1.5 * 2^52 and
+    1.5 * 2^57."""
     getcontext().prec = 60
     b = Decimal(2).ln() / 32
     fact = [Decimal(1)]
     for k in range(1, 20):
         fact.append(fact[-1] * k)
     text = HEADER.read_text()
-    checked = 0
-    for line in text.splitlines():
-        m = re.search(r"//\s*(c\d+(?:,\s*c\d+)*)\s*$", line)
-        if not m:
-            continue
-        ks = [int(x) for x in re.findall(r"c(\d+)", m.group(1))]
-        lits = [float.fromhex(h) for h in re.findall(r"0x[0-9a-f.]+p[+-]\d+", line)]
-        assert len(lits) == len(ks), line
-        for k, lit in zip(ks, lits):
-            assert lit == float(b ** k / fact[k]), (line, k)
-            checked += 1
-    assert checked == 6 + 13
+    c = annotated_literals(text, "c")
+    assert sorted(c) == list(range(1, 14))
+    for k, lit in c.items():
+        assert lit == float(b ** k / fact[k]), k
     assert float.fromhex(re.search(r"kExpShifter = (0x[0-9a-f.]+p[+-]?\d+)", text).group(1)) == 1.5 * 2.0 ** 52
     assert float.fromhex(re.search(r"kExpShifter32 = (0x[0-9a-f.]+p[+-]?\d+)", text).group(1)) == 1.5 * 2.0 ** 57
 
@@ -112,8 +150,9 @@ def test_error_within_one_ulp(exp_fn, lo, hi, count):
 
 def test_reduction_boundaries_and_special_values(exp_fn):
     k = np.arange(-4000, 4000, dtype=np.float64)
-    # interval ends of both reductions (N = nearest integer to x, resp. to x/32)
-    x = np.concatenate([k, k + 0.5, k - 0.5, np.nextafter(k + 0.5, 0), np.nextafter(k - 0.5, 0),
+    # interval ends of both reductions (N = nearest integer to 2x, resp. to x/32)
+    h = k / 2
+    x = np.concatenate([h, h + 0.25, h - 0.25, np.nextafter(h + 0.25, 0), np.nextafter(h - 0.25, 0),
                         32 * k[::8] + 15.9999999, 32 * k[::8] - 15.9999999, 32 * k[::8] + 16.0])
     x = x[np.abs(x) < 32000]
     err = ulp_errors(x, exp_fn(x))
