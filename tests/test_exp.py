@@ -91,50 +91,45 @@ def annotated_literals(text, tag):
     return out
 
 
-def test_interpolant_of_the_table_form():
-    """The table form's p_k: r (p0 + ... + p4 r^4) is 2^(r/32) - 1 within
-    2.5e-18 on |r| <= 1/4 (60-digit check on a grid with the interval ends);
-    the Taylor polynomial of the same degree misses by 3.5e-17, and p_k off by
-    10^(3k - 15) relative (the size of its term at r = 1/4) fails."""
+@pytest.mark.parametrize("tag,half,ncoef,bound,taylor_miss", [
+    ("p", 0.25, 5, 2.5e-18, 3e-17),   # table form: |r| <= 1/4
+    ("q", 16.0, 11, 2.1e-17, 1e-15),  # table-free form: |r| <= 16
+])
+def test_interpolants(tag, half, ncoef, bound, taylor_miss):
+    """r (c0 + ... + c_m r^m), with the literals annotated `tag`k in the header,
+    is 2^(r/32) - 1 within `bound` on |r| <= half (60-digit check on a grid with
+    the interval ends); the Taylor polynomial of the same degree misses by more
+    than `taylor_miss`; and each coefficient, moved by about 2x the bound over the
+    size of its term at r = half, fails."""
     getcontext().prec = 60
-    p = annotated_literals(HEADER.read_text(), "p")
-    assert sorted(p) == [0, 1, 2, 3, 4]
+    coef = annotated_literals(HEADER.read_text(), tag)
+    assert sorted(coef) == list(range(ncoef))
     ln2 = Decimal(2).ln()
+    h = Decimal(half)
 
-    def worst(coef):
+    def worst(c):
         w = Decimal(0)
         for i in range(-400, 401):
-            r = Decimal(i) / 1600
+            r = h * i / 400
             acc = Decimal(0)
-            for k in sorted(coef, reverse=True):
-                acc = acc * r + Decimal(coef[k])
-            approx = r * acc
-            w = max(w, abs(approx - ((r * ln2 / 32).exp() - 1)))
+            for k in sorted(c, reverse=True):
+                acc = acc * r + Decimal(c[k])
+            w = max(w, abs(r * acc - ((r * ln2 / 32).exp() - 1)))
         return w
 
-    assert worst(p) < Decimal("2.5e-18")
-    taylor = {k: float((ln2 / 32) ** (k + 1) / math.factorial(k + 1)) for k in range(5)}
-    assert worst(taylor) > Decimal("3e-17")
-    for k, rel in ((0, 1e-15), (1, 1e-12), (2, 1e-9), (3, 1e-6), (4, 1e-3)):
-        bad = dict(p)
-        bad[k] = p[k] * (1 + rel)
-        assert worst(bad) > Decimal("2.5e-18"), k
+    assert worst(coef) < Decimal(bound)
+    taylor = {k: float((ln2 / 32) ** (k + 1) / math.factorial(k + 1)) for k in range(ncoef)}
+    assert worst(taylor) > Decimal(taylor_miss)
+    for k in range(ncoef):
+        term = (half * math.log(2) / 32) ** (k + 1) / math.factorial(k + 1)
+        bad = dict(coef)
+        bad[k] = coef[k] * (1 + 10 ** math.ceil(math.log10(2 * bound / term)))
+        assert worst(bad) > Decimal(bound), k
 
 
-def test_polynomial_coefficients():
-    """Every literal annotated c_k in the header (the table-free form) is
-    (ln2/32)^k / k! rounded to nearest, and the shifters are 1.5 * 2^52 and
-    1.5 * 2^57."""
-    getcontext().prec = 60
-    b = Decimal(2).ln() / 32
-    fact = [Decimal(1)]
-    for k in range(1, 20):
-        fact.append(fact[-1] * k)
+def test_shifters():
+    """The rounding constants are 1.5 * 2^52 and 1.5 * 2^57."""
     text = HEADER.read_text()
-    c = annotated_literals(text, "c")
-    assert sorted(c) == list(range(1, 14))
-    for k, lit in c.items():
-        assert lit == float(b ** k / fact[k]), k
     assert float.fromhex(re.search(r"kExpShifter = (0x[0-9a-f.]+p[+-]?\d+)", text).group(1)) == 1.5 * 2.0 ** 52
     assert float.fromhex(re.search(r"kExpShifter32 = (0x[0-9a-f.]+p[+-]?\d+)", text).group(1)) == 1.5 * 2.0 ** 57
 
