@@ -339,9 +339,10 @@ hipError_t launch_ragged(hipStream_t st, double2 *state, const RaggedItem *items
 } // namespace
 
 #ifdef NLC_STEP_TILED
-// Toda: 64 x 16 sites with the table-driven exp (T = 9); the others 64 x 32
-// (T = 18: every launch reads and writes the whole state, about 0.14 ms at
-// 2^24 sites, so 64 x 16 with T = 9 measured 15 % slower).
+// Toda: 64 x 16 sites with the table-driven exp (T = 9; 64 x 32 with the
+// table-free exp measured 9 % slower); the others 64 x 32 (T = 18: every
+// launch reads and writes the whole state, about 0.14 ms at 2^24 sites, so
+// 64 x 16 with T = 9 measured 15 % slower).
 TiledLauncher NLC_STEP_TILED() {
 	constexpr int S = NLC_STEP_MODEL == TODA ? 16 : 32;
 	return &launch_tiled<NLC_STEP_MODEL, S>;
