@@ -246,7 +246,7 @@ int nlc_yoshida(double *c, double *d);
 int nlc_dispersion(int model, uint32_t n, double m, double *omega);
 /* Eigensystem of diag(mp2) - periodic adjacency (dDNKG/eigensystem.cpp:13-36):
  * omegas ascending = sqrt(eigenvalues); eigenvectors column-major n x n.
- * n <= 512, or no HIP device: Householder tridiagonalisation + implicit QL on
+ * n <= 512, or no HIP device: Householder tridiagonalisation + implicit QR on
  * the host; larger n with a device: rocSOLVER dsyevd (loaded on first use). */
 int nlc_eigensystem(uint32_t n, const double *mp2, double *omegas, double *eigenvectors);
 

@@ -18,11 +18,18 @@
 // here; 2^(N >> 6) is applied with ldexp, which saturates to +inf / 0 beyond
 // the double range.  Error about 1 ulp: the table entries are 2^(j/64)
 // rounded to nearest, one rounding in the last FMA, and |2^(r/32) - 1| <
-// 0.0055 carries the polynomial's rounding errors down to 1/100 ulp.  Valid
-// for |x| < 2^30 (larger magnitudes are reached only by a diverged chain,
-// whose state is non-finite long before); NaN propagates.
+// 0.0055 carries the polynomial's rounding errors down to 1/100 ulp.
 //
-// Cost on gfx950: 12 VALU + 1 LDS read per call, against about 25 VALU for the
+// Range: x is first clamped to [kExpArgMin, kExpArgMax] = [-34500, 33000], where
+// 2^(x/32) is already 0 (below the smallest subnormal, 2^-1074) and +inf
+// (above 2^1024), so every finite or infinite x gives the correctly saturated
+// result; without the clamp, N (taken from the low 32 bits of the shifted
+// argument) would wrap for |x| >= 2^30 and a bond stretched that far by a
+// blown-up step would get a finite force instead of +inf.  A NaN argument
+// gives 0 (the clamp's max returns the other operand): it only arises from a
+// state that is already NaN, and the drift keeps that NaN in the state.
+//
+// Cost on gfx950: 14 VALU + 1 LDS read per call, against about 25 VALU for the
 // device library's exp (a degree-12 polynomial on |r| <= ln2/2 plus range
 // checks).  The table lives in LDS (512 bytes: entry j in banks 2j mod 64 and
 // 2j + 1 mod 64, so a wavefront's reads conflict at most two ways).
@@ -60,12 +67,18 @@ constexpr int kExpTabSize = 64;
 
 constexpr double kExpShifter = 0x1.8p52;   // 1.5 * 2^52: y + kExpShifter rounds y to an integer
 constexpr double kExpShifter32 = 0x1.8p57; // 1.5 * 2^57: rounds x to a multiple of 32
+// 2^(x/32) rounds to +inf from x = 32768 on and to 0 below x = -34400 (where it
+// is under 2^-1075, half the smallest subnormal).
+constexpr double kExpArgMin = -34500.0, kExpArgMax = 33000.0;
+
+NLC_EXP_QUAL double exp2_32_clamp(double x) { return __builtin_fmin(__builtin_fmax(x, kExpArgMin), kExpArgMax); }
 
 // 2^(x/32) = e^(x ln2/32).
 NLC_EXP_QUAL double exp2_32_tab(double x, const double *tab) {
+	x = exp2_32_clamp(x);
 	const double s = __builtin_fma(x, 2.0, kExpShifter); // rounds 2x to the nearest integer N
 	const double nd = s - kExpShifter;                    // N as a double, exact
-	// low 32 bits of s's significand hold N in two's complement (|N| < 2^31)
+	// low 32 bits of s's significand hold N in two's complement (|N| < 2^17 after the clamp)
 	union {
 		double d;
 		unsigned long long u;
@@ -86,14 +99,15 @@ NLC_EXP_QUAL double exp2_32_tab(double x, const double *tab) {
 // ... + q10 r^10), the q_k interpolating (2^(r/32) - 1)/r at the eleven
 // Chebyshev nodes of |r| <= 16 (error < 2.1e-17 with the coefficients rounded
 // to double, against 1.3e-17 for the degree-13 Taylor polynomial), 2^N by
-// ldexp.  15 VALU and no memory access: with the table, the scheduler hoists
-// the table loads of a whole unrolled run of bonds to hide their latency and
-// the kernel loses occupancy; this form keeps a chain of bonds in few
-// registers.
+// ldexp; the same clamp as above.  17 VALU and no memory access: with the
+// table, the scheduler hoists the table loads of a whole unrolled run of bonds
+// to hide their latency and the kernel loses occupancy; this form keeps a
+// chain of bonds in few registers.
 NLC_EXP_QUAL double exp2_32_poly(double x) {
+	x = exp2_32_clamp(x);
 	const double s = x + kExpShifter32; // the unit in the last place of s is 32
 	const double nd = s - kExpShifter32; // 32 N, exact
-	// low 32 bits of s's significand (in units of 32) hold N in two's complement
+	// low 32 bits of s's significand (in units of 32) hold N in two's complement (|N| < 2^11)
 	union {
 		double d;
 		unsigned long long u;

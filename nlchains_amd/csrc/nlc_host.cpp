@@ -133,55 +133,83 @@ void tridiagonalize(uint32_t n, std::vector<double> &a, std::vector<double> &q, 
 // Largest chain length solved on the host when a device is present.
 constexpr uint32_t kHostEigenMax = 512;
 
-// Implicit symmetric QL iteration with Wilkinson-type shifts on the
-// tridiagonal (d, e) (Bowdler, Martin, Reinsch & Wilkinson, Numer. Math. 11,
-// 1968), applying every plane rotation to the columns of z.
-bool tridiagonal_ql(uint32_t n, std::vector<double> &d, std::vector<double> &e, std::vector<double> &z) {
-	const double eps = std::numeric_limits<double>::epsilon();
-	for (uint32_t l = 0; l < n; ++l) {
-		for (int iter = 0;; ++iter) {
-			uint32_t m = l;
-			for (; m + 1 < n; ++m) {
-				const double dd = std::fabs(d[m]) + std::fabs(d[m + 1]);
-				if (std::fabs(e[m]) <= eps * dd) break;
-			}
-			if (m == l) break;
-			if (iter == 64) return false;
-			// shift from the leading 2x2 block
-			double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
-			double r = std::hypot(g, 1.0);
-			g = d[m] - d[l] + e[l] / (g + std::copysign(r, g));
-			double s = 1.0, c = 1.0, p = 0.0;
-			bool deflated = false;
-			for (int64_t i = int64_t(m) - 1; i >= int64_t(l); --i) {
-				const double f = s * e[i], b = c * e[i];
-				r = std::hypot(f, g);
-				e[i + 1] = r;
-				if (r == 0.0) { // underflow: split the matrix here
-					d[i + 1] -= p;
-					e[m] = 0.0;
-					deflated = true;
-					break;
-				}
-				s = f / r;
-				c = g / r;
-				g = d[i + 1] - p;
-				r = (d[i] - g) * s + 2.0 * c * b;
-				p = s * r;
-				d[i + 1] = g + p;
-				g = c * r - b;
-				double *zi = &z[size_t(i) * n], *zi1 = &z[size_t(i + 1) * n];
-				for (uint32_t k = 0; k < n; ++k) {
-					const double t = zi1[k];
-					zi1[k] = s * zi[k] + c * t;
-					zi[k] = c * zi[k] - s * t;
-				}
-			}
-			if (deflated) continue;
-			d[l] -= p;
-			e[l] = g;
-			e[m] = 0.0;
+// Symmetric QR algorithm on the tridiagonal (d, e) (Golub & Van Loan, Matrix
+// Computations, 4th ed., Algorithms 8.3.2 and 8.3.3).  The active block
+// [lo, hi] is the trailing unreduced part: off-diagonals below the tolerance
+// are set to zero, which splits the matrix, and the block shrinks from the
+// bottom as its last eigenvalue converges.  Each sweep is one implicit QR step
+// with the Wilkinson shift: a plane rotation in (lo, lo + 1) that matches the
+// first column of T - mu I, then rotations (k, k + 1) that chase the bulge it
+// creates at T(k - 1, k + 1) down and out of the block.  Every rotation G is
+// applied as T <- G^T T G to the 2 x 2 diagonal block it touches and to the
+// two off-diagonal entries beside it, and accumulated into the eigenvectors as
+// Z <- Z G (columns of z, column-major n x n).
+struct Rotation {
+	double c, s; // G^T (x, y) = (hypot(x, y), 0) with G^T = [c s; -s c]
+	Rotation(double x, double y) {
+		const double r = std::hypot(x, y);
+		if (r == 0.0) c = 1.0, s = 0.0;
+		else c = x / r, s = y / r;
+	}
+};
+
+// The eigenvalue of [[d0, b], [b, d1]] nearer to d1 (Wilkinson's shift).
+double wilkinson_shift(double d0, double b, double d1) {
+	const double half_gap = 0.5 * (d0 - d1);
+	if (b == 0.0) return d1;
+	return d1 - b * b / (half_gap + std::copysign(std::hypot(half_gap, b), half_gap));
+}
+
+void qr_sweep(uint32_t n, uint32_t lo, uint32_t hi, std::vector<double> &d, std::vector<double> &e,
+              std::vector<double> &z) {
+	const double mu = wilkinson_shift(d[hi - 1], e[hi - 1], d[hi]);
+	double x = d[lo] - mu, bulge = e[lo];
+	for (uint32_t k = lo; k < hi; ++k) {
+		const Rotation g(x, bulge);
+		const double c = g.c, s = g.s;
+		// columns k, k+1 of row k-1: (x, bulge) -> (r, 0)
+		if (k > lo) e[k - 1] = c * x + s * bulge;
+		// the 2 x 2 diagonal block of rows and columns k, k+1
+		const double a = d[k], b = e[k], f = d[k + 1];
+		const double cc = c * c, ss = s * s, cs = c * s;
+		d[k] = cc * a + 2.0 * cs * b + ss * f;
+		d[k + 1] = ss * a - 2.0 * cs * b + cc * f;
+		e[k] = cs * (f - a) + (cc - ss) * b;
+		if (k + 1 < hi) {
+			// rows k, k+1 of column k+2: (0, e[k+1]) -> (s e[k+1], c e[k+1]); the
+			// first is the new bulge, removed by the next rotation
+			bulge = s * e[k + 1];
+			e[k + 1] *= c;
+			x = e[k];
 		}
+		double *zk = &z[size_t(k) * n], *zk1 = &z[size_t(k + 1) * n];
+		for (uint32_t i = 0; i < n; ++i) {
+			const double u = zk[i], v = zk1[i];
+			zk[i] = c * u + s * v;
+			zk1[i] = c * v - s * u;
+		}
+	}
+}
+
+bool negligible(double b, double d0, double d1) {
+	return std::fabs(b) <= std::numeric_limits<double>::epsilon() * (std::fabs(d0) + std::fabs(d1));
+}
+
+bool tridiagonal_qr(uint32_t n, std::vector<double> &d, std::vector<double> &e, std::vector<double> &z) {
+	const uint64_t max_sweeps = 30ull * n;
+	uint64_t sweeps = 0;
+	uint32_t hi = n - 1;
+	while (hi > 0) {
+		if (negligible(e[hi - 1], d[hi - 1], d[hi])) {
+			e[hi - 1] = 0.0; // d[hi] has converged
+			--hi;
+			continue;
+		}
+		uint32_t lo = hi - 1;
+		while (lo > 0 && !negligible(e[lo - 1], d[lo - 1], d[lo])) --lo;
+		if (lo > 0) e[lo - 1] = 0.0;
+		if (++sweeps > max_sweeps) return false;
+		qr_sweep(n, lo, hi, d, e, z);
 	}
 	return true;
 }
@@ -203,7 +231,7 @@ int eigensystem(uint32_t n, const double *mp2, double *omegas, double *V) {
 	}
 	a[size_t(n - 1) * n + 0] = a[0 * size_t(n) + n - 1] = -1.0;
 	tridiagonalize(n, a, q, d, e);
-	if (!tridiagonal_ql(n, d, e, q)) return NLC_EUNSUPPORTED;
+	if (!tridiagonal_qr(n, d, e, q)) return NLC_EUNSUPPORTED;
 	std::vector<uint32_t> ord(n);
 	for (uint32_t i = 0; i < n; ++i) ord[i] = i;
 	std::stable_sort(ord.begin(), ord.end(), [&](uint32_t x, uint32_t y) { return d[x] < d[y]; });

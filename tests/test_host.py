@@ -78,7 +78,7 @@ def test_yoshida_and_dispersion_match_oracle():
         np.testing.assert_array_equal(N.dispersion(N.DNLS, n), O.dispersion_dnls(n))
 
 
-@pytest.mark.parametrize("n", [2, 3, 17, 64, 200])
+@pytest.mark.parametrize("n", [2, 3, 17, 64, 200, 511, 512])
 def test_product_eigensystem_vs_lapack_and_oracle(n):
     rng = np.random.default_rng(n)
     mp2 = 2 + rng.uniform(0.5, 1.5, size=n)
@@ -99,6 +99,23 @@ def test_eigensystem_uniform_masses_degenerate_spectrum():
     om, V = N.eigensystem(np.full(n, 3.0))
     np.testing.assert_allclose(om, np.sort(np.sqrt(3 - 2 * np.cos(2 * np.pi * np.arange(n) / n))), rtol=1e-13)
     np.testing.assert_allclose(V.T @ V, np.eye(n), atol=1e-12)
+    A = 3 * np.eye(n) - np.eye(n, k=1) - np.eye(n, k=-1)
+    A[0, -1] = A[-1, 0] = -1
+    np.testing.assert_allclose(A @ V, V * om ** 2, atol=1e-12)
+
+
+def test_eigensystem_nearly_degenerate_and_graded_masses():
+    """Masses that split the degenerate pairs of the uniform chain by 1e-9,
+    and masses graded over three orders of magnitude: eigenvalues against
+    LAPACK, eigen-equation and orthonormality."""
+    n = 96
+    for mp2 in (3.0 + 1e-9 * np.sin(np.arange(n)), 2 + np.geomspace(1e-2, 10.0, n)):
+        om, V = N.eigensystem(mp2)
+        A = np.diag(mp2) - np.eye(n, k=1) - np.eye(n, k=-1)
+        A[0, -1] = A[-1, 0] = -1
+        np.testing.assert_allclose(om ** 2, np.linalg.eigvalsh(A), rtol=1e-13, atol=1e-13)
+        np.testing.assert_allclose(A @ V, V * om ** 2, atol=1e-11)
+        np.testing.assert_allclose(V.T @ V, np.eye(n), atol=1e-12)
 
 
 def test_ic_plane_waves_energy_and_determinism():
