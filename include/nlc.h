@@ -64,7 +64,7 @@ typedef enum { NLC_FP64 = 0, NLC_FP32C = 1 } nlc_precision;
  * launch (temporal blocking, any chain length from 256 sites; it needs a
  * second buffer of the state's size).  NLC_KERNEL_AUTO picks WAVE when a shape
  * exists for the chain length (every length up to 16384), else TILED (the
- * reference's move<> dispatch, DNKG_FPUT_Toda.cu:262-297, which falls back to
+ * reference's move<> dispatch, DNKG_FPUT_Toda.cu:254-293, which falls back to
  * its split kernel there); WAVE without a shape override does the same. */
 typedef enum { NLC_KERNEL_AUTO = 0, NLC_KERNEL_WAVE = 1, NLC_KERNEL_SPLIT = 2, NLC_KERNEL_TILED = 3 } nlc_kernel;
 
@@ -145,7 +145,7 @@ int nlc_device_state(nlc_ctx *ctx, void **dptr, size_t *bytes);
 /* ---- hot path ---------------------------------------------------------- */
 
 /* Asynchronous.  Advances every chain by `steps` Yoshida-6 steps: the
- * reference's move<Model> (DNKG_FPUT_Toda.cu:262-297), dDNKG::move
+ * reference's move<Model> (DNKG_FPUT_Toda.cu:254-293), dDNKG::move
  * (dDNKG.cu:199-234), and the DNLS split-step batch (DNLS.cpp:159-175), each
  * with kernel_batching = steps.  Drift/kick merging as the reference: the
  * c[7] drift of a step is fused with the c[0] drift of the next, and one final
@@ -173,7 +173,7 @@ int nlc_linenergies(nlc_ctx *ctx, double norm, double *linenergies, double *entr
 
 /* Synchronous.  Raw per-mode sums over copies, before normalisation (the
  * content of gres.linenergies after make_linenergies, e.g.
- * DNKG_FPUT_Toda.cu:304-321). */
+ * DNKG_FPUT_Toda.cu:296-323). */
 int nlc_linenergies_raw(nlc_ctx *ctx, double *e_raw);
 
 /* Synchronous.  Total Hamiltonian of every copy, FP64 on the GPU, with the

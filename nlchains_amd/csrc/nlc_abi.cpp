@@ -504,7 +504,7 @@ int setup_dnls(nlc_ctx *c) {
 }
 
 // Step kernel of a uniform real-model context (the reference's move<>
-// dispatch, DNKG_FPUT_Toda.cu:262-297).
+// dispatch, DNKG_FPUT_Toda.cu:254-293).
 int select_uniform(nlc_ctx *c) {
 	const nlc_params &p = c->p;
 	const uint32_t n = c->n;

@@ -4,7 +4,7 @@
 // Reference behaviour followed:
 //   plane-wave energies: DNKG_FPUT_Toda.cpp:113-123 (memset + 2x memcpy2D +
 //     cufftExecZ2Z of q and p separately) and make_linenergies_kernel
-//     DNKG_FPUT_Toda.cu:304-321 (thread per mode, serial loop over copies);
+//     DNKG_FPUT_Toda.cu:296-323 (thread per mode, serial loop over copies);
 //   DNLS energies: DNLS.cpp:142-144, DNLS.cu:70-90;
 //   normalisation and entropies: common/results_entropy.cpp:11-46 (host code
 //     in the reference, FP64 on the GPU here);

@@ -2,7 +2,7 @@
 // longer than the register- and LDS-resident kernels hold and for the
 // reference's --split_kernel.
 //
-// Reference behaviour followed: move_split, DNKG_FPUT_Toda.cu:210-259 and
+// Reference behaviour followed: move_split, DNKG_FPUT_Toda.cu:210-251 and
 // dDNKG.cu:154-196 (layout q[site][copy], p[site][copy], one pass per substep
 // that drifts q and kicks p); the planar<->split conversion of plane2split,
 // common/utilities_cuda.cu:27-62.

@@ -75,10 +75,10 @@ __device__ __forceinline__ double site_force(double l, double q, double r, doubl
 // Bond force term of FPUT / Toda for the bond x = q_{i} - q_{i-1}; the force on
 // site i is bond(x_{i+1}) - bond(x_i).
 //   FPUT: V'(x) = x + alpha x^2 + beta x^3  (Horner), the same force as
-//         (l+r-2c)(1+alpha(r-l)) + beta(d_r^3-d_l^3) of DNKG_FPUT_Toda.cu:41-58;
+//         (l+r-2c)(1+alpha(r-l)) + beta(d_r^3-d_l^3) of DNKG_FPUT_Toda.cu:33-50;
 //         on sigma q, kFormCubic (alpha = 0, sigma = sqrt(beta)) is x + x^3 and
 //         kFormQuad (beta = 0, sigma = alpha) is x + x^2;
-//   Toda: e^{2 alpha x}, the force of DNKG_FPUT_Toda.cu:62-74 without its factor
+//   Toda: e^{2 alpha x}, the force of DNKG_FPUT_Toda.cu:52-66 without its factor
 //         1/(2 alpha), which the kick folds into its weight (the constant
 //         -1/(2 alpha) of V' cancels in the difference), evaluated as
 //         2^(x'/32) on the bond x' = v_i - v_{i-1} of v = (64 alpha/ln2) q:

@@ -4,7 +4,7 @@
 // NLC_GROUP_LOOKUP defined.
 //
 // Reference behaviour followed: the same step as move_chain_in_warp
-// (DNKG_FPUT_Toda.cu:82-147, dDNKG.cu:25-128); the reference runs chains that
+// (DNKG_FPUT_Toda.cu:74-139, dDNKG.cu:24-131); the reference runs chains that
 // do not fit its warp kernel through the HBM split kernel (move_split,
 // DNKG_FPUT_Toda.cu:210-251), which this kernel replaces for n <= 16384.
 //

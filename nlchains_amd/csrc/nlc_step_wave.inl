@@ -4,8 +4,8 @@
 // NLC_STEP_LOOKUP defined, and by nlc_step16_<model>.hip with
 // NLC_STEP_LOOKUP16 (the 16-lane ragged shapes), so they compile in parallel.
 //
-// Reference behaviour followed: DNKG_FPUT_Toda.cu:82-147 (move_chain_in_warp),
-// :156-190 (move_chain_in_thread), dDNKG.cu:25-128; forces DNKG_FPUT_Toda.cu:21-74
+// Reference behaviour followed: DNKG_FPUT_Toda.cu:74-139 (move_chain_in_warp),
+// :148-182 (move_chain_in_thread), dDNKG.cu:24-131; forces DNKG_FPUT_Toda.cu:20-66
 // and dDNKG.cu:17-19.
 //
 // Design (MI355X-first, not a translation of the warp kernel):

@@ -164,7 +164,7 @@ ORC_HOT static void orc_chain_real(const orc_rhs_params *P, uint32_t n, double *
 /*
  * Advance `copies` chains of length n by `steps` Yoshida-6 steps.
  * planar: double[copies][n][2] = {q, p} (configuration.hpp:71, README.md:78).
- * Loop structure: DNKG_FPUT_Toda/cpu-implementation/DNKG_FPUT_Toda.cpp:283-304
+ * Loop structure: DNKG_FPUT_Toda/cpu-implementation/DNKG_FPUT_Toda.cpp:113-147
  * (dDNKG: dDNKG/cpu-implementation/dDNKG.cpp:86-110): drift with dt*c[k]
  * (doubled for k==0 after the first step, merging the c[7] drift of the
  * previous step), periodic halo, kick with dt*d[k]; a final dt*c[7] drift.

@@ -175,6 +175,32 @@ def test_cli_entropy_limit_stops_run(gpu, tmp_path):
 
 
 @pytest.mark.gpu
+def test_cli_blow_up_stops_run(gpu, tmp_path):
+    """A numerical blow-up makes the entropy non-finite and the run stops
+    (common/results.cpp:64-68): Toda alpha = 1 at dt = 1, far outside the
+    integrator's stability region (the oracle's chains are all non-finite
+    after the first 10-step batch), asked for 2000 steps."""
+    n, c, b = 64, 8, 10
+    # every mode carries energy, so the entropy at t = 0 is finite
+    s0 = gpu.ic_plane_waves(n, c, 0.0, 1, 5, 0.5, 1) + 0.01 * np.random.default_rng(4).normal(size=(c, n, 2))
+    s0.tofile(tmp_path / "init")
+    assert not np.isfinite(O.move_real(O.TODA, s0, b, 1.0, alpha=1.0)).all()
+    pre = tmp_path / "boom"
+    run("Toda", "-n", n, "-c", c, "--dt", 1.0, "-b", b, "-s", 2000, "--dump_interval", 2000, "-p", pre,
+        "-i", tmp_path / "init", "--alpha", 1, check=0)
+    ent = read_entropy(pre)
+    bad = ~np.isfinite(ent[:, 1:]).all(axis=1)
+    assert np.isfinite(ent[0, 1:]).all() and bad.any()
+    first = int(np.argmax(bad))
+    # the run ends once the batches queued at the first non-finite entropy are
+    # done (two ahead, as the reference's throttle), far before step 2000
+    assert first <= 3 and ent.shape[0] <= first + 3, ent[:, 0]
+    assert ent[-1, 0] < 2000 * 1.0
+    last = int(round(ent[-1, 0] / 1.0))
+    assert (tmp_path / f"boom-{last}").exists()  # the final dump at the step the run reached
+
+
+@pytest.mark.gpu
 def test_cli_ddnkg_eigensystem_files_and_run(gpu, tmp_path):
     n, c = 48, 6
     masses = gpu.ic_masses(n, 0.5, 1.5, 5)

@@ -96,6 +96,26 @@ def test_chain_energy_hand_values():
         kin + ((np.exp(2 * al * d) - 2 * al * d - 1) / (4 * al ** 2)).sum(), rel=1e-13)
 
 
+def test_chain_energy_hand_values_dnls_ddnkg():
+    """The DNLS and dDNKG branches, README.md:15-19 / nlchains.m:40-48, on
+    3-site chains whose energies are exact binary fractions.
+    DNLS: psi = (1, i, -1 + 2i), H = sum |psi_{x+1} - psi_x|^2 + beta/2 |psi_x|^4:
+    gradient terms 2 + 2 + 8 = 12, quartic sum 1 + 1 + 25 = 27, beta = 0.5 gives
+    12 + 6.75 = 18.75 (beta/4 instead of beta/2 would give 15.375).
+    dDNKG: q = (0, h, 3h), p = (1, 0, -1), masses m_x = (0.5, 1.5, 1.0), h = 0.5,
+    beta = 2: kinetic 1, bonds (1 + 4 + 9) h^2/2 = 1.75, masses
+    (1.5 h^2 + 9 h^2)/2 = 1.3125, quartic beta (h^4 + 81 h^4)/4 = 2.5625:
+    6.625 (the masses + 2 of the force in place of m_x would give 8.875)."""
+    psi = np.array([[[1.0, 0.0], [0.0, 1.0], [-1.0, 2.0]]])
+    assert O.chain_energy(O.DNLS, psi, beta=0.5)[0] == 18.75
+    assert O.chain_energy(O.DNLS, psi, beta=0.0)[0] == 12.0
+    h = 0.5
+    st = np.array([[[0.0, 1.0], [h, 0.0], [3 * h, -1.0]]])
+    masses = np.array([0.5, 1.5, 1.0])
+    assert O.chain_energy(O.DDNKG, st, beta=2.0, masses=masses)[0] == 6.625
+    assert O.chain_energy(O.DDNKG, st, beta=0.0, masses=masses)[0] == 6.625 - 2.5625
+
+
 def test_move_matches_composition_of_pinned_force():
     """The kick loop of orc_move_real (its own expressions) against a numpy
     Yoshida composition that uses the FD-pinned orc_forces."""
