@@ -32,6 +32,7 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "site-steps/s plain FP64 vs mixed/batched at stated energy drift; % of roofline"
 FP64_PEAK_TFLOPS = 78.6
+FP32_PEAK_TFLOPS = 157.3
 HBM_PEAK_GBS = 8000.0
 
 # algorithmic FLOP per site-step (FMA = 2, exp/sincos = 0): 7 kicks + 7 drifts;
@@ -47,6 +48,10 @@ def flop_per_site_step(w) -> int:
     return FLOP_PER_SITE_STEP[w["model"]]
 
 WORKLOADS = {
+    # FPUT-beta, 64 sites x 1024 copies, dt=0.1, energy density 0.1 over k = 1..4 (seed 1),
+    # energies + entropies every 100 steps: the batches are queued as the CLI queues them
+    "c1": dict(model="FPUT", n=64, copies=1024, dt=0.1, m=0.0, alpha=0.0, beta=1.0, k_lo=1, k_hi=5, dens=0.1,
+               seed=1, batch=100, pipelined=True),
     "dnkg": dict(model="DNKG", n=1024, copies=16384, dt=0.05, m=1.0, alpha=0.0, beta=1.0, k_lo=0, k_hi=128,
                  dens=0.5, seed=2, batch=1000),
     "fput": dict(model="FPUT", n=1024, copies=16384, dt=0.05, m=0.0, alpha=0.0, beta=1.0, k_lo=1, k_hi=128,
@@ -99,11 +104,16 @@ class HipEvents:
         self.hip.hipEventDestroy(ev)
 
 
+def host_cores() -> int:
+    """Host cores this process may run on (its CPU affinity)."""
+    return max(1, len(os.sched_getaffinity(0)))
+
+
 def cpu_baseline(w, s0, seconds_budget: float):
     """Oracle release build, OpenMP over copies, on a bounded sample of the workload."""
     from oracle import oracle as O
     lib = O.fast()
-    cores = max(1, min(len(os.sched_getaffinity(0)), 16))
+    cores = host_cores()
     def move(state, steps):
         if w["model"] == "DNLS":
             return O.move_dnls(state, steps, w["dt"], w["beta"], threads=cores, lib=lib)
@@ -130,9 +140,49 @@ def cpu_baseline(w, s0, seconds_budget: float):
         t = time.perf_counter()
         move(s0[:copies], steps)
         el = time.perf_counter() - t
-    return {"value": copies * w["n"] * steps / el, "unit": "site-steps/s", "cores": cores, "kind": "port",
+    return {"value": copies * w["n"] * steps / el, "unit": "site-steps/s", "cores": cores, "host_cores": host_cores(),
+            "kind": "port",
             "sample": f"oracle/liboracle_fast.so (-O3 -ffast-math, OpenMP over copies), {w['model']} "
                       f"N={w['n']} x {copies} copies x {steps} steps, {el:.1f} s"}
+
+
+def fp64_comparison(N, w, s0, b, args, E0, nrm0, E1, nrm1, e_lin, ent):
+    """The FP64 kernel on the same initial state for the same batches (timed
+    the same way, for the speed ratio), and the FP32C run's norm and energy
+    error against it: per-chain drift of the conserved norm and energy, the
+    mode energies and entropies at the end of the run."""
+    with N.Ensemble(N.MODELS[w["model"]], w["n"], w["copies"], w["dt"], beta=w["beta"]) as ref:
+        ref.upload(s0)
+        for _ in range(args.warmup):
+            ref.move(b)
+            ref.linenergies()
+        ref.sync()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            ref.move(b)
+            e64, s64 = ref.linenergies()
+        ref.sync()
+        el = time.perf_counter() - t0
+        E64, nrm64 = ref.chain_energies()
+    sites = float(w["n"]) * w["copies"]
+
+    def drift(a, a0):
+        r = np.abs(a / a0 - 1)
+        return {"max": float(r.max()), "mean": float(r.mean())}
+    return {
+        "kind": "fp32c",
+        "method": "state on chip as FP32 pairs hi + lo (48 significant bits); every substep's increment (nonlinear "
+                  "phase, FFTs, Fourier multiplier) in FP32, added by a compensated sum; state in HBM, energies, "
+                  "norms and entropies in FP64",
+        "over_steps": (args.warmup + args.steps) * b,
+        "norm_drift": drift(nrm1, nrm0), "energy_drift": drift(E1, E0),
+        "fp64": {"value": sites * b * args.steps / el, "norm_drift": drift(nrm64, nrm0),
+                 "energy_drift": drift(E64, E0)},
+        # the trajectories are chaotic: at the end of the run the FP32C and FP64
+        # states differ chain by chain, the ensemble statistics do not
+        "linenergies_vs_fp64": float(np.abs(e_lin - e64).max() / np.abs(e64).max()),
+        "entropies_vs_fp64": [float(abs(ent[i] - s64[i]) / abs(s64[i])) for i in range(2)],
+    }
 
 
 def copy_calibration(mib: int) -> None:
@@ -172,7 +222,7 @@ def cpu_baseline_ragged(w, lengths, s0, seconds_budget: float):
     of every length class, each class run as one oracle call."""
     from oracle import oracle as O
     lib = O.fast()
-    cores = max(1, min(len(os.sched_getaffinity(0)), 16))
+    cores = host_cores()
     offs = np.concatenate([[0], np.cumsum(lengths.astype(np.int64))])
     per_class = max(cores, 256)
     groups = []
@@ -195,7 +245,8 @@ def cpu_baseline_ragged(w, lengths, s0, seconds_budget: float):
     if el < 0.5 * seconds_budget:  # the 2-step probe underestimates the rate: rescale once
         steps = max(steps, min(int(steps * seconds_budget / el), 100 * w["batch"]))
         el = sample(steps)
-    return {"value": sites * steps / el, "unit": "site-steps/s", "cores": cores, "kind": "port",
+    return {"value": sites * steps / el, "unit": "site-steps/s", "cores": cores, "host_cores": host_cores(),
+            "kind": "port",
             "sample": f"oracle/liboracle_fast.so (-O3 -ffast-math, OpenMP over copies), Toda, first {per_class} "
                       f"chains of each length class ({sites} sites) x {steps} steps, {el:.1f} s"}
 
@@ -210,6 +261,9 @@ def main():
     ap.add_argument("--mode", default="plain", choices=["plain", "batched"],
                     help="plain: move + energy pass per batch; batched: autotuned shape, one hipGraph per batch")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU-baseline time budget (0 = skip)")
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32c"],
+                    help="fp32c: DNLS with FP32 fields and compensated accumulation (reductions in FP64); the "
+                         "line then also reports its norm and energy error against an FP64 run of the same steps")
     ap.add_argument("--projection", default="auto", choices=["auto", "mfma", "valu", "bf16"],
                     help="dDNKG energy projection: FP64 MFMA, FP64 VALU or split BF16 MFMA")
     ap.add_argument("--lanes", type=int, default=0)
@@ -233,7 +287,13 @@ def main():
     model = N.MODELS[w["model"]]
     b = w["batch"]
     ragged = w.get("ragged", False)
-    mode = "batched" if ragged else args.mode
+    pipelined = w.get("pipelined", False)
+    mode = "batched" if (ragged or pipelined) else args.mode
+    fp32c = args.precision == "fp32c"
+    if fp32c and w["model"] != "DNLS":
+        print(json.dumps({"error": "--precision fp32c is implemented for the DNLS workload"}))
+        sys.exit(2)
+    prec = N.FP32C if fp32c else N.FP64
     lengths = None
     if ragged:
         lengths = N.ic_ragged_lengths(w["nchains"], w["choices"], w["seed"])
@@ -257,7 +317,7 @@ def main():
         proj = {"auto": N.PROJ_AUTO, "mfma": N.PROJ_MFMA_F64, "valu": N.PROJ_VALU_F64,
                 "bf16": N.PROJ_SPLIT_BF16}[args.projection]
         ens = N.Ensemble(model, n, copies, w["dt"], m=w["m"], alpha=w["alpha"], beta=w["beta"], masses=masses,
-                         lanes_per_chain=args.lanes, sites_per_lane=args.sites, projection=proj)
+                         lanes_per_chain=args.lanes, sites_per_lane=args.sites, projection=proj, precision=prec)
         shape = f"N={n} copies={copies}"
         if w["model"] == "dDNKG":
             shape += f" masses U[{w['m_lo']},{w['m_hi']}] (seed {w['seed']}), {w['modes']} lowest eigenmodes excited"
@@ -270,23 +330,40 @@ def main():
             tune = ens.autotune(max(1, b // 20))
         desc = ens.describe()
         stream = ens.stream()
-        E0, _ = ens.chain_energies()
+        E0, nrm0 = ens.chain_energies()
 
-        def bench_step():
-            if mode == "batched":
-                return ens.batch(b)  # one hipGraph: move + energy pass + entropies
-            ens.move(b)
-            return ens.linenergies()  # energy pass + entropies, synchronous
+        def run_steps(k):
+            """k bench steps; returns the last one's energies and entropies."""
+            out = None
+            if pipelined:
+                # the CLI's pipeline: batches (one hipGraph each) queued up to three
+                # deep, results collected in submission order
+                queued = 0
+                for _ in range(k):
+                    ens.batch_submit(b)
+                    queued += 1
+                    if queued == 3:
+                        out = ens.batch_collect()
+                        queued -= 1
+                while queued:
+                    out = ens.batch_collect()
+                    queued -= 1
+                return out
+            for _ in range(k):
+                if mode == "batched":
+                    out = ens.batch(b)  # one hipGraph: move + energy pass + entropies
+                else:
+                    ens.move(b)
+                    out = ens.linenergies()  # energy pass + entropies, synchronous
+            return out
 
-        for _ in range(args.warmup):
-            bench_step()
+        run_steps(args.warmup)
         ens.sync()
         t0 = time.perf_counter()
-        for _ in range(args.steps):
-            e_lin, ent = bench_step()
+        e_lin, ent = run_steps(args.steps)
         ens.sync()
         elapsed = time.perf_counter() - t0
-        E1, _ = ens.chain_energies()
+        E1, nrm1 = ens.chain_energies()
         # step-kernel duration from HIP events on the context stream (separate launches)
         starts = [ev.create() for _ in range(3)]
         stops = [ev.create() for _ in range(3)]
@@ -314,6 +391,9 @@ def main():
         ens.upload(host_state)
         upload_ms = (time.perf_counter() - t) * 1e3
 
+    precision = None
+    if fp32c:
+        precision = fp64_comparison(N, w, s0, b, args, E0, nrm0, E1, nrm1, e_lin, ent)
     total_steps = (args.warmup + args.steps) * b
     rel = np.abs(E1 - E0) / np.abs(E0)
     drift = float(np.mean(rel[np.isfinite(rel)])) if np.isfinite(rel).any() else float("nan")
@@ -335,11 +415,13 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32" if fp32c else "f64",
         "data": "synthetic (seeded initial conditions: " + ("random-phase Fourier modes" if w["model"] == "DNLS" else
                  "eigenmodes" if w["model"] == "dDNKG" else "plane waves") + ")",
         "config": {"workload": f"{w['model']} m={w['m']} alpha={w['alpha']} beta={w['beta']} {shape} dt={w['dt']}, "
-                               f"{mode} FP64, {b} steps + energy pass + entropies per bench step",
+                               f"{mode} {'FP32C (FP32 fields, compensated accumulation)' if fp32c else 'FP64'}, "
+                               f"{b} steps + energy pass + entropies per bench step"
+                               + (", batches queued three deep" if pipelined else ""),
                    "kernel": desc, "yoshida_steps_per_bench_step": b, "mode": mode},
         "energy_drift": {"mean_rel": drift, "over_steps": total_steps,
                          "nonfinite_chains": int((~np.isfinite(rel)).sum())},
@@ -351,6 +433,17 @@ def main():
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic_bytes(desc),
                      "kernel": kname, "kernel_ms": k_ms, "flop_per_launch": flop},
     }
+    if fp32c:
+        # the FP32 vector peak is the packed-FP32 rate; compiled without packed
+        # FP32 ops, an FP32 instruction issues at the FP64 rate (78.6 TFLOP/s)
+        out["roofline"].update({"compute_unit": "fp32 vector ALU (157.3 TFLOP/s with packed FP32; non-packed "
+                                                "FP32, as compiled here, issues at 78.6 TFLOP/s)",
+                                "peak": FP32_PEAK_TFLOPS, "frac": achieved / FP32_PEAK_TFLOPS,
+                                "frac_of_nonpacked_fp32": achieved / FP64_PEAK_TFLOPS})
+        out["precision"] = precision
+        out["vs_fp64"] = value / precision["fp64"]["value"]
+    if pipelined:
+        out["us_per_batch"] = elapsed / args.steps * 1e6
     out["host_transfer"] = {"shard_bytes": int(host_state.nbytes), "download_ms": download_ms, "upload_ms": upload_ms,
                             "value_with_shard_download_every_bench_step":
                                 sites * b / ((elapsed / args.steps * 1e3 + download_ms) * 1e-3)}

@@ -47,11 +47,14 @@ extern "C" {
  * dDNKG.cpp:168-170, DNLS.cpp:193-195). */
 typedef enum { NLC_DNKG = 0, NLC_FPUT = 1, NLC_TODA = 2, NLC_DDNKG = 3, NLC_DNLS = 4 } nlc_model;
 
-/* Arithmetic of the step.  NLC_FP64 is the reference's all-double method.
- * NLC_FP32C (FP32 arithmetic with compensated double-single accumulation) is
- * reserved: nlc_create rejects it with NLC_EUNSUPPORTED, because on gfx950
- * without packed FP32 an FP32 instruction issues at the FP64 rate and the
- * compensated form needs more instructions per site than the FP64 step. */
+/* Arithmetic of the step.  NLC_FP64 is the reference's all-double method
+ * (DNLS.cpp:159-175 for DNLS).  NLC_FP32C: DNLS with 4096-site chains keeps
+ * the field on chip as two FP32 values (hi + lo), computes every substep's
+ * increment (nonlinear phase, FFTs, Fourier multiplier) in FP32 and adds it
+ * with a compensated double-single sum; the state in HBM, the energies, norms
+ * and entropies stay FP64.  Other models and lengths: NLC_EUNSUPPORTED (on
+ * gfx950 without packed FP32 an FP32 instruction issues at the FP64 rate, and
+ * the register-resident real-model steps are bound by those instructions). */
 typedef enum { NLC_FP64 = 0, NLC_FP32C = 1 } nlc_precision;
 
 /* Step-kernel family.  NLC_KERNEL_SPLIT is the reference's --split_kernel
@@ -186,15 +189,30 @@ int nlc_chain_energies(nlc_ctx *ctx, double *energies, double *norms);
 /* Ragged ensemble: nchains chains of individual lengths chain_lengths[i]
  * (DNKG, FPUT, Toda).  The reference runs one chain length per process
  * (gconf.chain_length, common/configuration.hpp:24); here chains of all lengths
- * share one context and one persistent launch per batch.  State layout for
+ * share one context and one persistent launch per batch (powers of two from 4
+ * to 2048 sites; other lengths run one wave-shaped launch per class).  State layout for
  * upload/download: the chains concatenated in the caller's order, each in the
  * reference layout.  p->chain_length and p->copies are ignored.  Energies and
  * entropies are computed per length class: nlc_linenergies writes, for every
  * class in ascending length, that class's linear energies (sum of the class
  * lengths doubles in total) and its two entropies (2 doubles per class). */
 int nlc_create_ragged(nlc_ctx **out, const nlc_params *p, const uint32_t *chain_lengths, uint32_t nchains);
-/* Length classes of a context (ascending length); a uniform context has one. */
+/* Mixed ensemble: chains of several models and lengths in one context, one
+ * persistent launch per batch.  params[0..nsets) (1 <= nsets <= 3): a model
+ * (DNKG, FPUT or Toda, FP64) with its parameters; chain i has length
+ * chain_lengths[i] and follows params[chain_set[i]].  The reference runs one
+ * model per process (common/main.cpp:157-160, a program per model in the
+ * registry of DNKG_FPUT_Toda.cpp:156-162); here the models share the state, the
+ * queue of the persistent kernel and the batch graph.  Length classes are the
+ * distinct (set, length) pairs in ascending order; energies, entropies and the
+ * state layout are as for nlc_create_ragged, each class with its own model's
+ * dispersion, normalisation and a0is0 default (m == 0). */
+int nlc_create_mixed(nlc_ctx **out, const nlc_params *params, uint32_t nsets, const uint32_t *chain_lengths,
+                     const uint32_t *chain_set, uint32_t nchains);
+/* Length classes of a context (ascending (set, length)); a uniform context has one. */
 int nlc_ragged_classes(const nlc_ctx *ctx, uint32_t *nclasses, uint32_t *lengths, uint32_t *counts, uint32_t cap);
+/* The parameter set of each class (0 unless mixed), in the order of nlc_ragged_classes. */
+int nlc_ragged_class_sets(const nlc_ctx *ctx, uint32_t *sets, uint32_t cap);
 
 /* Synchronous.  One dump interval: nlc_move(steps) followed by the energy pass,
  * normalisation and entropies, replayed from a hipGraph captured on the first

@@ -86,6 +86,8 @@ _sig("nlc_ic_masses", C.c_int, C.c_uint32, C.c_double, C.c_double, C.c_uint64, _
 _u32p = C.POINTER(C.c_uint32)
 _sig("nlc_create_ragged", C.c_int, C.POINTER(_ctx_p), C.POINTER(Params), _u32p, C.c_uint32)
 _sig("nlc_ragged_classes", C.c_int, _ctx_p, _u32p, _u32p, _u32p, C.c_uint32)
+_sig("nlc_create_mixed", C.c_int, C.POINTER(_ctx_p), C.POINTER(Params), C.c_uint32, _u32p, _u32p, C.c_uint32)
+_sig("nlc_ragged_class_sets", C.c_int, _ctx_p, _u32p, C.c_uint32)
 _sig("nlc_batch", C.c_int, _ctx_p, C.c_uint32, C.c_double, _dp, _dp)
 _sig("nlc_batch_submit", C.c_int, _ctx_p, C.c_uint32, C.c_double)
 _sig("nlc_batch_collect", C.c_int, _ctx_p, _dp, _dp)
@@ -208,15 +210,30 @@ class Ensemble:
     def ragged(cls, model: int, lengths, dt: float, *, m: float = 0.0, alpha: float = 0.0, beta: float = 0.0,
                precision: int = FP64):
         """A ragged ensemble: chains of the given lengths in one context."""
+        return cls.mixed([dict(model=model, m=m, alpha=alpha, beta=beta, precision=precision)], lengths, None, dt)
+
+    @classmethod
+    def mixed(cls, sets, lengths, chain_set, dt: float):
+        """A mixed ensemble: sets = [dict(model=..., m=..., alpha=..., beta=...)]
+        (1 to 3), chain i of length lengths[i] following sets[chain_set[i]];
+        chain_set None = a ragged ensemble of sets[0] (nlc_create_ragged)."""
         self = cls.__new__(cls)
-        p = Params()
-        _check(lib.nlc_params_init(C.byref(p)))
-        p.model, p.precision, p.dt, p.m, p.alpha, p.beta = model, precision, dt, m, alpha, beta
+        ps = (Params * len(sets))()
+        for p, kw in zip(ps, sets):
+            _check(lib.nlc_params_init(C.byref(p)))
+            p.model, p.precision, p.dt = kw["model"], kw.get("precision", FP64), dt
+            p.m, p.alpha, p.beta = kw.get("m", 0.0), kw.get("alpha", 0.0), kw.get("beta", 0.0)
         self._masses = None
         ln = np.ascontiguousarray(lengths, dtype=np.uint32)
         self._ctx = _ctx_p()
-        _check(lib.nlc_create_ragged(C.byref(self._ctx), C.byref(p), ln.ctypes.data_as(_u32p), ln.size))
-        self.model, self.lengths = model, ln
+        if chain_set is None:
+            _check(lib.nlc_create_ragged(C.byref(self._ctx), ps, ln.ctypes.data_as(_u32p), ln.size))
+        else:
+            cs = np.ascontiguousarray(chain_set, dtype=np.uint32)
+            assert cs.size == ln.size
+            _check(lib.nlc_create_mixed(C.byref(self._ctx), ps, len(sets), ln.ctypes.data_as(_u32p),
+                                        cs.ctypes.data_as(_u32p), ln.size))
+        self.model, self.lengths = sets[0]["model"], ln
         self.n, self.copies = int(ln.max()), int(ln.size)
         self._shape = (int(ln.sum()), 2)
         nc = C.c_uint32()
@@ -225,6 +242,9 @@ class Ensemble:
         _check(lib.nlc_ragged_classes(self._ctx, C.byref(nc), cl.ctypes.data_as(_u32p), ct.ctypes.data_as(_u32p),
                                       nc.value), self._ctx)
         self.class_lengths, self.class_counts = cl, ct
+        cset = np.zeros(nc.value, np.uint32)
+        _check(lib.nlc_ragged_class_sets(self._ctx, cset.ctypes.data_as(_u32p), nc.value), self._ctx)
+        self.class_sets = cset
         self._nlin, self._nent = int(cl.sum()), 2 * nc.value
         return self
 

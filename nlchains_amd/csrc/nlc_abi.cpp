@@ -31,12 +31,18 @@ struct Klass {
 	int L = 0, S = 0;
 	bool exact = false;
 	int side = 0; // per-class launches: 0 = context stream, 1 = the second stream
+	int set = 0;   // parameter set (mixed ensembles: one per model; otherwise 0)
+	int model = 0; // that set's model
+	int a0is0 = 0; // the results(a0is0) flag of the class
 };
 
 } // namespace
 
+void nlc::set_thread_error(const char *msg) { g_create_error = msg; }
+
 struct nlc_ctx {
-	nlc_params p{};
+	nlc_params p{};                // the context's parameters (mixed ensembles: set 0's)
+	std::vector<nlc_params> sets;  // parameter sets of the classes (one unless mixed)
 	std::vector<double> masses; // dDNKG m_x
 	hipStream_t stream = nullptr;
 	nlc::StepArgs args{};
@@ -52,10 +58,16 @@ struct nlc_ctx {
 	std::vector<uint64_t> caller_site0;  // caller order
 	std::vector<uint64_t> device_site0;  // caller order -> device offset
 	std::vector<uint32_t> device_chain;  // caller order -> internal chain index
+	// persistent kernel: items sorted most expensive first, S sites per lane
+	// (S/2 for the classes that fit in 64 lanes of S/2 when half_S), the
+	// parameter sets of its kinds, and its two queue counters
 	nlc::RaggedItem *items = nullptr;
 	uint32_t nitems = 0;
 	int ragged_S = 0;
-	nlc::RaggedLauncher ragged_launcher = nullptr;
+	bool ragged_half = false;
+	nlc::PersistentLauncher persistent = nullptr;
+	nlc::PersistentArgs pargs{};
+	uint32_t *queue = nullptr;
 	// per-class launches run on two streams (fork and join by events), so that
 	// the waves of one class fill the tail of another class's launch
 	hipStream_t aux_stream = nullptr;
@@ -81,6 +93,8 @@ struct nlc_ctx {
 	double *split_q[2] = {nullptr, nullptr}, *split_p[2] = {nullptr, nullptr}; // ping-pong pair
 	double *energy_d = nullptr, *norm_d = nullptr;
 	double2 *dnls_table = nullptr; // DNLS Fourier multipliers
+	float2 *dnls_tw32 = nullptr;   // FP32-compensated DNLS: FP32 twiddles
+	float4 *dnls_mult32 = nullptr; // and double-single increment multipliers
 	int dnls_variant = nlc::kDnlsRadix16; // n = 4096: register kernel, radix 16 or radix 8
 	double theta[9] = {};          // DNLS nonlinear angles
 	// results (per class: n energies + 2 entropies) are written by the results
@@ -198,11 +212,11 @@ nlc::TiledLauncher lookup_tiled(int model) {
 	return nullptr;
 }
 
-nlc::RaggedLauncher lookup_ragged(int model, int S) {
-	switch (model) {
-	case NLC_DNKG: return nlc::lookup_ragged_dnkg(S);
-	case NLC_FPUT: return nlc::lookup_ragged_fput(S);
-	case NLC_TODA: return nlc::lookup_ragged_toda(S);
+nlc::PersistentLauncher lookup_persistent(int S, uint32_t kind_mask) {
+	switch (S) {
+	case 8: return nlc::lookup_persistent_s8(kind_mask);
+	case 16: return nlc::lookup_persistent_s16(kind_mask);
+	case 32: return nlc::lookup_persistent_s32(kind_mask);
 	}
 	return nullptr;
 }
@@ -295,13 +309,22 @@ std::vector<Shape> wave_shapes(int model, uint32_t n) {
 	return out;
 }
 
-// The ragged persistent kernel with S sites per lane needs every class to be a
-// power of two in [S, 64 S].
-bool ragged_fits(const std::vector<Klass> &cls, int S) {
+// Sites per lane of a class in the persistent kernel with S sites per lane:
+// S, or with half_S the S/2 of a class that fits in 64 lanes of S/2; 0 if the
+// class has no such shape (the kernel needs powers of two, n = L S').
+int item_S(uint32_t n, int S, bool half_S) {
+	if (!is_pow2(n) || n > 64u * S) return 0;
+	if (half_S && S >= 2 && n <= 32u * S) return n >= uint32_t(S / 2) ? S / 2 : 0;
+	return n >= uint32_t(S) ? S : 0;
+}
+bool ragged_fits(const std::vector<Klass> &cls, int S, bool half_S) {
 	for (const auto &k : cls)
-		if (!is_pow2(k.n) || k.n < uint32_t(S) || k.n > 64u * S) return false;
+		if (!item_S(k.n, S, half_S)) return false;
 	return true;
 }
+// VALU instructions per site and substep of each kind (its kick and drift),
+// the cost weight of the persistent kernel's ordering
+constexpr double kKindCost[6] = {5, 6, 6, 6, 7, 19};
 
 void drop_graph(nlc_ctx *c) {
 	// a queued replay may still be running
@@ -320,8 +343,8 @@ void free_all(nlc_ctx *c) {
 	     {(void *) c->state, (void *) c->omega, (void *) c->tw, (void *) c->partial, (void *) c->reduce_scratch,
 	      (void *) c->e_raw, (void *) c->modes, (void *) c->mp2, (void *) c->masses_d,
 	      (void *) c->eigvec, (void *) c->split_q[0], (void *) c->split_p[0], (void *) c->split_q[1], (void *) c->split_p[1], (void *) c->energy_d, (void *) c->norm_d,
-	      (void *) c->dnls_table, (void *) c->fft_scratch, (void *) c->blue_table, (void *) c->items, (void *) c->vplanes, (void *) c->splanes,
-	      (void *) c->tile_buf})
+	      (void *) c->dnls_table, (void *) c->fft_scratch, (void *) c->blue_table, (void *) c->items, (void *) c->queue, (void *) c->vplanes, (void *) c->splanes,
+	      (void *) c->tile_buf, (void *) c->dnls_tw32, (void *) c->dnls_mult32})
 		if (ptr) (void) hipFree(ptr);
 	if (c->h_results) (void) hipHostFree(c->h_results);
 	if (c->h_slots) (void) hipHostFree(c->h_slots);
@@ -348,23 +371,47 @@ int dev_alloc(nlc_ctx *c, T **ptr, size_t count, const char *what) {
 	return NLC_OK;
 }
 
+// Step constants of one parameter set (DNKG_FPUT_Toda.cpp:76-84, dDNKG.cpp:93-97).
+nlc::StepArgs step_args(const nlc_params &p) {
+	nlc::StepArgs a{};
+	for (int i = 0; i < 8; ++i) {
+		a.dc[i] = nlc::kYoshidaC[i] * p.dt;
+		a.dd[i] = nlc::kYoshidaD[i] * p.dt;
+	}
+	a.m = p.m;
+	a.m2 = p.m + 2;
+	a.beta = p.beta;
+	a.alpha = p.alpha;
+	a.a2 = 2 * p.alpha;
+	a.a2i = 1 / (2 * p.alpha);
+	a.form = nlc::force_form(p.model, p.alpha, p.beta, &a.scale);
+	a.inv_scale = 1 / a.scale;
+	return a;
+}
+
+// The reference's results(a0is0) default of a model (DNKG_FPUT_Toda.cpp:52,
+// DNLS.cpp:49, dDNKG.cpp:83).
+int default_a0is0(const nlc_params &p) { return p.model == NLC_DNLS ? 1 : p.model == NLC_DDNKG ? 0 : p.m == 0.0; }
+
 // Buffers and tables shared by uniform and ragged contexts.
 int setup_common(nlc_ctx *c) {
 	const nlc_params &p = c->p;
-	// constants (DNKG_FPUT_Toda.cpp:76-84, dDNKG.cpp:93-97)
-	for (int i = 0; i < 8; ++i) {
-		c->args.dc[i] = nlc::kYoshidaC[i] * p.dt;
-		c->args.dd[i] = nlc::kYoshidaD[i] * p.dt;
+	if (c->sets.empty()) c->sets.push_back(p);
+	for (int s = 0; s < 3; ++s) {
+		if (s < int(c->sets.size())) {
+			c->pargs.a[s] = step_args(c->sets[s]);
+			c->pargs.kind[s] = nlc::kind_of(c->sets[s].model, c->pargs.a[s].form);
+		} else {
+			c->pargs.a[s] = c->pargs.a[0];
+			c->pargs.kind[s] = -1;
+		}
 	}
-	c->args.m = p.m;
-	c->args.m2 = p.m + 2;
-	c->args.beta = p.beta;
-	c->args.alpha = p.alpha;
-	c->args.a2 = 2 * p.alpha;
-	c->args.a2i = 1 / (2 * p.alpha);
-	c->args.form = nlc::force_form(p.model, p.alpha, p.beta, &c->args.scale);
-	c->args.inv_scale = 1 / c->args.scale;
-	c->a0is0 = (p.model == NLC_DNLS) ? 1 : (p.model == NLC_DDNKG ? 0 : (p.m == 0.0));
+	c->args = c->pargs.a[0];
+	c->a0is0 = default_a0is0(p);
+	for (auto &k : c->cls) {
+		k.model = c->sets[k.set].model;
+		k.a0is0 = default_a0is0(c->sets[k.set]);
+	}
 
 	NLC_HIP(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
 	c->state_bytes = sizeof(double2) * c->total_sites;
@@ -388,7 +435,7 @@ int setup_common(nlc_ctx *c) {
 			const double ang = 2.0 * M_PI * double(j) / double(k.n);
 			tw[k.tab0 + j] = make_double2(std::cos(ang), -std::sin(ang));
 		}
-		if (p.model != NLC_DDNKG) nlc_dispersion(p.model, k.n, p.m, omega.data() + k.tab0);
+		if (p.model != NLC_DDNKG) nlc_dispersion(k.model, k.n, c->sets[k.set].m, omega.data() + k.tab0);
 		k.rows = p.model == NLC_DDNKG ? nlc::ddnkg_partial_rows(k.count) : nlc::energy_rows(k.n, k.count);
 		if (p.model != NLC_DDNKG)
 			c->fft_scratch_sites = std::max(c->fft_scratch_sites, nlc::energy_scratch_sites(k.n, k.count));
@@ -494,6 +541,21 @@ int setup_dnls(nlc_ctx *c) {
 		nlc::dnls_permute_tables(n, nlc::kDnlsRadix8, tab.data(), perm.data() + size_t(4) * n);
 		tab.swap(perm);
 	}
+	if (p.precision == NLC_FP32C) {
+		// FP32 fields with compensated accumulation: the radix-16 register kernel
+		if (!nlc::dnls_fast_path(n) || (p.sites_per_lane && p.sites_per_lane != 16))
+			return c->fail(NLC_EUNSUPPORTED, "DNLS in FP32C precision runs 4096-site chains on the radix-16 kernel "
+			                                 "(chain_length %u, sites_per_lane %u)", n, p.sites_per_lane);
+		std::vector<float2> tw32(n);
+		std::vector<float4> mult32(size_t(4) * n);
+		nlc::dnls_fp32c_tables(n, p.dt, tw32.data(), mult32.data());
+		NLC_TRY(dev_alloc(c, &c->dnls_tw32, tw32.size(), "DNLS FP32 twiddles"));
+		NLC_TRY(dev_alloc(c, &c->dnls_mult32, mult32.size(), "DNLS FP32C multipliers"));
+		NLC_HIP(c, hipMemcpy(c->dnls_tw32, tw32.data(), sizeof(float2) * n, hipMemcpyHostToDevice));
+		NLC_HIP(c, hipMemcpy(c->dnls_mult32, mult32.data(), sizeof(float4) * mult32.size(), hipMemcpyHostToDevice));
+		c->dnls_variant = nlc::kDnlsRadix16;
+		return NLC_OK;
+	}
 	// sites_per_lane selects the register kernel: values per thread, 16 or 8
 	if (p.sites_per_lane == 8 && nlc::dnls_fast_path(n)) c->dnls_variant = nlc::kDnlsRadix8;
 	else if (p.sites_per_lane == 0 || (p.sites_per_lane == 16 && nlc::dnls_fast_path(n))) c->dnls_variant = nlc::kDnlsRadix16;
@@ -552,31 +614,52 @@ int select_uniform(nlc_ctx *c) {
 }
 
 // Ragged: the persistent kernel with S sites per lane (items built here), or
-// one uniform kernel per class when no S fits every class.
-int select_ragged(nlc_ctx *c, int S) {
+// one uniform kernel per class when S is 0 or no kernel fits every class.
+int select_ragged(nlc_ctx *c, int S, bool half_S) {
 	if (c->items) {
 		(void) hipFree(c->items);
 		c->items = nullptr;
 	}
-	c->ragged_launcher = S ? lookup_ragged(c->p.model, S) : nullptr;
-	if (c->ragged_launcher) {
-		std::vector<nlc::RaggedItem> items;
+	uint32_t kind_mask = 0;
+	for (int s = 0; s < 3; ++s)
+		if (c->pargs.kind[s] >= 0) kind_mask |= 1u << c->pargs.kind[s];
+	c->persistent = S && ragged_fits(c->cls, S, half_S) ? lookup_persistent(S, kind_mask) : nullptr;
+	if (c->persistent) {
+		struct Costed {
+			nlc::RaggedItem it;
+			double cost;
+		};
+		std::vector<Costed> all;
 		for (const auto &k : c->cls) {
-			const uint32_t L = k.n / uint32_t(S), cpw = 64 / L;
+			const int s_item = item_S(k.n, S, half_S);
+			const uint32_t L = k.n / uint32_t(s_item), cpw = 64 / L;
+			const uint8_t flags = uint8_t(k.set | (s_item < S ? nlc::kItemHalfS : 0));
+			const double cost = s_item * kKindCost[c->pargs.kind[k.set]];
 			for (uint32_t i = 0; i < k.count; i += cpw)
-				items.push_back({k.site0 + uint64_t(i) * k.n, k.n, uint16_t(std::min(cpw, k.count - i)),
-				                 uint16_t(log2u(L))});
+				all.push_back({{k.site0 + uint64_t(i) * k.n, k.n, uint16_t(std::min(cpw, k.count - i)),
+				                uint8_t(log2u(L)), flags},
+				               cost});
 		}
+		// most expensive first (stable: within a cost, device order)
+		std::stable_sort(all.begin(), all.end(), [](const Costed &a, const Costed &b) { return a.cost > b.cost; });
+		std::vector<nlc::RaggedItem> items(all.size());
+		for (size_t i = 0; i < all.size(); ++i) items[i] = all[i].it;
 		c->nitems = uint32_t(items.size());
 		NLC_TRY(dev_alloc(c, &c->items, items.size(), "ragged items"));
 		NLC_HIP(c, hipMemcpy(c->items, items.data(), sizeof(nlc::RaggedItem) * items.size(), hipMemcpyHostToDevice));
+		if (!c->queue) {
+			NLC_TRY(dev_alloc(c, &c->queue, 2, "persistent queue"));
+			NLC_HIP(c, hipMemset(c->queue, 0, 2 * sizeof(uint32_t)));
+		}
 		c->ragged_S = S;
+		c->ragged_half = half_S;
 		return NLC_OK;
 	}
 	c->ragged_S = 0;
+	c->ragged_half = false;
 	c->nitems = 0;
 	for (auto &k : c->cls) {
-		if (!default_shape(k.n, k.count, k.L, k.S, k.exact) || !(k.launcher = lookup(c->p.model, k.L, k.S, k.exact)))
+		if (!default_shape(k.n, k.count, k.L, k.S, k.exact) || !(k.launcher = lookup(k.model, k.L, k.S, k.exact)))
 			return c->fail(NLC_EUNSUPPORTED, "ragged ensembles need chain lengths with a wave shape (got %u)", k.n);
 	}
 	return NLC_OK;
@@ -584,14 +667,17 @@ int select_ragged(nlc_ctx *c, int S) {
 
 int enqueue_move(nlc_ctx *c, uint32_t steps) {
 	if (!steps) return c->fail(NLC_EARG, "--kernel_batching must be a positive number");
-	if (c->p.model == NLC_DNLS) {
+	if (c->p.model == NLC_DNLS && c->p.precision == NLC_FP32C) {
+		NLC_HIP(c, nlc::launch_dnls_fp32c(c->stream, c->state, c->copies, steps, c->theta, c->dnls_tw32,
+		                                  c->dnls_mult32));
+	} else if (c->p.model == NLC_DNLS) {
 		const bool r8 = nlc::dnls_fast_path(c->n) && c->dnls_variant == nlc::kDnlsRadix8;
 		NLC_HIP(c, nlc::launch_dnls(c->stream, c->state, c->n, c->copies, steps, c->theta, c->tw,
 		                            c->dnls_table + (r8 ? size_t(4) * c->n : 0), c->dnls_variant, c->fft_scratch,
 		                            c->blue_table));
 	} else if (c->ragged) {
-		if (c->ragged_launcher) {
-			NLC_HIP(c, c->ragged_launcher(c->stream, c->state, c->items, c->nitems, steps, c->args, 0));
+		if (c->persistent) {
+			NLC_HIP(c, c->persistent(c->stream, c->state, c->items, c->nitems, steps, c->pargs, c->queue, 0));
 		} else {
 			const bool fork = c->aux_stream != nullptr;
 			if (fork) {
@@ -600,7 +686,7 @@ int enqueue_move(nlc_ctx *c, uint32_t steps) {
 			}
 			for (const auto &k : c->cls)
 				NLC_HIP(c, k.launcher(fork && k.side ? c->aux_stream : c->stream, c->state + k.site0, k.count, k.n,
-				                      steps, c->args, nullptr));
+				                      steps, c->pargs.a[k.set], nullptr));
 			if (fork) {
 				NLC_HIP(c, hipEventRecord(c->join_ev, c->aux_stream));
 				NLC_HIP(c, hipStreamWaitEvent(c->stream, c->join_ev, 0));
@@ -666,9 +752,9 @@ int enqueue_energy(nlc_ctx *c, double norm = 0.0, double *dst = nullptr) {
 		}
 		double *er = c->e_raw + k.tab0;
 		if (dst) {
-			const double nm = norm > 0 ? norm : default_norm(c->p.model, k);
+			const double nm = norm > 0 ? norm : default_norm(k.model, k);
 			NLC_HIP(c, nlc::launch_reduce_results(c->stream, c->partial, k.rows, k.n, scale, er, c->reduce_scratch, nm,
-			                                      c->a0is0, c->modes, c->ragged ? 0 : c->nmodes,
+			                                      k.a0is0, c->modes, c->ragged ? 0 : c->nmodes,
 			                                      dst + k.tab0 + 2 * i));
 		} else {
 			NLC_HIP(c, nlc::launch_reduce_partial(c->stream, c->partial, k.rows, k.n, scale, er, c->reduce_scratch));
@@ -705,7 +791,9 @@ int validate_params(const nlc_params *p, bool ragged) {
 	if (!(p->dt > 0)) return reject(NLC_EARG, "--dt must be a positive number");
 	if (p->model == NLC_TODA && p->alpha == 0) return reject(NLC_EARG, "Toda needs --alpha != 0");
 	if (p->model == NLC_DDNKG && !p->masses) return reject(NLC_EARG, "dDNKG needs the masses (-m file content)");
-	if (p->precision != NLC_FP64) return reject(NLC_EUNSUPPORTED, "only FP64 precision (NLC_FP64) is implemented");
+	if (p->precision != NLC_FP64 && p->precision != NLC_FP32C) return reject(NLC_EARG, "unknown precision");
+	if (p->precision == NLC_FP32C && p->model != NLC_DNLS)
+		return reject(NLC_EUNSUPPORTED, "FP32C precision is implemented for DNLS (4096-site chains)");
 	if (p->kernel < NLC_KERNEL_AUTO || p->kernel > NLC_KERNEL_TILED) return reject(NLC_EARG, "unknown kernel family");
 	if (p->projection < NLC_PROJ_AUTO || p->projection > NLC_PROJ_SPLIT_BF16)
 		return reject(NLC_EARG, "unknown projection form");
@@ -813,43 +901,60 @@ int nlc_create(nlc_ctx **out, const nlc_params *p) {
 	return finish_create(c, out, rc);
 }
 
-int nlc_create_ragged(nlc_ctx **out, const nlc_params *p, const uint32_t *chain_lengths, uint32_t nchains) {
+} // extern "C"
+
+namespace {
+
+// Ragged and mixed ensembles: chain i has length chain_lengths[i] and
+// parameter set chain_set[i] (NULL: all set 0).  Classes are the distinct
+// (set, length) pairs in ascending order, each contiguous on the device.
+int create_classes(nlc_ctx **out, const nlc_params *params, uint32_t nsets, const uint32_t *chain_lengths,
+                   const uint32_t *chain_set, uint32_t nchains) {
 	g_create_error.clear();
 	if (!out) return reject(NLC_EARG, "out is NULL");
 	*out = nullptr;
 	if (!chain_lengths || !nchains) return reject(NLC_EARG, "no chains");
-	if (int rc = validate_params(p, true)) return rc;
-	if (p->model != NLC_DNKG && p->model != NLC_FPUT && p->model != NLC_TODA)
-		return reject(NLC_EUNSUPPORTED, "ragged ensembles are available for DNKG, FPUT and Toda");
+	if (!params || nsets < 1 || nsets > 3) return reject(NLC_EARG, "1 to 3 parameter sets");
+	for (uint32_t s = 0; s < nsets; ++s) {
+		if (int rc = validate_params(&params[s], true)) return rc;
+		if (params[s].model != NLC_DNKG && params[s].model != NLC_FPUT && params[s].model != NLC_TODA)
+			return reject(NLC_EUNSUPPORTED, "ragged and mixed ensembles are available for DNKG, FPUT and Toda");
+		if (params[s].precision != NLC_FP64) return reject(NLC_EUNSUPPORTED, "ragged and mixed ensembles are FP64");
+	}
 	uint64_t total = 0;
 	for (uint32_t i = 0; i < nchains; ++i) {
 		if (chain_lengths[i] < 2) return reject(NLC_EARG, "every chain length must be >= 2");
+		if (chain_set && chain_set[i] >= nsets) return reject(NLC_EARG, "chain parameter set out of range");
 		total += chain_lengths[i];
 	}
 	if (total > (1ULL << 31)) return reject(NLC_EARG, "total number of sites must be <= 2^31");
 	nlc_ctx *c = new (std::nothrow) nlc_ctx;
 	if (!c) return reject(NLC_ENOMEM, "context allocation failed");
-	c->p = *p;
+	c->p = params[0];
 	c->p.masses = nullptr;
+	c->sets.assign(params, params + nsets);
 	c->ragged = true;
 	c->lengths.assign(chain_lengths, chain_lengths + nchains);
 	c->copies = nchains;
 	c->total_sites = total;
-	// classes in ascending length; inside a class the caller's order
-	std::vector<uint32_t> distinct(c->lengths);
+	// classes in ascending (set, length); inside a class the caller's order
+	auto key = [&](uint32_t i) { return (uint64_t(chain_set ? chain_set[i] : 0) << 32) | chain_lengths[i]; };
+	std::vector<uint64_t> distinct(nchains);
+	for (uint32_t i = 0; i < nchains; ++i) distinct[i] = key(i);
 	std::sort(distinct.begin(), distinct.end());
 	distinct.erase(std::unique(distinct.begin(), distinct.end()), distinct.end());
-	c->n = distinct.back();
-	for (uint32_t len : distinct) {
+	for (uint64_t kv : distinct) {
 		Klass k;
-		k.n = len;
+		k.n = uint32_t(kv);
+		k.set = int(kv >> 32);
 		k.tab0 = c->table_len;
-		c->table_len += len;
+		c->table_len += k.n;
+		c->n = std::max(c->n, k.n);
 		c->cls.push_back(k);
 	}
 	std::vector<size_t> class_of(nchains);
 	for (uint32_t i = 0; i < nchains; ++i) {
-		class_of[i] = std::lower_bound(distinct.begin(), distinct.end(), c->lengths[i]) - distinct.begin();
+		class_of[i] = std::lower_bound(distinct.begin(), distinct.end(), key(i)) - distinct.begin();
 		c->cls[class_of[i]].count++;
 	}
 	uint64_t site = 0, chain = 0;
@@ -875,12 +980,14 @@ int nlc_create_ragged(nlc_ctx **out, const nlc_params *p, const uint32_t *chain_
 	int rc = setup_common(c);
 	if (!rc && c->cls.size() > 1) {
 		// split the classes over the two streams, longest processing time first
-		// (cost ~ sites of the class)
+		// (cost ~ sites of the class times its model's instructions per site)
 		std::vector<size_t> order(c->cls.size());
 		for (size_t i = 0; i < order.size(); ++i) order[i] = i;
-		auto sites = [&](size_t i) { return uint64_t(c->cls[i].n) * c->cls[i].count; };
+		auto sites = [&](size_t i) {
+			return double(c->cls[i].n) * c->cls[i].count * kKindCost[c->pargs.kind[c->cls[i].set]];
+		};
 		std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return sites(a) > sites(b); });
-		uint64_t load[2] = {0, 0};
+		double load[2] = {0, 0};
 		for (size_t i : order) {
 			const int side = load[1] < load[0];
 			c->cls[i].side = side;
@@ -892,14 +999,37 @@ int nlc_create_ragged(nlc_ctx **out, const nlc_params *p, const uint32_t *chain_
 			rc = c->fail(NLC_EHIP, "stream or event creation failed");
 	}
 	if (!rc) {
+		// the persistent kernel with the smallest S that fits (most chains per
+		// wave), items at S / 2 sites per lane where they fit; else per class
 		int S = 0;
 		for (int s : {32, 16, 8})
-			if (ragged_fits(c->cls, s)) S = s; // smallest S that fits: most chains per wave
-		rc = select_ragged(c, S);
+			if (ragged_fits(c->cls, s, true)) S = s;
+		rc = select_ragged(c, S, true);
 	}
-	if (!rc) rc = nlc_set_entropy_mask(c, nullptr, c->a0is0);
 	return finish_create(c, out, rc);
 }
+
+} // namespace
+
+extern "C" {
+
+int nlc_create_ragged(nlc_ctx **out, const nlc_params *p, const uint32_t *chain_lengths, uint32_t nchains) {
+	return create_classes(out, p, 1, chain_lengths, nullptr, nchains);
+}
+
+int nlc_create_mixed(nlc_ctx **out, const nlc_params *params, uint32_t nsets, const uint32_t *chain_lengths,
+                     const uint32_t *chain_set, uint32_t nchains) {
+	if (!chain_set) return reject(NLC_EARG, "chain_set is NULL");
+	return create_classes(out, params, nsets, chain_lengths, chain_set, nchains);
+}
+
+int nlc_ragged_class_sets(const nlc_ctx *c, uint32_t *sets, uint32_t cap) {
+	if (!c || (!sets && cap)) return NLC_EARG;
+	for (uint32_t i = 0; i < std::min<uint32_t>(cap, uint32_t(c->cls.size())); ++i) sets[i] = uint32_t(c->cls[i].set);
+	return NLC_OK;
+}
+
+
 
 int nlc_ragged_classes(const nlc_ctx *c, uint32_t *nclasses, uint32_t *lengths, uint32_t *counts, uint32_t cap) {
 	if (!c || !nclasses) return NLC_EARG;
@@ -1046,6 +1176,7 @@ int nlc_set_entropy_mask(nlc_ctx *c, const uint8_t *mask, int a0is0) {
 	}
 	c->nmodes = uint32_t(idx.size());
 	c->a0is0 = a0is0 ? 1 : 0;
+	for (auto &k : c->cls) k.a0is0 = c->a0is0;
 	drop_graph(c);
 	return NLC_OK;
 }
@@ -1112,7 +1243,7 @@ int nlc_batch_collect(nlc_ctx *c, double *linenergies, double *entropies) {
 int nlc_chain_energies(nlc_ctx *c, double *energies, double *norms) {
 	if (!c || !energies) return NLC_EARG;
 	for (const auto &k : c->cls)
-		NLC_HIP(c, nlc::launch_chain_energy(c->stream, c->p.model, c->state + k.site0, k.n, k.count, c->args,
+		NLC_HIP(c, nlc::launch_chain_energy(c->stream, k.model, c->state + k.site0, k.n, k.count, c->pargs.a[k.set],
 		                                    c->masses_d, c->energy_d + k.chain0, c->norm_d + k.chain0));
 	std::vector<double> e(c->copies), nr(c->copies);
 	NLC_HIP(c, hipMemcpyAsync(e.data(), c->energy_d, sizeof(double) * c->copies, hipMemcpyDeviceToHost, c->stream));
@@ -1166,16 +1297,23 @@ int nlc_autotune(nlc_ctx *c, uint32_t trial_steps, char *report, size_t len) {
 		float ms;
 		int dnls;
 		bool tiled;
+		bool half; // persistent kernel: the classes that fit at S/2 run there
 	};
 	std::vector<Cand> cands;
 	if (c->p.model == NLC_DNLS) {
-		if (nlc::dnls_fast_path(c->n))
+		if (nlc::dnls_fast_path(c->n) && c->p.precision != NLC_FP32C)
 			for (int v : {nlc::kDnlsRadix16, nlc::kDnlsRadix8}) cands.push_back({0, 0, false, false, 0, 0, v});
 		else
 			c->tune_report = "DNLS: a single kernel for this chain length, nothing to tune";
 	} else if (c->ragged) {
 		for (int S : {8, 16, 32})
-			if (ragged_fits(c->cls, S) && lookup_ragged(c->p.model, S)) cands.push_back({0, 0, false, false, S, 0, 0});
+			for (bool half : {false, true}) {
+				uint32_t kinds = 0;
+				for (int s = 0; s < 3; ++s)
+					if (c->pargs.kind[s] >= 0) kinds |= 1u << c->pargs.kind[s];
+				if (ragged_fits(c->cls, S, half) && lookup_persistent(S, kinds))
+					cands.push_back({0, 0, false, false, S, 0, 0, false, half});
+			}
 		bool per_class = true; // one uniform kernel per length class, launched in sequence
 		for (const auto &k : c->cls) {
 			int L, S;
@@ -1203,7 +1341,7 @@ int nlc_autotune(nlc_ctx *c, uint32_t trial_steps, char *report, size_t len) {
 			if (c->p.model == NLC_DNLS) {
 				c->dnls_variant = cd.dnls;
 			} else if (c->ragged) {
-				rc = select_ragged(c, cd.ragged_S);
+				rc = select_ragged(c, cd.ragged_S, cd.half);
 			} else {
 				c->p.kernel = cd.split ? NLC_KERNEL_SPLIT : cd.tiled ? NLC_KERNEL_TILED : NLC_KERNEL_WAVE;
 				c->p.lanes_per_chain = uint32_t(cd.L);
@@ -1239,7 +1377,8 @@ int nlc_autotune(nlc_ctx *c, uint32_t trial_steps, char *report, size_t len) {
 			if (c->p.model == NLC_DNLS)
 				snprintf(buf, sizeof buf, " %s %.3f ms%s", cd.dnls == nlc::kDnlsRadix8 ? "radix8x512" : "radix16x256", cd.ms,
 				         mark);
-			else if (c->ragged && cd.ragged_S) snprintf(buf, sizeof buf, " S=%d %.3f ms%s", cd.ragged_S, cd.ms, mark);
+			else if (c->ragged && cd.ragged_S)
+				snprintf(buf, sizeof buf, " S=%d%s %.3f ms%s", cd.ragged_S, cd.half ? "/half" : "", cd.ms, mark);
 			else if (c->ragged) snprintf(buf, sizeof buf, " per-class %.3f ms%s", cd.ms, mark);
 			else if (cd.split) snprintf(buf, sizeof buf, " split %.3f ms%s", cd.ms, mark);
 			else if (cd.tiled) snprintf(buf, sizeof buf, " tiled %.3f ms%s", cd.ms, mark);
@@ -1250,7 +1389,7 @@ int nlc_autotune(nlc_ctx *c, uint32_t trial_steps, char *report, size_t len) {
 		if (c->p.model == NLC_DNLS) {
 			c->dnls_variant = best->dnls;
 		} else if (c->ragged) {
-			NLC_TRY(select_ragged(c, best->ragged_S));
+			NLC_TRY(select_ragged(c, best->ragged_S, best->half));
 		} else {
 			c->p.kernel = best->split ? NLC_KERNEL_SPLIT : best->tiled ? NLC_KERNEL_TILED : NLC_KERNEL_WAVE;
 			c->p.lanes_per_chain = uint32_t(best->L);
@@ -1281,12 +1420,18 @@ int nlc_describe(const nlc_ctx *c, char *buf, size_t len) {
 		         : nlc::dnls_two_level(c->n)               ? "dnls_two_level_fft"
 		         : nlc::dnls_bluestein_size(c->n)          ? "dnls_lds_bluestein"
 		         : !nlc::dnls_fast_path(c->n)              ? "dnls_lds_stockham"
+		         : c->p.precision == NLC_FP32C        ? "dnls_radix16_registers_fp32c"
 		         : c->dnls_variant == nlc::kDnlsRadix8 ? "dnls_radix8_registers"
 		                                                : "dnls_radix16_registers");
-	else if (c->ragged)
-		snprintf(buf, len, "model=%s ragged chains=%u classes=%zu sites=%llu kernel=%s sites_per_lane=%d items=%u",
-		         names[c->p.model], c->copies, c->cls.size(), (unsigned long long) c->total_sites,
-		         c->ragged_launcher ? "ragged_persistent" : "per_class_wave", c->ragged_S, c->nitems);
+	else if (c->ragged) {
+		std::string models;
+		for (const auto &ps : c->sets) models += std::string(models.empty() ? "" : "+") + names[ps.model];
+		snprintf(buf, len,
+		         "model=%s %s chains=%u classes=%zu sites=%llu kernel=%s sites_per_lane=%d%s items=%u",
+		         models.c_str(), c->sets.size() > 1 ? "mixed" : "ragged", c->copies, c->cls.size(),
+		         (unsigned long long) c->total_sites, c->persistent ? "ragged_persistent" : "per_class_wave",
+		         c->ragged_S, c->persistent && c->ragged_half ? "/half" : "", c->nitems);
+	}
 	else if (c->use_split)
 		snprintf(buf, len, "model=%s n=%u copies=%u kernel=split", names[c->p.model], c->n, c->copies);
 	else if (c->use_tiled)
@@ -1296,12 +1441,14 @@ int nlc_describe(const nlc_ctx *c, char *buf, size_t len) {
 		         names[c->p.model], c->n, c->copies, c->L > 64 ? "group" : "wave", c->L, c->S, int(c->exact));
 	if (c->p.model != NLC_DNLS && !c->use_split) {
 		// force form of the register-resident kernels
+		std::string forms;
+		for (size_t s = 0; s < std::max<size_t>(1, c->sets.size()); ++s) {
+			const int f = c->pargs.a[s].form;
+			forms += std::string(s ? "+" : "") +
+			         (f == nlc::kFormCubic ? "cubic_unit" : f == nlc::kFormQuad ? "quadratic_unit" : "plain");
+		}
 		const size_t at = strlen(buf);
-		if (at + 1 < len)
-			snprintf(buf + at, len - at, " force=%s",
-			         c->args.form == nlc::kFormCubic  ? "cubic_unit"
-			         : c->args.form == nlc::kFormQuad ? "quadratic_unit"
-			                                          : "plain");
+		if (at + 1 < len) snprintf(buf + at, len - at, " force=%s", forms.c_str());
 	}
 	if (c->p.model == NLC_DDNKG) {
 		const size_t at = strlen(buf);

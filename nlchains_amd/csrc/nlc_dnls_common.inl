@@ -5,12 +5,33 @@
 // namespace nlc { namespace { ... } } by each DNLS translation unit.
 constexpr int kT = 256; // threads per chain
 
+// Complex values as double2 (FP64 kernels) or float2 (the FP32 transforms of
+// the compensated DNLS step); Cplx<C> gives the component type and a maker.
+template <class C>
+struct Cplx;
+template <>
+struct Cplx<double2> {
+	using R = double;
+	static __device__ __forceinline__ double2 make(double x, double y) { return make_double2(x, y); }
+};
+template <>
+struct Cplx<float2> {
+	using R = float;
+	static __device__ __forceinline__ float2 make(float x, float y) { return make_float2(x, y); }
+};
+
 __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
 	return make_double2(fma(a.x, b.x, -a.y * b.y), fma(a.x, b.y, a.y * b.x));
 }
 __device__ __forceinline__ double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
 __device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+	return make_float2(fmaf(a.x, b.x, -a.y * b.y), fmaf(a.x, b.y, a.y * b.x));
+}
+__device__ __forceinline__ float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
 
 // (cos r, sin r) for |r| <= pi/16: Taylor series about 0 in x = r^2 (degree
 // 12 and 11); the first omitted terms are below 1e-18 relative.
@@ -106,13 +127,13 @@ __device__ __forceinline__ void nonlinear_all(double2 (&v)[N], double theta) {
 }
 
 // 4-point DFT in place, SIGN = -1 forward, +1 inverse.
-template <int SIGN>
-__device__ __forceinline__ void dft4(double2 &a0, double2 &a1, double2 &a2, double2 &a3) {
-	const double2 t0 = cadd(a0, a2), t1 = csub(a0, a2), t2 = cadd(a1, a3), t3 = csub(a1, a3);
+template <int SIGN, class C>
+__device__ __forceinline__ void dft4(C &a0, C &a1, C &a2, C &a3) {
+	const C t0 = cadd(a0, a2), t1 = csub(a0, a2), t2 = cadd(a1, a3), t3 = csub(a1, a3);
 	a0 = cadd(t0, t2);
 	a2 = csub(t0, t2);
 	// (SIGN i) t3
-	const double2 it3 = SIGN < 0 ? make_double2(t3.y, -t3.x) : make_double2(-t3.y, t3.x);
+	const C it3 = SIGN < 0 ? Cplx<C>::make(t3.y, -t3.x) : Cplx<C>::make(-t3.y, t3.x);
 	a1 = cadd(t1, it3);
 	a3 = csub(t1, it3);
 }

@@ -10,6 +10,10 @@
 
 namespace nlc {
 
+// Message returned by nlc_last_error(NULL) in the calling thread (set by the
+// context-free entry points when they fail).
+void set_thread_error(const char *msg);
+
 enum Model { DNKG = 0, FPUT = 1, TODA = 2, DDNKG = 3, DNLS = 4 };
 
 // Scaled Yoshida weights and model constants, passed by value as a kernel
@@ -87,20 +91,49 @@ StepLauncher lookup_group_fput(int S, bool exact);
 StepLauncher lookup_group_toda(int S, bool exact);
 StepLauncher lookup_group_ddnkg(int S, bool exact);
 
-// Ragged ensembles: an item is one wavefront's worth (nchains <= 64/L) of
-// consecutive chains of n = L*S sites starting at site index site0.
+// Ragged and mixed ensembles: the persistent kernel (nlc_persist_s<S>.hip).
+// A kind is a model with its force form, compiled as its own code path.
+constexpr int kKindDnkgCubic = 0, kKindDnkgPlain = 1, kKindFputCubic = 2, kKindFputQuad = 3, kKindFputPlain = 4,
+              kKindToda = 5;
+constexpr uint32_t kAllKinds = 0x3F;
+__host__ __device__ constexpr int kind_model(int kind) { return kind <= 1 ? DNKG : kind <= 4 ? FPUT : TODA; }
+__host__ __device__ constexpr int kind_form(int kind) {
+	return (kind == kKindDnkgCubic || kind == kKindFputCubic) ? kFormCubic : kind == kKindFputQuad ? kFormQuad
+	                                                                                                : kFormPlain;
+}
+// The kind of a model with parameters giving force form `form`; -1 if none.
+inline int kind_of(int model, int form) {
+	if (model == DNKG) return form == kFormCubic ? kKindDnkgCubic : form == kFormPlain ? kKindDnkgPlain : -1;
+	if (model == FPUT)
+		return form == kFormCubic ? kKindFputCubic : form == kFormQuad ? kKindFputQuad : kKindFputPlain;
+	if (model == TODA) return form == kFormPlain ? kKindToda : -1;
+	return -1;
+}
+// Parameter sets of a persistent launch (at most 3: one per model of a mixed
+// ensemble), each with its kind.
+struct PersistentArgs {
+	StepArgs a[3];
+	int kind[3];
+};
+// An item: one wavefront's worth (nchains <= 64/L) of consecutive chains of n
+// sites of one class, L = 2^log2_lanes lanes per chain; flags bits 0-1 the
+// parameter set, bit 2: S/2 sites per lane instead of the kernel's S.
+constexpr uint8_t kItemSetMask = 3, kItemHalfS = 4;
 struct RaggedItem {
 	uint64_t site0;
 	uint32_t n;
 	uint16_t nchains;
-	uint16_t log2_lanes;
+	uint8_t log2_lanes;
+	uint8_t flags;
 };
-using RaggedLauncher = hipError_t (*)(hipStream_t, double2 *state, const RaggedItem *items, uint32_t nitems,
-                                      uint32_t steps, const StepArgs &args, uint32_t max_blocks);
-// persistent ragged kernels, S in {8, 16, 32}
-RaggedLauncher lookup_ragged_dnkg(int S);
-RaggedLauncher lookup_ragged_fput(int S);
-RaggedLauncher lookup_ragged_toda(int S);
+// queue: two zeroed counters owned by the caller (the kernel leaves them zero).
+using PersistentLauncher = hipError_t (*)(hipStream_t, double2 *state, const RaggedItem *items, uint32_t nitems,
+                                          uint32_t steps, const PersistentArgs &pa, uint32_t *queue,
+                                          uint32_t max_blocks);
+// S in {8, 16, 32}; kind_mask: one kind, or (S >= 16) any set of kinds
+PersistentLauncher lookup_persistent_s8(uint32_t kind_mask);
+PersistentLauncher lookup_persistent_s16(uint32_t kind_mask);
+PersistentLauncher lookup_persistent_s32(uint32_t kind_mask);
 
 // HBM-streaming step over the split layout q[n][copies], p[n][copies]: one
 // launch per substep from buffer pair cur to pair cur ^ 1 (cur is updated; the
@@ -221,6 +254,13 @@ hipError_t launch_dnls_rows4096(hipStream_t, double2 *rows, uint32_t nrows, uint
                                 const double2 *twn, const double2 *tk);
 bool dnls_fast_path(uint32_t n);
 void dnls_permute_tables(uint32_t n, int variant, const double2 *table, double2 *out);
+// FP32-compensated DNLS step of 4096-site chains (nlc_dnls_fp32c.hip): the
+// FP32 twiddles tw[n] and the double-single increment multipliers
+// mult[4][n] = {hi.x, hi.y, lo.x, lo.y} of (e^{-i dt d_k omega_j} - 1)/n in
+// the radix-16 kernel's order; theta9 as for launch_dnls.
+void dnls_fp32c_tables(uint32_t n, double dt, float2 *tw, float4 *mult);
+hipError_t launch_dnls_fp32c(hipStream_t, double2 *state, uint32_t copies, uint32_t steps, const double *theta9,
+                             const float2 *tw, const float4 *mult);
 hipError_t launch_dnls(hipStream_t, double2 *state, uint32_t n, uint32_t copies, uint32_t steps,
                        const double *theta9, const double2 *tw, const double2 *table, int variant,
                        double2 *scratch, const double2 *blue);

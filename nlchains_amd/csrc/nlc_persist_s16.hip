@@ -1,0 +1,5 @@
+// Persistent step kernels of ragged and mixed ensembles with 16 (and 8)
+// sites per lane (see nlc_step_wave.inl).
+#define NLC_PERSIST_S 16
+#define NLC_PERSIST_LOOKUP lookup_persistent_s16
+#include "nlc_step_wave.inl"

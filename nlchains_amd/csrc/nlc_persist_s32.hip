@@ -1,0 +1,5 @@
+// Persistent step kernels of ragged and mixed ensembles with 32 (and 16)
+// sites per lane (see nlc_step_wave.inl).
+#define NLC_PERSIST_S 32
+#define NLC_PERSIST_LOOKUP lookup_persistent_s32
+#include "nlc_step_wave.inl"

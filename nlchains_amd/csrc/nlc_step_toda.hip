@@ -1,5 +1,4 @@
 // Wave-resident step kernels, model TODA (see nlc_step_wave.inl).
 #define NLC_STEP_MODEL ::nlc::TODA
 #define NLC_STEP_LOOKUP lookup_step_toda
-#define NLC_STEP_RAGGED lookup_ragged_toda
 #include "nlc_step_wave.inl"

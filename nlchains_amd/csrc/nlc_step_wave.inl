@@ -278,61 +278,99 @@ hipError_t launch_tiled(hipStream_t st, double2 *state, double2 *scratch, uint32
 }
 #endif
 
-#ifdef NLC_STEP_RAGGED
-// Persistent step over a ragged ensemble: every item is one wavefront's worth
-// of chains of one length class (64 S / n chains of n = L S sites), so all
-// items cost the same; waves take items round-robin until none is left (no
-// atomics, deterministic, every wave reaches the end of the list).
-template <int MODEL, int S, int FORM>
+#ifdef NLC_PERSIST_S
+// Persistent step over a ragged (several chain lengths) or mixed (several
+// models) ensemble.  An item is one wavefront's worth of chains of one class:
+// 64/L chains of n = L S' sites, S' = S or S/2 sites per lane (flag bit 2), of
+// the model and parameters of its parameter set (flag bits 0-1).  Waves take
+// items from a queue (an atomic counter) in the order the host sorted them,
+// most expensive first, so the cheap items fill the end of the launch (the
+// longest-processing-time rule); which wave runs an item does not change its
+// result.  The queue counters are zero at every launch: the last wave to finish
+// resets them.
+//
+// KMASK: the kinds compiled in (bit k = kind k of PersistentArgs): one kind
+// for a single-model ensemble, so the kernel has that model's register
+// footprint; all six for a mixed one.
+template <int KIND, int S>
+__device__ __forceinline__ void run_item(double2 *__restrict__ state, const RaggedItem &it, uint32_t steps,
+                                         const StepArgs &a, const double *etab) {
+	constexpr int MODEL = kind_model(KIND), FORM = kind_form(KIND);
+	double2 *c0 = state + it.site0;
+	switch (it.log2_lanes) {
+	case 0: run_wave<MODEL, 1, S, true, FORM>(c0, it.n, it.nchains, steps, a, nullptr, etab); break;
+	case 1: run_wave<MODEL, 2, S, true, FORM>(c0, it.n, it.nchains, steps, a, nullptr, etab); break;
+	case 2: run_wave<MODEL, 4, S, true, FORM>(c0, it.n, it.nchains, steps, a, nullptr, etab); break;
+	case 3: run_wave<MODEL, 8, S, true, FORM>(c0, it.n, it.nchains, steps, a, nullptr, etab); break;
+	case 4: run_wave<MODEL, 16, S, true, FORM>(c0, it.n, it.nchains, steps, a, nullptr, etab); break;
+	case 5: run_wave<MODEL, 32, S, true, FORM>(c0, it.n, it.nchains, steps, a, nullptr, etab); break;
+	default: run_wave<MODEL, 64, S, true, FORM>(c0, it.n, it.nchains, steps, a, nullptr, etab); break;
+	}
+}
+
+template <int KIND, int S, uint32_t KMASK>
+__device__ __forceinline__ void run_kind(double2 *__restrict__ state, const RaggedItem &it, uint32_t steps,
+                                         const StepArgs &a, const double *etab) {
+	if constexpr ((KMASK >> KIND) & 1u) {
+		if constexpr (S >= 2) {
+			if (it.flags & kItemHalfS) {
+				run_item<KIND, S / 2>(state, it, steps, a, etab);
+				return;
+			}
+		}
+		run_item<KIND, S>(state, it, steps, a, etab);
+	}
+}
+
+template <int S, uint32_t KMASK>
 __global__ void __launch_bounds__(256)
-k_step_ragged(double2 *__restrict__ state, const RaggedItem *__restrict__ items, uint32_t nitems, uint32_t steps,
-              StepArgs a) {
-	__shared__ double etab[MODEL == TODA ? kExpTabSize : 1];
-	load_exp_table<MODEL>(etab);
-	const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-	const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
-	for (uint32_t i = wave; i < nitems; i += nwaves) {
+k_step_persistent(double2 *__restrict__ state, const RaggedItem *__restrict__ items, uint32_t nitems, uint32_t steps,
+                  PersistentArgs pa, uint32_t *__restrict__ queue) {
+	constexpr bool kHasToda = (KMASK >> kKindToda) & 1u;
+	__shared__ double etab[kHasToda ? kExpTabSize : 1];
+	if constexpr (kHasToda) load_exp_table<TODA>(etab);
+	const int lane = threadIdx.x & 63;
+	for (;;) {
+		uint32_t i = 0;
+		if (lane == 0) i = atomicAdd(&queue[0], 1u);
+		i = __builtin_amdgcn_readfirstlane(i);
+		if (i >= nitems) break; // wave-uniform
 		const RaggedItem it = items[i];
-		double2 *c0 = state + it.site0;
-		switch (it.log2_lanes) {
-		case 0: run_wave<MODEL, 1, S, true, FORM>(c0, it.n, it.nchains, steps, a, nullptr, etab); break;
-		case 1: run_wave<MODEL, 2, S, true, FORM>(c0, it.n, it.nchains, steps, a, nullptr, etab); break;
-		case 2: run_wave<MODEL, 4, S, true, FORM>(c0, it.n, it.nchains, steps, a, nullptr, etab); break;
-		case 3: run_wave<MODEL, 8, S, true, FORM>(c0, it.n, it.nchains, steps, a, nullptr, etab); break;
-		case 4: run_wave<MODEL, 16, S, true, FORM>(c0, it.n, it.nchains, steps, a, nullptr, etab); break;
-		case 5: run_wave<MODEL, 32, S, true, FORM>(c0, it.n, it.nchains, steps, a, nullptr, etab); break;
-		default: run_wave<MODEL, 64, S, true, FORM>(c0, it.n, it.nchains, steps, a, nullptr, etab); break;
+		const int set = it.flags & kItemSetMask;
+		const StepArgs &a = pa.a[set];
+		switch (pa.kind[set]) {
+		case 0: run_kind<0, S, KMASK>(state, it, steps, a, etab); break;
+		case 1: run_kind<1, S, KMASK>(state, it, steps, a, etab); break;
+		case 2: run_kind<2, S, KMASK>(state, it, steps, a, etab); break;
+		case 3: run_kind<3, S, KMASK>(state, it, steps, a, etab); break;
+		case 4: run_kind<4, S, KMASK>(state, it, steps, a, etab); break;
+		default: run_kind<5, S, KMASK>(state, it, steps, a, etab); break;
+		}
+	}
+	// the last wave to get here resets the queue for the next launch
+	if (lane == 0) {
+		const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
+		if (atomicAdd(&queue[1], 1u) == nwaves - 1) {
+			atomicExch(&queue[0], 0u);
+			atomicExch(&queue[1], 0u);
 		}
 	}
 }
 
-template <int MODEL, int S, int FORM>
-hipError_t launch_ragged_form(hipStream_t st, double2 *state, const RaggedItem *items, uint32_t nitems, uint32_t steps,
-                              const StepArgs &a, uint32_t max_blocks) {
+template <int S, uint32_t KMASK>
+hipError_t launch_persistent(hipStream_t st, double2 *state, const RaggedItem *items, uint32_t nitems, uint32_t steps,
+                             const PersistentArgs &pa, uint32_t *queue, uint32_t max_blocks) {
 	int per_cu = 0, cus = 0, dev = 0;
 	hipError_t e = hipGetDevice(&dev);
 	if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-	if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_step_ragged<MODEL, S, FORM>, 256, 0);
+	if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_step_persistent<S, KMASK>, 256, 0);
 	if (e != hipSuccess) return e;
-	// as many waves as fit at once, but no more rounds than the item count needs
-	const uint64_t slots = uint64_t(std::max(per_cu, 1)) * std::max(cus, 1) * 4;
-	const uint64_t rounds = (nitems + slots - 1) / slots;
-	uint64_t waves = rounds ? (nitems + rounds - 1) / rounds : 1;
-	uint64_t blocks = (waves + 3) / 4;
+	// as many waves as are resident at once, no more than there are items
+	uint64_t blocks = std::min<uint64_t>(uint64_t(std::max(per_cu, 1)) * std::max(cus, 1), (nitems + 3) / 4);
 	if (max_blocks) blocks = std::min<uint64_t>(blocks, max_blocks);
-	hipLaunchKernelGGL((k_step_ragged<MODEL, S, FORM>), dim3(uint32_t(std::max<uint64_t>(blocks, 1))), dim3(256), 0, st,
-	                   state, items, nitems, steps, a);
+	hipLaunchKernelGGL((k_step_persistent<S, KMASK>), dim3(uint32_t(std::max<uint64_t>(blocks, 1))), dim3(256), 0, st,
+	                   state, items, nitems, steps, pa, queue);
 	return hipGetLastError();
-}
-
-template <int MODEL, int S>
-hipError_t launch_ragged(hipStream_t st, double2 *state, const RaggedItem *items, uint32_t nitems, uint32_t steps,
-                         const StepArgs &a, uint32_t max_blocks) {
-	if constexpr (MODEL != TODA)
-		if (a.form == kFormCubic) return launch_ragged_form<MODEL, S, kFormCubic>(st, state, items, nitems, steps, a, max_blocks);
-	if constexpr (MODEL == FPUT)
-		if (a.form == kFormQuad) return launch_ragged_form<MODEL, S, kFormQuad>(st, state, items, nitems, steps, a, max_blocks);
-	return launch_ragged_form<MODEL, S, kFormPlain>(st, state, items, nitems, steps, a, max_blocks);
 }
 #endif
 
@@ -349,13 +387,21 @@ TiledLauncher NLC_STEP_TILED() {
 }
 #endif
 
-#ifdef NLC_STEP_RAGGED
-RaggedLauncher NLC_STEP_RAGGED(int S) {
-	switch (S) {
-	case 8: return &launch_ragged<NLC_STEP_MODEL, 8>;
-	case 16: return &launch_ragged<NLC_STEP_MODEL, 16>;
-	case 32: return &launch_ragged<NLC_STEP_MODEL, 32>;
+#ifdef NLC_PERSIST_S
+// The persistent kernels with S (and S/2) sites per lane: one per kind, and
+// for S >= 16 one with every kind (mixed ensembles).
+PersistentLauncher NLC_PERSIST_LOOKUP(uint32_t kind_mask) {
+	switch (kind_mask) {
+	case 1u << 0: return &launch_persistent<NLC_PERSIST_S, 1u << 0>;
+	case 1u << 1: return &launch_persistent<NLC_PERSIST_S, 1u << 1>;
+	case 1u << 2: return &launch_persistent<NLC_PERSIST_S, 1u << 2>;
+	case 1u << 3: return &launch_persistent<NLC_PERSIST_S, 1u << 3>;
+	case 1u << 4: return &launch_persistent<NLC_PERSIST_S, 1u << 4>;
+	case 1u << 5: return &launch_persistent<NLC_PERSIST_S, 1u << 5>;
 	}
+#if NLC_PERSIST_S >= 16
+	if (kind_mask && !(kind_mask & ~kAllKinds)) return &launch_persistent<NLC_PERSIST_S, kAllKinds>;
+#endif
 	return nullptr;
 }
 #endif

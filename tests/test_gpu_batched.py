@@ -190,7 +190,7 @@ def test_autotune_ragged(gpu):
         ens.upload(s0)
         report = ens.autotune(5)
         banner(f"autotune ragged: {report}")
-        assert all(f"S={s}" in report for s in (8, 16, 32))  # S <= n_min = 64 and 64 S >= n_max = 256
+        assert all(f"S={s} " in report for s in (8, 16, 32))  # S <= n_min = 64 and 64 S >= n_max = 256
         assert np.array_equal(ens.download(), s0)
         ens.move(20)
         got = ens.download()
@@ -223,3 +223,99 @@ def test_ragged_toda_full_size_sampled(gpu):
         err = chain_errors(g, want)
         banner(f"  n={n}: largest per-chain rel err {err.max():.3e}")
         assert err.max() <= 1e-10, n
+
+
+MIXED_SETS = [dict(model=0, m=1.0, beta=1.0), dict(model=1, alpha=0.0, beta=1.0), dict(model=2, alpha=1.0)]
+
+
+def oracle_mixed(sets, lengths, chain_set, s0, steps, dt):
+    """Every chain advanced with the oracle of its own model and parameters."""
+    chains = split_chains(s0, lengths)
+    res = [None] * len(lengths)
+    for si, kw in enumerate(sets):
+        kw = {k: v for k, v in kw.items() if k != "model"}
+        for n in sorted(set(lengths)):
+            idx = [i for i in range(len(lengths)) if lengths[i] == n and chain_set[i] == si]
+            if not idx:
+                continue
+            out = O.move_real(ORACLE_MODEL[sets[si]["model"]], np.stack([chains[i] for i in idx]), steps, dt, **kw)
+            for j, i in enumerate(idx):
+                res[i] = out[j]
+    return np.concatenate(res)
+
+
+@pytest.mark.parametrize("choices", [[32, 64, 128, 256, 512, 1024, 2048], [8, 16, 64, 100]])
+def test_mixed_models_parity(gpu, choices):
+    """DNKG, FPUT and Toda chains of several lengths in one context: one
+    persistent launch per move for powers of two (per-class launches for
+    lengths without a persistent shape), every chain against the oracle of its
+    own model after 2 x 50 steps (the second launch only advances the state if
+    the first reset the work queue), chain energies, and energies and
+    entropies per (model, length) class."""
+    N = gpu
+    nchains = 420
+    lengths = N.ic_ragged_lengths(nchains, choices, 11)
+    chain_set = np.random.default_rng(12).integers(0, 3, nchains).astype(np.uint32)
+    # DNKG chains need m > 0 for mode 0 (the IC helper takes the dispersion's m)
+    s0 = np.concatenate([N.ic_ragged_plane_waves([n], 1.0 if cs == 0 else 0.0, 0.05, 0.05, 0.3, 100 + i)
+                         for i, (n, cs) in enumerate(zip(lengths, chain_set))])
+    with N.Ensemble.mixed(MIXED_SETS, lengths, chain_set, 0.05) as ens:
+        desc = ens.describe()
+        banner(desc)
+        assert "mixed" in desc and "DNKG+FPUT+Toda" in desc
+        assert ("ragged_persistent" in desc) == (100 not in choices)
+        ens.upload(s0)
+        assert np.array_equal(ens.download(), s0)
+        ens.move(50)
+        ens.move(50)
+        got = ens.download()
+        E, _ = ens.chain_energies()
+        e_gpu, s_gpu = ens.linenergies()
+        cl, cset = list(ens.class_lengths), list(ens.class_sets)
+    want = oracle_mixed(MIXED_SETS, list(lengths), chain_set, s0, 100, 0.05)
+    err = chain_errors(got, want, lengths)
+    banner(f"  largest per-chain rel err {err.max():.3e}")
+    assert err.max() <= 1e-10
+    wchains = split_chains(want, lengths)
+    for i in range(nchains):
+        kw = {k: v for k, v in MIXED_SETS[chain_set[i]].items() if k != "model"}
+        Ew = O.chain_energy(ORACLE_MODEL[MIXED_SETS[chain_set[i]]["model"]], wchains[i][None], **kw)[0]
+        assert abs(E[i] - Ew) <= 1e-11 * abs(Ew), i
+    # classes in ascending (set, length); each with its model's dispersion and a0is0
+    assert cset == sorted(cset) and len(cl) == len(set(zip(cset, cl)))
+    at = 0
+    for ci, (n, si) in enumerate(zip(cl, cset)):
+        group = np.stack([wchains[i] for i in range(nchains) if lengths[i] == n and chain_set[i] == si])
+        m = MIXED_SETS[si].get("m", 0.0)
+        e = O.calc_linenergies(O.linenergies_real(group, O.dispersion_real(n, m)), 0.5 / len(group) / n, m == 0.0)
+        np.testing.assert_allclose(e_gpu[at:at + n], e, rtol=0, atol=1e-9 * e.max())
+        at += n
+
+
+def test_persistent_variants_and_queue_reset(gpu):
+    """Every persistent variant the autotuner offers (S sites per lane, with and
+    without the S/2 items) gives the oracle's trajectory, also over several
+    launches and graph replays (the work queue is reset by each launch)."""
+    N = gpu
+    lengths = N.ic_ragged_lengths(600, [32, 64, 128, 256, 512, 1024, 2048], 21)
+    s0 = N.ic_ragged_plane_waves(lengths, 0.0, 0.05, 0.05, 0.25, 21)
+    want = oracle_ragged(2, list(lengths), s0, 60, 0.05, dict(alpha=1.0))
+    with N.Ensemble.ragged(N.TODA, lengths, 0.05, alpha=1.0) as ens:
+        ens.upload(s0)
+        report = ens.autotune(10)
+        banner(f"autotune: {report} -> {ens.describe()}")
+        assert "S=32 " in report and "S=32/half" in report and "per-class" in report
+        ens.move(20)
+        ens.batch(20)
+        ens.batch(20)
+        got = ens.download()
+    assert chain_rel_err(got, want, lengths) <= 1e-10
+
+
+def test_mixed_rejects_bad_sets(gpu):
+    N = gpu
+    lengths = np.array([64, 128], np.uint32)
+    for sets, cs in (([dict(model=3, beta=1.0)], [0, 0]), ([dict(model=4, beta=1.0)], [0, 0]),
+                     (MIXED_SETS, [0, 3]), (MIXED_SETS + MIXED_SETS[:1], [0, 1])):
+        with pytest.raises(N.NlcError):
+            N.Ensemble.mixed(sets, lengths, cs, 0.05)
