@@ -117,7 +117,8 @@ typedef struct nlc_ctx nlc_ctx;
 int nlc_abi_version(void);
 const char *nlc_strerror(int code);
 /* Message of the last failure on ctx; with ctx == NULL, of the last failed
- * nlc_create in the calling thread. */
+ * nlc_create (or nlc_create_ragged, nlc_create_mixed, nlc_linenergies_noalloc)
+ * in the calling thread. */
 const char *nlc_last_error(const nlc_ctx *ctx);
 
 /* Defaults: FP64, AUTO kernel, everything else zero. */
@@ -241,6 +242,26 @@ int nlc_batch_collect(nlc_ctx *ctx, double *linenergies, double *entropies);
  * reference's compile-time optimized_chain_length choice (CMakeLists.txt:74-78)
  * and its launch heuristic (common/utilities_cuda.cpp:46-72). */
 int nlc_autotune(nlc_ctx *ctx, uint32_t trial_steps, char *report, size_t len);
+
+/* ---- allocation-free form ------------------------------------------------ */
+
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc).  The energy pass, normalisation and entropies of nlc_linenergies
+ * (all modes, minus mode 0 when a0is0) for a planar state already on the
+ * current device: d_state = double[copies][chain_length][2] (DNKG, FPUT, Toda:
+ * {q, p}; DNLS: psi), d_out = chain_length + 2 doubles of device memory
+ * (linear energies, S_WT, S_INF), m the DNKG linear parameter, norm <= 0 the
+ * model's default normalisation.  Asynchronous on `stream` (a hipStream_t, NULL
+ * for the null stream).  It allocates nothing: its work buffers live in one
+ * static device array of NLC_NOALLOC_SCRATCH_BYTES per GPU, held until the
+ * process ends; calls that overlap on one GPU race.  Chain lengths: powers of
+ * two from 8 to NLC_NOALLOC_MAX_CHAIN (NLC_EARG beyond, with the reason in
+ * nlc_last_error(NULL)). */
+#define NLC_NOALLOC_SCRATCH_BYTES (8u << 20)
+#define NLC_NOALLOC_MAX_CHAIN 4096
+int nlc_linenergies_noalloc(int model, const void *d_state, uint32_t chain_length, uint32_t copies, double m,
+                            double norm, int a0is0, double *d_out, void *stream);
 
 /* Waits for all work enqueued on the context. */
 int nlc_sync(nlc_ctx *ctx);
