@@ -347,13 +347,16 @@ int main_dnls(int argc, char **argv) {
 	add_common(o);
 	o.add("no_linear_callback", 0, false, false, "do not use cuFFT callback for linear evolution (no effect)")
 	    .add("no_nonlinear_callback", 0, false, false, "do not use cuFFT callback for nonlinear evolution (no effect)")
-	    .add("beta", 0, true, true, "fourth order nonlinearity");
+	    .add("beta", 0, true, true, "fourth order nonlinearity")
+	    .add("fp32c", 0, false, false,
+	         "FP32 fields with compensated accumulation, FP64 reductions (chain_length 4096)");
 	Config c;
 	run_common(o, argc, argv, c);
 	nlc_params p;
 	nlc_params_init(&p);
 	p.model = NLC_DNLS;
 	p.beta = o.get<double>("beta");
+	if (o.count("fp32c")) p.precision = NLC_FP32C;
 	return run_loop(c, p, false, "");
 }
 

@@ -12,6 +12,7 @@ import numpy as np
 import pytest
 
 from oracle import oracle as O
+from parity import chain_rel_err
 
 ROOT = Path(__file__).resolve().parent.parent
 CLI = ROOT / "nlchains_amd" / "nlchains"
@@ -38,7 +39,7 @@ def test_cli_lists_programs():
 
 @pytest.mark.parametrize("prog,opts", [("DNKG", ["-m", "--beta"]), ("FPUT", ["--alpha", "--beta"]),
                                        ("Toda", ["--alpha"]), ("dDNKG", ["-m", "--beta"]),
-                                       ("DNLS", ["--beta", "--no_linear_callback"])])
+                                       ("DNLS", ["--beta", "--no_linear_callback", "--fp32c"])])
 def test_cli_help_per_program(prog, opts):
     p = run(prog, check=0)
     assert f"Options for {prog}" in p.stdout
@@ -235,6 +236,24 @@ def test_cli_dnls_run(gpu, tmp_path):
     assert np.abs(got - want).max() <= 1e-10 * np.abs(want).max()
     e = O.calc_linenergies(O.linenergies_dnls(want), 1.0 / (n * c), True)
     np.testing.assert_allclose(np.fromfile(f"{pre}-linenergies-100"), e, rtol=0, atol=1e-9 * e.max())
+
+
+@pytest.mark.gpu
+def test_cli_dnls_fp32c(gpu, tmp_path):
+    """--fp32c: the FP32-compensated step (4096-site chains) through the CLI,
+    against the FP64 oracle at its stated precision; other lengths refused."""
+    n, c = 4096, 3
+    s0 = gpu.ic_dnls(n, c, 256, 1.0, 3)
+    s0.tofile(tmp_path / "init")
+    pre = tmp_path / "f"
+    run("DNLS", "-n", n, "-c", c, "--dt", 0.01, "-b", 50, "-s", 100, "-p", pre, "-i", tmp_path / "init", "--beta", 1,
+        "--fp32c", check=0)
+    want = O.move_dnls(s0, 100, 0.01, 1.0)
+    got = np.fromfile(f"{pre}-100").reshape(c, n, 2)
+    assert chain_rel_err(got, want) <= 3e-6
+    p = run("DNLS", "-n", 1024, "-c", c, "--dt", 0.01, "-b", 50, "-s", 100, "-p", pre, "-i", tmp_path / "init",
+            "--beta", 1, "--fp32c")
+    assert p.returncode != 0 and "FP32C" in (p.stdout + p.stderr)
 
 
 @pytest.mark.gpu
