@@ -206,14 +206,22 @@ def copy_calibration(mib: int) -> None:
     print(json.dumps({"copy_calibration_bytes": nbytes, "copies": 3}), flush=True)
 
 
+def traffic_key(desc: str) -> str:
+    """The kernel description without the fields that do not change the step
+    kernel's HBM traffic (the state is read and written once per launch whatever
+    the wave shape; the projection form belongs to the energy pass)."""
+    drop = ("lanes_per_chain=", "sites_per_lane=", "items=", "projection=")
+    return " ".join(t for t in desc.split() if not t.startswith(drop))
+
+
 def traffic_bytes(desc: str):
     """HBM bytes per launch of the step kernel(s) for this kernel configuration:
     rocprofv3 FETCH_SIZE and WRITE_SIZE counts scaled by a copy kernel of known
-    size, as recorded in profiles/traffic.json (None for an unprofiled shape)."""
+    size, as recorded in profiles/traffic.json (None for an unprofiled one)."""
     f = ROOT / "profiles" / "traffic.json"
     if not f.exists():
         return None
-    t = json.loads(f.read_text())["kernels"].get(desc)
+    t = json.loads(f.read_text())["kernels"].get(traffic_key(desc))
     return t["hbm_bytes"] if t else None
 
 
@@ -264,8 +272,9 @@ def main():
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32c"],
                     help="fp32c: DNLS with FP32 fields and compensated accumulation (reductions in FP64); the "
                          "line then also reports its norm and energy error against an FP64 run of the same steps")
-    ap.add_argument("--projection", default="auto", choices=["auto", "mfma", "valu", "bf16"],
-                    help="dDNKG energy projection: FP64 MFMA, FP64 VALU or split BF16 MFMA")
+    ap.add_argument("--projection", default="auto", choices=["auto", "mfma", "mfma_wide", "valu", "bf16"],
+                    help="dDNKG energy projection: FP64 MFMA (64 x 128 or 64 x 256 tiles), FP64 VALU or split BF16 "
+                         "MFMA; auto = the autotuned FP64 form in --mode batched, else FP64 MFMA")
     ap.add_argument("--lanes", type=int, default=0)
     ap.add_argument("--sites", type=int, default=0)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MiB",
@@ -314,8 +323,8 @@ def main():
         else:
             s0 = N.ic_plane_waves(n, copies, w["m"], w["k_lo"], w["k_hi"], w["dens"], w["seed"])
         sites = float(n) * copies
-        proj = {"auto": N.PROJ_AUTO, "mfma": N.PROJ_MFMA_F64, "valu": N.PROJ_VALU_F64,
-                "bf16": N.PROJ_SPLIT_BF16}[args.projection]
+        proj = {"auto": N.PROJ_AUTO, "mfma": N.PROJ_MFMA_F64, "mfma_wide": N.PROJ_MFMA_F64_WIDE,
+                "valu": N.PROJ_VALU_F64, "bf16": N.PROJ_SPLIT_BF16}[args.projection]
         ens = N.Ensemble(model, n, copies, w["dt"], m=w["m"], alpha=w["alpha"], beta=w["beta"], masses=masses,
                          lanes_per_chain=args.lanes, sites_per_lane=args.sites, projection=proj, precision=prec)
         shape = f"N={n} copies={copies}"
@@ -401,8 +410,9 @@ def main():
     k_ms = float(np.mean(kernel_ms))
     flop = flop_per_site_step(w) * sites * b
     achieved = flop / (k_ms * 1e-3) / 1e12
-    kname = ("k_dnls_4096_r8" if "radix8" in desc else "k_dnls_4096" if w["model"] == "DNLS"
-             else "k_step_ragged" if "ragged_persistent" in desc
+    kname = ("k_dnls_4096_r8" if "radix8" in desc else "k_dnls_4096_c" if fp32c
+             else "k_dnls_4096" if w["model"] == "DNLS"
+             else "k_step_persistent" if "ragged_persistent" in desc
              else "k_step_split" if "kernel=split" in desc else "k_step_wave")
     out = {
         "metric": METRIC,

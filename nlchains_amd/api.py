@@ -23,7 +23,7 @@ DNKG, FPUT, TODA, DDNKG, DNLS = range(5)
 MODELS = {"DNKG": DNKG, "FPUT": FPUT, "Toda": TODA, "dDNKG": DDNKG, "DNLS": DNLS}
 FP64, FP32C = 0, 1
 KERNEL_AUTO, KERNEL_WAVE, KERNEL_SPLIT, KERNEL_TILED = 0, 1, 2, 3
-PROJ_AUTO, PROJ_MFMA_F64, PROJ_VALU_F64, PROJ_SPLIT_BF16 = 0, 1, 2, 3
+PROJ_AUTO, PROJ_MFMA_F64, PROJ_VALU_F64, PROJ_SPLIT_BF16, PROJ_MFMA_F64_WIDE = 0, 1, 2, 3, 4
 
 NLC_OK, NLC_EARG, NLC_EHIP, NLC_ENOMEM, NLC_EUNSUPPORTED, NLC_ESTATE = 0, -1, -2, -3, -4, -5
 

@@ -795,7 +795,7 @@ int validate_params(const nlc_params *p, bool ragged) {
 	if (p->precision == NLC_FP32C && p->model != NLC_DNLS)
 		return reject(NLC_EUNSUPPORTED, "FP32C precision is implemented for DNLS (4096-site chains)");
 	if (p->kernel < NLC_KERNEL_AUTO || p->kernel > NLC_KERNEL_TILED) return reject(NLC_EARG, "unknown kernel family");
-	if (p->projection < NLC_PROJ_AUTO || p->projection > NLC_PROJ_SPLIT_BF16)
+	if (p->projection < NLC_PROJ_AUTO || p->projection > NLC_PROJ_MFMA_F64_WIDE)
 		return reject(NLC_EARG, "unknown projection form");
 	int ndev = 0;
 	if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
@@ -882,9 +882,10 @@ int nlc_create(nlc_ctx **out, const nlc_params *p) {
 	nlc_ctx *c = new (std::nothrow) nlc_ctx;
 	if (!c) return reject(NLC_ENOMEM, "context allocation failed");
 	c->p = *p;
-	c->proj_variant = p->projection == NLC_PROJ_VALU_F64     ? nlc::kProjValu
-	                  : p->projection == NLC_PROJ_SPLIT_BF16 ? nlc::kProjBf16
-	                                                         : nlc::kProjMfma;
+	c->proj_variant = p->projection == NLC_PROJ_VALU_F64       ? nlc::kProjValu
+	                  : p->projection == NLC_PROJ_SPLIT_BF16   ? nlc::kProjBf16
+	                  : p->projection == NLC_PROJ_MFMA_F64_WIDE ? nlc::kProjMfmaWide
+	                                                            : nlc::kProjMfma;
 	c->n = p->chain_length;
 	c->copies = p->copies;
 	c->total_sites = uint64_t(c->n) * c->copies;
@@ -1262,29 +1263,36 @@ int nlc_chain_energies(nlc_ctx *c, double *energies, double *norms) {
 static int tune_projection(nlc_ctx *c) {
 	hipEvent_t e0 = nullptr, e1 = nullptr;
 	int rc = NLC_OK;
-	float ms[2] = {0, 0};
+	const int forms[3] = {nlc::kProjMfma, nlc::kProjMfmaWide, nlc::kProjValu};
+	const char *names[3] = {"mfma", "mfma_wide", "valu"};
+	float ms[3] = {0, 0, 0};
 	if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
 		rc = c->fail(NLC_EHIP, "autotune: event creation failed");
-	for (int v : {nlc::kProjMfma, nlc::kProjValu}) {
-		if (rc) break;
-		c->proj_variant = v;
+	for (int f = 0; f < 3 && !rc; ++f) {
+		c->proj_variant = forms[f];
 		for (int rep = 0; rep < 2 && !rc; ++rep) { // the first run warms up
 			rc = enqueue_energy(c);
 			if (!rc && hipEventRecord(e0, c->stream) != hipSuccess) rc = c->fail(NLC_EHIP, "event record");
 			for (int i = 0; i < 3 && !rc; ++i) rc = enqueue_energy(c);
 			if (!rc && (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
-			            hipEventElapsedTime(&ms[v], e0, e1) != hipSuccess))
+			            hipEventElapsedTime(&ms[f], e0, e1) != hipSuccess))
 				rc = c->fail(NLC_EHIP, "autotune timing failed");
 		}
 	}
 	if (e0) (void) hipEventDestroy(e0);
 	if (e1) (void) hipEventDestroy(e1);
 	if (rc) return rc;
-	c->proj_variant = ms[nlc::kProjValu] < ms[nlc::kProjMfma] ? nlc::kProjValu : nlc::kProjMfma;
-	char buf[128];
-	snprintf(buf, sizeof buf, "; projection: mfma %.3f ms%s valu %.3f ms%s", ms[0] / 3,
-	         c->proj_variant == nlc::kProjMfma ? "*" : "", ms[1] / 3, c->proj_variant == nlc::kProjValu ? "*" : "");
-	c->tune_report += buf;
+	int best = 0;
+	for (int f = 1; f < 3; ++f)
+		if (ms[f] < ms[best]) best = f;
+	c->proj_variant = forms[best];
+	std::string rep = "; projection:";
+	char buf[64];
+	for (int f = 0; f < 3; ++f) {
+		snprintf(buf, sizeof buf, " %s %.3f ms%s", names[f], ms[f] / 3, f == best ? "*" : "");
+		rep += buf;
+	}
+	c->tune_report += rep;
 	return NLC_OK;
 }
 
@@ -1454,9 +1462,10 @@ int nlc_describe(const nlc_ctx *c, char *buf, size_t len) {
 		const size_t at = strlen(buf);
 		if (at + 1 < len)
 			snprintf(buf + at, len - at, " projection=%s",
-			         c->proj_variant == nlc::kProjMfma   ? "mfma_f64"
-			         : c->proj_variant == nlc::kProjBf16 ? "split_bf16"
-			                                             : "valu_f64");
+			         c->proj_variant == nlc::kProjMfma       ? "mfma_f64"
+			         : c->proj_variant == nlc::kProjMfmaWide ? "mfma_wide_f64"
+			         : c->proj_variant == nlc::kProjBf16     ? "split_bf16"
+			                                                 : "valu_f64");
 	}
 	return NLC_OK;
 }

@@ -192,6 +192,148 @@ k_proj_mfma(const double2 *__restrict__ state, uint32_t n, uint32_t copies, cons
 	}
 }
 
+// ------------------------------------------------------- FP64 MFMA, wide tile
+//
+// k_proj_mfma_wide: a workgroup of 8 wavefronts owns 64 copies x 256 modes;
+// wavefront w takes copy half w / 4 (32 copies, two 16-row blocks) and mode
+// group w % 4 (64 modes, four 16-column blocks).  The state tile of each 16
+// sites (64 copies x 16 sites of (q, p), 16 KB) is loaded once per workgroup
+// into LDS (double-buffered; rows padded to 17 entries, so the 16 copy rows
+// that one read instruction touches fall in distinct banks) and read from
+// there by the four mode groups, instead of being loaded by each wavefront;
+// each wavefront loads its eigenvector rows itself, one step ahead.  Global
+// loads per product fall to half of k_proj_mfma's, and with 256 modes per
+// workgroup a copy block is read by n/256 workgroups (two at 512 sites, on
+// the same XCD at about the same time).  Same K order and operand layout.
+constexpr int kWRows = 2, kWCols = 4;   // per wavefront: 32 copies x 64 modes
+constexpr int kWModes = 256;            // modes per workgroup
+constexpr int kWPad = 17;               // LDS row stride of the state tile, in (q, p) pairs
+
+__device__ __forceinline__ void lds_fence_barrier() {
+	__builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+	__builtin_amdgcn_s_barrier();
+	__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+template <bool FULL>
+__device__ __forceinline__ void wide_loop(const double2 *__restrict__ state, uint32_t n, uint32_t copies,
+                                          const double *__restrict__ V, uint32_t c0, uint32_t j_base, int half,
+                                          int r16, int g, double2 (&As)[2][64 * kWPad],
+                                          f64x4 (&acc_q)[kWRows][kWCols], f64x4 (&acc_p)[kWRows][kWCols]) {
+	const int tid = threadIdx.x;
+	// staging: thread tid carries copy tid / 8, sites 2 (tid % 8) and + 1 of each 16-site step
+	const uint32_t sc = uint32_t(tid >> 3), ss = 2u * uint32_t(tid & 7);
+	const double2 *srow = state + size_t(c0 + sc) * n + ss;
+	const bool s_ok = FULL || c0 + sc < copies;
+	auto stage = [&](uint32_t k0, double2 &x0, double2 &x1) {
+		if (FULL) {
+			x0 = srow[k0], x1 = srow[k0 + 1];
+		} else {
+			x0 = (s_ok && k0 + ss < n) ? srow[k0] : make_double2(0.0, 0.0);
+			x1 = (s_ok && k0 + ss + 1 < n) ? srow[k0 + 1] : make_double2(0.0, 0.0);
+		}
+	};
+	auto load_b = [&](uint32_t k0, double (&b)[kWCols][4]) {
+#pragma unroll
+		for (int cb = 0; cb < kWCols; ++cb) {
+			const uint32_t j = j_base + 16u * uint32_t(cb);
+			const double *src = V + size_t(j) * n + k0 + 4u * uint32_t(g);
+#pragma unroll
+			for (int t = 0; t < 4; ++t)
+				b[cb][t] = (FULL || (j < n && k0 + 4u * uint32_t(g) + t < n)) ? src[t] : 0.0;
+		}
+	};
+	double2 x0, x1;
+	stage(0, x0, x1);
+	As[0][sc * kWPad + ss] = x0;
+	As[0][sc * kWPad + ss + 1] = x1;
+	double b[kWCols][4];
+	load_b(0, b);
+	lds_fence_barrier();
+	const double2 *arow = &As[0][(32u * uint32_t(half) + uint32_t(r16)) * kWPad + 4u * uint32_t(g)];
+	uint32_t buf = 0;
+	for (uint32_t k0 = 0; k0 < n; k0 += 16) {
+		const bool more = k0 + 16u < n;
+		double bn[kWCols][4];
+		if (more) {
+			stage(k0 + 16u, x0, x1);
+			load_b(k0 + 16u, bn);
+		}
+		const double2 *ar = arow + buf * (64 * kWPad);
+#pragma unroll
+		for (int t = 0; t < 4; ++t) {
+			double2 a[kWRows];
+#pragma unroll
+			for (int rb = 0; rb < kWRows; ++rb) a[rb] = ar[16 * kWPad * rb + t];
+#pragma unroll
+			for (int rb = 0; rb < kWRows; ++rb)
+#pragma unroll
+				for (int cb = 0; cb < kWCols; ++cb) {
+					acc_q[rb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rb].x, b[cb][t], acc_q[rb][cb], 0, 0, 0);
+					acc_p[rb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rb].y, b[cb][t], acc_p[rb][cb], 0, 0, 0);
+				}
+		}
+		if (more) {
+			// the other buffer was last read before the previous barrier
+			As[buf ^ 1u][sc * kWPad + ss] = x0;
+			As[buf ^ 1u][sc * kWPad + ss + 1] = x1;
+#pragma unroll
+			for (int cb = 0; cb < kWCols; ++cb)
+#pragma unroll
+				for (int t = 0; t < 4; ++t) b[cb][t] = bn[cb][t];
+		}
+		lds_fence_barrier();
+		buf ^= 1u;
+	}
+}
+
+__global__ void __launch_bounds__(512, 1)
+k_proj_mfma_wide(const double2 *__restrict__ state, uint32_t n, uint32_t copies, const double *__restrict__ V,
+                 const double *__restrict__ omega, double *__restrict__ partial) {
+	__shared__ double2 As[2][64 * kWPad];
+	__shared__ double red[2][kWModes];
+	const uint32_t mode_blocks = (n + kWModes - 1) / kWModes;
+	const uint32_t xcd = blockIdx.x & 7, rest = blockIdx.x >> 3;
+	const uint32_t bx = rest % mode_blocks, by = (rest / mode_blocks) * 8u + xcd;
+	if (by * 64u >= copies) return; // padding workgroup of the last group of 8 copy blocks (uniform)
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+	const int half = wave >> 2, mg = wave & 3, r16 = lane & 15, g = lane >> 4;
+	const uint32_t c0 = by * 64u;
+	const uint32_t j_base = bx * uint32_t(kWModes) + uint32_t(mg) * 64u + uint32_t(r16);
+
+	f64x4 acc_q[kWRows][kWCols], acc_p[kWRows][kWCols];
+#pragma unroll
+	for (int rb = 0; rb < kWRows; ++rb)
+#pragma unroll
+		for (int cb = 0; cb < kWCols; ++cb) acc_q[rb][cb] = acc_p[rb][cb] = f64x4{0.0, 0.0, 0.0, 0.0};
+	const bool full = c0 + 64u <= copies && bx * uint32_t(kWModes) + uint32_t(kWModes) <= n && n % 16u == 0;
+	if (full) wide_loop<true>(state, n, copies, V, c0, j_base, half, r16, g, As, acc_q, acc_p);
+	else wide_loop<false>(state, n, copies, V, c0, j_base, half, r16, g, As, acc_q, acc_p);
+
+	// epilogue: lane holds modes j_base + 16 cb of copies 32 half + 16 rb + 4 g + r
+#pragma unroll
+	for (int cb = 0; cb < kWCols; ++cb) {
+		const uint32_t j = j_base + 16u * uint32_t(cb);
+		const double w = j < n ? omega[j] : 0.0;
+		double sum = 0.0;
+#pragma unroll
+		for (int rb = 0; rb < kWRows; ++rb)
+#pragma unroll
+			for (int r = 0; r < 4; ++r) {
+				const double wq = w * acc_q[rb][cb][r], pp = acc_p[rb][cb][r];
+				sum += fma(wq, wq, pp * pp);
+			}
+		sum += __shfl_xor(sum, 16, 64);
+		sum += __shfl_xor(sum, 32, 64);
+		if (g == 0) red[half][mg * 64 + 16 * cb + r16] = sum;
+	}
+	__syncthreads();
+	if (threadIdx.x < kWModes) {
+		const uint32_t j = bx * uint32_t(kWModes) + threadIdx.x;
+		if (j < n) partial[size_t(by) * n + j] = red[0][threadIdx.x] + red[1][threadIdx.x];
+	}
+}
+
 // ------------------------------------------------------- split BF16 form
 //
 // Each FP64 operand x is split into three BF16 terms, x = h + m + l (each the
@@ -356,6 +498,11 @@ hipError_t launch_energy_ddnkg(hipStream_t st, const double2 *state, uint32_t n,
 		const uint32_t mode_blocks = (n + 63) / 64, groups = (ddnkg_partial_rows(copies) + 7) / 8;
 		hipLaunchKernelGGL(k_proj_bf16x3, dim3(groups * 8 * mode_blocks), dim3(256), 0, st, (const __bf16 *) planes, n,
 		                   copies, (const __bf16 *) vplanes, omega, partial);
+	} else if (variant == kProjMfmaWide) {
+		const uint32_t mode_blocks = (n + kWModes - 1) / kWModes;
+		const uint32_t groups = (ddnkg_partial_rows(copies) + 7) / 8; // of 8 copy blocks, one per XCD
+		hipLaunchKernelGGL(k_proj_mfma_wide, dim3(groups * 8 * mode_blocks), dim3(512), 0, st, state, n, copies, V,
+		                   omega, partial);
 	} else {
 		const uint32_t mode_blocks = (n + kMfModes - 1) / kMfModes;
 		const uint32_t groups = (ddnkg_partial_rows(copies) + 7) / 8; // of 8 copy blocks, one per XCD

@@ -1,9 +1,14 @@
-// Double-precision exponential for the Toda bond force, table driven, on an
-// argument already expressed in 1/32 octaves: exp2_32(x) = 2^(x/32), so the
-// Toda kernels keep v = (64 alpha/ln2) q in registers and the bond term
-// e^{2 alpha (q_i - q_{i-1})} is exp2_32(v_i - v_{i-1}).
+// Double-precision exponentials for the Toda bond force, on an argument
+// already expressed in a power-of-two unit: exp2_32_tab(x) = 2^(x/32) (table
+// driven) and exp2_poly(y) = 2^y (table-free).  The Toda kernels keep
+// v = (64 alpha/ln2) q in registers for the first, (2 alpha/ln2) q for the
+// second (the two scalings differ by 2^5, so every product and sum of the step
+// scales exactly and both give the same state), and the bond term
+// e^{2 alpha (q_i - q_{i-1})} is then exp2_32_tab(v_i - v_{i-1}), resp.
+// exp2_poly(v_i - v_{i-1}).  Each unit lets the reduction's constants be the
+// instruction's inline constants (2, -1/2) or no constant at all.
 //
-// Method: P. T. P. Tang, "Table-driven implementation of the exponential
+// Table form: P. T. P. Tang, "Table-driven implementation of the exponential
 // function in IEEE floating-point arithmetic", ACM TOMS 15(2), 1989, with a
 // 64-entry table and a degree-5 polynomial, on the pre-scaled argument:
 //   2x = N + 2r,  N = nearest integer to 2x,  |r| <= 1/4,
@@ -12,30 +17,34 @@
 // the p_k interpolating (2^(r/32) - 1)/r at the five Chebyshev nodes of
 // |r| <= 1/4 (computed in 60-digit decimal arithmetic; error of the
 // polynomial < 2.4e-18 on the interval, against 3.5e-17 for the degree-5
-// Taylor polynomial).  N comes from the low word of 2x + 1.5 * 2^52 (one FMA,
-// no conversion instruction) and r = x - N/2 is exact (one FMA), so the
-// reduction of Tang's method (y - N ln2/64 in two FMAs) costs nothing extra
-// here; 2^(N >> 6) is applied with ldexp, which saturates to +inf / 0 beyond
-// the double range.  Error about 1 ulp: the table entries are 2^(j/64)
-// rounded to nearest, one rounding in the last FMA, and |2^(r/32) - 1| <
-// 0.0055 carries the polynomial's rounding errors down to 1/100 ulp.
+// Taylor polynomial).  N is 2x + 1.5 * 2^52 - 1.5 * 2^52 (an FMA and a
+// subtraction) and r = x - N/2 is exact (one FMA), so the reduction of Tang's
+// method (y - N ln2/64 in two FMAs) costs nothing extra here; 2^(N >> 6) is
+// applied with ldexp, which saturates to +inf / 0 beyond the double range.
+// Error about 1 ulp: the table entries are 2^(j/64) rounded to nearest, one
+// rounding in the last FMA, and |2^(r/32) - 1| < 0.0055 carries the
+// polynomial's rounding errors down to 1/100 ulp.
 //
-// Range: x is first clamped to [kExpArgMin, kExpArgMax] = [-34500, 33000], where
-// 2^(x/32) is already 0 (below the smallest subnormal, 2^-1074) and +inf
-// (above 2^1024), so every finite or infinite x gives the correctly saturated
-// result; without the clamp, N (taken from the low 32 bits of the shifted
-// argument) would wrap for |x| >= 2^30 and a bond stretched that far by a
-// blown-up step would get a finite force instead of +inf.  A NaN argument
-// gives 0 (the clamp's max returns the other operand): it only arises from a
-// state that is already NaN, and the drift keeps that NaN in the state.
+// Range: N is converted to an integer by the saturating conversion (values
+// beyond the int32 range clamp to INT32_MIN / INT32_MAX), so ldexp's exponent
+// N >> 6 saturates with it and every |x| < 2^50 gives the correctly rounded
+// result or the correctly saturated +inf / 0.  (Taking N from the low 32 bits
+// of the shifted argument, which costs no instruction, wrapped for |x| >= 2^30:
+// a bond stretched that far by a blown-up step got a finite force instead of
+// +inf, and the run's non-finite-entropy stop might never come.)  Beyond 2^50,
+// where x - N/2 is no longer exact, a positive x gives +inf or -inf and a
+// negative one 0 or a non-finite value; non-finite x give NaN.  Such arguments
+// come only from a chain that has already blown up; its state is non-finite
+// or becomes so.
 //
-// Cost on gfx950: 14 VALU + 1 LDS read per call, against about 25 VALU for the
+// Cost on gfx950: 13 VALU + 1 LDS read per call, against about 25 VALU for the
 // device library's exp (a degree-12 polynomial on |r| <= ln2/2 plus range
 // checks).  The table lives in LDS (512 bytes: entry j in banks 2j mod 64 and
 // 2j + 1 mod 64, so a wavefront's reads conflict at most two ways).
 //
-// The header is plain C++ apart from NLC_EXP_QUAL, so the same code can be
-// compiled on the host to measure its error.
+// The header is plain C++ apart from NLC_EXP_QUAL and the saturating
+// conversion, so the same code can be compiled on the host (with NLC_EXP_HOST
+// defined) to measure its error.
 #pragma once
 
 #ifndef NLC_EXP_QUAL
@@ -43,6 +52,23 @@
 #endif
 
 namespace nlc {
+
+// double -> int32 with saturation (NaN gives 0).  On the device this is HIP's
+// __double2int_rz, which compiles to v_cvt_i32_f64: the instruction clamps
+// out-of-range values to INT32_MIN / INT32_MAX.  (An inline-asm form of the same
+// instruction kept the compiler from interleaving the 32 exponentials of a
+// 32-site lane and doubled the register use of those kernels.)  The host
+// build, used to measure the error, spells the saturation out.
+#ifdef NLC_EXP_HOST
+inline int cvt_sat_i32(double x) {
+	if (!(x == x)) return 0;
+	if (x >= 2147483647.0) return 2147483647;
+	if (x <= -2147483648.0) return -2147483647 - 1;
+	return int(x);
+}
+#else
+NLC_EXP_QUAL int cvt_sat_i32(double x) { return __double2int_rz(x); }
+#endif
 
 constexpr int kExpTabSize = 64;
 
@@ -65,25 +91,13 @@ constexpr int kExpTabSize = 64;
 	 0x1.d5818dcfba487p+0, 0x1.da9e603db3285p+0, 0x1.dfc97337b9b5fp+0, 0x1.e502ee78b3ff6p+0,                           \
 	 0x1.ea4afa2a490dap+0, 0x1.efa1bee615a27p+0, 0x1.f50765b6e4540p+0, 0x1.fa7c1819e90d8p+0}
 
-constexpr double kExpShifter = 0x1.8p52;   // 1.5 * 2^52: y + kExpShifter rounds y to an integer
-constexpr double kExpShifter32 = 0x1.8p57; // 1.5 * 2^57: rounds x to a multiple of 32
-// 2^(x/32) rounds to +inf from x = 32768 on and to 0 below x = -34400 (where it
-// is under 2^-1075, half the smallest subnormal).
-constexpr double kExpArgMin = -34500.0, kExpArgMax = 33000.0;
-
-NLC_EXP_QUAL double exp2_32_clamp(double x) { return __builtin_fmin(__builtin_fmax(x, kExpArgMin), kExpArgMax); }
+constexpr double kExpShifter = 0x1.8p52; // 1.5 * 2^52: y + kExpShifter rounds y to an integer
 
 // 2^(x/32) = e^(x ln2/32).
 NLC_EXP_QUAL double exp2_32_tab(double x, const double *tab) {
-	x = exp2_32_clamp(x);
 	const double s = __builtin_fma(x, 2.0, kExpShifter); // rounds 2x to the nearest integer N
-	const double nd = s - kExpShifter;                    // N as a double, exact
-	// low 32 bits of s's significand hold N in two's complement (|N| < 2^17 after the clamp)
-	union {
-		double d;
-		unsigned long long u;
-	} bits = {s};
-	const int n = int(unsigned(bits.u));
+	const double nd = s - kExpShifter;                    // N as a double, exact for |x| < 2^50
+	const int n = cvt_sat_i32(nd);
 	const double r = __builtin_fma(nd, -0.5, x); // x - N/2, exact, |r| <= 1/4
 	// (2^(r/32) - 1) / r = sum_k p_k r^k
 	double t = __builtin_fma(r, 0x1.5d8812769ed72p-35, 0x1.3b2ad0385b409p-27); // p4, p3
@@ -95,35 +109,32 @@ NLC_EXP_QUAL double exp2_32_tab(double x, const double *tab) {
 	return __builtin_ldexp(__builtin_fma(T, q, T), n >> 6);
 }
 
-// Table-free variant: x = 32 N + r, |r| <= 16, 2^(r/32) - 1 = r (q0 + q1 r +
-// ... + q10 r^10), the q_k interpolating (2^(r/32) - 1)/r at the eleven
-// Chebyshev nodes of |r| <= 16 (error < 2.1e-17 with the coefficients rounded
-// to double, against 1.3e-17 for the degree-13 Taylor polynomial), 2^N by
-// ldexp; the same clamp as above.  17 VALU and no memory access: with the
-// table, the scheduler hoists the table loads of a whole unrolled run of bonds
-// to hide their latency and the kernel loses occupancy; this form keeps a
-// chain of bonds in few registers.
-NLC_EXP_QUAL double exp2_32_poly(double x) {
-	x = exp2_32_clamp(x);
-	const double s = x + kExpShifter32; // the unit in the last place of s is 32
-	const double nd = s - kExpShifter32; // 32 N, exact
-	// low 32 bits of s's significand (in units of 32) hold N in two's complement (|N| < 2^11)
-	union {
-		double d;
-		unsigned long long u;
-	} bits = {s};
-	const int n = int(unsigned(bits.u));
-	const double r = x - nd; // exact, |r| <= 16
-	double t = __builtin_fma(r, 0x1.e9d3fe3952179p-87, 0x1.e6063f7217bc6p-78);     // q10, q9
-	t = __builtin_fma(t, r, 0x1.b524fae627834p-69);                                // q8
-	t = __builtin_fma(t, r, 0x1.62bfd47773353p-60);                                // q7
-	t = __builtin_fma(t, r, 0x1.ffcbfc670dcd4p-52);                                // q6
-	t = __builtin_fma(t, r, 0x1.430913096fd9fp-43);                                // q5
-	t = __builtin_fma(t, r, 0x1.5d87fe78a5276p-35);                                // q4
-	t = __builtin_fma(t, r, 0x1.3b2ab6fba1ddap-27);                                // q3
-	t = __builtin_fma(t, r, 0x1.c6b08d704a0c2p-20);                                // q2
-	t = __builtin_fma(t, r, 0x1.ebfbdff82c598p-13);                                // q1
-	t = __builtin_fma(t, r, 0x1.62e42fefa39efp-6);                                 // q0
+// Table-free variant: y = N + r, N = y rounded to the nearest integer (ties
+// to even, by the rounding instruction), |r| <= 1/2, 2^r - 1 = r (q0 + q1 r +
+// ... + q10 r^10), the q_k interpolating (2^r - 1)/r at the eleven Chebyshev
+// nodes of |r| <= 1/2 (error < 2.1e-17 with the coefficients rounded to
+// double, against 1.3e-17 for the degree-13 Taylor polynomial), 2^N by ldexp
+// with N converted as for the table form.  15 VALU and no memory access (on
+// an argument in 1/32 octaves it takes 16: the scaling to octaves before the
+// rounding; here the state carries it).  With the table, the scheduler hoists
+// the table loads of a whole unrolled run of bonds to hide their latency and
+// the kernel loses occupancy; this form keeps a chain of bonds in few
+// registers.  r = y - N is exact for every finite y, so this form saturates
+// exactly to +inf / 0 for all finite arguments.
+NLC_EXP_QUAL double exp2_poly(double y) {
+	const double nd = __builtin_rint(y);
+	const int n = cvt_sat_i32(nd);
+	const double r = y - nd; // exact, |r| <= 1/2
+	double t = __builtin_fma(r, 0x1.e9d3fe3952179p-32, 0x1.e6063f7217bc6p-28); // q10, q9
+	t = __builtin_fma(t, r, 0x1.b524fae627834p-24);                                // q8
+	t = __builtin_fma(t, r, 0x1.62bfd47773353p-20);                                // q7
+	t = __builtin_fma(t, r, 0x1.ffcbfc670dcd4p-17);                                // q6
+	t = __builtin_fma(t, r, 0x1.430913096fd9fp-13);                                // q5
+	t = __builtin_fma(t, r, 0x1.5d87fe78a5276p-10);                                // q4
+	t = __builtin_fma(t, r, 0x1.3b2ab6fba1ddap-7);                                 // q3
+	t = __builtin_fma(t, r, 0x1.c6b08d704a0c2p-5);                                 // q2
+	t = __builtin_fma(t, r, 0x1.ebfbdff82c598p-3);                                 // q1
+	t = __builtin_fma(t, r, 0x1.62e42fefa39efp-1);                                 // q0
 	return __builtin_ldexp(__builtin_fma(t, r, 1.0), n);
 }
 

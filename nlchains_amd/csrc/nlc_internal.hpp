@@ -39,7 +39,8 @@ struct StepArgs {
 //               sigma = sqrt(beta);
 //   kFormQuad   FPUT with beta = 0 != alpha: sigma = alpha;
 //   kFormPlain  every other case; Toda keeps v = sigma q with unscaled p,
-//               sigma = 64 alpha/ln2 (its bond exponent in 1/32 octaves).
+//               sigma = 64 alpha/ln2 (its bond exponent in 1/32 octaves;
+//               kernels with 25+ sites per lane use sigma/32, octaves).
 enum ForceForm { kFormPlain = 0, kFormCubic = 1, kFormQuad = 2 };
 
 // The form and sigma for a model's parameters.
@@ -190,10 +191,12 @@ constexpr uint32_t kEnergyFftMax = 4096;
 
 namespace nlc {
 // dDNKG projection energies: partial[rows][n] with rows = ddnkg_partial_rows(copies),
-// on FP64 matrix cores (kProjMfma), the FP64 vector ALU (kProjValu) or BF16
+// on FP64 matrix cores (kProjMfma: 64 copies x 128 modes per workgroup;
+// kProjMfmaWide: 64 x 256 with the state tile shared through LDS), the FP64
+// vector ALU (kProjValu) or BF16
 // matrix cores with three-term operand splits (kProjBf16: n % 32 == 0; vplanes =
 // ddnkg_split_host of V, 3 n^2 values; planes = scratch for 6 n copies values).
-constexpr int kProjMfma = 0, kProjValu = 1, kProjBf16 = 2;
+constexpr int kProjMfma = 0, kProjValu = 1, kProjBf16 = 2, kProjMfmaWide = 3;
 uint32_t ddnkg_partial_rows(uint32_t copies);
 bool ddnkg_bf16_supported(uint32_t n);
 void ddnkg_split_host(const double *x, size_t count, uint16_t *planes);

@@ -171,7 +171,7 @@ k_step_group(double2 *__restrict__ state, uint32_t n, uint32_t steps, StepArgs a
 	for (int j = 0; j < S; ++j) {
 		const bool real_site = EXACT || j < S - 1 || ch.full;
 		double2 v = make_double2(0.0, 0.0);
-		if (real_site) v = to_registers<MODEL, FORM>(base[j], a);
+		if (real_site) v = to_registers<MODEL, FORM, S>(base[j], a);
 		ch.q[j] = v.x;
 		ch.p[j] = v.y;
 		if constexpr (MODEL == DDNKG) ch.mp[j] = real_site ? mp2g[off + j] : 0.0;
@@ -184,7 +184,7 @@ k_step_group(double2 *__restrict__ state, uint32_t n, uint32_t steps, StepArgs a
 		ch.m_l = ch.m_r = 0.0;
 	}
 
-	const double cs = (MODEL == TODA) ? a.scale : 1.0;
+	const double cs = (MODEL == TODA) ? a.scale * toda_unit<MODEL, S>() : 1.0;
 	const double dc0 = cs * a.dc[0];
 	ch.drift(dc0);
 	for (uint32_t s = 0; s < steps; ++s) {
@@ -208,7 +208,7 @@ k_step_group(double2 *__restrict__ state, uint32_t n, uint32_t steps, StepArgs a
 
 #pragma unroll
 	for (int j = 0; j < S; ++j)
-		if (EXACT || j < S - 1 || ch.full) base[j] = from_registers<MODEL, FORM>(ch.q[j], ch.p[j], a);
+		if (EXACT || j < S - 1 || ch.full) base[j] = from_registers<MODEL, FORM, S>(ch.q[j], ch.p[j], a);
 }
 
 // W = ceil(n / (64 S)) wavefronts; the sites must fill every lane with S or

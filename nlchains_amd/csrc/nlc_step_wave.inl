@@ -21,7 +21,7 @@
 //    FPUT and Toda exchange the bond force of the lane's first bond instead of
 //    recomputing it, so each bond term (a cubic or an FP64 exp) is evaluated
 //    once per substep; Toda works on v = (64 alpha/ln2) q with the 2^(x/32) of
-//    nlc_exp.hpp;
+//    nlc_exp.hpp (with 25+ sites per lane on v/32 and its 2^x);
 //  * the 7 substeps are unrolled with their weights in SGPRs; the c7 drift of a
 //    step is fused with the c0 drift of the next (the reference's 2*c0 merge).
 //  * HBM is touched once per batch: 16 B/site in, 16 B/site out.
@@ -39,6 +39,7 @@ namespace {
 
 template <int MODEL, int L, int S, bool EXACT, int FORM>
 struct Chain {
+	static constexpr int kSites = S;
 	double q[S], p[S];
 	double mp[MODEL == DDNKG ? S : 1];
 	bool full; // lane holds S real sites (else S-1 and a dummy last slot)
@@ -87,10 +88,11 @@ struct Chain {
 
 // `steps` Yoshida steps of the chain part held by ch: the c7 drift of a step
 // fused with the c0 drift of the next (the reference's 2 c0 merge).  Toda
-// drifts v = (64 alpha/ln2) q: its drift weights carry that factor.
+// drifts v = (64 alpha/ln2) q (times toda_unit): its drift weights carry that
+// factor.
 template <int MODEL, class C>
 __device__ __forceinline__ void step_loop(C &ch, uint32_t steps, const StepArgs &a) {
-	const double cs = (MODEL == TODA) ? a.scale : 1.0;
+	const double cs = (MODEL == TODA) ? a.scale * toda_unit<MODEL, C::kSites>() : 1.0;
 	const double dc0 = cs * a.dc[0];
 	ch.drift(dc0);
 	for (uint32_t s = 0; s < steps; ++s) {
@@ -116,8 +118,9 @@ __device__ __forceinline__ void step_loop(C &ch, uint32_t steps, const StepArgs 
 // length n starting at chain0 for `steps` steps.  All 64 lanes take part in
 // the halo exchange even when the wave is only partly filled.  Toda keeps
 // v = (64 alpha/ln2) q in registers (the exponent of its bond term is then a
-// plain difference in 1/32 octaves) and drifts v with (64 alpha/ln2) c;
-// q = v / (64 alpha/ln2) is stored back.
+// plain difference in 1/32 octaves; v/32 and octaves from 25 sites per lane
+// on) and drifts v with the same factor times c; q = v / (64 alpha/ln2) is
+// stored back.
 // The scaled force forms hold sigma q and sigma p.
 template <int MODEL, int L, int S, bool EXACT, int FORM>
 __device__ __forceinline__ void run_wave(double2 *__restrict__ chain0, uint32_t n, uint32_t live_chains, uint32_t steps,
@@ -139,7 +142,7 @@ __device__ __forceinline__ void run_wave(double2 *__restrict__ chain0, uint32_t 
 	for (int j = 0; j < S; ++j) {
 		const bool real_site = EXACT || j < S - 1 || ch.full;
 		double2 v = make_double2(0.0, 0.0);
-		if (live && real_site) v = to_registers<MODEL, FORM>(base[j], a);
+		if (live && real_site) v = to_registers<MODEL, FORM, S>(base[j], a);
 		ch.q[j] = v.x;
 		ch.p[j] = v.y;
 		if constexpr (MODEL == DDNKG) ch.mp[j] = real_site ? mp2g[off + j] : 0.0;
@@ -151,7 +154,7 @@ __device__ __forceinline__ void run_wave(double2 *__restrict__ chain0, uint32_t 
 #pragma unroll
 		for (int j = 0; j < S; ++j)
 			if (EXACT || j < S - 1 || ch.full)
-				base[j] = from_registers<MODEL, FORM>(ch.q[j], ch.p[j], a);
+				base[j] = from_registers<MODEL, FORM, S>(ch.q[j], ch.p[j], a);
 	}
 }
 
@@ -226,7 +229,7 @@ k_step_tiled(const double2 *__restrict__ src, double2 *__restrict__ dst, uint32_
 	uint32_t g = uint32_t((uint64_t(seg) * C + n - H + uint32_t(lane * S)) % n);
 #pragma unroll
 	for (int j = 0; j < S; ++j, g = (g + 1 == n) ? 0 : g + 1) {
-		const double2 v = to_registers<MODEL, FORM>(in[g], a);
+		const double2 v = to_registers<MODEL, FORM, S>(in[g], a);
 		ch.q[j] = v.x;
 		ch.p[j] = v.y;
 		if constexpr (MODEL == DDNKG) ch.mp[j] = mp2g[g];
@@ -237,7 +240,7 @@ k_step_tiled(const double2 *__restrict__ src, double2 *__restrict__ dst, uint32_
 	for (int j = 0; j < S; ++j) {
 		const uint32_t e = uint32_t(lane * S + j);
 		const uint64_t g = uint64_t(seg) * C + e - H;
-		if (e >= H && e < H + C && g < n) out[g] = from_registers<MODEL, FORM>(ch.q[j], ch.p[j], a);
+		if (e >= H && e < H + C && g < n) out[g] = from_registers<MODEL, FORM, S>(ch.q[j], ch.p[j], a);
 	}
 }
 

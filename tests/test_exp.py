@@ -1,7 +1,9 @@
 """The two FP64 exponential forms of the Toda kernels
-(nlchains_amd/csrc/nlc_exp.hpp: table-driven and table-free 2^(x/32)), compiled
-on the host from the same header, against 2^(x/32) computed to 40 significant
-digits with Python's decimal module."""
+(nlchains_amd/csrc/nlc_exp.hpp: table-driven 2^(x/32) and table-free 2^y),
+compiled on the host from the same header, against 2^(x/32) computed to 40
+significant digits with Python's decimal module.  The tests state every
+argument as x (1/32 octaves); the table-free wrapper passes y = x/32, exactly,
+as the kernels' state scaling does."""
 import ctypes
 import math
 import re
@@ -16,13 +18,14 @@ HEADER = Path(__file__).resolve().parents[1] / "nlchains_amd" / "csrc" / "nlc_ex
 
 WRAPPER = r"""
 #define NLC_EXP_QUAL inline
+#define NLC_EXP_HOST 1
 #include "nlc_exp.hpp"
 static const double tab[nlc::kExpTabSize] = NLC_EXP2_TABLE;
 extern "C" void exp_many(const double *y, double *out, long n) {
 	for (long i = 0; i < n; ++i) out[i] = nlc::exp2_32_tab(y[i], tab);
 }
 extern "C" void exp_poly_many(const double *y, double *out, long n) {
-	for (long i = 0; i < n; ++i) out[i] = nlc::exp2_32_poly(y[i]);
+	for (long i = 0; i < n; ++i) out[i] = nlc::exp2_poly(y[i] * 0x1p-5);
 }
 """
 
@@ -91,20 +94,20 @@ def annotated_literals(text, tag):
     return out
 
 
-@pytest.mark.parametrize("tag,half,ncoef,bound,taylor_miss", [
-    ("p", 0.25, 5, 2.5e-18, 3e-17),   # table form: |r| <= 1/4
-    ("q", 16.0, 11, 2.1e-17, 1e-15),  # table-free form: |r| <= 16
+@pytest.mark.parametrize("tag,half,unit,ncoef,bound,taylor_miss", [
+    ("p", 0.25, 1 / 32, 5, 2.5e-18, 3e-17),  # table form: 2^(r/32), |r| <= 1/4
+    ("q", 0.5, 1, 11, 2.1e-17, 1e-15),       # table-free form: 2^r, |r| <= 1/2
 ])
-def test_interpolants(tag, half, ncoef, bound, taylor_miss):
+def test_interpolants(tag, half, unit, ncoef, bound, taylor_miss):
     """r (c0 + ... + c_m r^m), with the literals annotated `tag`k in the header,
-    is 2^(r/32) - 1 within `bound` on |r| <= half (60-digit check on a grid with
-    the interval ends); the Taylor polynomial of the same degree misses by more
-    than `taylor_miss`; and each coefficient, moved by about 2x the bound over the
-    size of its term at r = half, fails."""
+    is 2^(unit r) - 1 within `bound` on |r| <= half (60-digit check on a grid
+    with the interval ends); the Taylor polynomial of the same degree misses by
+    more than `taylor_miss`; and each coefficient, moved by about 2x the bound
+    over the size of its term at r = half, fails."""
     getcontext().prec = 60
     coef = annotated_literals(HEADER.read_text(), tag)
     assert sorted(coef) == list(range(ncoef))
-    ln2 = Decimal(2).ln()
+    a = Decimal(2).ln() * Decimal(unit)
     h = Decimal(half)
 
     def worst(c):
@@ -114,24 +117,23 @@ def test_interpolants(tag, half, ncoef, bound, taylor_miss):
             acc = Decimal(0)
             for k in sorted(c, reverse=True):
                 acc = acc * r + Decimal(c[k])
-            w = max(w, abs(r * acc - ((r * ln2 / 32).exp() - 1)))
+            w = max(w, abs(r * acc - ((r * a).exp() - 1)))
         return w
 
     assert worst(coef) < Decimal(bound)
-    taylor = {k: float((ln2 / 32) ** (k + 1) / math.factorial(k + 1)) for k in range(ncoef)}
+    taylor = {k: float(a ** (k + 1) / math.factorial(k + 1)) for k in range(ncoef)}
     assert worst(taylor) > Decimal(taylor_miss)
     for k in range(ncoef):
-        term = (half * math.log(2) / 32) ** (k + 1) / math.factorial(k + 1)
+        term = (half * math.log(2) * unit) ** (k + 1) / math.factorial(k + 1)
         bad = dict(coef)
         bad[k] = coef[k] * (1 + 10 ** math.ceil(math.log10(2 * bound / term)))
         assert worst(bad) > Decimal(bound), k
 
 
-def test_shifters():
-    """The rounding constants are 1.5 * 2^52 and 1.5 * 2^57."""
+def test_shifter():
+    """The rounding constant of the table form is 1.5 * 2^52."""
     text = HEADER.read_text()
     assert float.fromhex(re.search(r"kExpShifter = (0x[0-9a-f.]+p[+-]?\d+)", text).group(1)) == 1.5 * 2.0 ** 52
-    assert float.fromhex(re.search(r"kExpShifter32 = (0x[0-9a-f.]+p[+-]?\d+)", text).group(1)) == 1.5 * 2.0 ** 57
 
 
 # x = 32 y / ln2 for y in (-1, 1), (-30, 30), (-700, 700), (-1e-6, 1e-6)
@@ -159,25 +161,42 @@ def test_reduction_boundaries_and_special_values(exp_fn):
 
 
 def test_saturation_far_out_of_range(exp_fn):
-    """Arguments far beyond the double range saturate: +inf above, 0 below, at
-    every magnitude (without the clamp the integer part wrapped from |x| = 2^30
-    on: 1.2e9 and 1e11 gave 0, some large negative x gave +inf).  The
-    thresholds: 2^(x/32) is the largest finite double's neighbourhood at
-    x = 32767.99 and under half the smallest subnormal below x = -34400."""
-    big = np.array([40000.0, 2.0 ** 30, 1.2e9, 1e9, 2.0 ** 31, 1e11, 2.0 ** 36, 1e15, 2.0 ** 52, 1e300,
-                    np.finfo(np.float64).max, np.inf])
+    """Arguments far beyond the double range saturate: +inf above, 0 below, for
+    every |x| < 2^50 (taking the integer part from the low bits of the shifted
+    argument wrapped it from |x| = 2^30 on: 1.2e9 and 1e11 gave 0, some large
+    negative x gave +inf).  The thresholds: 2^(x/32) is the largest finite
+    double's neighbourhood at x = 32767.99 and under half the smallest
+    subnormal below x = -34400.  The integer conversion of the host build
+    saturates like the device's conversion instruction."""
+    big = np.array([40000.0, 2.0 ** 30, 1.2e9, 1e9, 2.0 ** 31, 1e11, 2.0 ** 36, 1e15, 2.0 ** 49,
+                    2.0 ** 50 - 1])
     assert np.all(exp_fn(np.concatenate([[32768.0, 33000.0], big])) == np.inf)
     assert np.all(exp_fn(np.concatenate([[-34401.0, -34500.0], -big])) == 0.0)
     near = np.array([32767.9, -34399.0, -34300.0])
     out = exp_fn(near)
     assert np.isfinite(out[0]) and out[0] > 1.7e308
     assert 0.0 < out[1] < 1e-323 and out[2] > 0.0
-    # x = +-(random magnitudes from 1e5 to 1e300)
-    x = 10.0 ** np.random.default_rng(7).uniform(5, 300, 2000)
+    x = 10.0 ** np.random.default_rng(7).uniform(5, np.log10(2.0 ** 50), 2000)
     assert np.all(exp_fn(x) == np.inf) and np.all(exp_fn(-x) == 0.0)
 
 
-def test_nan_argument_gives_zero(exp_fn):
-    """The clamp's max maps a NaN argument to the lower bound, so the bond
-    term is 0; a NaN argument only comes from a state that is already NaN."""
-    assert exp_fn(np.array([np.nan]))[0] == 0.0
+def test_beyond_exact_reduction_range(exp_tab):
+    """From |x| = 2^50 on, the table form's remainder x - N/2 is no longer
+    exact: a positive argument still gives a non-finite value (the bond force
+    of a blown-up chain is never finite) and a negative one 0 or a non-finite
+    value, never a finite nonzero number.  The table-free form's remainder is
+    exact for every finite x, so it saturates exactly."""
+    x = 10.0 ** np.random.default_rng(8).uniform(np.log10(2.0 ** 50), 300, 4000)
+    x = np.concatenate([x, [2.0 ** 52, 2.0 ** 60, 1e100, 1e300, np.finfo(np.float64).max]])
+    exp_fn, exp_poly = exp_tab["exp_many"], exp_tab["exp_poly_many"]
+    for out in (exp_fn(x), exp_poly(x)):
+        assert not np.isfinite(out).any()
+    neg = exp_fn(-x)
+    assert np.all((neg == 0.0) | ~np.isfinite(neg))
+    assert np.all(exp_poly(x) == np.inf) and np.all(exp_poly(-x) == 0.0)
+
+
+def test_nonfinite_arguments_give_nan(exp_fn):
+    """NaN and +-inf give NaN: such an argument only comes from a state that is
+    already non-finite, and the run's non-finite-entropy check stops it."""
+    assert np.isnan(exp_fn(np.array([np.nan, np.inf, -np.inf]))).all()

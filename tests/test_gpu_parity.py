@@ -323,6 +323,33 @@ def test_move_batches_compose(gpu):
         assert chain_rel_err(a.download(), b.download()) < 1e-13
 
 
+@pytest.mark.parametrize("shape", [dict(), dict(lanes_per_chain=2, sites_per_lane=32)])
+def test_toda_far_stretched_bond_gives_nonfinite_state(gpu, shape):
+    """A Toda bond stretched by 1e9 or 1e12 (exponent arguments of 9e10 and
+    9e13 in the kernel's units, beyond the 2^30 where a wrapped integer part
+    made the exponential finite) gives an infinite force and a non-finite chain
+    after one step, in both exponential forms (the table form on the default
+    64 x 1 shape, the table-free one on 2 x 32); the other chains of the
+    ensemble stay on the oracle's trajectory."""
+    N = gpu
+    n, copies = 64, 6
+    model, kw, masses, s0 = make_case(N, "Toda", n, copies, 8)
+    s0 = s0.copy()
+    s0[1, 20, 0] += 1e9
+    s0[3, 41, 0] -= 1e12
+    with N.Ensemble(model, n, copies, 0.05, **kw, **shape) as ens:
+        ens.upload(s0)
+        ens.move(1)
+        got = ens.download()
+        desc = ens.describe()
+    banner(f"  {desc}: finite chains {[bool(np.isfinite(got[c]).all()) for c in range(copies)]}")
+    good = [0, 2, 4, 5]
+    assert not np.isfinite(got[1]).all() and not np.isfinite(got[3]).all()
+    assert np.isfinite(got[good]).all()
+    want = oracle_move(model, kw, masses, s0[good], 1, 0.05)
+    assert chain_rel_err(got[good], want) <= STATE_TOL
+
+
 def test_move_is_deterministic(gpu):
     N = gpu
     model, kw, masses, s0 = make_case(N, "Toda", 512, 64, 4)
@@ -515,7 +542,7 @@ def test_full_size_dnkg_sampled_parity_and_drift(gpu):
 
 # ------------------------------------------------ dDNKG projection forms
 
-@pytest.mark.parametrize("proj", ["mfma", "valu"])
+@pytest.mark.parametrize("proj", ["mfma", "mfma_wide", "valu"])
 @pytest.mark.parametrize("n,copies", [(512, 70), (100, 9), (130, 129), (48, 200), (16, 64)])
 def test_ddnkg_projection_forms(gpu, proj, n, copies):
     """Both forms of the fused projection (FP64 matrix cores, FP64 vector ALU)
@@ -525,7 +552,7 @@ def test_ddnkg_projection_forms(gpu, proj, n, copies):
     banner(f"dDNKG projection {proj} n={n} copies={copies}")
     model, kw, masses, s0 = make_case(N, "dDNKG", n, copies, 11)
     s0 = s0 + 0.01 * np.random.default_rng(3).normal(size=s0.shape)
-    p = N.PROJ_MFMA_F64 if proj == "mfma" else N.PROJ_VALU_F64
+    p = {"mfma": N.PROJ_MFMA_F64, "mfma_wide": N.PROJ_MFMA_F64_WIDE, "valu": N.PROJ_VALU_F64}[proj]
     with N.Ensemble(model, n, copies, 0.05, masses=masses, projection=p, **kw) as ens:
         ens.upload(s0)
         raw_gpu = ens.linenergies_raw()
@@ -546,7 +573,7 @@ def test_ddnkg_projection_full_size(gpu):
     s0 = N.ic_eigenmodes(n, copies, om, V, n // 8, 0.5, 5)
     s0 = s0 + 0.01 * np.random.default_rng(4).normal(size=s0.shape)
     raws = {}
-    for name, p in (("mfma", N.PROJ_MFMA_F64), ("valu", N.PROJ_VALU_F64)):
+    for name, p in (("mfma", N.PROJ_MFMA_F64), ("mfma_wide", N.PROJ_MFMA_F64_WIDE), ("valu", N.PROJ_VALU_F64)):
         with N.Ensemble(N.DDNKG, n, copies, 0.05, masses=masses, beta=1.0, projection=p) as ens:
             ens.upload(s0)
             raws[name] = ens.linenergies_raw()
@@ -555,6 +582,7 @@ def test_ddnkg_projection_full_size(gpu):
     banner(f"  mfma vs valu {vec_rel_err(raws['mfma'], raws['valu']):.3e}; "
            f"sum {raws['mfma'].sum():.15g} vs harmonic {harmonic:.15g}")
     assert vec_rel_err(raws["mfma"], raws["valu"]) <= 1e-12
+    assert vec_rel_err(raws["mfma_wide"], raws["valu"]) <= 1e-12
     assert abs(raws["mfma"].sum() - harmonic) <= 1e-11 * harmonic
 
 
