@@ -136,7 +136,7 @@ def cpu_baseline(w, s0, seconds_budget: float):
     move(s0[:copies], steps)
     el = time.perf_counter() - t
     if el < 0.5 * seconds_budget:  # the small probe underestimates the rate: rescale once
-        steps = max(steps, min(int(steps * seconds_budget / el), 100 * w["batch"]))
+        steps = max(steps, min(int(steps * seconds_budget / el), 10000 * w["batch"]))
         t = time.perf_counter()
         move(s0[:copies], steps)
         el = time.perf_counter() - t
@@ -251,7 +251,7 @@ def cpu_baseline_ragged(w, lengths, s0, seconds_budget: float):
         return time.perf_counter() - t
     el = sample(steps)
     if el < 0.5 * seconds_budget:  # the 2-step probe underestimates the rate: rescale once
-        steps = max(steps, min(int(steps * seconds_budget / el), 100 * w["batch"]))
+        steps = max(steps, min(int(steps * seconds_budget / el), 10000 * w["batch"]))
         el = sample(steps)
     return {"value": sites * steps / el, "unit": "site-steps/s", "cores": cores, "host_cores": host_cores(),
             "kind": "port",

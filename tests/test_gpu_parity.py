@@ -94,6 +94,18 @@ def test_move_wave_shapes(gpu, n, L, S):
     check_move(gpu, "FPUT", n, 37, steps=60, lanes_per_chain=L, sites_per_lane=S)
 
 
+@pytest.mark.parametrize("n,L,S", [(64, 4, 16), (1024, 64, 16), (1500, 64, 24), (64, 2, 32), (25, 1, 25),
+                                   (30, 1, 30), (512, 16, 32), (500, 16, 32), (2048, 64, 32), (2000, 64, 32)])
+def test_toda_exp_forms(gpu, n, L, S):
+    """Toda on both exponential forms: up to 24 sites per lane the table form
+    on v = (64 alpha/ln2) q, from 25 on the table-free form on the state
+    scaled to octaves (v/32); 100 steps against the oracle.  Energy density
+    0.25: at 0.5 the 2048-site plane waves of this seed start with bond
+    stretches up to 3.4 and one chain goes non-finite within 100 steps of
+    dt = 0.05 in the oracle too."""
+    check_move(gpu, "Toda", n, 23, steps=100, dens=0.25, lanes_per_chain=L, sites_per_lane=S)
+
+
 # Parameters that select each force form of the register-resident kernels;
 # the scaled forms hold sigma q, sigma p with sigma = sqrt(beta) (cubic) or
 # alpha (quadratic), != 1 here.  n = 12: thread per chain; 1000 / 1024: wave
