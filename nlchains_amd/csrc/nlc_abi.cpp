@@ -1356,13 +1356,21 @@ int nlc_autotune(nlc_ctx *c, uint32_t trial_steps, char *report, size_t len) {
 				c->p.sites_per_lane = uint32_t(cd.S);
 				rc = select_uniform(c);
 			}
-			for (int rep = 0; rep < 2 && !rc; ++rep) { // the first run warms up
+			// the first run warms up and sizes the second: short trials (small
+			// ensembles) are repeated back to back until the timed region is
+			// about 0.2 ms, so that launch gaps and timer resolution do not
+			// decide between shapes; ms is the time per trial launch
+			int nrep = 1;
+			for (int rep = 0; rep < 2 && !rc; ++rep) {
 				rc = enqueue_move(c, trial_steps);
 				if (!rc && (hipEventRecord(e0, c->stream) != hipSuccess)) rc = c->fail(NLC_EHIP, "event record");
-				if (!rc) rc = enqueue_move(c, trial_steps);
+				for (int i = 0; i < nrep && !rc; ++i) rc = enqueue_move(c, trial_steps);
+				float ms = 0;
 				if (!rc && (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
-				            hipEventElapsedTime(&cd.ms, e0, e1) != hipSuccess))
+				            hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
 					rc = c->fail(NLC_EHIP, "autotune timing failed");
+				cd.ms = ms / float(nrep);
+				if (rep == 0 && cd.ms > 0) nrep = std::max(1, std::min(256, int(0.2f / cd.ms + 0.5f)));
 			}
 			if (hipMemcpyAsync(c->state, backup, c->state_bytes, hipMemcpyDeviceToDevice, c->stream) && !rc)
 				rc = c->fail(NLC_EHIP, "autotune restore failed");
