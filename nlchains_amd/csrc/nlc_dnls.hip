@@ -18,9 +18,13 @@
 //    time, conjugate twiddles) returns natural order, so no reordering pass is
 //    ever made.  LDS: one padded region per wavefront holding what that
 //    wavefront reads in the cross-wave exchange plus the rows of its in-wave
-//    16 x 16 transposes, so a substep needs two workgroup barriers (one per
-//    cross-wave exchange), every slot is a per-lane base plus a compile-time
-//    offset, and neither exchange has bank conflicts.
+//    16 x 16 transposes; every slot is a per-lane base plus a compile-time
+//    offset, and neither exchange has bank conflicts.  Each exchange moves the
+//    real parts, then the imaginary parts, through 34 KB (component-split
+//    exchanges of nlc_fft4096.inl): three chains fit a CU, and at 168 VGPRs
+//    three wavefronts per SIMD, for six workgroup barriers per substep
+//    instead of two (the 68 KB form with double2 slots held two chains per
+//    CU and measured 2 % slower; results are the same bit for bit).
 //  * k_dnls_4096_r8: the same transform as four radix-8 passes on 512 threads
 //    x 8 values (more wavefronts per SIMD, one more LDS exchange).
 //  * other lengths: nlc_dnls_generic.hip (LDS kernels up to 8192 sites) and
@@ -43,10 +47,10 @@ namespace {
 #include "nlc_fft4096.inl"
 
 
-__global__ void __launch_bounds__(kT, 2)
+__global__ void __launch_bounds__(kT, 3)
 k_dnls_4096(double2 *__restrict__ state, uint32_t copies, uint32_t steps, DnlsArgs a, const double2 *__restrict__ tw,
             const double2 *__restrict__ mult) {
-	__shared__ Lds4352<double2> L;
+	__shared__ Lds4352d L;
 	const uint32_t chain = blockIdx.x, t = threadIdx.x;
 	double2 *psi = state + size_t(chain) * 4096;
 	double2 v[16]; // psi[t + 256 j] at v[pos(j)]
@@ -60,13 +64,19 @@ k_dnls_4096(double2 *__restrict__ state, uint32_t copies, uint32_t steps, DnlsAr
 			// in registers across the loop: registers are the occupancy limit here
 			const double2 *twk = tw;
 			asm volatile("" : "+s"(twk));
-			nonlinear_all(v, th);
-			fft4096_fwd(v, L, twk);
-			// Fourier multiplier, stored per thread in the digit-reversed order
+			nonlinear_all_halves(v, th);
+			fft4096_fwd_split(v, L, twk);
+			// Fourier multiplier, stored per thread in the digit-reversed order;
+			// loaded and applied in two halves (the register budget of three
+			// wavefronts per SIMD)
 			const double2 *m = mult + size_t(k < 4 ? k : 6 - k) * 4096 + t;
 #pragma unroll
-			for (int j = 0; j < 16; ++j) v[pos(j)] = cmul(v[pos(j)], m[256 * j]);
-			fft4096_inv(v, L, twk);
+			for (int h = 0; h < 2; ++h) {
+#pragma unroll
+				for (int j = 8 * h; j < 8 * h + 8; ++j) v[pos(j)] = cmul(v[pos(j)], m[256 * j]);
+				__builtin_amdgcn_sched_barrier(0);
+			}
+			fft4096_inv_split(v, L, twk);
 		}
 	}
 #pragma unroll

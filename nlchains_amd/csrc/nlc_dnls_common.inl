@@ -126,6 +126,31 @@ __device__ __forceinline__ void nonlinear_all(double2 (&v)[N], double theta) {
 	}
 }
 
+// nonlinear_all in two halves of 8 values, the scheduler kept from
+// interleaving them: half the polynomial temporaries live at a time, for a
+// kernel that runs at 168 VGPRs (three wavefronts per SIMD hide the latency
+// the interleaving hid).
+__device__ __forceinline__ void nonlinear_all_halves(double2 (&v)[16], double theta) {
+	bool small = true;
+#pragma unroll
+	for (int j = 0; j < 16; ++j) small = small && fabs(theta * fma(v[j].x, v[j].x, v[j].y * v[j].y)) <= kExpiSmall;
+	const bool all_small = __all(small);
+#pragma unroll
+	for (int h = 0; h < 2; ++h) {
+		double a[8];
+#pragma unroll
+		for (int j = 0; j < 8; ++j) a[j] = -theta * fma(v[8 * h + j].x, v[8 * h + j].x, v[8 * h + j].y * v[8 * h + j].y);
+		if (all_small) {
+#pragma unroll
+			for (int j = 0; j < 8; ++j) v[8 * h + j] = rotate(v[8 * h + j], expi_small(a[j]));
+		} else {
+#pragma unroll
+			for (int j = 0; j < 8; ++j) v[8 * h + j] = rotate(v[8 * h + j], expi_reduced(a[j]));
+		}
+		__builtin_amdgcn_sched_barrier(0);
+	}
+}
+
 // 4-point DFT in place, SIGN = -1 forward, +1 inverse.
 template <int SIGN, class C>
 __device__ __forceinline__ void dft4(C &a0, C &a1, C &a2, C &a3) {

@@ -208,3 +208,144 @@ __device__ __forceinline__ void fft4096_inv(C v[16], Lds4352<C> &L, const C *__r
 	// stage 1': over k0 -> x2 transposed
 	dft16_n2t<+1>(v);
 }
+
+// ------------------------------------------------ component-split exchanges
+//
+// The same FP64 transforms with every LDS exchange made twice, real parts
+// then imaginary parts, through an array of 4352 doubles (34 KB instead of
+// the 68 KB of Lds4352<double2>): three chains' LDS fit a CU, so with at most
+// 168 VGPRs three wavefronts per SIMD hide the LDS and L2 latency that two
+// leave exposed.  Each cross-wave exchange takes three workgroup barriers
+// instead of one (write re, read re, write im, read im: the imaginary parts
+// reuse the slots the real parts were read from), each in-wave exchange three
+// wave syncs instead of two, and the LDS operations move 8 bytes instead of 16.
+//
+// Slot of element a in the cross-wave exchanges: a + 64 (a >> 10) + 16 ((a >> 8)
+// & 3), injective (the 256-element blocks of a 1024-element quarter start 272
+// slots apart) and inside a 1088-slot region per quarter: each wavefront reads
+// only its own region (as in Lds4352).  The second term moves consecutive
+// 16-lane groups of a wavefront by 16 doubles, so the two groups that one
+// 8-byte access instruction serves at a time (32 lanes, 256 bytes) fall in
+// different banks.  In-wave exchanges use
+// the same 17-entry rows inside the region, now conflict-free for 32 lanes
+// (17 is odd, and the next group starts 272 = 16 mod 32 doubles later).
+//
+// LDS hazards, as for fft4096_fwd: every slot that a thread writes first in
+// an exchange is one it read last itself; the other writes follow a
+// workgroup barrier (cross-wave) or a wave sync (in-wave, own region).
+struct Lds4352d {
+	double a[4352];
+};
+
+template <int PART>
+__device__ __forceinline__ double &part(double2 &z) {
+	if constexpr (PART == 0) return z.x;
+	else return z.y;
+}
+
+template <int PART>
+__device__ __forceinline__ void xch1_fwd_write(const double2 v[16], double *L, uint32_t t) {
+#pragma unroll
+	for (int k0 = 0; k0 < 16; ++k0) L[t + 256 * k0 + 64 * (k0 >> 2) + 16 * (k0 & 3)] = PART == 0 ? v[k0].x : v[k0].y;
+}
+
+template <int PART>
+__device__ __forceinline__ void xch1_fwd_read(double2 v[16], const double *L, uint32_t lo, uint32_t hi) {
+	const double *r = L + lo + 256 * hi + 64 * (hi >> 2) + 16 * (hi & 3);
+#pragma unroll
+	for (int i = 0; i < 16; ++i) {
+		const int x1 = 4 * (i & 3) + (i >> 2);
+		part<PART>(v[x1]) = r[16 * x1];
+	}
+}
+
+__device__ __forceinline__ void fft4096_fwd_split(double2 v[16], Lds4352d &L, const double2 *__restrict__ tw) {
+	const uint32_t t = threadIdx.x, lo = t & 15, hi = t >> 4;
+	const uint32_t w = t >> 6, g = hi & 3;
+	dft16_t2n<-1>(v);
+	twiddle_run<false, true, false>(v, tw, 0, t);
+	// exchange 1 (cross-wave), real then imaginary parts
+	xch1_fwd_write<0>(v, L.a, t);
+	lds_barrier();
+	xch1_fwd_read<0>(v, L.a, lo, hi);
+	lds_barrier();
+	xch1_fwd_write<1>(v, L.a, t);
+	lds_barrier();
+	xch1_fwd_read<1>(v, L.a, lo, hi);
+	dft16_n2t<-1>(v);
+	twiddle_run<false, true, true>(v, tw, 0, 16 * lo);
+	// exchange 2 (within the 16-lane group)
+	double *grp = L.a + 1088 * w + 272 * g;
+	wave_sync();
+#pragma unroll
+	for (int k1 = 0; k1 < 16; ++k1) grp[lo + 17 * k1] = v[pos(k1)].x;
+	wave_sync();
+#pragma unroll
+	for (int i = 0; i < 16; ++i) {
+		const int x0 = 4 * (i & 3) + (i >> 2);
+		v[x0].x = grp[x0 + 17 * lo];
+	}
+	wave_sync();
+	// the real pass touched only the .x halves: v[pos(k1)].y still hold the
+	// imaginary parts in the old order
+#pragma unroll
+	for (int k1 = 0; k1 < 16; ++k1) grp[lo + 17 * k1] = v[pos(k1)].y;
+	wave_sync();
+#pragma unroll
+	for (int i = 0; i < 16; ++i) {
+		const int x0 = 4 * (i & 3) + (i >> 2);
+		v[x0].y = grp[x0 + 17 * lo];
+	}
+	dft16_n2t<-1>(v);
+}
+
+__device__ __forceinline__ void fft4096_inv_split(double2 v[16], Lds4352d &L, const double2 *__restrict__ tw) {
+	const uint32_t t = threadIdx.x, lo = t & 15, hi = t >> 4;
+	const uint32_t w = t >> 6, g = hi & 3;
+	dft16_t2n<+1>(v);
+	twiddle_run<true, true, false>(v, tw, 0, 16 * lo);
+	// exchange 2' (within the 16-lane group)
+	double *grp = L.a + 1088 * w + 272 * g;
+	wave_sync();
+#pragma unroll
+	for (int x0 = 0; x0 < 16; ++x0) grp[x0 + 17 * lo] = v[x0].x;
+	wave_sync();
+#pragma unroll
+	for (int i = 0; i < 16; ++i) {
+		const int k1 = 4 * (i & 3) + (i >> 2);
+		v[k1].x = grp[lo + 17 * k1];
+	}
+	wave_sync();
+#pragma unroll
+	for (int x0 = 0; x0 < 16; ++x0) grp[x0 + 17 * lo] = v[x0].y;
+	wave_sync();
+#pragma unroll
+	for (int i = 0; i < 16; ++i) {
+		const int k1 = 4 * (i & 3) + (i >> 2);
+		v[k1].y = grp[lo + 17 * k1];
+	}
+	dft16_n2t<+1>(v);
+	twiddle_run<true, false, true>(v, tw, lo * hi, 16 * hi);
+	// exchange 1' (cross-wave), written into the wavefront's own region
+	double *e1w = L.a + lo + 256 * hi + 64 * w + 16 * (hi & 3);
+	const double *e1r = L.a + t;
+	wave_sync();
+#pragma unroll
+	for (int x1 = 0; x1 < 16; ++x1) e1w[16 * x1] = v[pos(x1)].x;
+	lds_barrier();
+#pragma unroll
+	for (int i = 0; i < 16; ++i) {
+		const int k0 = 4 * (i & 3) + (i >> 2);
+		v[k0].x = e1r[256 * k0 + 64 * (k0 >> 2) + 16 * (k0 & 3)];
+	}
+	lds_barrier();
+#pragma unroll
+	for (int x1 = 0; x1 < 16; ++x1) e1w[16 * x1] = v[pos(x1)].y;
+	lds_barrier();
+#pragma unroll
+	for (int i = 0; i < 16; ++i) {
+		const int k0 = 4 * (i & 3) + (i >> 2);
+		v[k0].y = e1r[256 * k0 + 64 * (k0 >> 2) + 16 * (k0 & 3)];
+	}
+	dft16_n2t<+1>(v);
+}
