@@ -23,6 +23,26 @@ def test_bench_refuses_more_than_one_gpu():
     assert "one GPU" in msg["error"]
 
 
+def test_traffic_table_keys_and_profiles():
+    """profiles/traffic.json: every key is a kernel description reduced by
+    bench.traffic_key (so a description that differs only in wave shape, item
+    count or projection form finds it), every entry's bytes follow from its
+    FETCH_SIZE / WRITE_SIZE and the copy calibration, and its profile exists."""
+    sys.path.insert(0, str(ROOT))
+    import bench
+    t = json.loads((ROOT / "profiles" / "traffic.json").read_text())
+    assert abs(t["fetch_scale"] - 2.0) < 1e-3 and t["write_scale"] == 1.0
+    for key, e in t["kernels"].items():
+        assert bench.traffic_key(key) == key, key
+        want = 1024 * (e["fetch_kib"] * t["fetch_scale"] + e["write_kib"] * t["write_scale"])
+        assert abs(e["hbm_bytes"] - want) <= 1e-6 * want, key
+        assert (ROOT / "profiles" / e["profile"]).exists(), e["profile"]
+    desc = ("model=DNKG n=1024 copies=16384 kernel=wave lanes_per_chain=64 sites_per_lane=16 exact=1 "
+            "force=cubic_unit")
+    assert bench.traffic_bytes(desc) == t["kernels"][bench.traffic_key(desc)]["hbm_bytes"]
+    assert bench.traffic_key(desc + " projection=mfma_wide_f64") == bench.traffic_key(desc)
+
+
 @pytest.mark.gpu
 def test_bench_json_contract(gpu):
     p = run_bench("--steps", "3", "--warmup", "1", "--cpu-seconds", "1")
