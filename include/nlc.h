@@ -237,6 +237,15 @@ int nlc_batch(nlc_ctx *ctx, uint32_t steps, double norm, double *linenergies, do
 int nlc_batch_submit(nlc_ctx *ctx, uint32_t steps, double norm);
 int nlc_batch_collect(nlc_ctx *ctx, double *linenergies, double *entropies);
 
+/* How nlc_batch_submit issues a batch: 0 = replay of the batch's cached
+ * hipGraph (the default), 1 = direct launches of the same kernels (the same
+ * results).  A graph replay costs a few microseconds more than the launches it
+ * holds, which matters for batches of tens of microseconds (64 x 1024 chains);
+ * nlc_autotune times both and sets the faster.  No counterpart in the
+ * reference (its batches are direct launches and cuFFT calls).  NLC_ESTATE
+ * while batches are queued. */
+int nlc_set_batch_mode(nlc_ctx *ctx, int direct);
+
 /* Times every applicable step-kernel shape (wave and workgroup shapes lanes x
  * sites and the split kernel for uniform ensembles; sites per lane of the persistent kernel
  * for ragged ones) on the context's own state for trial_steps steps, keeps the

@@ -91,6 +91,7 @@ _sig("nlc_ragged_class_sets", C.c_int, _ctx_p, _u32p, C.c_uint32)
 _sig("nlc_batch", C.c_int, _ctx_p, C.c_uint32, C.c_double, _dp, _dp)
 _sig("nlc_batch_submit", C.c_int, _ctx_p, C.c_uint32, C.c_double)
 _sig("nlc_batch_collect", C.c_int, _ctx_p, _dp, _dp)
+_sig("nlc_set_batch_mode", C.c_int, _ctx_p, C.c_int)
 BATCH_SLOTS = 4  # NLC_BATCH_SLOTS
 _sig("nlc_autotune", C.c_int, _ctx_p, C.c_uint32, C.c_char_p, C.c_size_t)
 _sig("nlc_ic_ragged_lengths", C.c_int, C.c_uint32, _u32p, C.c_uint32, C.c_uint64, _u32p)
@@ -313,6 +314,10 @@ class Ensemble:
         e, s = np.zeros(self._nlin), np.zeros(self._nent)
         _check(lib.nlc_batch_collect(self._ctx, _ptr(e), _ptr(s)), self._ctx)
         return e, s
+
+    def set_batch_mode(self, direct: bool) -> None:
+        """Queued batches as graph replays (False, the default) or direct launches."""
+        _check(lib.nlc_set_batch_mode(self._ctx, int(bool(direct))), self._ctx)
 
     def autotune(self, trial_steps: int = 20) -> str:
         buf = C.create_string_buffer(1024)

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -124,6 +125,10 @@ struct nlc_ctx {
 		uint32_t steps = 0;
 		double norm = 0;
 	} graphs[NLC_BATCH_SLOTS + 1];
+	// queued batches as direct kernel launches instead of graph replays (the
+	// autotuner times both: a replay costs a few microseconds more than the
+	// launches it holds, which matters only for batches of tens of microseconds)
+	bool batch_direct = false;
 
 	std::string err, tune_report;
 
@@ -1223,11 +1228,24 @@ int nlc_batch_submit(nlc_ctx *c, uint32_t steps, double norm) {
 		for (hipEvent_t &ev : c->slot_ev) NLC_HIP(c, hipEventCreateWithFlags(&ev, hipEventDisableTiming));
 	}
 	const uint32_t slot = c->slot_head;
-	NLC_TRY(ensure_batch_graph(c, slot, steps, norm, c->h_slots + c->results_len() * slot));
-	NLC_HIP(c, hipGraphLaunch(c->graphs[slot].exec, c->stream));
+	double *dst = c->h_slots + c->results_len() * slot;
+	if (c->batch_direct) {
+		NLC_TRY(enqueue_move(c, steps));
+		NLC_TRY(enqueue_energy(c, norm, dst));
+	} else {
+		NLC_TRY(ensure_batch_graph(c, slot, steps, norm, dst));
+		NLC_HIP(c, hipGraphLaunch(c->graphs[slot].exec, c->stream));
+	}
 	NLC_HIP(c, hipEventRecord(c->slot_ev[slot], c->stream));
 	c->slot_head = (slot + 1) % NLC_BATCH_SLOTS;
 	++c->slot_count;
+	return NLC_OK;
+}
+
+int nlc_set_batch_mode(nlc_ctx *c, int direct) {
+	if (!c || (direct != 0 && direct != 1)) return NLC_EARG;
+	if (c->slot_count) return c->fail(NLC_ESTATE, "nlc_set_batch_mode: %u batches are queued", c->slot_count);
+	c->batch_direct = direct == 1;
 	return NLC_OK;
 }
 
@@ -1255,6 +1273,47 @@ int nlc_chain_energies(nlc_ctx *c, double *energies, double *norms) {
 		energies[i] = e[j];
 		if (norms && c->p.model == NLC_DNLS) norms[i] = nr[j];
 	}
+	return NLC_OK;
+}
+
+// Queued batches (nlc_batch_submit): time a pipeline of batches of
+// trial_steps, three in flight as the CLI keeps them, as graph replays and as
+// direct launches, on a copy of the state, and keep the faster.  Skipped while
+// batches are queued.
+static int tune_submission(nlc_ctx *c, uint32_t steps) {
+	if (c->slot_count) return NLC_OK;
+	double2 *backup = nullptr;
+	NLC_TRY(dev_alloc(c, &backup, c->total_sites, "autotune backup"));
+	int rc = NLC_OK;
+	if (hipMemcpyAsync(backup, c->state, c->state_bytes, hipMemcpyDeviceToDevice, c->stream) != hipSuccess)
+		rc = c->fail(NLC_EHIP, "autotune setup failed");
+	double us[2] = {0, 0};
+	constexpr int kBatches = 16;
+	for (int mode = 0; mode < 2 && !rc; ++mode) {
+		c->batch_direct = mode == 1;
+		for (int rep = 0; rep < 2 && !rc; ++rep) { // the first run builds the graphs and warms up
+			if (hipStreamSynchronize(c->stream) != hipSuccess) rc = c->fail(NLC_EHIP, "autotune sync failed");
+			const auto t0 = std::chrono::steady_clock::now();
+			for (int i = 0; i < kBatches && !rc; ++i) {
+				if (c->slot_count == 3) rc = nlc_batch_collect(c, nullptr, nullptr);
+				if (!rc) rc = nlc_batch_submit(c, steps, 0.0);
+			}
+			while (c->slot_count && !rc) rc = nlc_batch_collect(c, nullptr, nullptr);
+			us[mode] = std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count() /
+			           kBatches;
+		}
+	}
+	while (c->slot_count) (void) nlc_batch_collect(c, nullptr, nullptr);
+	if (hipMemcpyAsync(c->state, backup, c->state_bytes, hipMemcpyDeviceToDevice, c->stream) != hipSuccess && !rc)
+		rc = c->fail(NLC_EHIP, "autotune restore failed");
+	(void) hipStreamSynchronize(c->stream);
+	(void) hipFree(backup);
+	if (rc) return rc;
+	c->batch_direct = us[1] < us[0];
+	char buf[96];
+	snprintf(buf, sizeof buf, "; queued batches: graph %.1f us%s direct %.1f us%s", us[0], c->batch_direct ? "" : "*",
+	         us[1], c->batch_direct ? "*" : "");
+	c->tune_report += buf;
 	return NLC_OK;
 }
 
@@ -1414,6 +1473,7 @@ int nlc_autotune(nlc_ctx *c, uint32_t trial_steps, char *report, size_t len) {
 		}
 	}
 	if (c->p.model == NLC_DDNKG && c->p.projection == NLC_PROJ_AUTO) NLC_TRY(tune_projection(c));
+	NLC_TRY(tune_submission(c, trial_steps));
 	if (report && len) snprintf(report, len, "%s", c->tune_report.c_str());
 	return NLC_OK;
 }

@@ -121,11 +121,14 @@ def test_graph_batches_match_eager(gpu, kind):
             e.chain_energies()  # unrelated eager work between replays
 
 
+@pytest.mark.parametrize("direct", [False, True])
 @pytest.mark.parametrize("ragged", [False, True])
-def test_queued_batches_match_synchronous(gpu, ragged):
-    """nlc_batch_submit / nlc_batch_collect: results come back in submission
-    order, equal to nlc_batch's; the slot ring wraps; the state after the
-    queue is the state after all queued batches; the queue depth is enforced."""
+def test_queued_batches_match_synchronous(gpu, ragged, direct):
+    """nlc_batch_submit / nlc_batch_collect, as graph replays and as direct
+    launches: results come back in submission order, equal to nlc_batch's; the
+    slot ring wraps; the state after the queue is the state after all queued
+    batches; the queue depth is enforced; the mode cannot change while batches
+    are queued."""
     N = gpu
     if ragged:
         lengths = N.ic_ragged_lengths(100, [32, 64, 256], 3)
@@ -135,6 +138,7 @@ def test_queued_batches_match_synchronous(gpu, ragged):
         s0 = N.ic_plane_waves(64, 1024, 0.0, 1, 5, 0.1, 1)
         mk = lambda: N.Ensemble(N.FPUT, 64, 1024, 0.1, beta=1.0)  # noqa: E731
     with mk() as q, mk() as s:
+        q.set_batch_mode(direct)
         q.upload(s0)
         s.upload(s0)
         want = [s.batch(20) for _ in range(10)]
@@ -143,6 +147,9 @@ def test_queued_batches_match_synchronous(gpu, ragged):
             q.batch_submit(20)
         with pytest.raises(N.NlcError):
             q.batch_submit(20)
+        with pytest.raises(N.NlcError) as ex:
+            q.set_batch_mode(not direct)
+        assert ex.value.code == N.NLC_ESTATE
         got.append(q.batch_collect())
         for i in range(6):  # keep three queued while collecting, wrapping the ring
             q.batch_submit(20)
@@ -174,7 +181,7 @@ def test_autotune_keeps_state_and_parity(gpu):
         ens.upload(s0)
         report = ens.autotune(10)
         banner(f"autotune: {report} -> {ens.describe()}")
-        assert "*" in report
+        assert "*" in report and "queued batches: graph" in report
         assert np.array_equal(ens.download(), s0)
         ens.move(50)
         got = ens.download()
