@@ -492,6 +492,62 @@ k_reduce_results(const double *__restrict__ partial, uint32_t rows, uint32_t n, 
 	results_body(e, n, norm, a0is0, idx, nidx, out);
 }
 
+// k_reduce_results for n <= 64 modes: the four wavefronts sum every fourth
+// row (the row split of k_reduce_results at n = 64), wavefront 0 adds the four
+// parts in order and does the normalisation and the entropy sums with one mode
+// per lane, reduced by lane exchanges (pairs i, i + 32, then + 16, ...: the
+// additions of k_reduce_results' tree, which holds at most 64 nonzero values)
+// instead of eight workgroup barriers.
+__global__ void __launch_bounds__(kThreads)
+k_reduce_results64(const double *__restrict__ partial, uint32_t rows, uint32_t n, const double *__restrict__ scale,
+                   double *__restrict__ e_raw, double norm, int a0is0, const uint32_t *__restrict__ idx,
+                   uint32_t nidx, double *__restrict__ out) {
+	__shared__ double part[kThreads / 64][64];
+	const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+	double sum = 0.0;
+	if (lane < n) {
+#pragma unroll 8
+		for (uint32_t g = wave; g < rows; g += kThreads / 64) sum += partial[size_t(g) * n + lane];
+	}
+	part[wave][lane] = sum;
+	__syncthreads();
+	if (wave != 0) return;
+	double t = 0.0;
+#pragma unroll
+	for (uint32_t q = 0; q < kThreads / 64; ++q) t += part[q][lane];
+	if (lane < n) {
+		t = scale ? t * scale[lane] : t;
+		e_raw[lane] = t;
+		out[lane] = (a0is0 && lane == 0) ? 0.0 : t * norm;
+	}
+	part[1][lane] = t; // this wavefront's LDS writes are visible to its own later reads
+	__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+	__builtin_amdgcn_wave_barrier();
+	__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+	const uint32_t first = a0is0 ? 1 : 0;
+	const uint32_t cnt = nidx ? nidx : n - first;
+	double s_e = 0, s_log = 0, s_elog = 0;
+	if (lane < cnt) {
+		const uint32_t k = nidx ? idx[lane] : first + lane;
+		const double e = (a0is0 && k == 0) ? 0.0 : part[1][k] * norm;
+		const double lg = log(e);
+		s_e = e;
+		s_log = lg;
+		s_elog = e * lg;
+	}
+#pragma unroll
+	for (int w = 32; w > 0; w >>= 1) {
+		s_e += __shfl_xor(s_e, w, 64);
+		s_log += __shfl_xor(s_log, w, 64);
+		s_elog += __shfl_xor(s_elog, w, 64);
+	}
+	if (lane == 0) {
+		const double normalization = double(cnt) / s_e, lognorm = log(normalization);
+		out[n] = -(s_log + double(cnt) * lognorm);
+		out[n + 1] = normalization * (s_elog + lognorm * s_e);
+	}
+}
+
 // One workgroup per chain: total energy (and DNLS norm), fixed-order reduction.
 __global__ void __launch_bounds__(kThreads)
 k_chain_energy(int model, const double2 *__restrict__ state, uint32_t n, uint32_t copies, StepArgs a,
@@ -689,8 +745,12 @@ hipError_t launch_reduce_results(hipStream_t st, const double *partial, uint32_t
 		partial = scratch;
 		grid = chunks;
 	}
-	hipLaunchKernelGGL(k_reduce_results, dim3(1), dim3(kThreads), 0, st, partial, grid, n, scale, e_raw, norm, a0is0,
-	                   idx, nidx, out);
+	if (n <= 64)
+		hipLaunchKernelGGL(k_reduce_results64, dim3(1), dim3(kThreads), 0, st, partial, grid, n, scale, e_raw, norm,
+		                   a0is0, idx, nidx, out);
+	else
+		hipLaunchKernelGGL(k_reduce_results, dim3(1), dim3(kThreads), 0, st, partial, grid, n, scale, e_raw, norm,
+		                   a0is0, idx, nidx, out);
 	return hipGetLastError();
 }
 
