@@ -1303,7 +1303,10 @@ static int tune_submission(nlc_ctx *c, uint32_t steps) {
 			           kBatches;
 		}
 	}
-	while (c->slot_count) (void) nlc_batch_collect(c, nullptr, nullptr);
+	if (c->slot_count) { // only after an error: drop what is still queued
+		(void) hipStreamSynchronize(c->stream);
+		c->slot_count = 0;
+	}
 	if (hipMemcpyAsync(c->state, backup, c->state_bytes, hipMemcpyDeviceToDevice, c->stream) != hipSuccess && !rc)
 		rc = c->fail(NLC_EHIP, "autotune restore failed");
 	(void) hipStreamSynchronize(c->stream);
