@@ -249,6 +249,14 @@ __device__ __forceinline__ void wide_loop(const double2 *__restrict__ state, uin
 	As[0][sc * kWPad + ss + 1] = x1;
 	double b[kWCols][4];
 	load_b(0, b);
+	// wait for the first operands here: otherwise the wait for them, placed at
+	// the top of the loop, also holds every later step until that step's own
+	// prefetch arrives (the loop body's operand registers are written by
+	// loads in the first step and by copies after)
+#pragma unroll
+	for (int cb = 0; cb < kWCols; ++cb)
+#pragma unroll
+		for (int t = 0; t < 4; ++t) asm volatile("" : "+v"(b[cb][t]));
 	lds_fence_barrier();
 	const double2 *arow = &As[0][(32u * uint32_t(half) + uint32_t(r16)) * kWPad + 4u * uint32_t(g)];
 	uint32_t buf = 0;
