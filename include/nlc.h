@@ -73,8 +73,8 @@ typedef enum { NLC_KERNEL_AUTO = 0, NLC_KERNEL_WAVE = 1, NLC_KERNEL_SPLIT = 2, N
 
 /* dDNKG energy projection (the two cublasDgemm of dDNKG.cpp:118-135 and the
  * reduction of dDNKG.cu:237-265, fused).  NLC_PROJ_AUTO = FP64 matrix cores
- * until nlc_autotune times the three FP64 forms (matrix cores with 64 x 128 or
- * 64 x 256 tiles, vector ALU).  NLC_PROJ_SPLIT_BF16 (chain
+ * with 64 x 256 tiles until nlc_autotune times the three FP64 forms (matrix
+ * cores with 64 x 128 or 64 x 256 tiles, vector ALU).  NLC_PROJ_SPLIT_BF16 (chain
  * lengths divisible by 32, chosen explicitly, never by the autotuner) runs the
  * products on BF16 matrix cores with three-term splits of both operands and
  * FP32 accumulation: mode energies differ from the FP64 forms at the 1e-7

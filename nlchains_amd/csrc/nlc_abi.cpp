@@ -887,10 +887,10 @@ int nlc_create(nlc_ctx **out, const nlc_params *p) {
 	nlc_ctx *c = new (std::nothrow) nlc_ctx;
 	if (!c) return reject(NLC_ENOMEM, "context allocation failed");
 	c->p = *p;
-	c->proj_variant = p->projection == NLC_PROJ_VALU_F64       ? nlc::kProjValu
-	                  : p->projection == NLC_PROJ_SPLIT_BF16   ? nlc::kProjBf16
-	                  : p->projection == NLC_PROJ_MFMA_F64_WIDE ? nlc::kProjMfmaWide
-	                                                            : nlc::kProjMfma;
+	c->proj_variant = p->projection == NLC_PROJ_VALU_F64   ? nlc::kProjValu
+	                  : p->projection == NLC_PROJ_SPLIT_BF16 ? nlc::kProjBf16
+	                  : p->projection == NLC_PROJ_MFMA_F64   ? nlc::kProjMfma
+	                                                         : nlc::kProjMfmaWide; // auto: the faster measured form
 	c->n = p->chain_length;
 	c->copies = p->copies;
 	c->total_sites = uint64_t(c->n) * c->copies;
