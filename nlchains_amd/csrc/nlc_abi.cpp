@@ -90,7 +90,7 @@ struct nlc_ctx {
 	double *masses_d = nullptr;
 	double *eigvec = nullptr;  // dDNKG V, column-major
 	int proj_variant = nlc::kProjMfma; // dDNKG projection: FP64 matrix cores, vector ALU or split BF16
-	uint16_t *vplanes = nullptr, *splanes = nullptr; // split BF16 form: V and state splits
+	uint16_t *vplanes = nullptr; // split BF16 form: the eigenvector splits
 	double *split_q[2] = {nullptr, nullptr}, *split_p[2] = {nullptr, nullptr}; // ping-pong pair
 	double *energy_d = nullptr, *norm_d = nullptr;
 	double2 *dnls_table = nullptr; // DNLS Fourier multipliers
@@ -348,7 +348,7 @@ void free_all(nlc_ctx *c) {
 	     {(void *) c->state, (void *) c->omega, (void *) c->tw, (void *) c->partial, (void *) c->reduce_scratch,
 	      (void *) c->e_raw, (void *) c->modes, (void *) c->mp2, (void *) c->masses_d,
 	      (void *) c->eigvec, (void *) c->split_q[0], (void *) c->split_p[0], (void *) c->split_q[1], (void *) c->split_p[1], (void *) c->energy_d, (void *) c->norm_d,
-	      (void *) c->dnls_table, (void *) c->fft_scratch, (void *) c->blue_table, (void *) c->items, (void *) c->queue, (void *) c->vplanes, (void *) c->splanes,
+	      (void *) c->dnls_table, (void *) c->fft_scratch, (void *) c->blue_table, (void *) c->items, (void *) c->queue, (void *) c->vplanes,
 	      (void *) c->tile_buf, (void *) c->dnls_tw32, (void *) c->dnls_mult32})
 		if (ptr) (void) hipFree(ptr);
 	if (c->h_results) (void) hipHostFree(c->h_results);
@@ -466,7 +466,6 @@ int setup_common(nlc_ctx *c) {
 			std::vector<uint16_t> vp(size_t(3) * n * n);
 			nlc::ddnkg_split_host(V.data(), size_t(n) * n, vp.data());
 			NLC_TRY(dev_alloc(c, &c->vplanes, vp.size(), "eigenvector splits"));
-			NLC_TRY(dev_alloc(c, &c->splanes, size_t(6) * n * c->copies, "state splits"));
 			NLC_HIP(c, hipMemcpy(c->vplanes, vp.data(), sizeof(uint16_t) * vp.size(), hipMemcpyHostToDevice));
 		}
 	}
@@ -746,7 +745,7 @@ int enqueue_energy(nlc_ctx *c, double norm = 0.0, double *dst = nullptr) {
 			break;
 		case NLC_DDNKG:
 			NLC_HIP(c, nlc::launch_energy_ddnkg(c->stream, st, k.n, k.count, c->eigvec, c->omega, c->partial,
-			                                    c->proj_variant, c->vplanes, c->splanes));
+			                                    c->proj_variant, c->vplanes));
 			break;
 		case NLC_DNLS:
 			NLC_HIP(c, nlc::launch_energy_dnls(c->stream, st, k.n, k.count, c->tw + k.tab0, c->partial, k.rows,

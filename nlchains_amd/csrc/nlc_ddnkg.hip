@@ -9,16 +9,20 @@
 // Design: one fused kernel reads the planar shard directly (no transpose, no
 // projection matrices in HBM) and its epilogue squares, applies omega and
 // reduces 64 copies into one partial row; the per-tile rows are reduced in a
-// fixed order afterwards.  Two forms of the product:
-//  * k_proj_mfma (default): FP64 matrix cores, v_mfma_f64_16x16x4f64.  A
-//    wavefront owns 64 copies x 32 modes; the K index of the instruction runs
-//    over sites in a lane-private order (lane group g = lane/16 takes sites
-//    4g..4g+3 of each 16-site chunk, one per instruction), the same order for
-//    the state operand and for V, so every lane loads 64 contiguous bytes of its
-//    copy's (q,p) pairs and 32 contiguous bytes of its mode's eigenvector, with
-//    no LDS staging and no transpose;
+// fixed order afterwards.  Forms of the product:
+//  * k_proj_mfma: FP64 matrix cores, v_mfma_f64_16x16x4f64.  A wavefront owns
+//    64 copies x 32 modes; the K index of the instruction runs over sites in a
+//    lane-private order (lane group g = lane/16 takes sites 4g..4g+3 of each
+//    16-site chunk, one per instruction), the same order for the state operand
+//    and for V, so every lane loads 64 contiguous bytes of its copy's (q,p)
+//    pairs and 32 contiguous bytes of its mode's eigenvector, with no LDS
+//    staging and no transpose;
+//  * k_proj_mfma_wide (default): the same products on 64-copy x 256-mode
+//    workgroup tiles, the state tile shared by the wavefronts through LDS;
 //  * k_proj_energy: the FP64 vector-ALU form (64x64 tile staged through LDS),
-//    kept as the autotuner's alternative.
+//    kept as the autotuner's alternative;
+//  * k_proj_bf16x3_wide: BF16 matrix cores with three-term operand splits
+//    (mode energies within about 1e-8 of FP64; never chosen automatically).
 #include <cstring>
 
 #include "nlc_internal.hpp"
@@ -350,12 +354,21 @@ k_proj_mfma_wide(const double2 *__restrict__ state, uint32_t n, uint32_t copies,
 // hh + (hm + mh + hl + lh + mm), on v_mfma_f32_16x16x32_bf16 with the leading
 // term and the corrections in separate FP32 accumulators (the corrections are
 // 2^-8 smaller, so their rounding is negligible), summed in FP64 at the end.
-// The state is split once per pass by k_split_state into six planes
-// [qh qm ql ph pm pl][copy][site]; the eigenvectors once on the host.  Requires
-// n % 32 == 0.  Workgroup: 64 copies x 64 modes, wavefront (wr, wc) owns copies
-// 32 wr + [0, 32) and modes 32 wc + [0, 32); operand layout of the instruction:
-// lane l supplies A[l % 16][8 (l / 16) + e] and B[8 (l / 16) + e][l % 16],
-// e = 0..7, and holds D[4 (l / 16) + r][l % 16] in accumulator element r.
+// The eigenvectors are split once on the host into planes [h m l][mode][site];
+// the state is split inside the kernel.  Requires n % 32 == 0.  Operand layout
+// of the instruction: lane l supplies A[l % 16][8 (l / 16) + e] and
+// B[8 (l / 16) + e][l % 16], e = 0..7, and holds D[4 (l / 16) + r][l % 16] in
+// accumulator element r.
+//
+// k_proj_bf16x3_wide: a 64-copy x 256-mode workgroup tile, eight wavefronts of
+// 32 copies x 64 modes, with both operands staged through LDS one 32-site step
+// ahead.  The state is read in FP64 and split in registers while the matrix
+// cores run the step before, so there is no split pass and no split planes in
+// HBM; the eigenvector splits are read once per workgroup.  LDS per buffer:
+// state splits [qh qm ql ph pm pl][64 copies][32 sites] (24 KB) and
+// eigenvector splits [h m l][256 modes][32 sites] (48 KB); two buffers and the
+// copy reduction take 148 KB, one workgroup per CU.  Each fragment read is 64
+// lanes x 16 bytes (row r16 at a 64-byte stride, lane group g at 16 bytes).
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
@@ -366,61 +379,83 @@ __device__ __forceinline__ void split3(double x, __bf16 &h, __bf16 &m, __bf16 &l
 	l = __bf16(float(r1 - double(float(m))));
 }
 
-__global__ void __launch_bounds__(256)
-k_split_state(const double2 *__restrict__ state, size_t sites, __bf16 *__restrict__ planes) {
-	const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-	if (i >= sites) return;
-	const double2 v = state[i];
-	__bf16 h, m, l;
-	split3(v.x, h, m, l);
-	planes[i] = h, planes[sites + i] = m, planes[2 * sites + i] = l;
-	split3(v.y, h, m, l);
-	planes[3 * sites + i] = h, planes[4 * sites + i] = m, planes[5 * sites + i] = l;
-}
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 
-__global__ void __launch_bounds__(256)
-k_proj_bf16x3(const __bf16 *__restrict__ planes, uint32_t n, uint32_t copies, const __bf16 *__restrict__ vplanes,
-              const double *__restrict__ omega, double *__restrict__ partial) {
-	__shared__ double red[2][64];
-	const uint32_t mode_blocks = (n + 63) / 64;
-	const uint32_t xcd = blockIdx.x & 7, rest = blockIdx.x >> 3;
-	const uint32_t bx = rest % mode_blocks, by = (rest / mode_blocks) * 8u + xcd;
-	if (by * 64u >= copies) return; // padding workgroup (uniform over the group)
-	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, wr = wave >> 1, wc = wave & 1;
-	const int r16 = lane & 15, g = lane >> 4;
-	const size_t sites = size_t(n) * copies, vsz = size_t(n) * n;
-	const uint32_t c_base = by * 64u + 32u * wr + r16, j_base = bx * 64u + 32u * wc + r16;
+constexpr int kBModes = 256;                                                 // modes per workgroup
+constexpr uint32_t kBfA = 6u * 64u * 64u, kBfB = 3u * uint32_t(kBModes) * 64u; // bytes per buffer
+constexpr uint32_t kBfLds = 2u * (kBfA + kBfB) + 2u * uint32_t(kBModes) * uint32_t(sizeof(double));
 
-	f32x4 hq[2][2], cq[2][2], hp[2][2], cp[2][2];
-#pragma unroll
-	for (int rb = 0; rb < 2; ++rb)
-#pragma unroll
-		for (int cb = 0; cb < 2; ++cb)
-			hq[rb][cb] = cq[rb][cb] = hp[rb][cb] = cp[rb][cb] = f32x4{0.f, 0.f, 0.f, 0.f};
+// 16-byte piece c of LDS row r sits at piece c ^ swz(r): a fragment read puts
+// 16 rows at a 64-byte stride in one access, which would otherwise hit each
+// bank four times
+__device__ __forceinline__ uint32_t swz(uint32_t row) { return (row >> 2) & 3u; }
 
-	const bf16x8 zero = bf16x8{};
-	for (uint32_t k0 = 0; k0 < n; k0 += 32) {
-		const uint32_t site = k0 + 8u * uint32_t(g);
-		bf16x8 a[2][6], b[2][3];
+template <bool FULL>
+__device__ __forceinline__ void bf16_wide_loop(const double2 *__restrict__ state, uint32_t n, uint32_t copies,
+                                               const __bf16 *__restrict__ vplanes, uint32_t c0, uint32_t jw0,
+                                               int half, int mg, int r16, int g, unsigned char *lds,
+                                               f32x4 (&hq)[2][4], f32x4 (&cq)[2][4], f32x4 (&hp)[2][4],
+                                               f32x4 (&cp)[2][4]) {
+	const int tid = threadIdx.x;
+	const size_t vsz = size_t(n) * n;
+	// staging: state copy tid / 8, sites 4 (tid % 8) + [0, 4); eigenvector row
+	// tid / 2, sites 16 (tid % 2) + [0, 16), two 16-byte pieces per plane
+	const uint32_t sc = uint32_t(tid >> 3), ss = 4u * uint32_t(tid & 7);
+	const uint32_t vr = uint32_t(tid >> 1), vs = 16u * uint32_t(tid & 1);
+	const bool s_ok = FULL || c0 + sc < copies;
+	const bool v_ok = FULL || jw0 + vr < n;
+	const double2 *srow = state + size_t(s_ok ? c0 + sc : 0u) * n + ss;
+	const __bf16 *vrow = vplanes + size_t(v_ok ? jw0 + vr : 0u) * n + vs;
+	auto load = [&](uint32_t k0, double2 (&x)[4], uint4 (&v)[3][2]) {
 #pragma unroll
-		for (int rb = 0; rb < 2; ++rb) {
-			const uint32_t c = c_base + 16u * rb;
+		for (int t = 0; t < 4; ++t) x[t] = s_ok ? srow[k0 + t] : make_double2(0.0, 0.0);
 #pragma unroll
-			for (int pl = 0; pl < 6; ++pl)
-				a[rb][pl] = c < copies ? *(const bf16x8 *) (planes + pl * sites + size_t(c) * n + site) : zero;
+		for (int pl = 0; pl < 3; ++pl)
+#pragma unroll
+			for (int u = 0; u < 2; ++u)
+				v[pl][u] = v_ok ? *(const uint4 *) (vrow + pl * vsz + k0 + 8u * uint32_t(u)) : make_uint4(0, 0, 0, 0);
+	};
+	auto store = [&](unsigned char *buf, const double2 (&x)[4], const uint4 (&v)[3][2]) {
+		bf16x4 h[2], m[2], l[2];
+#pragma unroll
+		for (int t = 0; t < 4; ++t) {
+			__bf16 a, b, c;
+			split3(x[t].x, a, b, c);
+			h[0][t] = a, m[0][t] = b, l[0][t] = c;
+			split3(x[t].y, a, b, c);
+			h[1][t] = a, m[1][t] = b, l[1][t] = c;
 		}
+		unsigned char *srow_l = buf + sc * 64u + 16u * ((ss >> 3) ^ swz(sc)) + 2u * (ss & 7u);
 #pragma unroll
-		for (int cb = 0; cb < 2; ++cb) {
-			const uint32_t j = j_base + 16u * cb;
-#pragma unroll
-			for (int pl = 0; pl < 3; ++pl)
-				b[cb][pl] = j < n ? *(const bf16x8 *) (vplanes + pl * vsz + size_t(j) * n + site) : zero;
+		for (int part = 0; part < 2; ++part) {
+			*(bf16x4 *) (srow_l + (3 * part + 0) * 4096) = h[part];
+			*(bf16x4 *) (srow_l + (3 * part + 1) * 4096) = m[part];
+			*(bf16x4 *) (srow_l + (3 * part + 2) * 4096) = l[part];
 		}
+		unsigned char *vrow_l = buf + kBfA + vr * 64u;
+#pragma unroll
+		for (int pl = 0; pl < 3; ++pl)
+#pragma unroll
+			for (int u = 0; u < 2; ++u)
+				*(uint4 *) (vrow_l + pl * (kBModes * 64) + 16u * (((vs >> 3) + uint32_t(u)) ^ swz(vr))) = v[pl][u];
+	};
+	// rows 16 apart have the same swizzle, so one offset serves every rb and cb
+	const uint32_t a_off = (32u * uint32_t(half) + uint32_t(r16)) * 64u + 16u * (uint32_t(g) ^ swz(uint32_t(r16)));
+	const uint32_t b_off = kBfA + (64u * uint32_t(mg) + uint32_t(r16)) * 64u + 16u * (uint32_t(g) ^ swz(uint32_t(r16)));
+	auto compute = [&](const unsigned char *buf) {
+		bf16x8 a[2][6];
 #pragma unroll
 		for (int rb = 0; rb < 2; ++rb)
 #pragma unroll
-			for (int cb = 0; cb < 2; ++cb) {
-				const bf16x8 *q = a[rb], *p = a[rb] + 3, *v = b[cb];
+			for (int pl = 0; pl < 6; ++pl) a[rb][pl] = *(const bf16x8 *) (buf + a_off + rb * 1024 + pl * 4096);
+#pragma unroll
+		for (int cb = 0; cb < 4; ++cb) {
+			bf16x8 v[3];
+#pragma unroll
+			for (int pl = 0; pl < 3; ++pl) v[pl] = *(const bf16x8 *) (buf + b_off + cb * 1024 + pl * (kBModes * 64));
+#pragma unroll
+			for (int rb = 0; rb < 2; ++rb) {
+				const bf16x8 *q = a[rb], *p = a[rb] + 3;
 				hq[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(q[0], v[0], hq[rb][cb], 0, 0, 0);
 				hp[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(p[0], v[0], hp[rb][cb], 0, 0, 0);
 				f32x4 x = cq[rb][cb], y = cp[rb][cb];
@@ -436,12 +471,53 @@ k_proj_bf16x3(const __bf16 *__restrict__ planes, uint32_t n, uint32_t copies, co
 				y = __builtin_amdgcn_mfma_f32_16x16x32_bf16(p[1], v[1], y, 0, 0, 0);
 				cq[rb][cb] = x, cp[rb][cb] = y;
 			}
+		}
+	};
+	double2 x[4];
+	uint4 v[3][2];
+	load(0, x, v);
+	store(lds, x, v);
+	lds_fence_barrier();
+	uint32_t buf = 0;
+	for (uint32_t k0 = 0; k0 < n; k0 += 32) {
+		const bool more = k0 + 32u < n;
+		if (more) load(k0 + 32u, x, v);
+		compute(lds + buf * (kBfA + kBfB));
+		// the other buffer was last read before the previous barrier
+		if (more) store(lds + (buf ^ 1u) * (kBfA + kBfB), x, v);
+		lds_fence_barrier();
+		buf ^= 1u;
 	}
+}
 
-	// epilogue: e(c, j) = (omega_j Q)^2 + P^2 over the wave's 32 copies, then the two copy halves
+__global__ void __launch_bounds__(512, 1)
+k_proj_bf16x3_wide(const double2 *__restrict__ state, uint32_t n, uint32_t copies,
+                   const __bf16 *__restrict__ vplanes, const double *__restrict__ omega, double *__restrict__ partial) {
+	extern __shared__ __align__(16) unsigned char lds_bf[]; // kBfLds bytes
+	double(*red)[kBModes] = (double(*)[kBModes])(lds_bf + 2u * (kBfA + kBfB));
+	const uint32_t mode_blocks = (n + kBModes - 1) / kBModes;
+	const uint32_t xcd = blockIdx.x & 7, rest = blockIdx.x >> 3;
+	const uint32_t bx = rest % mode_blocks, by = (rest / mode_blocks) * 8u + xcd;
+	if (by * 64u >= copies) return; // padding workgroup of the last group of 8 copy blocks (uniform)
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+	const int half = wave >> 2, mg = wave & 3, r16 = lane & 15, g = lane >> 4;
+	const uint32_t c0 = by * 64u, jw0 = bx * uint32_t(kBModes);
+
+	f32x4 hq[2][4], cq[2][4], hp[2][4], cp[2][4];
 #pragma unroll
-	for (int cb = 0; cb < 2; ++cb) {
-		const uint32_t j = j_base + 16u * cb;
+	for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+		for (int cb = 0; cb < 4; ++cb)
+			hq[rb][cb] = cq[rb][cb] = hp[rb][cb] = cp[rb][cb] = f32x4{0.f, 0.f, 0.f, 0.f};
+	const bool full = c0 + 64u <= copies && jw0 + uint32_t(kBModes) <= n;
+	if (full) bf16_wide_loop<true>(state, n, copies, vplanes, c0, jw0, half, mg, r16, g, lds_bf, hq, cq, hp, cp);
+	else bf16_wide_loop<false>(state, n, copies, vplanes, c0, jw0, half, mg, r16, g, lds_bf, hq, cq, hp, cp);
+
+	// epilogue: lane holds modes jw0 + 64 mg + 16 cb + r16 of copies
+	// c0 + 32 half + 16 rb + 4 g + r
+#pragma unroll
+	for (int cb = 0; cb < 4; ++cb) {
+		const uint32_t j = jw0 + 64u * uint32_t(mg) + 16u * uint32_t(cb) + uint32_t(r16);
 		const double w = j < n ? omega[j] : 0.0;
 		double sum = 0.0;
 #pragma unroll
@@ -454,11 +530,11 @@ k_proj_bf16x3(const __bf16 *__restrict__ planes, uint32_t n, uint32_t copies, co
 			}
 		sum += __shfl_xor(sum, 16, 64);
 		sum += __shfl_xor(sum, 32, 64);
-		if (g == 0) red[wr][32 * wc + 16 * cb + r16] = sum;
+		if (g == 0) red[half][64 * mg + 16 * cb + r16] = sum;
 	}
 	__syncthreads();
-	if (threadIdx.x < 64) {
-		const uint32_t j = bx * 64u + threadIdx.x;
+	if (threadIdx.x < kBModes) {
+		const uint32_t j = jw0 + threadIdx.x;
 		if (j < n) partial[size_t(by) * n + j] = red[0][threadIdx.x] + red[1][threadIdx.x];
 	}
 }
@@ -493,19 +569,19 @@ void ddnkg_split_host(const double *x, size_t count, uint16_t *planes) {
 }
 
 hipError_t launch_energy_ddnkg(hipStream_t st, const double2 *state, uint32_t n, uint32_t copies, const double *V,
-                               const double *omega, double *partial, int variant, const uint16_t *vplanes,
-                               uint16_t *planes) {
+                               const double *omega, double *partial, int variant, const uint16_t *vplanes) {
 	if (variant == kProjValu) {
 		const dim3 grid((n + kTj - 1) / kTj, ddnkg_partial_rows(copies));
 		hipLaunchKernelGGL(k_proj_energy, grid, dim3(256), 0, st, state, n, copies, V, omega, partial);
 	} else if (variant == kProjBf16) {
-		if (!ddnkg_bf16_supported(n) || !vplanes || !planes) return hipErrorInvalidValue;
-		const size_t sites = size_t(n) * copies;
-		hipLaunchKernelGGL(k_split_state, dim3(uint32_t((sites + 255) / 256)), dim3(256), 0, st, state, sites,
-		                   (__bf16 *) planes);
-		const uint32_t mode_blocks = (n + 63) / 64, groups = (ddnkg_partial_rows(copies) + 7) / 8;
-		hipLaunchKernelGGL(k_proj_bf16x3, dim3(groups * 8 * mode_blocks), dim3(256), 0, st, (const __bf16 *) planes, n,
-		                   copies, (const __bf16 *) vplanes, omega, partial);
+		if (!ddnkg_bf16_supported(n) || !vplanes) return hipErrorInvalidValue;
+		hipError_t e = hipFuncSetAttribute((const void *) k_proj_bf16x3_wide,
+		                                   hipFuncAttributeMaxDynamicSharedMemorySize, int(kBfLds));
+		if (e != hipSuccess) return e;
+		const uint32_t mode_blocks = (n + kBModes - 1) / kBModes;
+		const uint32_t groups = (ddnkg_partial_rows(copies) + 7) / 8; // of 8 copy blocks, one per XCD
+		hipLaunchKernelGGL(k_proj_bf16x3_wide, dim3(groups * 8 * mode_blocks), dim3(512), kBfLds, st, state, n, copies,
+		                   (const __bf16 *) vplanes, omega, partial);
 	} else if (variant == kProjMfmaWide) {
 		const uint32_t mode_blocks = (n + kWModes - 1) / kWModes;
 		const uint32_t groups = (ddnkg_partial_rows(copies) + 7) / 8; // of 8 copy blocks, one per XCD

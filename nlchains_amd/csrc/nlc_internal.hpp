@@ -193,16 +193,15 @@ namespace nlc {
 // dDNKG projection energies: partial[rows][n] with rows = ddnkg_partial_rows(copies),
 // on FP64 matrix cores (kProjMfma: 64 copies x 128 modes per workgroup;
 // kProjMfmaWide: 64 x 256 with the state tile shared through LDS), the FP64
-// vector ALU (kProjValu) or BF16
-// matrix cores with three-term operand splits (kProjBf16: n % 32 == 0; vplanes =
-// ddnkg_split_host of V, 3 n^2 values; planes = scratch for 6 n copies values).
+// vector ALU (kProjValu) or BF16 matrix cores with three-term operand splits
+// (kProjBf16: n % 32 == 0; vplanes = ddnkg_split_host of V, 3 n^2 values; the
+// state is split inside the kernel).
 constexpr int kProjMfma = 0, kProjValu = 1, kProjBf16 = 2, kProjMfmaWide = 3;
 uint32_t ddnkg_partial_rows(uint32_t copies);
 bool ddnkg_bf16_supported(uint32_t n);
 void ddnkg_split_host(const double *x, size_t count, uint16_t *planes);
 hipError_t launch_energy_ddnkg(hipStream_t, const double2 *state, uint32_t n, uint32_t copies, const double *V,
-                               const double *omega, double *partial, int variant, const uint16_t *vplanes,
-                               uint16_t *planes);
+                               const double *omega, double *partial, int variant, const uint16_t *vplanes);
 // Host eigensystem (nlc_host.cpp).
 int eigensystem(uint32_t n, const double *mp2, double *omegas, double *V);
 extern const double kYoshidaC[8], kYoshidaD[8];
