@@ -89,7 +89,7 @@ struct nlc_ctx {
 	double *mp2 = nullptr;     // dDNKG m_x + 2
 	double *masses_d = nullptr;
 	double *eigvec = nullptr;  // dDNKG V, column-major
-	int proj_variant = nlc::kProjMfma; // dDNKG projection: FP64 matrix cores, vector ALU or split BF16
+	int proj_variant = nlc::kProjMfmaWide; // dDNKG projection: FP64 matrix cores, vector ALU or split BF16
 	uint16_t *vplanes = nullptr; // split BF16 form: the eigenvector splits
 	double *split_q[2] = {nullptr, nullptr}, *split_p[2] = {nullptr, nullptr}; // ping-pong pair
 	double *energy_d = nullptr, *norm_d = nullptr;
