@@ -146,37 +146,69 @@ def cpu_baseline(w, s0, seconds_budget: float):
                       f"N={w['n']} x {copies} copies x {steps} steps, {el:.1f} s"}
 
 
-def fp64_comparison(N, w, s0, b, args, E0, nrm0, E1, nrm1, e_lin, ent):
+FP32C_METHOD = {
+    "DNLS": "state on chip as FP32 pairs hi + lo (48 significant bits); every substep's increment (nonlinear "
+            "phase, FFTs, Fourier multiplier) in FP32, added by a compensated sum; state in HBM, energies, norms "
+            "and entropies in FP64",
+    "FPUT": "q and p on chip as FP32 pairs hi + lo; bond forces and every drift and kick increment in FP32, added "
+            "by Kahan's compensated sum; state in HBM, energies and entropies in FP64",
+    "Toda": "q and p on chip as FP32 pairs hi + lo; the bond exponent and its exponential in FP64, every drift and "
+            "kick increment rounded to FP32 and added by Kahan's compensated sum; state in HBM, energies and "
+            "entropies in FP64",
+}
+
+
+def fp64_comparison(N, make_fp64, sites, w, s0, b, args, mode, pipelined, E0, nrm0, E1, nrm1, e_lin, ent):
     """The FP64 kernel on the same initial state for the same batches (timed
     the same way, for the speed ratio), and the FP32C run's norm and energy
     error against it: per-chain drift of the conserved norm and energy, the
     mode energies and entropies at the end of the run."""
-    with N.Ensemble(N.MODELS[w["model"]], w["n"], w["copies"], w["dt"], beta=w["beta"]) as ref:
+    with make_fp64() as ref:
         ref.upload(s0)
-        for _ in range(args.warmup):
-            ref.move(b)
-            ref.linenergies()
+        if mode == "batched":
+            ref.autotune(max(1, b // 20))
+
+        def batches(k):
+            out = None
+            if pipelined:  # queued three deep, as the FP32C run
+                queued = 0
+                for _ in range(k):
+                    ref.batch_submit(b)
+                    queued += 1
+                    if queued == 3:
+                        out = ref.batch_collect()
+                        queued -= 1
+                while queued:
+                    out = ref.batch_collect()
+                    queued -= 1
+                return out
+            for _ in range(k):
+                if mode == "batched":
+                    out = ref.batch(b)
+                else:
+                    ref.move(b)
+                    out = ref.linenergies()
+            return out
+        batches(args.warmup)
         ref.sync()
         t0 = time.perf_counter()
-        for _ in range(args.steps):
-            ref.move(b)
-            e64, s64 = ref.linenergies()
+        e64, s64 = batches(args.steps)
         ref.sync()
         el = time.perf_counter() - t0
         E64, nrm64 = ref.chain_energies()
-    sites = float(w["n"]) * w["copies"]
 
     def drift(a, a0):
-        r = np.abs(a / a0 - 1)
-        return {"max": float(r.max()), "mean": float(r.mean())}
+        with np.errstate(divide="ignore", invalid="ignore"):
+            r = np.abs(a / a0 - 1)
+        r = r[np.isfinite(r)]
+        return {"max": float(r.max()), "mean": float(r.mean())} if r.size else None
     return {
         "kind": "fp32c",
-        "method": "state on chip as FP32 pairs hi + lo (48 significant bits); every substep's increment (nonlinear "
-                  "phase, FFTs, Fourier multiplier) in FP32, added by a compensated sum; state in HBM, energies, "
-                  "norms and entropies in FP64",
+        "method": FP32C_METHOD[w["model"]],
         "over_steps": (args.warmup + args.steps) * b,
-        "norm_drift": drift(nrm1, nrm0), "energy_drift": drift(E1, E0),
-        "fp64": {"value": sites * b * args.steps / el, "norm_drift": drift(nrm64, nrm0),
+        "norm_drift": drift(nrm1, nrm0) if w["model"] == "DNLS" else None, "energy_drift": drift(E1, E0),
+        "fp64": {"value": sites * b * args.steps / el,
+                 "norm_drift": drift(nrm64, nrm0) if w["model"] == "DNLS" else None,
                  "energy_drift": drift(E64, E0)},
         # the trajectories are chaotic: at the end of the run the FP32C and FP64
         # states differ chain by chain, the ensemble statistics do not
@@ -299,8 +331,8 @@ def main():
     pipelined = w.get("pipelined", False)
     mode = "batched" if (ragged or pipelined) else args.mode
     fp32c = args.precision == "fp32c"
-    if fp32c and w["model"] != "DNLS":
-        print(json.dumps({"error": "--precision fp32c is implemented for the DNLS workload"}))
+    if fp32c and w["model"] not in ("DNLS", "FPUT", "Toda"):
+        print(json.dumps({"error": "--precision fp32c is implemented for the DNLS, FPUT and Toda workloads"}))
         sys.exit(2)
     prec = N.FP32C if fp32c else N.FP64
     lengths = None
@@ -308,7 +340,10 @@ def main():
         lengths = N.ic_ragged_lengths(w["nchains"], w["choices"], w["seed"])
         s0 = N.ic_ragged_plane_waves(lengths, w["m"], w["mode_fraction"], w["e_lo"], w["e_hi"], w["seed"])
         sites = float(lengths.sum())
-        ens = N.Ensemble.ragged(model, lengths, w["dt"], m=w["m"], alpha=w["alpha"], beta=w["beta"])
+        def make(precision):
+            return N.Ensemble.ragged(model, lengths, w["dt"], m=w["m"], alpha=w["alpha"], beta=w["beta"],
+                                     precision=precision)
+        ens = make(prec)
         shape = f"{w['nchains']} chains, lengths uniform over {w['choices']}"
     else:
         n, copies = w["n"], w["copies"]
@@ -325,8 +360,12 @@ def main():
         sites = float(n) * copies
         proj = {"auto": N.PROJ_AUTO, "mfma": N.PROJ_MFMA_F64, "mfma_wide": N.PROJ_MFMA_F64_WIDE,
                 "valu": N.PROJ_VALU_F64, "bf16": N.PROJ_SPLIT_BF16}[args.projection]
-        ens = N.Ensemble(model, n, copies, w["dt"], m=w["m"], alpha=w["alpha"], beta=w["beta"], masses=masses,
-                         lanes_per_chain=args.lanes, sites_per_lane=args.sites, projection=proj, precision=prec)
+        def make(precision, shape_override=True):
+            return N.Ensemble(model, n, copies, w["dt"], m=w["m"], alpha=w["alpha"], beta=w["beta"], masses=masses,
+                              lanes_per_chain=args.lanes if shape_override else 0,
+                              sites_per_lane=args.sites if shape_override else 0, projection=proj,
+                              precision=precision)
+        ens = make(prec)
         shape = f"N={n} copies={copies}"
         if w["model"] == "dDNKG":
             shape += f" masses U[{w['m_lo']},{w['m_hi']}] (seed {w['seed']}), {w['modes']} lowest eigenmodes excited"
@@ -402,7 +441,8 @@ def main():
 
     precision = None
     if fp32c:
-        precision = fp64_comparison(N, w, s0, b, args, E0, nrm0, E1, nrm1, e_lin, ent)
+        precision = fp64_comparison(N, lambda: make(N.FP64) if ragged else make(N.FP64, False), sites, w, s0, b,
+                                    args, mode, pipelined, E0, nrm0, E1, nrm1, e_lin, ent)
     total_steps = (args.warmup + args.steps) * b
     rel = np.abs(E1 - E0) / np.abs(E0)
     drift = float(np.mean(rel[np.isfinite(rel)])) if np.isfinite(rel).any() else float("nan")
@@ -410,9 +450,10 @@ def main():
     k_ms = float(np.mean(kernel_ms))
     flop = flop_per_site_step(w) * sites * b
     achieved = flop / (k_ms * 1e-3) / 1e12
-    kname = ("k_dnls_4096_r8" if "radix8" in desc else "k_dnls_4096_c" if fp32c
+    kname = ("k_dnls_4096_r8" if "radix8" in desc else "k_dnls_4096_c" if fp32c and w["model"] == "DNLS"
              else "k_dnls_4096" if w["model"] == "DNLS"
              else "k_step_persistent" if "ragged_persistent" in desc
+             else "k_step_wave32" if fp32c
              else "k_step_split" if "kernel=split" in desc else "k_step_wave")
     out = {
         "metric": METRIC,
@@ -447,7 +488,8 @@ def main():
         # the FP32 vector peak is the packed-FP32 rate; compiled without packed
         # FP32 ops, an FP32 instruction issues at the FP64 rate (78.6 TFLOP/s)
         out["roofline"].update({"compute_unit": "fp32 vector ALU (157.3 TFLOP/s with packed FP32; non-packed "
-                                                "FP32, as compiled here, issues at 78.6 TFLOP/s)",
+                                                "FP32, as compiled here, issues at 78.6 TFLOP/s)"
+                                                + ("; the bond exponentials in fp64" if w["model"] == "Toda" else ""),
                                 "peak": FP32_PEAK_TFLOPS, "frac": achieved / FP32_PEAK_TFLOPS,
                                 "frac_of_nonpacked_fp32": achieved / FP64_PEAK_TFLOPS})
         out["precision"] = precision

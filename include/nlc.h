@@ -48,13 +48,19 @@ extern "C" {
 typedef enum { NLC_DNKG = 0, NLC_FPUT = 1, NLC_TODA = 2, NLC_DDNKG = 3, NLC_DNLS = 4 } nlc_model;
 
 /* Arithmetic of the step.  NLC_FP64 is the reference's all-double method
- * (DNLS.cpp:159-175 for DNLS).  NLC_FP32C: DNLS with 4096-site chains keeps
- * the field on chip as two FP32 values (hi + lo), computes every substep's
- * increment (nonlinear phase, FFTs, Fourier multiplier) in FP32 and adds it
- * with a compensated double-single sum; the state in HBM, the energies, norms
- * and entropies stay FP64.  Other models and lengths: NLC_EUNSUPPORTED (on
- * gfx950 without packed FP32 an FP32 instruction issues at the FP64 rate, and
- * the register-resident real-model steps are bound by those instructions). */
+ * (DNLS.cpp:159-175 for DNLS, DNKG_FPUT_Toda.cu:33-66 for the FPUT and Toda
+ * kicks).  NLC_FP32C keeps the state on chip as two FP32 values (hi + lo),
+ * forms every substep's increment in FP32 and adds it with a compensated
+ * sum; the state in HBM, the energies, norms and entropies stay FP64:
+ *   DNLS with 4096-site chains: nonlinear phase, FFTs and Fourier multiplier
+ *   in FP32;
+ *   FPUT and Toda with power-of-two chain lengths up to 2048 (uniform, ragged,
+ *   and mixed FPUT + Toda ensembles, every set FP32C): FPUT bond forces in
+ *   FP32, the Toda bond exponential in FP64, q and p updated by Kahan's
+ *   compensated summation.
+ * Other models and lengths: NLC_EUNSUPPORTED.  On gfx950 without packed FP32
+ * an FP32 instruction issues at the FP64 rate, so FP32C is faster only where
+ * the FP64 step is not bound by its instructions (DNLS). */
 typedef enum { NLC_FP64 = 0, NLC_FP32C = 1 } nlc_precision;
 
 /* Step-kernel family.  NLC_KERNEL_SPLIT is the reference's --split_kernel
@@ -190,7 +196,8 @@ int nlc_chain_energies(nlc_ctx *ctx, double *energies, double *norms);
 /* ---- batched form ------------------------------------------------------ */
 
 /* Ragged ensemble: nchains chains of individual lengths chain_lengths[i]
- * (DNKG, FPUT, Toda).  The reference runs one chain length per process
+ * (DNKG, FPUT, Toda; FP32C for FPUT and Toda with power-of-two lengths up to
+ * 2048).  The reference runs one chain length per process
  * (gconf.chain_length, common/configuration.hpp:24); here chains of all lengths
  * share one context and one persistent launch per batch (powers of two from 4
  * to 2048 sites; other lengths run one wave-shaped launch per class).  State layout for
@@ -202,7 +209,8 @@ int nlc_chain_energies(nlc_ctx *ctx, double *energies, double *norms);
 int nlc_create_ragged(nlc_ctx **out, const nlc_params *p, const uint32_t *chain_lengths, uint32_t nchains);
 /* Mixed ensemble: chains of several models and lengths in one context, one
  * persistent launch per batch.  params[0..nsets) (1 <= nsets <= 3): a model
- * (DNKG, FPUT or Toda, FP64) with its parameters; chain i has length
+ * (DNKG, FPUT or Toda) with its parameters, all sets FP64 or all FP32C (FPUT
+ * and Toda only, powers of two up to 2048 sites); chain i has length
  * chain_lengths[i] and follows params[chain_set[i]].  The reference runs one
  * model per process (common/main.cpp:157-160, a program per model in the
  * registry of DNKG_FPUT_Toda.cpp:156-162); here the models share the state, the

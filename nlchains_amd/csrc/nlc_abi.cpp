@@ -207,6 +207,19 @@ nlc::StepLauncher lookup(int model, int L, int S, bool exact) {
 	return l;
 }
 
+// FP32-compensated wave kernels: FPUT and Toda, exact power-of-two shapes.
+nlc::StepLauncher lookup32(int model, int L, int S) {
+	if (model == NLC_FPUT) return nlc::lookup_step32_fput(L, S);
+	if (model == NLC_TODA) return nlc::lookup_step32_toda(L, S);
+	return nullptr;
+}
+
+// The wave kernel of a shape in the given precision (real models).
+nlc::StepLauncher lookup_prec(int model, int L, int S, bool exact, bool c32) {
+	if (!c32) return lookup(model, L, S, exact);
+	return exact && L <= 64 ? lookup32(model, L, S) : nullptr;
+}
+
 nlc::TiledLauncher lookup_tiled(int model) {
 	switch (model) {
 	case NLC_DNKG: return nlc::lookup_tiled_dnkg();
@@ -217,11 +230,11 @@ nlc::TiledLauncher lookup_tiled(int model) {
 	return nullptr;
 }
 
-nlc::PersistentLauncher lookup_persistent(int S, uint32_t kind_mask) {
+nlc::PersistentLauncher lookup_persistent(int S, uint32_t kind_mask, bool c32) {
 	switch (S) {
-	case 8: return nlc::lookup_persistent_s8(kind_mask);
-	case 16: return nlc::lookup_persistent_s16(kind_mask);
-	case 32: return nlc::lookup_persistent_s32(kind_mask);
+	case 8: return c32 ? nlc::lookup_persistent32_s8(kind_mask) : nlc::lookup_persistent_s8(kind_mask);
+	case 16: return c32 ? nlc::lookup_persistent32_s16(kind_mask) : nlc::lookup_persistent_s16(kind_mask);
+	case 32: return c32 ? nlc::lookup_persistent32_s32(kind_mask) : nlc::lookup_persistent_s32(kind_mask);
 	}
 	return nullptr;
 }
@@ -296,15 +309,16 @@ struct Shape {
 	int L, S;
 	bool exact;
 };
-std::vector<Shape> wave_shapes(int model, uint32_t n) {
+std::vector<Shape> wave_shapes(int model, uint32_t n, bool c32) {
 	std::vector<Shape> out;
 	for (int L = 1; L <= 64; L *= 2) {
 		const int S = int((n + L - 1) / L);
 		if (S < 1 || S > 32) continue;
 		const bool exact = uint32_t(L * S) == n;
 		if (!exact && S < 2) continue;
-		if (lookup(model, L, S, exact)) out.push_back({L, S, exact});
+		if (lookup_prec(model, L, S, exact, c32)) out.push_back({L, S, exact});
 	}
+	if (c32) return out;
 	for (int L = 128; L <= 512; L += 64) {
 		const int S = int((n + L - 1) / L);
 		if (!group_shape_ok(n, L, S)) continue;
@@ -328,8 +342,9 @@ bool ragged_fits(const std::vector<Klass> &cls, int S, bool half_S) {
 	return true;
 }
 // VALU instructions per site and substep of each kind (its kick and drift),
-// the cost weight of the persistent kernel's ordering
-constexpr double kKindCost[6] = {5, 6, 6, 6, 7, 19};
+// the cost weight of the persistent kernel's ordering; FP32-compensated FPUT
+// and Toda kinds in the second row
+constexpr double kKindCost[2][6] = {{5, 6, 6, 6, 7, 19}, {5, 6, 14, 14, 15, 30}};
 
 void drop_graph(nlc_ctx *c) {
 	// a queued replay may still be running
@@ -571,11 +586,19 @@ int setup_dnls(nlc_ctx *c) {
 
 // Step kernel of a uniform real-model context (the reference's move<>
 // dispatch, DNKG_FPUT_Toda.cu:254-293).
+// FP32-compensated real models (the DNLS form has its own kernel).
+bool is_c32(const nlc_ctx *c) { return c->p.precision == NLC_FP32C && c->p.model != NLC_DNLS; }
+
 int select_uniform(nlc_ctx *c) {
 	const nlc_params &p = c->p;
 	const uint32_t n = c->n;
+	const bool c32 = is_c32(c);
 	bool want_split = p.kernel == NLC_KERNEL_SPLIT, want_tiled = p.kernel == NLC_KERNEL_TILED;
 	c->use_split = c->use_tiled = false;
+	if (c32 && (want_split || want_tiled))
+		return c->fail(NLC_EUNSUPPORTED, "FP32C precision runs on the wave kernels (no split or tiled form)");
+	if (c32 && !(is_pow2(n) && n <= 2048))
+		return c->fail(NLC_EUNSUPPORTED, "FP32C precision of FPUT and Toda needs a power-of-two chain_length <= 2048");
 	if (!want_split && !want_tiled) {
 		int L = 0, S = 0;
 		bool exact = false, have;
@@ -588,12 +611,12 @@ int select_uniform(nlc_ctx *c) {
 			       group_shape_ok(n, L, S);
 			if (!have) return c->fail(NLC_EARG, "wave shape %d lanes x %d sites cannot hold chain_length %u", L, S, n);
 		} else {
-			have = default_shape(n, c->copies, L, S, exact) || group_default(p.model, n, L, S, exact);
+			have = default_shape(n, c->copies, L, S, exact) || (!c32 && group_default(p.model, n, L, S, exact));
 		}
-		c->launcher = have ? lookup(p.model, L, S, exact) : nullptr;
+		c->launcher = have ? lookup_prec(p.model, L, S, exact, c32) : nullptr;
 		if (c->launcher) {
 			c->L = L, c->S = S, c->exact = exact;
-		} else if (p.lanes_per_chain) {
+		} else if (p.lanes_per_chain || c32) {
 			return c->fail(NLC_EUNSUPPORTED, "no wave-resident kernel for chain_length %u (shape %d x %d)", n, L, S);
 		} else {
 			// longer than a workgroup holds: segments in registers with temporal blocking
@@ -627,7 +650,8 @@ int select_ragged(nlc_ctx *c, int S, bool half_S) {
 	uint32_t kind_mask = 0;
 	for (int s = 0; s < 3; ++s)
 		if (c->pargs.kind[s] >= 0) kind_mask |= 1u << c->pargs.kind[s];
-	c->persistent = S && ragged_fits(c->cls, S, half_S) ? lookup_persistent(S, kind_mask) : nullptr;
+	const bool c32 = is_c32(c);
+	c->persistent = S && ragged_fits(c->cls, S, half_S) ? lookup_persistent(S, kind_mask, c32) : nullptr;
 	if (c->persistent) {
 		struct Costed {
 			nlc::RaggedItem it;
@@ -638,7 +662,7 @@ int select_ragged(nlc_ctx *c, int S, bool half_S) {
 			const int s_item = item_S(k.n, S, half_S);
 			const uint32_t L = k.n / uint32_t(s_item), cpw = 64 / L;
 			const uint8_t flags = uint8_t(k.set | (s_item < S ? nlc::kItemHalfS : 0));
-			const double cost = s_item * kKindCost[c->pargs.kind[k.set]];
+			const double cost = s_item * kKindCost[c32][c->pargs.kind[k.set]];
 			for (uint32_t i = 0; i < k.count; i += cpw)
 				all.push_back({{k.site0 + uint64_t(i) * k.n, k.n, uint16_t(std::min(cpw, k.count - i)),
 				                uint8_t(log2u(L)), flags},
@@ -663,7 +687,8 @@ int select_ragged(nlc_ctx *c, int S, bool half_S) {
 	c->ragged_half = false;
 	c->nitems = 0;
 	for (auto &k : c->cls) {
-		if (!default_shape(k.n, k.count, k.L, k.S, k.exact) || !(k.launcher = lookup(k.model, k.L, k.S, k.exact)))
+		if (!default_shape(k.n, k.count, k.L, k.S, k.exact) ||
+		    !(k.launcher = lookup_prec(k.model, k.L, k.S, k.exact, c32)))
 			return c->fail(NLC_EUNSUPPORTED, "ragged ensembles need chain lengths with a wave shape (got %u)", k.n);
 	}
 	return NLC_OK;
@@ -796,8 +821,8 @@ int validate_params(const nlc_params *p, bool ragged) {
 	if (p->model == NLC_TODA && p->alpha == 0) return reject(NLC_EARG, "Toda needs --alpha != 0");
 	if (p->model == NLC_DDNKG && !p->masses) return reject(NLC_EARG, "dDNKG needs the masses (-m file content)");
 	if (p->precision != NLC_FP64 && p->precision != NLC_FP32C) return reject(NLC_EARG, "unknown precision");
-	if (p->precision == NLC_FP32C && p->model != NLC_DNLS)
-		return reject(NLC_EUNSUPPORTED, "FP32C precision is implemented for DNLS (4096-site chains)");
+	if (p->precision == NLC_FP32C && p->model != NLC_DNLS && p->model != NLC_FPUT && p->model != NLC_TODA)
+		return reject(NLC_EUNSUPPORTED, "FP32C precision is implemented for FPUT, Toda and DNLS");
 	if (p->kernel < NLC_KERNEL_AUTO || p->kernel > NLC_KERNEL_TILED) return reject(NLC_EARG, "unknown kernel family");
 	if (p->projection < NLC_PROJ_AUTO || p->projection > NLC_PROJ_MFMA_F64_WIDE)
 		return reject(NLC_EARG, "unknown projection form");
@@ -924,7 +949,10 @@ int create_classes(nlc_ctx **out, const nlc_params *params, uint32_t nsets, cons
 		if (int rc = validate_params(&params[s], true)) return rc;
 		if (params[s].model != NLC_DNKG && params[s].model != NLC_FPUT && params[s].model != NLC_TODA)
 			return reject(NLC_EUNSUPPORTED, "ragged and mixed ensembles are available for DNKG, FPUT and Toda");
-		if (params[s].precision != NLC_FP64) return reject(NLC_EUNSUPPORTED, "ragged and mixed ensembles are FP64");
+		if (params[s].precision != params[0].precision)
+			return reject(NLC_EARG, "the parameter sets of a mixed ensemble share one precision");
+		if (params[s].precision == NLC_FP32C && params[s].model == NLC_DNKG)
+			return reject(NLC_EUNSUPPORTED, "FP32C ragged and mixed ensembles hold FPUT and Toda chains");
 	}
 	uint64_t total = 0;
 	for (uint32_t i = 0; i < nchains; ++i) {
@@ -989,7 +1017,7 @@ int create_classes(nlc_ctx **out, const nlc_params *params, uint32_t nsets, cons
 		std::vector<size_t> order(c->cls.size());
 		for (size_t i = 0; i < order.size(); ++i) order[i] = i;
 		auto sites = [&](size_t i) {
-			return double(c->cls[i].n) * c->cls[i].count * kKindCost[c->pargs.kind[c->cls[i].set]];
+			return double(c->cls[i].n) * c->cls[i].count * kKindCost[is_c32(c)][c->pargs.kind[c->cls[i].set]];
 		};
 		std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return sites(a) > sites(b); });
 		double load[2] = {0, 0};
@@ -1380,20 +1408,24 @@ int nlc_autotune(nlc_ctx *c, uint32_t trial_steps, char *report, size_t len) {
 				uint32_t kinds = 0;
 				for (int s = 0; s < 3; ++s)
 					if (c->pargs.kind[s] >= 0) kinds |= 1u << c->pargs.kind[s];
-				if (ragged_fits(c->cls, S, half) && lookup_persistent(S, kinds))
+				if (ragged_fits(c->cls, S, half) && lookup_persistent(S, kinds, is_c32(c)))
 					cands.push_back({0, 0, false, false, S, 0, 0, false, half});
 			}
 		bool per_class = true; // one uniform kernel per length class, launched in sequence
 		for (const auto &k : c->cls) {
 			int L, S;
 			bool exact;
-			per_class = per_class && default_shape(k.n, k.count, L, S, exact) && lookup(c->p.model, L, S, exact);
+			per_class = per_class && default_shape(k.n, k.count, L, S, exact) &&
+			            lookup_prec(k.model, L, S, exact, is_c32(c));
 		}
 		if (per_class) cands.push_back({0, 0, false, false, 0, 0, 0});
 	} else {
-		for (const Shape &s : wave_shapes(c->p.model, c->n)) cands.push_back({s.L, s.S, s.exact, false, 0, 0, 0});
-		if (c->n > 2048 && lookup_tiled(c->p.model)) cands.push_back({0, 0, false, false, 0, 0, 0, true});
-		cands.push_back({0, 0, false, true, 0, 0, 0});
+		for (const Shape &s : wave_shapes(c->p.model, c->n, is_c32(c)))
+			cands.push_back({s.L, s.S, s.exact, false, 0, 0, 0});
+		if (!is_c32(c)) {
+			if (c->n > 2048 && lookup_tiled(c->p.model)) cands.push_back({0, 0, false, false, 0, 0, 0, true});
+			cands.push_back({0, 0, false, true, 0, 0, 0});
+		}
 	}
 	if (!cands.empty()) {
 		drop_graph(c);
@@ -1526,7 +1558,7 @@ int nlc_describe(const nlc_ctx *c, char *buf, size_t len) {
 			         (f == nlc::kFormCubic ? "cubic_unit" : f == nlc::kFormQuad ? "quadratic_unit" : "plain");
 		}
 		const size_t at = strlen(buf);
-		if (at + 1 < len) snprintf(buf + at, len - at, " force=%s", forms.c_str());
+		if (at + 1 < len) snprintf(buf + at, len - at, " force=%s%s", forms.c_str(), is_c32(c) ? " precision=fp32c" : "");
 	}
 	if (c->p.model == NLC_DDNKG) {
 		const size_t at = strlen(buf);

@@ -68,6 +68,10 @@ StepLauncher lookup_step_dnkg(int L, int S, bool exact);
 StepLauncher lookup_step_fput(int L, int S, bool exact);
 StepLauncher lookup_step_toda(int L, int S, bool exact);
 StepLauncher lookup_step_ddnkg(int L, int S, bool exact);
+// FP32-compensated wave kernels (FPUT, Toda; p.precision = NLC_FP32C): exact
+// power-of-two shapes n = L S, L <= 64, S <= 32 (StepLauncher's mp2 unused).
+StepLauncher lookup_step32_fput(int L, int S);
+StepLauncher lookup_step32_toda(int L, int S);
 // Temporally blocked step for chains longer than a workgroup holds: `steps`
 // steps in launches of a few steps each, ping-ponging between state and
 // scratch (same size as state); the result ends in state.
@@ -135,6 +139,10 @@ using PersistentLauncher = hipError_t (*)(hipStream_t, double2 *state, const Rag
 PersistentLauncher lookup_persistent_s8(uint32_t kind_mask);
 PersistentLauncher lookup_persistent_s16(uint32_t kind_mask);
 PersistentLauncher lookup_persistent_s32(uint32_t kind_mask);
+// The same, FP32-compensated (FPUT and Toda kinds).
+PersistentLauncher lookup_persistent32_s8(uint32_t kind_mask);
+PersistentLauncher lookup_persistent32_s16(uint32_t kind_mask);
+PersistentLauncher lookup_persistent32_s32(uint32_t kind_mask);
 
 // HBM-streaming step over the split layout q[n][copies], p[n][copies]: one
 // launch per substep from buffer pair cur to pair cur ^ 1 (cur is updated; the

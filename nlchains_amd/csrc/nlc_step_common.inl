@@ -62,6 +62,36 @@ __device__ __forceinline__ double from_right(double v, int lane) {
 	else return bperm_f64(v, (lane & ~(L - 1)) | ((lane + 1) & (L - 1)));
 }
 
+// The same moves for FP32 values (the FP32-compensated kernels).
+template <int CTRL>
+__device__ __forceinline__ float dpp_f32(float v) {
+	return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, true));
+}
+
+template <int L>
+__device__ __forceinline__ float from_left(float v, int lane) {
+	if constexpr (L == 1) return v;
+	else if constexpr (L == 2) return dpp_f32<kQuadSwap>(v);
+	else if constexpr (L == 4) return dpp_f32<kQuadLeft>(v);
+	else if constexpr (L == 16) return dpp_f32<kRowRor1>(v);
+	else if constexpr (L == 64) return dpp_f32<kWaveRor1>(v);
+	else
+		return __int_as_float(__builtin_amdgcn_ds_bpermute(((lane & ~(L - 1)) | ((lane - 1) & (L - 1))) << 2,
+		                                                   __float_as_int(v)));
+}
+
+template <int L>
+__device__ __forceinline__ float from_right(float v, int lane) {
+	if constexpr (L == 1) return v;
+	else if constexpr (L == 2) return dpp_f32<kQuadSwap>(v);
+	else if constexpr (L == 4) return dpp_f32<kQuadRight>(v);
+	else if constexpr (L == 16) return dpp_f32<kRowRor15>(v);
+	else if constexpr (L == 64) return dpp_f32<kWaveRol1>(v);
+	else
+		return __int_as_float(__builtin_amdgcn_ds_bpermute(((lane & ~(L - 1)) | ((lane + 1) & (L - 1))) << 2,
+		                                                   __float_as_int(v)));
+}
+
 // Force of DNKG / dDNKG on a site with neighbours l, r and mass = m_x + 2:
 // l + r - q (m_x + 2 + beta q^2) (DNKG_FPUT_Toda.cu:24-31, dDNKG.cu:17-19).
 // kFormCubic works on sigma q with sigma = sqrt(beta), where the cubic

@@ -158,6 +158,143 @@ __device__ __forceinline__ void run_wave(double2 *__restrict__ chain0, uint32_t 
 	}
 }
 
+// FP32-compensated chain state (FPUT and Toda, n = L S): q and p as FP32 pairs
+// (h, l) whose sum is the value.  Every increment of a drift or kick is formed
+// in FP32 and added with Kahan's compensated summation, l carrying what h
+// could not hold, so the state keeps about 2^-48 of its value while the
+// increments are rounded to FP32.  FPUT evaluates its bond force in FP32 on
+// x = (h_i - h_{i-1}) + (l_i - l_{i-1}); Toda forms the bond exponent and the
+// exponential in FP64 (the exponential amplifies an exponent error), and only
+// its kick increment is rounded to FP32.  Drift and kick weights are rounded
+// to FP32.  HBM holds the FP64 state; loads split it, stores sum the pair.
+template <int MODEL, int L, int S, int FORM>
+struct Chain32 {
+	static_assert(MODEL == FPUT || MODEL == TODA, "FP32-compensated kernels: FPUT and Toda");
+	static constexpr int kSites = S;
+	float qh[S], ql[S], ph[S], pl[S];
+	int lane;
+	const double *etab; // Toda: exp table in LDS
+
+	// (h, l) += d
+	__device__ __forceinline__ static void add(float &h, float &l, float d) {
+		const float y = d + l;
+		const float s = h + y;
+		l = y - (s - h);
+		h = s;
+	}
+
+	__device__ __forceinline__ void drift(double c) {
+		const float cf = float(c);
+#pragma unroll
+		for (int j = 0; j < S; ++j) add(qh[j], ql[j], cf * ph[j]);
+	}
+
+	__device__ __forceinline__ static float fput_bond(float x, const StepArgs &a) {
+		if constexpr (FORM == kFormCubic) return fmaf(x * x, x, x);
+		else if constexpr (FORM == kFormQuad) return fmaf(x, x, x);
+		else return x * fmaf(x, fmaf(float(a.beta), x, float(a.alpha)), 1.0f);
+	}
+
+	__device__ __forceinline__ double toda_bond(float h1, float l1, float h0, float l0) const {
+		// both differences are exact in FP64; one rounding in their sum
+		const double x = (double(h1) - double(h0)) + (double(l1) - double(l0));
+		if constexpr (S >= 25) return exp2_poly(x);
+		else return exp2_32_tab(x, etab);
+	}
+
+	__device__ __forceinline__ void kick(double d, const StepArgs &a) {
+		const float lh = from_left<L>(qh[S - 1], lane), ll = from_left<L>(ql[S - 1], lane);
+		if constexpr (MODEL == FPUT) {
+			const float w = float(d);
+			const float b0 = fput_bond((qh[0] - lh) + (ql[0] - ll), a);
+			const float bs = from_right<L>(b0, lane); // right bond of the lane's last site
+			float bl = b0;
+#pragma unroll
+			for (int j = 0; j < S; ++j) {
+				const float br = (j == S - 1) ? bs : fput_bond((qh[j + 1] - qh[j]) + (ql[j + 1] - ql[j]), a);
+				add(ph[j], pl[j], w * (br - bl));
+				bl = br;
+			}
+		} else {
+			const double w = d * a.a2i;
+			const double b0 = toda_bond(qh[0], ql[0], lh, ll);
+			const double bs = from_right<L>(b0, lane);
+			double bl = b0;
+#pragma unroll
+			for (int j = 0; j < S; ++j) {
+				const double br = (j == S - 1) ? bs : toda_bond(qh[j + 1], ql[j + 1], qh[j], ql[j]);
+				add(ph[j], pl[j], float(w * (br - bl)));
+				bl = br;
+			}
+		}
+	}
+};
+
+__device__ __forceinline__ void split_f32(double v, float &h, float &l) {
+	h = float(v);
+	l = float(v - double(h));
+}
+
+// run_wave for the FP32-compensated state (exact shapes only).
+template <int MODEL, int L, int S, int FORM>
+__device__ __forceinline__ void run_wave32(double2 *__restrict__ chain0, uint32_t n, uint32_t live_chains,
+                                           uint32_t steps, const StepArgs &a, const double *etab) {
+	const int lane = threadIdx.x & 63;
+	const uint32_t sub = uint32_t(lane / L);
+	const bool live = sub < live_chains;
+	Chain32<MODEL, L, S, FORM> ch;
+	ch.lane = lane;
+	ch.etab = etab;
+	double2 *base = chain0 + size_t(live ? sub : 0) * n + uint32_t((lane & (L - 1)) * S);
+#pragma unroll
+	for (int j = 0; j < S; ++j) {
+		const double2 v = live ? to_registers<MODEL, FORM, S>(base[j], a) : make_double2(0.0, 0.0);
+		split_f32(v.x, ch.qh[j], ch.ql[j]);
+		split_f32(v.y, ch.ph[j], ch.pl[j]);
+	}
+	step_loop<MODEL>(ch, steps, a);
+	if (live) {
+#pragma unroll
+		for (int j = 0; j < S; ++j)
+			base[j] = from_registers<MODEL, FORM, S>(double(ch.qh[j]) + double(ch.ql[j]),
+			                                         double(ch.ph[j]) + double(ch.pl[j]), a);
+	}
+}
+
+template <int MODEL, int L, int S, int FORM>
+__global__ void __launch_bounds__(256)
+k_step_wave32(double2 *__restrict__ state, uint32_t copies, uint32_t n, uint32_t steps, StepArgs a) {
+	constexpr uint32_t CPW = 64 / L;
+	__shared__ double etab[MODEL == TODA ? kExpTabSize : 1];
+	load_exp_table<MODEL>(etab);
+	const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+	const uint32_t first_chain = wave * CPW;
+	if (first_chain >= copies) return; // whole wavefront idle: uniform exit
+	run_wave32<MODEL, L, S, FORM>(state + size_t(first_chain) * n, n, min(CPW, copies - first_chain), steps, a, etab);
+}
+
+template <int MODEL, int L, int S, int FORM>
+hipError_t launch_wave32_form(hipStream_t st, double2 *state, uint32_t copies, uint32_t n, uint32_t steps,
+                              const StepArgs &a) {
+	if (n != uint32_t(L * S)) return hipErrorInvalidValue;
+	constexpr uint32_t CPW = 64 / L;
+	const uint64_t waves = (uint64_t(copies) + CPW - 1) / CPW;
+	const uint64_t blocks = (waves + 3) / 4;
+	hipLaunchKernelGGL((k_step_wave32<MODEL, L, S, FORM>), dim3(uint32_t(blocks)), dim3(256), 0, st, state, copies,
+	                   n, steps, a);
+	return hipGetLastError();
+}
+
+template <int MODEL, int L, int S>
+hipError_t launch_wave32(hipStream_t st, double2 *state, uint32_t copies, uint32_t n, uint32_t steps,
+                         const StepArgs &a, const double *) {
+	if constexpr (MODEL == FPUT) {
+		if (a.form == kFormCubic) return launch_wave32_form<MODEL, L, S, kFormCubic>(st, state, copies, n, steps, a);
+		if (a.form == kFormQuad) return launch_wave32_form<MODEL, L, S, kFormQuad>(st, state, copies, n, steps, a);
+	}
+	return launch_wave32_form<MODEL, L, S, kFormPlain>(st, state, copies, n, steps, a);
+}
+
 template <int MODEL, int L, int S, bool EXACT, int FORM>
 __global__ void __launch_bounds__(256)
 k_step_wave(double2 *__restrict__ state, uint32_t copies, uint32_t n, uint32_t steps, StepArgs a,
@@ -295,19 +432,34 @@ hipError_t launch_tiled(hipStream_t st, double2 *state, double2 *scratch, uint32
 // KMASK: the kinds compiled in (bit k = kind k of PersistentArgs): one kind
 // for a single-model ensemble, so the kernel has that model's register
 // footprint; all six for a mixed one.
+#ifndef NLC_PERSIST_C32
+#define NLC_PERSIST_C32 0
+#endif
 template <int KIND, int S>
 __device__ __forceinline__ void run_item(double2 *__restrict__ state, const RaggedItem &it, uint32_t steps,
                                          const StepArgs &a, const double *etab) {
 	constexpr int MODEL = kind_model(KIND), FORM = kind_form(KIND);
 	double2 *c0 = state + it.site0;
-	switch (it.log2_lanes) {
-	case 0: run_wave<MODEL, 1, S, true, FORM>(c0, it.n, it.nchains, steps, a, nullptr, etab); break;
-	case 1: run_wave<MODEL, 2, S, true, FORM>(c0, it.n, it.nchains, steps, a, nullptr, etab); break;
-	case 2: run_wave<MODEL, 4, S, true, FORM>(c0, it.n, it.nchains, steps, a, nullptr, etab); break;
-	case 3: run_wave<MODEL, 8, S, true, FORM>(c0, it.n, it.nchains, steps, a, nullptr, etab); break;
-	case 4: run_wave<MODEL, 16, S, true, FORM>(c0, it.n, it.nchains, steps, a, nullptr, etab); break;
-	case 5: run_wave<MODEL, 32, S, true, FORM>(c0, it.n, it.nchains, steps, a, nullptr, etab); break;
-	default: run_wave<MODEL, 64, S, true, FORM>(c0, it.n, it.nchains, steps, a, nullptr, etab); break;
+	if constexpr (NLC_PERSIST_C32) {
+		switch (it.log2_lanes) {
+		case 0: run_wave32<MODEL, 1, S, FORM>(c0, it.n, it.nchains, steps, a, etab); break;
+		case 1: run_wave32<MODEL, 2, S, FORM>(c0, it.n, it.nchains, steps, a, etab); break;
+		case 2: run_wave32<MODEL, 4, S, FORM>(c0, it.n, it.nchains, steps, a, etab); break;
+		case 3: run_wave32<MODEL, 8, S, FORM>(c0, it.n, it.nchains, steps, a, etab); break;
+		case 4: run_wave32<MODEL, 16, S, FORM>(c0, it.n, it.nchains, steps, a, etab); break;
+		case 5: run_wave32<MODEL, 32, S, FORM>(c0, it.n, it.nchains, steps, a, etab); break;
+		default: run_wave32<MODEL, 64, S, FORM>(c0, it.n, it.nchains, steps, a, etab); break;
+		}
+	} else {
+		switch (it.log2_lanes) {
+		case 0: run_wave<MODEL, 1, S, true, FORM>(c0, it.n, it.nchains, steps, a, nullptr, etab); break;
+		case 1: run_wave<MODEL, 2, S, true, FORM>(c0, it.n, it.nchains, steps, a, nullptr, etab); break;
+		case 2: run_wave<MODEL, 4, S, true, FORM>(c0, it.n, it.nchains, steps, a, nullptr, etab); break;
+		case 3: run_wave<MODEL, 8, S, true, FORM>(c0, it.n, it.nchains, steps, a, nullptr, etab); break;
+		case 4: run_wave<MODEL, 16, S, true, FORM>(c0, it.n, it.nchains, steps, a, nullptr, etab); break;
+		case 5: run_wave<MODEL, 32, S, true, FORM>(c0, it.n, it.nchains, steps, a, nullptr, etab); break;
+		default: run_wave<MODEL, 64, S, true, FORM>(c0, it.n, it.nchains, steps, a, nullptr, etab); break;
+		}
 	}
 }
 
@@ -392,8 +544,23 @@ TiledLauncher NLC_STEP_TILED() {
 
 #ifdef NLC_PERSIST_S
 // The persistent kernels with S (and S/2) sites per lane: one per kind, and
-// for S >= 16 one with every kind (mixed ensembles).
+// for S >= 16 one with every kind (mixed ensembles); the FP32-compensated
+// build (NLC_PERSIST_C32) has the FPUT and Toda kinds.
 PersistentLauncher NLC_PERSIST_LOOKUP(uint32_t kind_mask) {
+#if NLC_PERSIST_C32
+	constexpr uint32_t kC32Kinds = (1u << kKindFputCubic) | (1u << kKindFputQuad) | (1u << kKindFputPlain) |
+	                               (1u << kKindToda);
+	switch (kind_mask) {
+	case 1u << kKindFputCubic: return &launch_persistent<NLC_PERSIST_S, 1u << kKindFputCubic>;
+	case 1u << kKindFputQuad: return &launch_persistent<NLC_PERSIST_S, 1u << kKindFputQuad>;
+	case 1u << kKindFputPlain: return &launch_persistent<NLC_PERSIST_S, 1u << kKindFputPlain>;
+	case 1u << kKindToda: return &launch_persistent<NLC_PERSIST_S, 1u << kKindToda>;
+	}
+#if NLC_PERSIST_S >= 16
+	if (kind_mask && !(kind_mask & ~kC32Kinds)) return &launch_persistent<NLC_PERSIST_S, kC32Kinds>;
+#endif
+	return nullptr;
+#else
 	switch (kind_mask) {
 	case 1u << 0: return &launch_persistent<NLC_PERSIST_S, 1u << 0>;
 	case 1u << 1: return &launch_persistent<NLC_PERSIST_S, 1u << 1>;
@@ -405,6 +572,21 @@ PersistentLauncher NLC_PERSIST_LOOKUP(uint32_t kind_mask) {
 #if NLC_PERSIST_S >= 16
 	if (kind_mask && !(kind_mask & ~kAllKinds)) return &launch_persistent<NLC_PERSIST_S, kAllKinds>;
 #endif
+	return nullptr;
+#endif
+}
+#endif
+
+#ifdef NLC_STEP_LOOKUP32
+// FP32-compensated wave kernels (FPUT, Toda): every power-of-two shape with
+// n = L S, L <= 64, S <= 32.
+StepLauncher NLC_STEP_LOOKUP32(int L, int S) {
+#define NLC_C32(LL, SS)                                                                                            \
+	if (L == LL && S == SS) return &launch_wave32<NLC_STEP_MODEL, LL, SS>;
+#define NLC_C32_ROW(LL) NLC_C32(LL, 1) NLC_C32(LL, 2) NLC_C32(LL, 4) NLC_C32(LL, 8) NLC_C32(LL, 16) NLC_C32(LL, 32)
+	NLC_C32_ROW(1) NLC_C32_ROW(2) NLC_C32_ROW(4) NLC_C32_ROW(8) NLC_C32_ROW(16) NLC_C32_ROW(32) NLC_C32_ROW(64)
+#undef NLC_C32_ROW
+#undef NLC_C32
 	return nullptr;
 }
 #endif

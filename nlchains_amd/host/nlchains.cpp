@@ -299,11 +299,15 @@ int main_real(int model, int argc, char **argv) {
 		break;
 	}
 	o.add("split_kernel", 0, false, false, "force use of split kernel");
+	if (model != NLC_DNKG)
+		o.add("fp32c", 0, false, false,
+		      "FP32 fields with compensated accumulation, FP64 reductions (power-of-two chain_length <= 2048)");
 	Config c;
 	run_common(o, argc, argv, c);
 	nlc_params p;
 	nlc_params_init(&p);
 	p.model = model;
+	if (model != NLC_DNKG && o.count("fp32c")) p.precision = NLC_FP32C;
 	if (model == NLC_DNKG) p.m = o.get<double>("-m");
 	if (model != NLC_TODA) p.beta = o.get<double>("beta");
 	if (model != NLC_DNKG) p.alpha = o.get<double>("alpha");

@@ -80,6 +80,21 @@ def test_bench_dnls_fp32c_line(gpu):
 
 
 @pytest.mark.gpu
+def test_bench_toda_ragged_fp32c_line(gpu):
+    """--precision fp32c on the ragged Toda workload: the FP32-compensated
+    persistent launch, its rate next to the FP64 one and its energy drift."""
+    p = run_bench("--workload", "toda_ragged", "--precision", "fp32c", "--steps", "2", "--warmup", "1", "--batch",
+                  "20", "--cpu-seconds", "0")
+    assert p.returncode == 0, p.stderr[-2000:]
+    d = json.loads(p.stdout.strip().splitlines()[-1])
+    assert d["dtype"] == "f32" and "precision=fp32c" in d["config"]["kernel"]
+    assert "ragged_persistent" in d["config"]["kernel"] or "per_class" in d["config"]["kernel"]
+    pr = d["precision"]
+    assert pr["over_steps"] == 60 and pr["fp64"]["value"] > 0 and d["vs_fp64"] > 0 and pr["norm_drift"] is None
+    assert pr["energy_drift"]["mean"] < 1e-5 and pr["fp64"]["energy_drift"]["mean"] < 1e-5
+
+
+@pytest.mark.gpu
 def test_bench_c1_pipelined(gpu):
     p = run_bench("--workload", "c1", "--steps", "20", "--warmup", "2", "--cpu-seconds", "1")
     assert p.returncode == 0, p.stderr[-2000:]

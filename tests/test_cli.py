@@ -37,8 +37,8 @@ def test_cli_lists_programs():
     assert "DNKG, DNLS, FPUT, Toda, dDNKG" in p.stdout
 
 
-@pytest.mark.parametrize("prog,opts", [("DNKG", ["-m", "--beta"]), ("FPUT", ["--alpha", "--beta"]),
-                                       ("Toda", ["--alpha"]), ("dDNKG", ["-m", "--beta"]),
+@pytest.mark.parametrize("prog,opts", [("DNKG", ["-m", "--beta"]), ("FPUT", ["--alpha", "--beta", "--fp32c"]),
+                                       ("Toda", ["--alpha", "--fp32c"]), ("dDNKG", ["-m", "--beta"]),
                                        ("DNLS", ["--beta", "--no_linear_callback", "--fp32c"])])
 def test_cli_help_per_program(prog, opts):
     p = run(prog, check=0)
@@ -253,6 +253,26 @@ def test_cli_dnls_fp32c(gpu, tmp_path):
     assert chain_rel_err(got, want) <= 3e-6
     p = run("DNLS", "-n", 1024, "-c", c, "--dt", 0.01, "-b", 50, "-s", 100, "-p", pre, "-i", tmp_path / "init",
             "--beta", 1, "--fp32c")
+    assert p.returncode != 0 and "FP32C" in (p.stdout + p.stderr)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prog,extra,model,kw", [("FPUT", ["--alpha", 0, "--beta", 1], O.FPUT, dict(beta=1.0)),
+                                                ("Toda", ["--alpha", 1], O.TODA, dict(alpha=1.0))])
+def test_cli_real_fp32c(gpu, tmp_path, prog, extra, model, kw):
+    """--fp32c for FPUT and Toda: the FP32-compensated step through the CLI,
+    against the FP64 oracle at its stated precision; other lengths refused."""
+    n, c = 256, 8
+    s0 = gpu.ic_plane_waves(n, c, 0.0, 1, 32, 0.25, 3)
+    s0.tofile(tmp_path / "init")
+    pre = tmp_path / "f"
+    run(prog, "-n", n, "-c", c, "--dt", 0.05, "-b", 50, "-s", 100, "-p", pre, "-i", tmp_path / "init", *extra,
+        "--fp32c", check=0)
+    want = O.move_real(model, s0, 100, 0.05, **kw)
+    got = np.fromfile(f"{pre}-100").reshape(c, n, 2)
+    assert chain_rel_err(got, want) <= 5e-6
+    p = run(prog, "-n", 100, "-c", c, "--dt", 0.05, "-b", 50, "-s", 100, "-p", pre, "-i", tmp_path / "init", *extra,
+            "--fp32c")
     assert p.returncode != 0 and "FP32C" in (p.stdout + p.stderr)
 
 

@@ -147,7 +147,8 @@ def test_fp32c_batches_and_graph(gpu):
 
 def test_fp32c_unsupported_combinations(gpu):
     N = gpu
-    for model, n, kw in ((N.DNKG, 1024, dict(m=1.0, beta=1.0)), (N.TODA, 1024, dict(alpha=1.0)),
+    # (FPUT and Toda FP32C: power-of-two chains up to 2048 sites, tests/test_gpu_fp32c_real.py)
+    for model, n, kw in ((N.DNKG, 1024, dict(m=1.0, beta=1.0)), (N.TODA, 4096, dict(alpha=1.0)),
                          (N.DNLS, 1024, dict(beta=1.0)), (N.DNLS, 8192, dict(beta=1.0))):
         with pytest.raises(N.NlcError) as ex:
             N.Ensemble(model, n, 4, 0.01, precision=N.FP32C, **kw)

@@ -598,7 +598,7 @@ def test_ddnkg_projection_full_size(gpu):
     assert abs(raws["mfma"].sum() - harmonic) <= 1e-11 * harmonic
 
 
-@pytest.mark.parametrize("n,copies", [(512, 70), (96, 130), (64, 8192), (512, 8192)])
+@pytest.mark.parametrize("n,copies", [(32, 5), (512, 70), (96, 130), (64, 8192), (320, 200), (512, 8192)])
 def test_ddnkg_projection_split_bf16(gpu, n, copies):
     """BF16 matrix cores with three-term operand splits: mode energies within
     1e-6 of the largest mode energy of the FP64 result (measured: about 1e-7),
