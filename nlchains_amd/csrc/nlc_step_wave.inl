@@ -432,15 +432,14 @@ hipError_t launch_tiled(hipStream_t st, double2 *state, double2 *scratch, uint32
 // KMASK: the kinds compiled in (bit k = kind k of PersistentArgs): one kind
 // for a single-model ensemble, so the kernel has that model's register
 // footprint; all six for a mixed one.
-#ifndef NLC_PERSIST_C32
-#define NLC_PERSIST_C32 0
-#endif
-template <int KIND, int S>
+//
+// C32: the FP32-compensated chain state (FPUT and Toda kinds).
+template <int KIND, int S, bool C32>
 __device__ __forceinline__ void run_item(double2 *__restrict__ state, const RaggedItem &it, uint32_t steps,
                                          const StepArgs &a, const double *etab) {
 	constexpr int MODEL = kind_model(KIND), FORM = kind_form(KIND);
 	double2 *c0 = state + it.site0;
-	if constexpr (NLC_PERSIST_C32) {
+	if constexpr (C32) {
 		switch (it.log2_lanes) {
 		case 0: run_wave32<MODEL, 1, S, FORM>(c0, it.n, it.nchains, steps, a, etab); break;
 		case 1: run_wave32<MODEL, 2, S, FORM>(c0, it.n, it.nchains, steps, a, etab); break;
@@ -463,21 +462,21 @@ __device__ __forceinline__ void run_item(double2 *__restrict__ state, const Ragg
 	}
 }
 
-template <int KIND, int S, uint32_t KMASK>
+template <int KIND, int S, uint32_t KMASK, bool C32>
 __device__ __forceinline__ void run_kind(double2 *__restrict__ state, const RaggedItem &it, uint32_t steps,
                                          const StepArgs &a, const double *etab) {
 	if constexpr ((KMASK >> KIND) & 1u) {
 		if constexpr (S >= 2) {
 			if (it.flags & kItemHalfS) {
-				run_item<KIND, S / 2>(state, it, steps, a, etab);
+				run_item<KIND, S / 2, C32>(state, it, steps, a, etab);
 				return;
 			}
 		}
-		run_item<KIND, S>(state, it, steps, a, etab);
+		run_item<KIND, S, C32>(state, it, steps, a, etab);
 	}
 }
 
-template <int S, uint32_t KMASK>
+template <int S, uint32_t KMASK, bool C32>
 __global__ void __launch_bounds__(256)
 k_step_persistent(double2 *__restrict__ state, const RaggedItem *__restrict__ items, uint32_t nitems, uint32_t steps,
                   PersistentArgs pa, uint32_t *__restrict__ queue) {
@@ -494,12 +493,12 @@ k_step_persistent(double2 *__restrict__ state, const RaggedItem *__restrict__ it
 		const int set = it.flags & kItemSetMask;
 		const StepArgs &a = pa.a[set];
 		switch (pa.kind[set]) {
-		case 0: run_kind<0, S, KMASK>(state, it, steps, a, etab); break;
-		case 1: run_kind<1, S, KMASK>(state, it, steps, a, etab); break;
-		case 2: run_kind<2, S, KMASK>(state, it, steps, a, etab); break;
-		case 3: run_kind<3, S, KMASK>(state, it, steps, a, etab); break;
-		case 4: run_kind<4, S, KMASK>(state, it, steps, a, etab); break;
-		default: run_kind<5, S, KMASK>(state, it, steps, a, etab); break;
+		case 0: run_kind<0, S, KMASK, C32>(state, it, steps, a, etab); break;
+		case 1: run_kind<1, S, KMASK, C32>(state, it, steps, a, etab); break;
+		case 2: run_kind<2, S, KMASK, C32>(state, it, steps, a, etab); break;
+		case 3: run_kind<3, S, KMASK, C32>(state, it, steps, a, etab); break;
+		case 4: run_kind<4, S, KMASK, C32>(state, it, steps, a, etab); break;
+		default: run_kind<5, S, KMASK, C32>(state, it, steps, a, etab); break;
 		}
 	}
 	// the last wave to get here resets the queue for the next launch
@@ -512,18 +511,18 @@ k_step_persistent(double2 *__restrict__ state, const RaggedItem *__restrict__ it
 	}
 }
 
-template <int S, uint32_t KMASK>
+template <int S, uint32_t KMASK, bool C32>
 hipError_t launch_persistent(hipStream_t st, double2 *state, const RaggedItem *items, uint32_t nitems, uint32_t steps,
                              const PersistentArgs &pa, uint32_t *queue, uint32_t max_blocks) {
 	int per_cu = 0, cus = 0, dev = 0;
 	hipError_t e = hipGetDevice(&dev);
 	if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-	if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_step_persistent<S, KMASK>, 256, 0);
+	if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_step_persistent<S, KMASK, C32>, 256, 0);
 	if (e != hipSuccess) return e;
 	// as many waves as are resident at once, no more than there are items
 	uint64_t blocks = std::min<uint64_t>(uint64_t(std::max(per_cu, 1)) * std::max(cus, 1), (nitems + 3) / 4);
 	if (max_blocks) blocks = std::min<uint64_t>(blocks, max_blocks);
-	hipLaunchKernelGGL((k_step_persistent<S, KMASK>), dim3(uint32_t(std::max<uint64_t>(blocks, 1))), dim3(256), 0, st,
+	hipLaunchKernelGGL((k_step_persistent<S, KMASK, C32>), dim3(uint32_t(std::max<uint64_t>(blocks, 1))), dim3(256), 0, st,
 	                   state, items, nitems, steps, pa, queue);
 	return hipGetLastError();
 }
@@ -543,6 +542,9 @@ TiledLauncher NLC_STEP_TILED() {
 #endif
 
 #ifdef NLC_PERSIST_S
+#ifndef NLC_PERSIST_C32
+#define NLC_PERSIST_C32 0
+#endif
 // The persistent kernels with S (and S/2) sites per lane: one per kind, and
 // for S >= 16 one with every kind (mixed ensembles); the FP32-compensated
 // build (NLC_PERSIST_C32) has the FPUT and Toda kinds.
@@ -551,26 +553,26 @@ PersistentLauncher NLC_PERSIST_LOOKUP(uint32_t kind_mask) {
 	constexpr uint32_t kC32Kinds = (1u << kKindFputCubic) | (1u << kKindFputQuad) | (1u << kKindFputPlain) |
 	                               (1u << kKindToda);
 	switch (kind_mask) {
-	case 1u << kKindFputCubic: return &launch_persistent<NLC_PERSIST_S, 1u << kKindFputCubic>;
-	case 1u << kKindFputQuad: return &launch_persistent<NLC_PERSIST_S, 1u << kKindFputQuad>;
-	case 1u << kKindFputPlain: return &launch_persistent<NLC_PERSIST_S, 1u << kKindFputPlain>;
-	case 1u << kKindToda: return &launch_persistent<NLC_PERSIST_S, 1u << kKindToda>;
+	case 1u << kKindFputCubic: return &launch_persistent<NLC_PERSIST_S, 1u << kKindFputCubic, true>;
+	case 1u << kKindFputQuad: return &launch_persistent<NLC_PERSIST_S, 1u << kKindFputQuad, true>;
+	case 1u << kKindFputPlain: return &launch_persistent<NLC_PERSIST_S, 1u << kKindFputPlain, true>;
+	case 1u << kKindToda: return &launch_persistent<NLC_PERSIST_S, 1u << kKindToda, true>;
 	}
 #if NLC_PERSIST_S >= 16
-	if (kind_mask && !(kind_mask & ~kC32Kinds)) return &launch_persistent<NLC_PERSIST_S, kC32Kinds>;
+	if (kind_mask && !(kind_mask & ~kC32Kinds)) return &launch_persistent<NLC_PERSIST_S, kC32Kinds, true>;
 #endif
 	return nullptr;
 #else
 	switch (kind_mask) {
-	case 1u << 0: return &launch_persistent<NLC_PERSIST_S, 1u << 0>;
-	case 1u << 1: return &launch_persistent<NLC_PERSIST_S, 1u << 1>;
-	case 1u << 2: return &launch_persistent<NLC_PERSIST_S, 1u << 2>;
-	case 1u << 3: return &launch_persistent<NLC_PERSIST_S, 1u << 3>;
-	case 1u << 4: return &launch_persistent<NLC_PERSIST_S, 1u << 4>;
-	case 1u << 5: return &launch_persistent<NLC_PERSIST_S, 1u << 5>;
+	case 1u << 0: return &launch_persistent<NLC_PERSIST_S, 1u << 0, false>;
+	case 1u << 1: return &launch_persistent<NLC_PERSIST_S, 1u << 1, false>;
+	case 1u << 2: return &launch_persistent<NLC_PERSIST_S, 1u << 2, false>;
+	case 1u << 3: return &launch_persistent<NLC_PERSIST_S, 1u << 3, false>;
+	case 1u << 4: return &launch_persistent<NLC_PERSIST_S, 1u << 4, false>;
+	case 1u << 5: return &launch_persistent<NLC_PERSIST_S, 1u << 5, false>;
 	}
 #if NLC_PERSIST_S >= 16
-	if (kind_mask && !(kind_mask & ~kAllKinds)) return &launch_persistent<NLC_PERSIST_S, kAllKinds>;
+	if (kind_mask && !(kind_mask & ~kAllKinds)) return &launch_persistent<NLC_PERSIST_S, kAllKinds, false>;
 #endif
 	return nullptr;
 #endif
