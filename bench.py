@@ -217,6 +217,35 @@ def fp64_comparison(N, make_fp64, sites, w, s0, b, args, mode, pipelined, E0, nr
     }
 
 
+def batched_form(ens, s0, sites, b, args, plain_value):
+    """The batched form of the same workload and initial state, in the same
+    process after the plain run: the autotuned kernel shape, every bench step
+    one hipGraph replay (move + energy pass + entropies), timed the same way;
+    with its energy drift beside the plain run's."""
+    with ens:
+        ens.upload(s0)
+        report = ens.autotune(max(1, b // 20))
+        ens.upload(s0)
+        E0, _ = ens.chain_energies()
+        for _ in range(args.warmup):
+            ens.batch(b)
+        ens.sync()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            ens.batch(b)
+        ens.sync()
+        el = time.perf_counter() - t0
+        E1, _ = ens.chain_energies()
+        desc = ens.describe()
+    rel = np.abs(E1 - E0) / np.abs(E0)
+    value = sites * b * args.steps / el
+    return {"value": value, "ms_per_step": el / args.steps * 1e3, "vs_plain": value / plain_value,
+            "kernel": desc, "autotune": report,
+            "energy_drift": {"mean_rel": float(np.mean(rel[np.isfinite(rel)])) if np.isfinite(rel).any() else None,
+                             "over_steps": (args.warmup + args.steps) * b,
+                             "nonfinite_chains": int((~np.isfinite(rel)).sum())}}
+
+
 def copy_calibration(mib: int) -> None:
     """Device-to-device copies of a known size (ROCm runs them as a blit
     kernel on the compute queue): the reference point that scales the
@@ -302,8 +331,11 @@ def main():
                     help="plain: move + energy pass per batch; batched: autotuned shape, one hipGraph per batch")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU-baseline time budget (0 = skip)")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32c"],
-                    help="fp32c: DNLS with FP32 fields and compensated accumulation (reductions in FP64); the "
-                         "line then also reports its norm and energy error against an FP64 run of the same steps")
+                    help="fp32c: FP32 fields with compensated accumulation (DNLS, FPUT, Toda; reductions in FP64); "
+                         "the line then also reports its energy (and DNLS norm) error against an FP64 run of the "
+                         "same steps")
+    ap.add_argument("--no-batched", action="store_true",
+                    help="plain mode: skip the batched form of the same workload reported beside the plain one")
     ap.add_argument("--projection", default="auto", choices=["auto", "mfma", "mfma_wide", "valu", "bf16"],
                     help="dDNKG energy projection: FP64 MFMA (64 x 128 or 64 x 256 tiles), FP64 VALU or split BF16 "
                          "MFMA; auto = the autotuned FP64 form in --mode batched, else FP64 MFMA")
@@ -513,6 +545,8 @@ def main():
                                    "frac": hw / (e_ms * 1e-3) / 1e12 / peak})
     if tune:
         out["config"]["autotune"] = tune
+    if mode == "plain" and not fp32c and not args.no_batched:
+        out["batched"] = batched_form(make(prec, False), s0, sites, b, args, value)
     if args.cpu_seconds > 0:
         out["cpu_baseline"] = (cpu_baseline_ragged(w, lengths, s0, args.cpu_seconds) if ragged
                                else cpu_baseline(w, s0, args.cpu_seconds))

@@ -163,8 +163,9 @@ __device__ __forceinline__ void run_wave(double2 *__restrict__ chain0, uint32_t 
 // in FP32 and added with Kahan's compensated summation, l carrying what h
 // could not hold, so the state keeps about 2^-48 of its value while the
 // increments are rounded to FP32.  FPUT evaluates its bond force in FP32 on
-// x = (h_i - h_{i-1}) + (l_i - l_{i-1}); Toda forms the bond exponent and the
-// exponential in FP64 (the exponential amplifies an exponent error), and only
+// x = (h_i - h_{i-1}) + (l_i - l_{i-1}); Toda forms the bond exponent (from
+// the pairs, in FP64) and the exponential in FP64 (the exponential amplifies
+// an exponent error), and only
 // its kick increment is rounded to FP32.  Drift and kick weights are rounded
 // to FP32.  HBM holds the FP64 state; loads split it, stores sum the pair.
 template <int MODEL, int L, int S, int FORM>
@@ -196,7 +197,8 @@ struct Chain32 {
 	}
 
 	__device__ __forceinline__ double toda_bond(float h1, float l1, float h0, float l0) const {
-		// both differences are exact in FP64; one rounding in their sum
+		// each difference of two FP32 values is exact in FP64 (unless their
+		// magnitudes differ by more than 2^29); one rounding in the sum
 		const double x = (double(h1) - double(h0)) + (double(l1) - double(l0));
 		if constexpr (S >= 25) return exp2_poly(x);
 		else return exp2_32_tab(x, etab);

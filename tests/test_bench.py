@@ -62,6 +62,9 @@ def test_bench_json_contract(gpu):
     c = d["cpu_baseline"]
     assert c["kind"] in ("reference", "port") and c["value"] > 0 and c["sample"]
     assert 1 <= c["cores"] <= c["host_cores"]
+    bt = d["batched"]  # the batched form of the same workload, same process
+    assert bt["value"] > 1e11 and bt["vs_plain"] > 0 and "wave shapes" in bt["autotune"]
+    assert bt["energy_drift"]["over_steps"] == 4000 and bt["energy_drift"]["nonfinite_chains"] == 0
 
 
 @pytest.mark.gpu
