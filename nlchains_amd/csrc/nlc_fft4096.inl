@@ -75,21 +75,34 @@ __device__ __forceinline__ void dft16_t2n(C v[16]) {
 // v[j] *= W^{e0 + j de} for j = 0..15 with W = exp(-2 pi i/4096) = tw[1]
 // (conjugated when CONJ); ONE0 asserts e0 == 0, so v[0] is left alone.  The
 // running power is refreshed from the exact table every 4 steps to bound
-// rounding.
+// rounding.  In FP64 the third and fourth power of each run of four come from
+// the three-term recurrence r_{j+1} = 2 cos(theta) r_j - r_{j-1} (two FMAs
+// instead of a 4-instruction complex product, a few ulp of FP64 at most);
+// the FP32 kernel keeps the products, whose smaller rounding error it needs.
 template <bool CONJ, bool ONE0, bool TRANSPOSED, class C>
 __device__ __forceinline__ void twiddle_run(C v[16], const C *__restrict__ tw, uint32_t e0, uint32_t de) {
+	using R = typename Cplx<C>::R;
+	constexpr bool kRecur = sizeof(R) == 8;
+	const C w = tw[de & 4095u];
+	const C wc = CONJ ? cconj(w) : w;
+	const R c2 = R(2) * w.x; // 2 cos(theta), the same for W and its conjugate
 #pragma unroll
 	for (int q = 0; q < 4; ++q) {
 		C r = tw[(e0 + uint32_t(4 * q) * de) & 4095u];
-		const C w = tw[de & 4095u];
-		const C wc = CONJ ? cconj(w) : w;
 		if (CONJ) r = cconj(r);
+		C rp = r; // the power before r (used from j = 1 on)
 #pragma unroll
 		for (int j = 0; j < 4; ++j) {
 			const int idx = 4 * q + j;
 			const int at = TRANSPOSED ? pos(idx) : idx;
 			if (idx || !ONE0) v[at] = cmul(v[at], r);
-			if (j < 3) r = cmul(r, wc);
+			if (j < 3) {
+				C rn;
+				if (kRecur && j >= 1) rn = Cplx<C>::make(fma(c2, r.x, -rp.x), fma(c2, r.y, -rp.y));
+				else rn = cmul(r, wc);
+				rp = r;
+				r = rn;
+			}
 		}
 	}
 }
