@@ -689,7 +689,10 @@ int select_ragged(nlc_ctx *c, int S, bool half_S) {
 	for (auto &k : c->cls) {
 		if (!default_shape(k.n, k.count, k.L, k.S, k.exact) ||
 		    !(k.launcher = lookup_prec(k.model, k.L, k.S, k.exact, c32)))
-			return c->fail(NLC_EUNSUPPORTED, "ragged ensembles need chain lengths with a wave shape (got %u)", k.n);
+			return c->fail(NLC_EUNSUPPORTED,
+			               c32 ? "FP32C ragged ensembles need power-of-two chain lengths <= 2048 (got %u)"
+			                   : "ragged ensembles need chain lengths with a wave shape (got %u)",
+			               k.n);
 	}
 	return NLC_OK;
 }

@@ -210,3 +210,8 @@ def test_fp32c_refusals(gpu, model, n, kw):
     if model == N.FPUT:
         with pytest.raises(N.NlcError):
             N.Ensemble(N.FPUT, 64, 4, 0.05, beta=1.0, precision=N.FP32C, kernel=N.KERNEL_SPLIT)
+        with pytest.raises(N.NlcError, match="power-of-two"):
+            N.Ensemble.ragged(N.FPUT, np.array([64, 100, 64], dtype=np.uint32), 0.05, beta=1.0, precision=N.FP32C)
+        with pytest.raises(N.NlcError):  # one precision per mixed ensemble
+            N.Ensemble.mixed([dict(model=N.FPUT, beta=1.0, precision=N.FP32C), dict(model=N.TODA, alpha=1.0)],
+                             np.array([64, 64], dtype=np.uint32), [0, 1], 0.05)
