@@ -139,6 +139,32 @@ def test_fp32c_ragged_toda_persistent(gpu):
     assert err.max() <= RAGGED_TOL and d64.max() <= RAGGED_TOL
 
 
+def test_fp32c_ragged_short_chains_every_persistent_width(gpu):
+    """Short FPUT chains (8..512 sites) in FP32C: the persistent kernel with
+    8 sites per lane by default, then the widths that hold 8-site chains (8,
+    and 16 at half width) and the per-class launches, which the autotuner
+    times; runs before and after it checked against the oracle."""
+    N = gpu
+    lengths = N.ic_ragged_lengths(500, [8, 16, 32, 64, 128, 256, 512], 9)
+    s0 = N.ic_ragged_plane_waves(lengths, 0.0, 0.05, 0.05, 0.5, 9)
+    want = oracle_ragged({0: 1}, {0: dict(alpha=0.25, beta=1.0)}, [0] * len(lengths), list(lengths), s0, 60)
+    with N.Ensemble.ragged(N.FPUT, lengths, 0.05, alpha=0.25, beta=1.0, precision=N.FP32C) as ens:
+        desc = ens.describe()
+        ens.upload(s0)
+        ens.move(60)
+        err0 = chain_errors(ens.download(), want, list(lengths)).max()
+        report = ens.autotune(5)
+        ens.upload(s0)
+        ens.move(60)
+        err1 = chain_errors(ens.download(), want, list(lengths)).max()
+        desc2 = ens.describe()
+    banner(f"{desc}: {err0:.2e}; autotune: {report}; {desc2}: {err1:.2e}")
+    assert "ragged_persistent sites_per_lane=8" in desc and "precision=fp32c" in desc
+    for w in ("S=8 ", "S=8/half", "S=16/half", "per-class"):
+        assert w in report, report
+    assert err0 <= RAGGED_TOL and err1 <= RAGGED_TOL
+
+
 def test_fp32c_mixed_fput_toda(gpu):
     """FPUT and Toda chains of three lengths in one FP32C context: per-chain
     parity with the oracle, and the kernel the autotuner picks giving the same
