@@ -40,6 +40,10 @@ def test_strerror_and_params_init():
     (dict(n=1), N.NLC_EARG), (dict(copies=0), N.NLC_EARG), (dict(dt=0.0), N.NLC_EARG),
     (dict(dt=-1.0), N.NLC_EARG), (dict(model=N.TODA, alpha=0.0), N.NLC_EARG),
     (dict(model=N.DDNKG), N.NLC_EARG), (dict(n=1 << 16, copies=(1 << 15) + 1), N.NLC_EARG),
+    (dict(precision=7), N.NLC_EARG),
+    # FP32C: DNLS, FPUT and Toda only
+    (dict(model=N.DNKG, precision=N.FP32C), N.NLC_EUNSUPPORTED),
+    (dict(model=N.DDNKG, masses=np.ones(64), precision=N.FP32C), N.NLC_EUNSUPPORTED),
 ])
 def test_create_rejects_invalid_arguments(kw, code):
     """The argument rules of common/main.cpp:83-103 (checked before any device call)."""
