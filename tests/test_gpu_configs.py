@@ -110,3 +110,35 @@ def test_ddnkg_512x8192_dump_interval(gpu):
     e = O.calc_linenergies(ref, 0.5 / copies, False)
     wt, inf = O.entropies(e, False)
     assert abs(ent[0] - wt) <= 1e-9 * abs(wt) and abs(ent[1] - inf) <= 1e-9 * abs(inf)
+
+
+@pytest.mark.parametrize("name,model,kw,published", [
+    # the reference's published energy errors at N = 64 (supporting-material/post-processing/examples.nb:1997-2256):
+    # dt = 0.1 and 0.05
+    ("FPUT", 1, dict(alpha=0.1, beta=0.1), (9.749e-9, 1.511e-10)),
+    ("Toda", 2, dict(alpha=0.1), (9.213e-9, 1.408e-10)),
+    ("DNKG", 0, dict(m=1.0, beta=1.0), (2.083e-8, 3.190e-10)),
+])
+def test_energy_error_scales_as_dt6(gpu, name, model, kw, published):
+    """The 6th-order method's energy error: mean over copies of |E(T) - E(0)| / E(0)
+    after T = 1000 at dt = 0.1 and 0.05, N = 64.  The ratio of the two is 2^6 within
+    a factor of two, as in the reference's published table (ratio 64.5, 65.4 and 65.3
+    there); the reference's initial states are not published, so only the ratio is
+    compared, the magnitudes are printed beside the published ones.  Measured on
+    MI355X: ratios 64.0 (FPUT), 64.0 (Toda), 64.3 (DNKG); errors at dt = 0.05 of
+    6.5e-11, 1.7e-11 and 1.0e-10 for these seeded states of energy density 0.1."""
+    N = gpu
+    n, copies, T = 64, 256, 1000.0
+    s0 = N.ic_plane_waves(n, copies, kw.get("m", 0.0), 1, 4, 0.1, 11)
+    eps = []
+    for dt in (0.1, 0.05):
+        with N.Ensemble(model, n, copies, dt, **kw) as ens:
+            ens.upload(s0)
+            E0, _ = ens.chain_energies()
+            ens.move(int(round(T / dt)))
+            E1, _ = ens.chain_energies()
+        eps.append(float(np.mean(np.abs(E1 - E0) / np.abs(E0))))
+    ratio = eps[0] / eps[1]
+    print(f"[gpu-configs] {name}: eps(0.1) {eps[0]:.3e} eps(0.05) {eps[1]:.3e} ratio {ratio:.1f}; published "
+          f"{published[0]:.3e} {published[1]:.3e} ratio {published[0] / published[1]:.1f}", flush=True)
+    assert 32 <= ratio <= 128
