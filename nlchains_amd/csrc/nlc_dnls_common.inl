@@ -1,6 +1,6 @@
 // Device pieces shared by the DNLS kernels: complex helpers, the nonlinear
-// phase e^{-i theta |psi|^2} (Taylor polynomials of sin and cos on |r| <= pi/16,
-// degree 11 and 12, after a Cody & Waite reduction by pi/8 for larger angles),
+// phase e^{-i theta |psi|^2} (Taylor polynomials of sin and cos on |r| <= pi/32,
+// degree 9 and 10, after a Cody & Waite reduction by pi/16 for larger angles),
 // the 4-point DFT and the LDS Stockham FFT.  Included inside
 // namespace nlc { namespace { ... } } by each DNLS translation unit.
 constexpr int kT = 256; // threads per chain
@@ -33,17 +33,16 @@ __device__ __forceinline__ float2 cconj(float2 a) { return make_float2(a.x, -a.y
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
 
-// (cos r, sin r) for |r| <= pi/16: Taylor series about 0 in x = r^2 (degree
-// 12 and 11); the first omitted terms are below 1e-18 relative.
+// (cos r, sin r) for |r| <= pi/32: Taylor series about 0 in x = r^2 (degree
+// 10 and 9); the first omitted terms, r^12/12! and r^11/11!, are below 3e-18
+// relative.
 __device__ __forceinline__ double2 expi_small(double r) {
 	const double x = r * r;
-	double s = -2.505210838544172e-08;         // -1/11!
-	s = fma(s, x, 2.7557319223985893e-06);     // 1/9!
+	double s = 2.7557319223985893e-06;         // 1/9!
 	s = fma(s, x, -0.0001984126984126984);     // -1/7!
 	s = fma(s, x, 0.008333333333333333);       // 1/5!
 	s = fma(s, x, -0.16666666666666666);       // -1/3!
-	double c = 2.08767569878681e-09;           // 1/12!
-	c = fma(c, x, -2.755731922398589e-07);     // -1/10!
+	double c = -2.755731922398589e-07;         // -1/10!
 	c = fma(c, x, 2.48015873015873e-05);       // 1/8!
 	c = fma(c, x, -0.001388888888888889);      // -1/6!
 	c = fma(c, x, 0.041666666666666664);       // 1/4!
@@ -51,43 +50,61 @@ __device__ __forceinline__ double2 expi_small(double r) {
 	return make_double2(fma(c, x, 1.0), fma(r * x, s, r));
 }
 
-// e^{i k pi/8}, k = 0..15
-__constant__ double2 kOctant[16] = {
+// e^{i k pi/16}, k = 0..31, correctly rounded (the first octant's values
+// carried to the others by the quarter-turn symmetries)
+__constant__ double2 kSixteenth[32] = {
     {1.0, 0.0},
+    {0.9807852804032304, 0.19509032201612828},
     {0.9238795325112867, 0.3826834323650898},
+    {0.8314696123025452, 0.5555702330196022},
     {0.7071067811865476, 0.7071067811865476},
+    {0.5555702330196022, 0.8314696123025452},
     {0.3826834323650898, 0.9238795325112867},
+    {0.19509032201612828, 0.9807852804032304},
     {0.0, 1.0},
+    {-0.19509032201612828, 0.9807852804032304},
     {-0.3826834323650898, 0.9238795325112867},
+    {-0.5555702330196022, 0.8314696123025452},
     {-0.7071067811865476, 0.7071067811865476},
+    {-0.8314696123025452, 0.5555702330196022},
     {-0.9238795325112867, 0.3826834323650898},
+    {-0.9807852804032304, 0.19509032201612828},
     {-1.0, 0.0},
+    {-0.9807852804032304, -0.19509032201612828},
     {-0.9238795325112867, -0.3826834323650898},
+    {-0.8314696123025452, -0.5555702330196022},
     {-0.7071067811865476, -0.7071067811865476},
+    {-0.5555702330196022, -0.8314696123025452},
     {-0.3826834323650898, -0.9238795325112867},
+    {-0.19509032201612828, -0.9807852804032304},
     {0.0, -1.0},
+    {0.19509032201612828, -0.9807852804032304},
     {0.3826834323650898, -0.9238795325112867},
+    {0.5555702330196022, -0.8314696123025452},
     {0.7071067811865476, -0.7071067811865476},
+    {0.8314696123025452, -0.5555702330196022},
     {0.9238795325112867, -0.3826834323650898},
+    {0.9807852804032304, -0.19509032201612828},
 };
 
-// (cos a, sin a).  |a| <= pi/16 directly (the nonlinear angles of a DNLS run
+// (cos a, sin a).  |a| <= 0.098 directly (the nonlinear angles of a DNLS run
 // are far smaller, so the other branch is normally skipped by the whole
-// wavefront); otherwise Cody & Waite reduction a = k pi/8 + r, |r| <= pi/16,
-// with pi/8 split in three parts (33, 33 and 53 significant bits, so k*Q1 and
-// k*Q2 are exact for |k| < 2^20), and e^{ia} = e^{i k pi/8} e^{ir}.
-constexpr double kExpiSmall = 0.19634954084936207; // pi/16
+// wavefront); otherwise Cody & Waite reduction a = k pi/16 + r, |r| <= pi/32,
+// with pi/16 split in three parts (33, 33 and 53 significant bits, so k*Q1 and
+// k*Q2 are exact for |k| < 2^20), and e^{ia} = e^{i k pi/16} e^{ir}.  The bound
+// 0.098 sits just under pi/32 so that every angle it admits has k = 0.
+constexpr double kExpiSmall = 0.098;
 
-// The reduced form, valid for every angle; for |a| <= pi/16 it gives exactly
-// expi_small(a) (k = 0, r = a, and the octant factor is 1 + 0i).
+// The reduced form, valid for every angle; for |a| <= kExpiSmall it gives
+// exactly expi_small(a) (k = 0, r = a, and the table factor is 1 + 0i).
 __device__ __forceinline__ double2 expi_reduced(double a) {
-	constexpr double kEightOverPi = 2.5464790894703255, kQ1 = 0x1.921fb54400000p-2, kQ2 = 0x1.0b4611a600000p-36,
-	                 kQ3 = 0x1.3198a2e037073p-71;
-	const double k = rint(a * kEightOverPi);
+	constexpr double kSixteenOverPi = 5.092958178940651, kQ1 = 0x1.921fb54400000p-3, kQ2 = 0x1.0b4611a600000p-37,
+	                 kQ3 = 0x1.3198a2e037073p-72;
+	const double k = rint(a * kSixteenOverPi);
 	double r = fma(-k, kQ1, a);
 	r = fma(-k, kQ2, r);
 	r = fma(-k, kQ3, r);
-	return cmul(expi_small(r), kOctant[int(k) & 15]);
+	return cmul(expi_small(r), kSixteenth[int(k) & 31]);
 }
 
 __device__ __forceinline__ double2 expi(double a) {
@@ -105,7 +122,7 @@ __device__ __forceinline__ double2 nonlinear(double2 p, double theta) {
 
 // The nonlinear step on all N values of a thread.  The branch between the
 // short and the reduced form is taken once for the whole wavefront (the
-// reduced form only when some angle of some lane exceeds pi/16), so each path
+// reduced form only when some angle of some lane exceeds kExpiSmall), so each path
 // is straight-line code over the N values: their polynomial chains interleave
 // instead of running one after another behind a per-value branch.
 template <int N>
@@ -129,18 +146,20 @@ __device__ __forceinline__ void nonlinear_all(double2 (&v)[N], double theta) {
 // nonlinear_all in two halves of 8 values, the scheduler kept from
 // interleaving them: half the polynomial temporaries live at a time, for a
 // kernel that runs at 168 VGPRs (three wavefronts per SIMD hide the latency
-// the interleaving hid).
+// the interleaving hid).  Each half takes its own wave-uniform branch, so the
+// angles are computed once (both forms agree bit for bit up to kExpiSmall, so the
+// result does not depend on how the values are grouped).
 __device__ __forceinline__ void nonlinear_all_halves(double2 (&v)[16], double theta) {
-	bool small = true;
-#pragma unroll
-	for (int j = 0; j < 16; ++j) small = small && fabs(theta * fma(v[j].x, v[j].x, v[j].y * v[j].y)) <= kExpiSmall;
-	const bool all_small = __all(small);
 #pragma unroll
 	for (int h = 0; h < 2; ++h) {
 		double a[8];
+		bool small = true;
 #pragma unroll
-		for (int j = 0; j < 8; ++j) a[j] = -theta * fma(v[8 * h + j].x, v[8 * h + j].x, v[8 * h + j].y * v[8 * h + j].y);
-		if (all_small) {
+		for (int j = 0; j < 8; ++j) {
+			a[j] = -theta * fma(v[8 * h + j].x, v[8 * h + j].x, v[8 * h + j].y * v[8 * h + j].y);
+			small = small && fabs(a[j]) <= kExpiSmall;
+		}
+		if (__all(small)) {
 #pragma unroll
 			for (int j = 0; j < 8; ++j) v[8 * h + j] = rotate(v[8 * h + j], expi_small(a[j]));
 		} else {

@@ -69,7 +69,7 @@ __device__ __forceinline__ float2 expim1_small(float a) {
 
 // The nonlinear increment of all 16 values of a thread, folded into lo; w gets
 // the FP32 input of the forward transform, psi + d_N.  One wave-uniform branch
-// as in the FP64 kernel: angles above pi/16 (rare: |psi|^2 far above its mean)
+// as in the FP64 kernel: angles above pi/16 here (rare: |psi|^2 far above its mean)
 // take the FP64 reduced form for the whole wavefront.
 __device__ __forceinline__ void nonlinear_increment(const float2 (&hi)[16], float2 (&lo)[16], float2 (&w)[16],
                                                     float th_hi, float th_lo) {
