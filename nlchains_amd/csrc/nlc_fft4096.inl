@@ -29,11 +29,77 @@ __device__ __forceinline__ C w16() {
 // tracked for free instead of copying registers.
 __host__ __device__ constexpr int pos(int k) { return 4 * (k & 3) + (k >> 2); }
 
+// DFT4 (signs as dft4) of (x0, w^A x1, w^2A x2, w^3A x3), w = exp(SIGN 2 pi i/16),
+// A = 1, 2, 3, in FMA form (Linzer & Feig, IBM J. Res. Dev. 37 (1993)): each
+// twiddle is a real factor, cos(pi/4) or cos(pi/8), times a complex number
+// with components +-1 and +-tan(pi/8), so the rotation costs two adds or two
+// FMAs and the factor rides on the FMAs that combine the results.  20 or 22
+// instructions where the complex products and adds take 24 or 28; the same
+// transform up to rounding.
+template <int SIGN, int A, class C>
+__device__ __forceinline__ void dft4_tw(C &x0, C &x1, C &x2, C &x3) {
+	using R = typename Cplx<C>::R;
+	constexpr R h = R(0.70710678118654752440), c1 = R(0.92387953251128675613), tau = R(0.41421356237309504880);
+	constexpr R S = R(SIGN);
+	// t0, t1 = x0 +- w^{2A} x2
+	C t0, t1;
+	if constexpr (A == 2) { // w^4 = S i
+		const C z = Cplx<C>::make(-S * x2.y, S * x2.x);
+		t0 = cadd(x0, z);
+		t1 = csub(x0, z);
+	} else { // w^2 = h (1 + S i), w^6 = h (-1 + S i)
+		constexpr R g = A == 1 ? R(1) : R(-1);
+		const C z = Cplx<C>::make(g * x2.x - S * x2.y, g * x2.y + S * x2.x);
+		t0 = Cplx<C>::make(fma(h, z.x, x0.x), fma(h, z.y, x0.y));
+		t1 = Cplx<C>::make(fma(-h, z.x, x0.x), fma(-h, z.y, x0.y));
+	}
+	// t2, t3 = w^A x1 +- w^{3A} x3 = f (up, um)
+	C up, um;
+	R f;
+	if constexpr (A == 2) { // w^2 (x1 +- S i x3), w^2 = h (1 + S i)
+		const C p = Cplx<C>::make(x1.x - S * x3.y, x1.y + S * x3.x);
+		const C m = Cplx<C>::make(x1.x + S * x3.y, x1.y - S * x3.x);
+		up = Cplx<C>::make(p.x - S * p.y, p.y + S * p.x);
+		um = Cplx<C>::make(m.x - S * m.y, m.y + S * m.x);
+		f = h;
+	} else {
+		// w^1 = c1 (1 + S i tau), w^3 = c1 (tau + S i), w^9 = -c1 (1 + S i tau)
+		C a, b;
+		if constexpr (A == 1) {
+			a = Cplx<C>::make(fma(-S * tau, x1.y, x1.x), fma(S * tau, x1.x, x1.y));
+			b = Cplx<C>::make(fma(tau, x3.x, -S * x3.y), fma(tau, x3.y, S * x3.x));
+		} else {
+			a = Cplx<C>::make(fma(tau, x1.x, -S * x1.y), fma(tau, x1.y, S * x1.x));
+			b = Cplx<C>::make(fma(S * tau, x3.y, -x3.x), fma(-S * tau, x3.x, -x3.y));
+		}
+		up = cadd(a, b);
+		um = csub(a, b);
+		f = c1;
+	}
+	// y0, y2 = t0 +- f up; y1, y3 = t1 +- S i f um
+	x0 = Cplx<C>::make(fma(f, up.x, t0.x), fma(f, up.y, t0.y));
+	x2 = Cplx<C>::make(fma(-f, up.x, t0.x), fma(-f, up.y, t0.y));
+	x1 = Cplx<C>::make(fma(-S * f, um.y, t1.x), fma(S * f, um.x, t1.y));
+	x3 = Cplx<C>::make(fma(S * f, um.y, t1.x), fma(-S * f, um.x, t1.y));
+}
+
+// The second layer's twiddles are folded into its DFT4s (dft4_tw) in FP64;
+// the FP32 transforms of the compensated step keep the complex products.
+template <class C>
+constexpr bool kFoldTwiddles = sizeof(typename Cplx<C>::R) == 8;
+
 // natural in -> transposed out
 template <int SIGN, class C>
 __device__ __forceinline__ void dft16_n2t(C v[16]) {
 #pragma unroll
 	for (int b = 0; b < 4; ++b) dft4<SIGN>(v[b], v[4 + b], v[8 + b], v[12 + b]); // u[b][c] at 4c + b
+	if constexpr (kFoldTwiddles<C>) { // DFT4 over b of W16^{bc} u[b][c]
+		dft4<SIGN>(v[0], v[1], v[2], v[3]);
+		dft4_tw<SIGN, 1>(v[4], v[5], v[6], v[7]);
+		dft4_tw<SIGN, 2>(v[8], v[9], v[10], v[11]);
+		dft4_tw<SIGN, 3>(v[12], v[13], v[14], v[15]);
+		return;
+	}
 	v[4 * 1 + 1] = cmul(v[4 * 1 + 1], w16<SIGN, 1, C>());
 	v[4 * 2 + 1] = cmul(v[4 * 2 + 1], w16<SIGN, 2, C>());
 	v[4 * 3 + 1] = cmul(v[4 * 3 + 1], w16<SIGN, 3, C>());
@@ -56,6 +122,13 @@ template <int SIGN, class C>
 __device__ __forceinline__ void dft16_t2n(C v[16]) {
 #pragma unroll
 	for (int c = 0; c < 4; ++c) dft4<SIGN>(v[4 * c], v[4 * c + 1], v[4 * c + 2], v[4 * c + 3]); // Y[c][b'] at 4c + b'
+	if constexpr (kFoldTwiddles<C>) { // DFT4 over c of W16^{cb} Y[c][b]
+		dft4<SIGN>(v[0], v[4], v[8], v[12]);
+		dft4_tw<SIGN, 1>(v[1], v[5], v[9], v[13]);
+		dft4_tw<SIGN, 2>(v[2], v[6], v[10], v[14]);
+		dft4_tw<SIGN, 3>(v[3], v[7], v[11], v[15]);
+		return;
+	}
 	v[4 * 1 + 1] = cmul(v[4 * 1 + 1], w16<SIGN, 1, C>());
 	v[4 * 2 + 1] = cmul(v[4 * 2 + 1], w16<SIGN, 2, C>());
 	v[4 * 3 + 1] = cmul(v[4 * 3 + 1], w16<SIGN, 3, C>());
