@@ -83,10 +83,9 @@ __device__ __forceinline__ void dft4_tw(C &x0, C &x1, C &x2, C &x3) {
 	x3 = Cplx<C>::make(fma(S * f, um.y, t1.x), fma(-S * f, um.x, t1.y));
 }
 
-// The second layer's twiddles are folded into its DFT4s (dft4_tw) in FP64;
-// the FP32 transforms of the compensated step keep the complex products.
+// The second layer's twiddles are folded into its DFT4s (dft4_tw).
 template <class C>
-constexpr bool kFoldTwiddles = sizeof(typename Cplx<C>::R) == 8;
+constexpr bool kFoldTwiddles = true;
 
 // natural in -> transposed out
 template <int SIGN, class C>
