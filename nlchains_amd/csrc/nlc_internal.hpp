@@ -39,13 +39,13 @@ struct StepArgs {
 //               sigma = sqrt(beta);
 //   kFormQuad   FPUT with beta = 0 != alpha: sigma = alpha;
 //   kFormPlain  every other case; Toda keeps v = sigma q with unscaled p,
-//               sigma = 64 alpha/ln2 (its bond exponent in 1/32 octaves;
-//               kernels with 25+ sites per lane use sigma/32, octaves).
+//               sigma = 256 alpha/ln2 (its bond exponent in 1/128 octaves;
+//               kernels with 25+ sites per lane use sigma/128, octaves).
 enum ForceForm { kFormPlain = 0, kFormCubic = 1, kFormQuad = 2 };
 
 // The form and sigma for a model's parameters.
 inline ForceForm force_form(int model, double alpha, double beta, double *sigma) {
-	*sigma = model == TODA ? alpha * 0x1.71547652b82fep+6 : 1.0; // 64/ln2
+	*sigma = model == TODA ? alpha * 0x1.71547652b82fep+8 : 1.0; // 256/ln2
 	if ((model == DNKG || model == DDNKG || (model == FPUT && alpha == 0.0)) && beta > 0.0) {
 		*sigma = __builtin_sqrt(beta);
 		return kFormCubic;

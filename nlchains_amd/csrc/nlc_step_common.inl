@@ -9,7 +9,7 @@ __constant__ double kExp2Tab[kExpTabSize] = NLC_EXP2_TABLE;
 template <int MODEL>
 __device__ __forceinline__ void load_exp_table(double *lds) {
 	if constexpr (MODEL == TODA) {
-		if (threadIdx.x < kExpTabSize) lds[threadIdx.x] = kExp2Tab[threadIdx.x];
+		for (uint32_t i = threadIdx.x; i < uint32_t(kExpTabSize); i += blockDim.x) lds[i] = kExp2Tab[i];
 		__syncthreads();
 	}
 }
@@ -111,7 +111,7 @@ __device__ __forceinline__ double site_force(double l, double q, double r, doubl
 //   Toda: e^{2 alpha x}, the force of DNKG_FPUT_Toda.cu:52-66 without its factor
 //         1/(2 alpha), which the kick folds into its weight (the constant
 //         -1/(2 alpha) of V' cancels in the difference), evaluated as
-//         2^(x'/32) on the bond x' = v_i - v_{i-1} of v = (64 alpha/ln2) q,
+//         2^(x'/128) on the bond x' = v_i - v_{i-1} of v = (256 alpha/ln2) q,
 //         table driven (fewest instructions), or for S >= 25 sites per lane,
 //         where the state takes 100+ VGPRs and hoisted table loads would
 //         leave one wave per SIMD, the table-free form, 2^x' on the bond of
@@ -123,21 +123,21 @@ __device__ __forceinline__ double bond_force(double x, const StepArgs &a, const 
 		else if constexpr (FORM == kFormQuad) return fma(x, x, x);
 		else return x * fma(x, fma(a.beta, x, a.alpha), 1.0);
 	} else if constexpr (S >= 25) return exp2_poly(x); // Toda: x = v_i - v_{i-1}
-	else return exp2_32_tab(x, etab);
+	else return exp2_128_tab(x, etab);
 }
 
 // Unit of the Toda state in a kernel with S sites per lane: 1 for the table
-// form's v = (64 alpha/ln2) q (bond exponent in 1/32 octaves), 2^-5 for the
+// form's v = (256 alpha/ln2) q (bond exponent in 1/128 octaves), 2^-7 for the
 // table-free form's (2 alpha/ln2) q (octaves).  Scaling by a power of two is
 // exact, so both hold the same trajectory.
 template <int MODEL, int S>
 __device__ __forceinline__ constexpr double toda_unit() {
-	return (MODEL == TODA && S >= 25) ? 0x1p-5 : 1.0;
+	return (MODEL == TODA && S >= 25) ? 0x1p-7 : 1.0;
 }
 
 // The state as the kernels hold it: sigma q and sigma p for the scaled forms
 // (then sigma F(q) = F~(sigma q), so drift and kick weights stay as they
-// are), v = sigma q with unscaled p for Toda (sigma = 64 alpha/ln2 times
+// are), v = sigma q with unscaled p for Toda (sigma = 256 alpha/ln2 times
 // toda_unit), else q and p.  The way back multiplies by 1/sigma: a division
 // would put its 10-instruction sequence between the stores of a lane, and the
 // L2 then writes partly filled lines back (a third more HBM write traffic

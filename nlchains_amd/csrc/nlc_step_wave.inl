@@ -20,7 +20,7 @@
 //    coefficient 1), the drift 1 FMA;
 //    FPUT and Toda exchange the bond force of the lane's first bond instead of
 //    recomputing it, so each bond term (a cubic or an FP64 exp) is evaluated
-//    once per substep; Toda works on v = (64 alpha/ln2) q with the 2^(x/32) of
+//    once per substep; Toda works on v = (256 alpha/ln2) q with the 2^(x/128) of
 //    nlc_exp.hpp (with 25+ sites per lane on v/32 and its 2^x);
 //  * the 7 substeps are unrolled with their weights in SGPRs; the c7 drift of a
 //    step is fused with the c0 drift of the next (the reference's 2*c0 merge).
@@ -88,7 +88,7 @@ struct Chain {
 
 // `steps` Yoshida steps of the chain part held by ch: the c7 drift of a step
 // fused with the c0 drift of the next (the reference's 2 c0 merge).  Toda
-// drifts v = (64 alpha/ln2) q (times toda_unit): its drift weights carry that
+// drifts v = (256 alpha/ln2) q (times toda_unit): its drift weights carry that
 // factor.
 template <int MODEL, class C>
 __device__ __forceinline__ void step_loop(C &ch, uint32_t steps, const StepArgs &a) {
@@ -117,9 +117,9 @@ __device__ __forceinline__ void step_loop(C &ch, uint32_t steps, const StepArgs 
 // One wavefront integrates `live_chains` (<= 64/L) consecutive chains of
 // length n starting at chain0 for `steps` steps.  All 64 lanes take part in
 // the halo exchange even when the wave is only partly filled.  Toda keeps
-// v = (64 alpha/ln2) q in registers (the exponent of its bond term is then a
-// plain difference in 1/32 octaves; v/32 and octaves from 25 sites per lane
-// on) and drifts v with the same factor times c; q = v / (64 alpha/ln2) is
+// v = (256 alpha/ln2) q in registers (the exponent of its bond term is then a
+// plain difference in 1/128 octaves; v/128 and octaves from 25 sites per lane
+// on) and drifts v with the same factor times c; q = v / (256 alpha/ln2) is
 // stored back.
 // The scaled force forms hold sigma q and sigma p.
 template <int MODEL, int L, int S, bool EXACT, int FORM>
@@ -201,7 +201,7 @@ struct Chain32 {
 		// magnitudes differ by more than 2^29); one rounding in the sum
 		const double x = (double(h1) - double(h0)) + (double(l1) - double(l0));
 		if constexpr (S >= 25) return exp2_poly(x);
-		else return exp2_32_tab(x, etab);
+		else return exp2_128_tab(x, etab);
 	}
 
 	__device__ __forceinline__ void kick(double d, const StepArgs &a) {

@@ -1,9 +1,10 @@
 """The two FP64 exponential forms of the Toda kernels
-(nlchains_amd/csrc/nlc_exp.hpp: table-driven 2^(x/32) and table-free 2^y),
+(nlchains_amd/csrc/nlc_exp.hpp: table-driven 2^(x/128) and table-free 2^y),
 compiled on the host from the same header, against 2^(x/32) computed to 40
 significant digits with Python's decimal module.  The tests state every
-argument as x (1/32 octaves); the table-free wrapper passes y = x/32, exactly,
-as the kernels' state scaling does."""
+argument as x (1/32 octaves); the table form's wrapper passes 4x (1/128
+octaves) and the table-free wrapper y = x/32, both exact, as the kernels'
+state scaling does."""
 import ctypes
 import math
 import re
@@ -22,7 +23,7 @@ WRAPPER = r"""
 #include "nlc_exp.hpp"
 static const double tab[nlc::kExpTabSize] = NLC_EXP2_TABLE;
 extern "C" void exp_many(const double *y, double *out, long n) {
-	for (long i = 0; i < n; ++i) out[i] = nlc::exp2_32_tab(y[i], tab);
+	for (long i = 0; i < n; ++i) out[i] = nlc::exp2_128_tab(y[i] * 4.0, tab);
 }
 extern "C" void exp_poly_many(const double *y, double *out, long n) {
 	for (long i = 0; i < n; ++i) out[i] = nlc::exp2_poly(y[i] * 0x1p-5);
@@ -58,6 +59,14 @@ def exp_fn(request, exp_tab):
     return exp_tab[request.param]
 
 
+# Bound on the error in ulps: the rounded table entry (at most 1/2 ulp), the
+# last FMA's rounding (1/2 ulp) and the polynomial (under 1/50 ulp) add up to
+# just over 1 ulp where the two roundings meet; 60 000 random arguments per
+# range reach 1.016 for the 256-entry table and 1.008 for the former 64-entry
+# one.
+ULP_BOUND = 1.02
+
+
 def ulp_errors(x, got):
     """Error of got = 2^(x/32) in units of the last place of the exact value."""
     getcontext().prec = 40
@@ -75,9 +84,9 @@ def test_table_entries_are_correctly_rounded():
     block = text[text.index("#define NLC_EXP2_TABLE"):]
     block = block[:block.index("}") + 1]
     vals = [float.fromhex(h) for h in re.findall(r"0x[0-9a-f.]+p[+-]\d+", block)]
-    assert len(vals) == 64
+    assert len(vals) == 256
     for j, v in enumerate(vals):
-        assert v == float(Decimal(2) ** (Decimal(j) / 64)), j
+        assert v == float(Decimal(2) ** (Decimal(j) / 256)), j
 
 
 def annotated_literals(text, tag):
@@ -95,7 +104,7 @@ def annotated_literals(text, tag):
 
 
 @pytest.mark.parametrize("tag,half,unit,ncoef,bound,taylor_miss", [
-    ("p", 0.25, 1 / 32, 5, 2.5e-18, 3e-17),  # table form: 2^(r/32), |r| <= 1/4
+    ("p", 0.25, 1 / 128, 4, 5e-18, 3e-17),  # table form: 2^(r/128), |r| <= 1/4
     ("q", 0.5, 1, 11, 2.1e-17, 1e-15),       # table-free form: 2^r, |r| <= 1/2
 ])
 def test_interpolants(tag, half, unit, ncoef, bound, taylor_miss):
@@ -139,21 +148,23 @@ def test_shifter():
 # x = 32 y / ln2 for y in (-1, 1), (-30, 30), (-700, 700), (-1e-6, 1e-6)
 @pytest.mark.parametrize("lo,hi,count", [(-46.0, 46.0, 4000), (-1385.0, 1385.0, 4000), (-32300.0, 32300.0, 2000),
                                          (-4.6e-5, 4.6e-5, 500)])
-def test_error_within_one_ulp(exp_fn, lo, hi, count):
+def test_error_within_ulp_bound(exp_fn, lo, hi, count):
     x = np.random.default_rng(int(abs(hi) * 10)).uniform(lo, hi, count)
     err = ulp_errors(x, exp_fn(x))
-    assert err.max() < 1.0, (err.max(), x[err.argmax()])
+    assert err.max() < ULP_BOUND, (err.max(), x[err.argmax()])
 
 
 def test_reduction_boundaries_and_special_values(exp_fn):
     k = np.arange(-4000, 4000, dtype=np.float64)
-    # interval ends of both reductions (N = nearest integer to 2x, resp. to x/32)
-    h = k / 2
-    x = np.concatenate([h, h + 0.25, h - 0.25, np.nextafter(h + 0.25, 0), np.nextafter(h - 0.25, 0),
+    # interval ends of both reductions (N = nearest integer to 8x, the table
+    # form's 2x' on x' = 4x in 1/128 octaves, resp. to x/32)
+    h = k / 8
+    x = np.concatenate([h, h + 0.0625, h - 0.0625, np.nextafter(h + 0.0625, 0), np.nextafter(h - 0.0625, 0),
+                        k / 2 + 0.25, k / 2 - 0.25,
                         32 * k[::8] + 15.9999999, 32 * k[::8] - 15.9999999, 32 * k[::8] + 16.0])
     x = x[np.abs(x) < 32000]
     err = ulp_errors(x, exp_fn(x))
-    assert err.max() < 1.0
+    assert err.max() < ULP_BOUND
     out = exp_fn(np.array([0.0, -0.0, 32767.0, 32768.0, 50000.0, -34500.0, -50000.0]))
     assert out[0] == 1.0 and out[1] == 1.0
     assert np.isfinite(out[2]) and out[3] == np.inf and out[4] == np.inf
@@ -181,7 +192,7 @@ def test_saturation_far_out_of_range(exp_fn):
 
 
 def test_beyond_exact_reduction_range(exp_tab):
-    """From |x| = 2^50 on, the table form's remainder x - N/2 is no longer
+    """From |4x| = 2^50 on, the table form's remainder 4x - N/2 is no longer
     exact: a positive argument still gives a non-finite value (the bond force
     of a blown-up chain is never finite) and a negative one 0 or a non-finite
     value, never a finite nonzero number.  The table-free form's remainder is

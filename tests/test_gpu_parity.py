@@ -98,7 +98,7 @@ def test_move_wave_shapes(gpu, n, L, S):
                                    (30, 1, 30), (512, 16, 32), (500, 16, 32), (2048, 64, 32), (2000, 64, 32)])
 def test_toda_exp_forms(gpu, n, L, S):
     """Toda on both exponential forms: up to 24 sites per lane the table form
-    on v = (64 alpha/ln2) q, from 25 on the table-free form on the state
+    on v = (256 alpha/ln2) q, from 25 on the table-free form on the state
     scaled to octaves (v/32); 100 steps against the oracle.  Energy density
     0.25: at 0.5 the 2048-site plane waves of this seed start with bond
     stretches up to 3.4 and one chain goes non-finite within 100 steps of
