@@ -15,7 +15,8 @@ from oracle import oracle as O
 from parity import chain_errors, chain_rel_err, oracle_parallel
 
 CHOICES = [32, 64, 128, 256, 512, 1024, 2048]
-# chains (caller order) that the verdict of the half-density run named
+# chains (caller order) of the seed-4 ensemble at energy density up to 0.5 whose
+# FP64 oracle run goes non-finite within the first 1000-step batch
 DIVERGING = [4634, 9226, 19670, 39921, 44023]
 
 
