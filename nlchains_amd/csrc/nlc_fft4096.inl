@@ -160,8 +160,17 @@ __device__ __forceinline__ void twiddle_run(C v[16], const C *__restrict__ tw, u
 	const R c2 = R(2) * w.x; // 2 cos(theta), the same for W and its conjugate
 #pragma unroll
 	for (int q = 0; q < 4; ++q) {
-		C r = tw[(e0 + uint32_t(4 * q) * de) & 4095u];
-		if (CONJ) r = cconj(r);
+		// with e0 == 0 the first run starts at the table's entry 0, (1, -0)
+		// (conjugated when CONJ), and its next power is W^{de} itself: the same
+		// bits as the load and the product, without either
+		const bool unit = ONE0 && q == 0;
+		C r;
+		if (unit) {
+			r = Cplx<C>::make(R(1), CONJ ? R(0) : -R(0));
+		} else {
+			r = tw[(e0 + uint32_t(4 * q) * de) & 4095u];
+			if (CONJ) r = cconj(r);
+		}
 		C rp = r; // the power before r (used from j = 1 on)
 #pragma unroll
 		for (int j = 0; j < 4; ++j) {
@@ -171,6 +180,7 @@ __device__ __forceinline__ void twiddle_run(C v[16], const C *__restrict__ tw, u
 			if (j < 3) {
 				C rn;
 				if (kRecur && j >= 1) rn = Cplx<C>::make(fma(c2, r.x, -rp.x), fma(c2, r.y, -rp.y));
+				else if (unit && j == 0) rn = wc;
 				else rn = cmul(r, wc);
 				rp = r;
 				r = rn;
