@@ -162,8 +162,10 @@ __device__ __forceinline__ void twiddle_run(C v[16], const C *__restrict__ tw, u
 	for (int q = 0; q < 4; ++q) {
 		// with e0 == 0 the first run starts at the table's entry 0, (1, -0)
 		// (conjugated when CONJ), and its next power is W^{de} itself: the same
-		// bits as the load and the product, without either
-		const bool unit = ONE0 && q == 0;
+		// bits as the load and the product, without either (FP64 only: in the
+		// FP32 kernel the shorter run changed the register allocation and put
+		// spill stores inside the step loop)
+		const bool unit = kRecur && ONE0 && q == 0;
 		C r;
 		if (unit) {
 			r = Cplx<C>::make(R(1), CONJ ? R(0) : -R(0));
